@@ -1,0 +1,314 @@
+"""Benchmark: particle-steps/s of the FP32-force / FP64-state HIP plate step.
+
+Workload (default): the large single ice plate, a 1024 x 128 reference lattice
+(131 072 disks, 520 834 bonds), both ends clamped, circular indenter of
+radius 0.5 m (20 disk radii) moving down at 0.5 m/s, axial / shear breaking
+strains 0.01 / 0.02, degree-bounded dt (8 partners per disk; the reference's
+all-pairs bound would give 5.5e-8 s and the indenter would never reach the
+plate), 20 000 steps.  Inputs are the deterministic lattice: no random data.
+
+Reported beside the kernel rate:
+  * plain_fp64_gpu: the same step written as eager PyTorch FP64 on the same
+    GPU (sort-based cell list, the reference's ops), timed on a short run;
+  * fp64_kernel: the HIP step with FP64 forces;
+  * roofline: algorithmic bytes per launch / mean kernel time (HIP events on
+    the launch stream) against the 8 TB/s HBM peak;
+  * cpu_baseline: the C oracle on one host core over a bounded sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "particle-steps/s FP32-force vs FP64 plain on 1 GPU; p50/p99 batched solve latency"
+HBM_PEAK_GBS = 8000.0
+# compulsory bytes per disk per step of plate_step_kernel: x, v read (32) and
+# written (32), link bits read+written (2), fixed flag (1), candidate count (1),
+# list-build offset xb (8)
+PLATE_BYTES_PER_DISK = 76
+
+
+def c2_config(device="cuda"):
+    from tensordem_amd import DEMConfig
+    return DEMConfig(nx=1024, ny=128, radius=0.025, tool_radius=0.5, tool_speed=0.5,
+                     breaking_strain=0.01, shear_breaking_strain=0.02, device=device)
+
+
+# ---------------------------------------------------------------------------
+# plain FP64 eager PyTorch step (the speedup denominator)
+# ---------------------------------------------------------------------------
+class PlainFP64:
+    """IceDEM.step (src/tensordem/dem.py:226-382) as eager FP64 PyTorch ops on
+    the GPU, with a per-step sort-based cell list instead of all N^2 pairs.
+    Ledger sums stay on the device (no host syncs)."""
+
+    def __init__(self, cfg, dt, device="cuda"):
+        import torch
+        from tensordem_amd.lattice import bonds, fixed_mask, positions
+        dev = torch.device(device)
+        self.cfg, self.dt = cfg, dt
+        f64 = dict(dtype=torch.float64, device=dev)
+        self.x0 = torch.tensor(positions(cfg.nx, cfg.ny, cfg.radius), **f64)
+        self.x = self.x0.clone()
+        self.v = torch.zeros_like(self.x)
+        self.fixed = torch.tensor(fixed_mask(cfg.nx, cfg.ny, cfg.fix_edges), device=dev)
+        bi, bj, rest, _ = bonds(cfg.nx, cfg.ny, cfg.radius)
+        self.i = torch.tensor(bi, device=dev)
+        self.j = torch.tensor(bj, device=dev)
+        self.rest = torch.tensor(rest, **f64)
+        self.alive = torch.ones(len(bi), dtype=torch.bool, device=dev)
+        self.refv = self.x0[self.j] - self.x0[self.i]
+        n = len(self.x)
+        self.n = n
+        self.bond_keys = torch.sort(self.i * n + self.j).values
+        self.time = torch.zeros((), **f64)
+        self.acc = torch.zeros(5, **f64)
+        self.tool_start = torch.tensor([0.0, (cfg.ny - 1) * 2 * cfg.radius + cfg.radius
+                                        + cfg.tool_radius + cfg.tool_gap], **f64)
+        self.tool_v = torch.tensor([0.0, -cfg.tool_speed], **f64)
+
+    def _contacts(self):
+        import torch
+        c, x, n = self.cfg, self.x, self.n
+        h = 2 * c.radius
+        cell = torch.floor(x / h).long()
+        cell -= cell.min(0).values
+        ncx = int(cell[:, 0].max()) + 3  # host sync, as a plain implementation would
+        key = (cell[:, 1] + 1) * ncx + cell[:, 0] + 1
+        order = torch.argsort(key)
+        skey = key[order]
+        pi, pj = [], []
+        for oy in (-1, 0, 1):
+            for ox in (-1, 0, 1):
+                nk = key + oy * ncx + ox
+                lo = torch.searchsorted(skey, nk)
+                hi = torch.searchsorted(skey, nk, right=True)
+                cnt = hi - lo
+                src = torch.repeat_interleave(torch.arange(n, device=x.device), cnt)
+                start = torch.repeat_interleave(lo, cnt)
+                off = torch.arange(len(src), device=x.device) - torch.repeat_interleave(
+                    torch.cumsum(cnt, 0) - cnt, cnt)
+                dst = order[start + off]
+                pi.append(src)
+                pj.append(dst)
+        a, b = torch.cat(pi), torch.cat(pj)
+        keep = a < b
+        a, b = a[keep], b[keep]
+        d = torch.linalg.vector_norm(x[b] - x[a], dim=1)
+        keep = d < 2 * c.radius
+        a, b = a[keep], b[keep]
+        bonded = torch.isin(a * n + b, self.bond_keys)
+        return a[~bonded], b[~bonded]
+
+    def step(self):
+        import torch
+        c, x, v, i, j = self.cfg, self.x, self.v, self.i, self.j
+        n = self.n
+        # shear strain (dem.py:182-208)
+        cur = x[j] - x[i]
+        ro = self.refv[:, :, None] * self.refv[:, None, :]
+        cr = cur[:, :, None] * self.refv[:, None, :]
+        cov = torch.zeros((n, 2, 2), dtype=x.dtype, device=x.device)
+        crs = torch.zeros_like(cov)
+        cov.index_add_(0, i, ro)
+        cov.index_add_(0, j, ro)
+        crs.index_add_(0, i, cr)
+        crs.index_add_(0, j, cr)
+        F = crs @ torch.linalg.pinv(cov)
+        E = 0.5 * (F.transpose(1, 2) @ F - torch.eye(2, dtype=x.dtype, device=x.device))
+        s = torch.sqrt((E[:, 0, 0] - E[:, 1, 1]).square() + 4.0 * E[:, 0, 1].square())
+        # fracture (dem.py:238-277)
+        dist = torch.linalg.vector_norm(cur, dim=1)
+        nrm = cur / dist.clamp_min(torch.finfo(x.dtype).eps)[:, None]
+        ext = dist - self.rest
+        fail = ((ext / self.rest) > c.breaking_strain) | (0.5 * (s[i] + s[j]) > c.shear_breaking_strain)
+        newly = self.alive & fail
+        self.acc[0] += (0.5 * c.bond_stiffness * ext[newly].square()).sum()
+        self.alive &= ~fail
+        # bond + broken-bond contact forces
+        ov = (2 * c.radius - dist).clamp_min(0)
+        vn = ((v[j] - v[i]) * nrm).sum(1)
+        rep = (c.contact_stiffness * ov - c.contact_damping * vn).clamp_min(0)
+        rep = torch.where((ov > 0) & ~self.alive, rep, torch.zeros_like(rep))
+        mag = torch.where(self.alive, c.bond_stiffness * ext, -rep)
+        pf = mag[:, None] * nrm
+        force = -c.drag * v
+        force.index_add_(0, i, pf)
+        force.index_add_(0, j, -pf)
+        # unbonded contacts
+        a, b = self._contacts()
+        d = x[b] - x[a]
+        dd = torch.linalg.vector_norm(d, dim=1)
+        nn = d / dd.clamp_min(torch.finfo(x.dtype).eps)[:, None]
+        ovc = 2 * c.radius - dd
+        vnc = ((v[b] - v[a]) * nn).sum(1)
+        repc = (c.contact_stiffness * ovc - c.contact_damping * vnc).clamp_min(0)
+        cf = -repc[:, None] * nn
+        force.index_add_(0, a, cf)
+        force.index_add_(0, b, -cf)
+        # tool (dem.py:294-312)
+        tp = self.tool_start + self.time * self.tool_v
+        td = x - tp
+        tdist = torch.linalg.vector_norm(td, dim=1)
+        tn = td / tdist.clamp_min(torch.finfo(x.dtype).eps)[:, None]
+        tov = (c.radius + c.tool_radius - tdist).clamp_min(0)
+        tsp = ((v - self.tool_v) * tn).sum(1)
+        tmag = torch.where(tov > 0, (c.contact_stiffness * tov - c.contact_damping * tsp).clamp_min(0),
+                           torch.zeros_like(tov))
+        tf = tmag[:, None] * tn
+        force += tf
+        self.acc[1] += self.dt * (tf.sum(0) @ self.tool_v)
+        # integrate (dem.py:363-371)
+        self.v += self.dt * force / c.mass
+        self.v[self.fixed] = 0
+        self.x += self.dt * self.v
+        self.x[self.fixed] = self.x0[self.fixed]
+        self.time += self.dt
+
+
+def time_plain(cfg, dt, steps, warmup):
+    import torch
+    p = PlainFP64(cfg, dt)
+    for _ in range(warmup):
+        p.step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        p.step()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps
+
+
+# ---------------------------------------------------------------------------
+def kernel_event_time(eng, launches):
+    """Mean duration of single step launches, bracketed by HIP events on the
+    launch stream."""
+    import torch
+    stream = torch.cuda.current_stream()
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(launches)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(launches)]
+    for k in range(launches):
+        starts[k].record(stream)
+        eng.step(1)
+        ends[k].record(stream)
+    torch.cuda.synchronize()
+    return sum(s.elapsed_time(e) for s, e in zip(starts, ends)) / launches * 1e-3
+
+
+def cpu_baseline(cfg, dt, budget_s=12.0):
+    """C oracle (FP64, one core) on the same plate, run for ~budget_s."""
+    from oracle.plate import OraclePlate
+    sim = OraclePlate(cfg, dt=dt)
+    sim.step()
+    t0 = time.perf_counter()
+    steps = 0
+    while time.perf_counter() - t0 < budget_s:
+        sim.step()
+        steps += 1
+    el = time.perf_counter() - t0
+    n = cfg.nx * cfg.ny
+    return dict(value=n * steps / el, unit="particle-steps/s", cores=1, kind="port",
+                sample=f"{steps} steps of the {cfg.nx}x{cfg.ny} plate from rest "
+                       f"(FP64 C oracle, cell grid rebuilt every step), {el:.1f} s")
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--chunk", type=int, default=100, help="steps per captured graph")
+    ap.add_argument("--plain-steps", type=int, default=30)
+    ap.add_argument("--event-launches", type=int, default=200)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    args = ap.parse_args(argv)
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: every plate and request of this workload runs on a "
+              "single MI355X and no multi-GPU path is built (see DESIGN.md).", file=sys.stderr)
+        return 2
+    import torch
+    from tensordem_amd.engine import PlateEngine, PlateSpec
+
+    cfg = c2_config()
+    dt = cfg.degree_bound_dt()
+    n = cfg.nx * cfg.ny
+    chunk = max(2, args.chunk - args.chunk % 2)
+    steps = max(chunk, args.steps - args.steps % chunk)
+
+    def engine(fp32):
+        return PlateEngine([PlateSpec(cfg, dt=dt)], fp32=fp32)
+
+    results = {}
+    for fp32 in (True, False):
+        eng = engine(fp32)
+        eng.step(args.warmup - args.warmup % 2)
+        eng.graph(chunk // 2)
+        torch.cuda.synchronize()
+        stream = torch.cuda.current_stream()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        e0.record(stream)
+        eng.run_graph(chunk // 2, repeats=steps // chunk)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        gpu_s = e0.elapsed_time(e1) * 1e-3
+        st = eng.status()[0]
+        broken = int((~eng.alive).sum())
+        k_s = kernel_event_time(eng, args.event_launches)
+        results["fp32" if fp32 else "fp64"] = dict(
+            seconds=gpu_s, wall_s=wall, ms_per_step=gpu_s / steps * 1e3,
+            rate=n * steps / gpu_s, kernel_s=k_s, status=int(st), broken=broken)
+        eng.close()
+        del eng
+    r32, r64 = results["fp32"], results["fp64"]
+
+    plain = None
+    if not args.no_plain:
+        ps = time_plain(cfg, dt, args.plain_steps, 3)
+        plain = dict(value=n / ps, unit="particle-steps/s", ms_per_step=ps * 1e3,
+                     steps=args.plain_steps,
+                     what="eager PyTorch FP64 restatement of IceDEM.step on the same GPU "
+                          "(per-step sort-based cell list)")
+    bytes_per_launch = PLATE_BYTES_PER_DISK * n
+    achieved = bytes_per_launch / r32["kernel_s"] / 1e9
+    line = {
+        "metric": METRIC, "value": r32["rate"], "unit": "particle-steps/s", "n_gpus": 1,
+        "steps": steps, "warmup": args.warmup, "ms_per_step": r32["ms_per_step"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32-force/f64-state", "data": "synthetic: deterministic reference lattice",
+        "config": {"workload": "large single ice plate 1024x128 (131072 disks, 520834 bonds), "
+                               "indenter r=0.5 m at 0.5 m/s, strains 0.01/0.02, "
+                               f"degree-bounded dt={dt:.6g} s, 20000 steps",
+                   "nx": 1024, "ny": 128, "dt_s": dt, "graph_chunk_steps": chunk,
+                   "broken_bonds_end": r32["broken"], "status_bits": r32["status"]},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "plate_step_kernel<float>",
+                     "kernel_us": r32["kernel_s"] * 1e6,
+                     "bytes_per_launch": bytes_per_launch},
+        "fp64_kernel": {"value": r64["rate"], "ms_per_step": r64["ms_per_step"],
+                        "kernel_us": r64["kernel_s"] * 1e6, "broken_bonds_end": r64["broken"]},
+    }
+    if plain:
+        line["plain_fp64_gpu"] = plain
+        line["speedup_vs_plain_fp64"] = r32["rate"] / plain["value"]
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(c2_config("cpu"), dt, args.cpu_seconds)
+    print(json.dumps(line))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,198 @@
+/*
+ * tensordem_amd: MI355X (gfx950) kernels for the force-and-integrate step of
+ * the 2-D bonded-disk ice solver, behind a plain C ABI.
+ *
+ * The reference solver is pure Python/PyTorch and exposes no foreign-function
+ * interface; the entry points below replace the bodies of its Python methods
+ * and are bound from Python with ctypes (tensordem_amd/_lib.py).  Every
+ * pointer argument is a device pointer (HIP global memory) unless the comment
+ * says "host".  Arrays are owned by the caller; the library allocates nothing
+ * except graph objects inside a tdm_graph.  `stream` is a hipStream_t
+ * (NULL = default stream).  Every function returns TDM_OK (0) or a negative
+ * TDM_E* code; tdm_last_error() describes the most recent failure.
+ *
+ * Particle layout: one lattice ("member") of nx*ny disks is stored row-major
+ * (particle = first + iy*nx + ix), exactly the order of the reference
+ * (src/tensordem/dem.py:105-110).  Bonds of a member are its 8-neighbour
+ * lattice links in the reference's triu order (dem.py:116-124); a bond is
+ * identified by (lower particle, direction code 4..7) and its global index is
+ * fwd_bond[lower] + rank of the code among the lower particle's links.
+ */
+#ifndef TENSORDEM_AMD_H
+#define TENSORDEM_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TDM_OK 0
+#define TDM_EINVAL (-1)   /* bad argument or shape */
+#define TDM_EHIP (-2)     /* a HIP runtime call failed */
+#define TDM_ESPACE (-3)   /* workspace too small */
+
+/* status bits accumulated per member by the kernels (tdm_plate_member.status) */
+#define TDM_ST_NONFINITE 1      /* x or v became non-finite (dem.py:379-382) */
+#define TDM_ST_STALE_LIST 2     /* a disk moved >= skin/2 since the contact list was built */
+#define TDM_ST_CAND_OVERFLOW 4  /* more unbonded contact candidates than slots */
+#define TDM_ST_CELL_OVERFLOW 8  /* more disks in one hash cell than its capacity */
+
+/* ledger layout (accounting, per member, float64) -- the dict of dem.py:139-144 */
+#define TDM_ACC_EXT_WORK 0
+#define TDM_ACC_RELEASE 1
+#define TDM_ACC_ACQUIRED 2
+#define TDM_ACC_TOOL_WORK 3
+#define TDM_ACC_DAMPING 4
+#define TDM_ACC_EXT_IMP 5   /* x, y */
+#define TDM_ACC_SUP_IMP 7   /* x, y */
+#define TDM_ACC_DRAG_IMP 9  /* x, y */
+#define TDM_ACC_TOOL_IMP 11 /* x, y */
+#define TDM_NACC 13
+
+/* last-step record (per member, float64) -- the dict of dem.py:372-376 */
+#define TDM_LAST_T0 0
+#define TDM_LAST_T1 1
+#define TDM_LAST_WORK 2
+#define TDM_LAST_SUPPORT 3  /* x, y */
+#define TDM_LAST_REACTION 5 /* x, y */
+#define TDM_LAST_EXTERNAL 7 /* x, y */
+#define TDM_LAST_MOMENT 9
+#define TDM_NLAST 10
+
+/*
+ * One plate.  Lives in device memory (array of nmembers); the step kernel
+ * advances `time`, `step_count`, `mode` and `tab` and ORs into `status`.
+ * Field meaning follows DEMConfig (dem.py:17-35) unless noted.
+ */
+typedef struct tdm_plate_member {
+  double radius, bond_stiffness, contact_stiffness;
+  double breaking_strain, shear_breaking_strain;
+  double contact_damping, drag, tool_radius;
+  double dt, mass;
+  double tool_start[2];   /* circle: tool start; hull: hull_start */
+  double tool_velocity[2];/* circle: (0,-tool_speed); hull: hull_velocity */
+  double time;            /* simulated time at the start of the next step */
+  double skin;            /* Verlet skin for unbonded contact candidates [m] */
+  double cell_size;       /* hash cell edge, >= 2*radius + skin [m] */
+  double grid_origin[2];  /* lower-left corner of the member's hash grid [m] */
+  double fp32_margin;     /* relative band around a break threshold re-checked in FP64 */
+  int64_t step_count;
+  int32_t nx, ny;         /* lattice size */
+  int32_t first;          /* first particle index */
+  int32_t bond_first;     /* first global bond index */
+  int32_t nbonds;
+  int32_t hull_first, hull_m; /* hull vertices [hull_first, +hull_m) in `hull`; 0 = circle */
+  int32_t ncx, ncy, cell_first; /* hash grid size and first cell index */
+  int32_t tile_x;         /* tile width in disks (16..256); tile height = 256 / tile_x */
+  int32_t block_first, nblocks; /* this member's range in the block table */
+  int32_t mode;           /* 0 idle, 1 search candidates this step, 2 bin new positions */
+  int32_t tab;            /* hash table holding the current candidate source */
+  int32_t status;         /* TDM_ST_* bits */
+  int32_t has_loads;      /* loads array rows for this member are valid */
+  int32_t per_disk_strength; /* eps_t/eps_s come from the per-disk arrays */
+  int32_t pad_;
+} tdm_plate_member;
+
+/* Device arrays of a plate batch (all members share them). */
+typedef struct tdm_plate_arrays {
+  double *x, *v;            /* (N,2) state, updated by tdm_plate_step */
+  double *x_alt, *v_alt;    /* (N,2) second buffers (double buffering) */
+  const uint8_t *fixed;     /* (N,) clamped disks (dem.py:113-115) */
+  uint8_t *links;           /* (N,) alive bit per direction code 0..7 */
+  const int32_t *fwd_bond;  /* (N,) first global bond index owned by the disk */
+  uint8_t *alive;           /* (B,) dem.py:130 (bond-indexed) */
+  uint8_t *failure_mode;    /* (B,) 0 intact, 1 tensile, 2 shear, 3 both (dem.py:131-133) */
+  double *failure_time_s, *failure_extension_m, *failure_energy_J; /* (B,) */
+  const double *loads;      /* (N,2) external forces or NULL (dem.py:210-224) */
+  const double *hull;       /* (H,2) hull profile vertices or NULL (dem.py:154-169) */
+  const double *eps_t_disk, *eps_s_disk; /* (N,) per-disk thresholds or NULL */
+  double *accounting;       /* (nmembers, TDM_NACC) */
+  double *last;             /* (nmembers, TDM_NLAST) */
+  double *force;            /* (N,2) output of tdm_plate_forces, else may be NULL */
+  double *reaction;         /* (nmembers,2) output of tdm_plate_forces */
+  double *energy;           /* (nmembers,5) output of tdm_plate_energy */
+  tdm_plate_member *members;/* (nmembers,) */
+  void *workspace;          /* tdm_plate_workspace_bytes() bytes, 256-byte aligned */
+  size_t workspace_bytes;
+  int32_t n, nbonds, nmembers, nblocks, ncells;
+  int32_t cand_slots;       /* unbonded candidate slots per disk */
+  int32_t cell_cap;         /* disks per hash cell */
+  int32_t pad_;
+} tdm_plate_arrays;
+
+/* Library version (major*10000 + minor*100 + patch). */
+int tdm_version(void);
+
+/* Copies the last error message into buf (host), NUL-terminated. */
+int tdm_last_error(char *buf, size_t len);
+
+/* sizeof(tdm_plate_member) and sizeof(tdm_plate_arrays) as compiled, so a
+ * binding can check its mirror of the two structs. */
+int tdm_abi_sizes(size_t *member_bytes, size_t *arrays_bytes);
+
+/* Number of 256-disk tiles (thread blocks) a member needs for tile width tile_x. */
+int tdm_plate_tiles(int nx, int ny, int tile_x);
+
+/* Workspace bytes for a batch (block table, ledger partials, counters, hash
+ * tables, candidate lists). */
+size_t tdm_plate_workspace_bytes(int n, int nblocks, int nmembers, int ncells, int cand_slots,
+                                 int cell_cap);
+
+/*
+ * Prepares a batch: writes the XCD-aware block table into the workspace,
+ * derives per-disk link bits from `alive`, clears counters and hash tables
+ * and bins the current positions so the next step builds fresh contact
+ * candidate lists.  `members` must already hold the member table (device).
+ * Replaces the topology part of IceDEM.__init__ (dem.py:105-153).
+ */
+int tdm_plate_setup(const tdm_plate_arrays *a, void *stream);
+
+/* Re-derives link bits from `alive` after the caller edited `alive`
+ * (the reference lets callers assign IceDEM.alive directly). */
+int tdm_plate_sync_links(const tdm_plate_arrays *a, void *stream);
+
+/* Re-bins the current positions and schedules a candidate search; call after
+ * the caller moved disks (assigned positions). */
+int tdm_plate_rebin(const tdm_plate_arrays *a, void *stream);
+
+/*
+ * Advances every member by `nsteps` steps of IceDEM.step (dem.py:347-382):
+ * fracture commit, bond/contact/tool/hull forces, semi-implicit Euler, clamp,
+ * impulse/work/energy ledger.  State alternates between (x,v) and
+ * (x_alt,v_alt); after an odd nsteps the new state is in the _alt buffers.
+ * fp32 != 0 evaluates bond, contact and tool forces in FP32 (state, sums and
+ * ledger stay FP64; break decisions within fp32_margin of a threshold are
+ * re-evaluated in FP64).  `parity` (0/1) says which buffer holds the state on
+ * entry (0 = x/v).
+ */
+int tdm_plate_step(const tdm_plate_arrays *a, int nsteps, int parity, int fp32, void *stream);
+
+/*
+ * IceDEM.forces (dem.py:226-345): commits fracture, writes `force` (N,2) and
+ * the ice-on-tool `reaction` (nmembers,2); positions and time do not move.
+ */
+int tdm_plate_forces(const tdm_plate_arrays *a, int parity, int fp32, void *stream);
+
+/*
+ * IceDEM.mechanical_energy (dem.py:603-624): writes per member
+ * (kinetic, bond, pair contact, tool contact, total) into `energy`; commits
+ * nothing.
+ */
+int tdm_plate_energy(const tdm_plate_arrays *a, int parity, void *stream);
+
+/* Opaque captured sequence of steps (a hipGraph of 2*pairs steps). */
+typedef struct tdm_graph tdm_graph;
+
+/* Captures 2*pairs steps starting from parity 0 into a graph (the state ends
+ * in x/v again).  Kernel arguments are frozen at capture time. */
+int tdm_plate_graph_create(const tdm_plate_arrays *a, int pairs, int fp32, void *stream,
+                           tdm_graph **out);
+int tdm_graph_launch(tdm_graph *g, void *stream);
+int tdm_graph_destroy(tdm_graph *g);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TENSORDEM_AMD_H */

@@ -1,0 +1,21 @@
+"""MI355X-native force-and-integrate step for the TensorDEM 2-D bonded-disk ice solver.
+
+`DEMConfig` and `IceDEM` are drop-ins for `tensordem.DEMConfig` /
+`tensordem.IceDEM` (src/tensordem/dem.py); `PlateEngine` runs many plates in
+one launch.  The compute path is the HIP library built by
+`tensordem_amd.build.build()`; importing the package does not need it, but
+constructing an engine does, and fails loudly without it.
+"""
+from .config import DEMConfig
+
+__all__ = ["DEMConfig", "IceDEM", "PlateEngine", "PlateSpec"]
+
+
+def __getattr__(name):
+    if name == "IceDEM":
+        from .icedem import IceDEM
+        return IceDEM
+    if name in ("PlateEngine", "PlateSpec"):
+        from . import engine
+        return getattr(engine, name)
+    raise AttributeError(name)

@@ -1,0 +1,1151 @@
+// Fused force-and-integrate step for bonded ice plates on MI355X (gfx950).
+//
+// One launch advances every member (plate) of a batch by one step of the
+// reference solver IceDEM.step (src/tensordem/dem.py:347-382):
+//   * each 256-thread block owns a TX x TY tile of one lattice and stages the
+//     tile plus a 2-disk halo of positions in LDS;
+//   * the objective shear strain (dem.py:182-208) of the tile and a 1-disk
+//     ring is computed from LDS, so it never round-trips through HBM;
+//   * every disk gathers its bonds (8-neighbour stencil, no index lists) and
+//     its unbonded contact candidates in the reference's summation order, so
+//     forces are deterministic without float atomics;
+//   * fracture is committed by both ends of a bond with identical arithmetic
+//     (the lower-index end writes the bond's failure record);
+//   * tool or hull contact, loads, semi-implicit Euler, fixed clamp and the
+//     per-block ledger partials are fused in; the last block of each member
+//     sums that member's partials in a fixed order and advances its clock;
+//   * the cell hash for unbonded contacts is rebuilt in two pipelined phases
+//     (bin, then search) folded into ordinary steps, triggered on device when
+//     a disk has moved a quarter of the Verlet skin.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "../../include/tensordem_amd.h"
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kHalo = 2;
+constexpr int kMaxLds = 1300;   // (TX+4)*(TY+4) for TX in 16..256, TY = 256/TX
+constexpr int kMaxRing = 800;   // (TX+2)*(TY+2)
+constexpr int kCh = 16;         // ledger partial channels
+constexpr int kMaxCand = 16;    // upper bound for cand_slots
+constexpr double kEps64 = 2.220446049250313e-16;  // torch.finfo(float64).eps
+constexpr float kEps32 = 1.1920928955078125e-07f;
+
+// ledger partial channels
+enum {
+  C_RELEASED = 0, C_ACQUIRED, C_PAIR_DISS, C_TOOL_DISS, C_VSQ, C_TOOL_FX, C_TOOL_FY,
+  C_SUP_X, C_SUP_Y, C_VSUM_X, C_VSUM_Y, C_LOAD_X, C_LOAD_Y, C_WORK, C_MOMENT, C_BREAKS
+};
+
+enum { MODE_IDLE = 0, MODE_SEARCH = 1, MODE_BIN = 2 };
+enum { OP_FORCES = 0, OP_STEP = 1, OP_ENERGY = 2 };
+// energy channels (OP_ENERGY reuses the partial buffer)
+enum { E_BOND = 0, E_CONTACT, E_TOOL, E_VSQ };
+
+// direction codes in ascending flat-index offset: -nx-1, -nx, -nx+1, -1, +1, nx-1, nx, nx+1
+__host__ __device__ constexpr int dxc(int c) { return c < 3 ? c - 1 : (c == 3 ? -1 : (c == 4 ? 1 : c - 6)); }
+__host__ __device__ constexpr int dyc(int c) { return c < 3 ? -1 : (c < 5 ? 0 : 1); }
+
+struct Workspace {
+  int4 *blocks;          // (member, tile x0, tile y0, block rank in member)
+  double *partials;      // [nblocks][kCh]
+  unsigned *counters;    // [nmembers]
+  int *req;              // [nmembers] rebuild requests
+  int *cell_count;       // [2][ncells]
+  int *cell_tab;         // [2][ncells][cap]
+  int *cand;             // [slots][n]
+  uint8_t *ncand;        // [n]
+  float2 *xb;            // [n] position - lattice site when the list was built
+};
+
+__host__ __device__ inline size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
+
+__host__ __device__ inline void carve(void *base, int n, int nblocks, int nmembers, int ncells,
+                                      int slots, int cap, Workspace *w, size_t *total)
+{
+  size_t off = 0;
+  char *b = static_cast<char *>(base);
+  auto take = [&](size_t bytes) {
+    char *p = b ? b + off : nullptr;
+    off += align256(bytes);
+    return p;
+  };
+  w->blocks = reinterpret_cast<int4 *>(take(sizeof(int4) * size_t(nblocks)));
+  w->partials = reinterpret_cast<double *>(take(sizeof(double) * kCh * size_t(nblocks)));
+  w->counters = reinterpret_cast<unsigned *>(take(sizeof(unsigned) * size_t(nmembers)));
+  w->req = reinterpret_cast<int *>(take(sizeof(int) * size_t(nmembers)));
+  w->cell_count = reinterpret_cast<int *>(take(sizeof(int) * 2 * size_t(ncells)));
+  w->cell_tab = reinterpret_cast<int *>(take(sizeof(int) * 2 * size_t(ncells) * size_t(cap)));
+  w->cand = reinterpret_cast<int *>(take(sizeof(int) * size_t(slots) * size_t(n)));
+  w->ncand = reinterpret_cast<uint8_t *>(take(size_t(n)));
+  w->xb = reinterpret_cast<float2 *>(take(sizeof(float2) * size_t(n)));
+  *total = off;
+}
+
+__device__ inline double2 lattice_site(int ix, int iy, double half_span, double two_r)
+{
+  // dem.py:105-110: (ix - (nx-1)/2) * 2r, iy * 2r
+  return make_double2(((double)ix - half_span) * two_r, (double)iy * two_r);
+}
+
+__device__ inline bool on_lattice(int ix, int iy, int nx, int ny)
+{
+  return ix >= 0 && ix < nx && iy >= 0 && iy < ny;
+}
+
+__device__ inline uint32_t present_links(int ix, int iy, int nx, int ny)
+{
+  uint32_t m = 0;
+#pragma unroll
+  for (int c = 0; c < 8; ++c)
+    if (on_lattice(ix + dxc(c), iy + dyc(c), nx, ny)) m |= 1u << c;
+  return m;
+}
+
+// ---------------------------------------------------------------------------
+// Shear strain (dem.py:182-208).  Bonds are visited as index_add_ visits
+// them: partners above the disk by ascending index (codes 4..7), then
+// partners below (codes 0..3).  pinv of the 2x2 covariance is the closed-form
+// inverse.
+// ---------------------------------------------------------------------------
+struct Tile {
+  const double2 *sx;  // LDS positions, row stride lw, origin (tx0-2, ty0-2)
+  int lw, tx0, ty0, nx, ny;
+  double half_span, two_r;
+  __device__ double2 x(int ix, int iy) const { return sx[(iy - ty0 + kHalo) * lw + (ix - tx0 + kHalo)]; }
+};
+
+__device__ double strain64(const Tile &t, int ix, int iy)
+{
+  double c00 = 0, c01 = 0, c10 = 0, c11 = 0, x00 = 0, x01 = 0, x10 = 0, x11 = 0;
+  const double2 xp = t.x(ix, iy);
+  const double2 sp = lattice_site(ix, iy, t.half_span, t.two_r);
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const int c = (k + 4) & 7;
+    const int jx = ix + dxc(c), jy = iy + dyc(c);
+    if (!on_lattice(jx, jy, t.nx, t.ny)) continue;
+    const double2 xq = t.x(jx, jy);
+    const double2 sq = lattice_site(jx, jy, t.half_span, t.two_r);
+    double rx, ry, qx, qy;
+    if (c >= 4) { rx = sq.x - sp.x; ry = sq.y - sp.y; qx = xq.x - xp.x; qy = xq.y - xp.y; }
+    else        { rx = sp.x - sq.x; ry = sp.y - sq.y; qx = xp.x - xq.x; qy = xp.y - xq.y; }
+    c00 += rx * rx; c01 += rx * ry; c10 += ry * rx; c11 += ry * ry;
+    x00 += qx * rx; x01 += qx * ry; x10 += qy * rx; x11 += qy * ry;
+  }
+  double det = c00 * c11 - c01 * c10;
+  double i00, i01, i10, i11;
+  if (det != 0.0) {
+    i00 = c11 / det; i01 = -c01 / det; i10 = -c10 / det; i11 = c00 / det;
+  } else {
+    double tr = c00 + c11;
+    if (tr > 0.0) {
+      double t2 = tr * tr;
+      i00 = c00 / t2; i01 = c01 / t2; i10 = c10 / t2; i11 = c11 / t2;
+    } else {
+      i00 = i01 = i10 = i11 = 0.0;
+    }
+  }
+  double f00 = x00 * i00 + x01 * i10;
+  double f01 = x00 * i01 + x01 * i11;
+  double f10 = x10 * i00 + x11 * i10;
+  double f11 = x10 * i01 + x11 * i11;
+  double g00 = f00 * f00 + f10 * f10;
+  double g01 = f00 * f01 + f10 * f11;
+  double g11 = f01 * f01 + f11 * f11;
+  double exx = 0.5 * (g00 - 1.0), exy = 0.5 * (g01 - 0.0), eyy = 0.5 * (g11 - 1.0);
+  double dxy = exx - eyy;
+  return sqrt(dxy * dxy + 4.0 * (exy * exy));
+}
+
+// FP32 form of the same invariant.  Writing F = I + H with H = U C^-1,
+// U = sum (q - r) r^T, keeps the small displacement u = q - r free of
+// cancellation: E = (H + H^T + H^T H) / 2.
+__device__ float strain32(const Tile &t, int ix, int iy)
+{
+  float c00 = 0, c01 = 0, c11 = 0, u00 = 0, u01 = 0, u10 = 0, u11 = 0;
+  const double2 xp = t.x(ix, iy);
+  const double2 sp = lattice_site(ix, iy, t.half_span, t.two_r);
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const int c = (k + 4) & 7;
+    const int jx = ix + dxc(c), jy = iy + dyc(c);
+    if (!on_lattice(jx, jy, t.nx, t.ny)) continue;
+    const double2 xq = t.x(jx, jy);
+    const double2 sq = lattice_site(jx, jy, t.half_span, t.two_r);
+    double rx = sq.x - sp.x, ry = sq.y - sp.y;
+    float ux = (float)((xq.x - xp.x) - rx), uy = (float)((xq.y - xp.y) - ry);
+    float fx = (float)rx, fy = (float)ry;
+    c00 += fx * fx; c01 += fx * fy; c11 += fy * fy;
+    u00 += ux * fx; u01 += ux * fy; u10 += uy * fx; u11 += uy * fy;
+  }
+  float det = c00 * c11 - c01 * c01;
+  float i00, i01, i11;
+  if (det != 0.0f) {
+    i00 = c11 / det; i01 = -c01 / det; i11 = c00 / det;
+  } else {
+    float tr = c00 + c11;
+    float t2 = tr > 0.0f ? tr * tr : 1.0f;
+    i00 = tr > 0.0f ? c00 / t2 : 0.0f; i01 = tr > 0.0f ? c01 / t2 : 0.0f;
+    i11 = tr > 0.0f ? c11 / t2 : 0.0f;
+  }
+  float h00 = u00 * i00 + u01 * i01;
+  float h01 = u00 * i01 + u01 * i11;
+  float h10 = u10 * i00 + u11 * i01;
+  float h11 = u10 * i01 + u11 * i11;
+  float exx = h00 + 0.5f * (h00 * h00 + h10 * h10);
+  float eyy = h11 + 0.5f * (h01 * h01 + h11 * h11);
+  float exy = 0.5f * (h01 + h10) + 0.5f * (h00 * h01 + h10 * h11);
+  float dxy = exx - eyy;
+  return sqrtf(dxy * dxy + 4.0f * (exy * exy));
+}
+
+struct KArgs {
+  tdm_plate_arrays a;
+  Workspace w;
+  const double2 *x_in, *v_in;
+  double2 *x_out, *v_out;
+  int op;  // OP_FORCES, OP_STEP or OP_ENERGY
+};
+
+__device__ inline double wave_sum(double v)
+{
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
+// Deterministic block sum of kCh channels; result valid in thread 0.
+__device__ void block_sum(double (&ch)[kCh], double (*lds)[kCh])
+{
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < kCh; ++k) {
+    double s = wave_sum(ch[k]);
+    if (lane == 0) lds[wave][k] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < kCh; ++k) ch[k] = ((lds[0][k] + lds[1][k]) + lds[2][k]) + lds[3][k];
+  }
+}
+
+template <typename R> struct Prec;
+template <> struct Prec<double> { static constexpr bool fp64 = true; };
+template <> struct Prec<float> { static constexpr bool fp64 = false; };
+
+// One bond's or contact's contribution, written for both ends with the same
+// operand order (lo -> hi) so the two ends agree bit for bit.
+template <typename R>
+__device__ void contact_pair(double2 x_lo, double2 x_hi, double2 v_lo, double2 v_hi, double2 n0,
+                             const tdm_plate_member &m, bool owner, double (&ch)[kCh],
+                             double &fx, double &fy, bool self_lo)
+{
+  const double two_r = 2 * m.radius;
+  if constexpr (Prec<R>::fp64) {
+    double dx = x_hi.x - x_lo.x, dy = x_hi.y - x_lo.y;
+    double d = sqrt(dx * dx + dy * dy);
+    if (!(d < two_r)) return;
+    double nx, ny;
+    if (d > 0) { double dd = d > kEps64 ? d : kEps64; nx = dx / dd; ny = dy / dd; }
+    else { nx = n0.x; ny = n0.y; }
+    double ov = two_r - d;
+    double vn = (v_hi.x - v_lo.x) * nx + (v_hi.y - v_lo.y) * ny;
+    double rep = m.contact_stiffness * ov - m.contact_damping * vn;
+    if (rep < 0) rep = 0;
+    double mag = -rep;
+    double Fx = mag * nx, Fy = mag * ny;
+    if (self_lo) { fx += Fx; fy += Fy; } else { fx += -Fx; fy += -Fy; }
+    if (owner) ch[C_PAIR_DISS] += (rep - m.contact_stiffness * ov) * (-vn);
+  } else {
+    float dx = (float)(x_hi.x - x_lo.x), dy = (float)(x_hi.y - x_lo.y);
+    float d = sqrtf(dx * dx + dy * dy);
+    float tr = (float)two_r;
+    if (!(d < tr)) return;
+    float nx, ny;
+    if (d > 0) { float dd = fmaxf(d, kEps32); nx = dx / dd; ny = dy / dd; }
+    else { nx = (float)n0.x; ny = (float)n0.y; }
+    float ov = tr - d;
+    float vn = (float)(v_hi.x - v_lo.x) * nx + (float)(v_hi.y - v_lo.y) * ny;
+    float kc = (float)m.contact_stiffness;
+    float rep = fmaxf(kc * ov - (float)m.contact_damping * vn, 0.0f);
+    float Fx = -rep * nx, Fy = -rep * ny;
+    if (self_lo) { fx += (double)Fx; fy += (double)Fy; } else { fx -= (double)Fx; fy -= (double)Fy; }
+    if (owner) ch[C_PAIR_DISS] += (double)((rep - kc * ov) * (-vn));
+  }
+}
+
+template <typename R>
+__global__ __launch_bounds__(kThreads) void plate_step_kernel(KArgs k)
+{
+  __shared__ double2 sx[kMaxLds];
+  __shared__ R ss[kMaxRing];
+  __shared__ double red[4][kCh];
+  __shared__ int last_flag;
+
+  const tdm_plate_arrays &a = k.a;
+  const int4 blk = k.w.blocks[blockIdx.x];
+  const int mi = blk.x;
+  const tdm_plate_member &m = a.members[mi];
+  const int nx = m.nx, ny = m.ny, TX = m.tile_x, TY = kThreads / m.tile_x;
+  const int tx0 = blk.y, ty0 = blk.z;
+  const int lw = TX + 2 * kHalo, lh = TY + 2 * kHalo;
+  const double two_r = 2 * m.radius;
+  const double half_span = (double)(nx - 1) / 2;
+  const int mode = m.mode;
+  const double time = m.time;
+  const int first = m.first;
+
+  // stage positions of tile + 2-disk halo
+  for (int t = threadIdx.x; t < lw * lh; t += kThreads) {
+    int ix = tx0 - kHalo + t % lw, iy = ty0 - kHalo + t / lw;
+    double2 v = make_double2(0.0, 0.0);
+    if (on_lattice(ix, iy, nx, ny)) v = k.x_in[first + iy * nx + ix];
+    sx[t] = v;
+  }
+  __syncthreads();
+  Tile tile{sx, lw, tx0, ty0, nx, ny, half_span, two_r};
+  const int rw = TX + 2;
+  for (int t = threadIdx.x; t < rw * (TY + 2); t += kThreads) {
+    int ix = tx0 - 1 + t % rw, iy = ty0 - 1 + t / rw;
+    R s = 0;
+    if (on_lattice(ix, iy, nx, ny)) {
+      if constexpr (Prec<R>::fp64) s = strain64(tile, ix, iy);
+      else s = strain32(tile, ix, iy);
+    }
+    ss[t] = s;
+  }
+  __syncthreads();
+
+  double ch[kCh];
+#pragma unroll
+  for (int c = 0; c < kCh; ++c) ch[c] = 0.0;
+
+  const int ix = tx0 + (int)threadIdx.x % TX, iy = ty0 + (int)threadIdx.x / TX;
+  const bool valid = ix < nx && iy < ny;
+  const int n = a.n;
+  int status = 0, req = 0;
+  if (valid) {
+    const int p = first + iy * nx + ix;
+    const double2 xp = tile.x(ix, iy);
+    const double2 vp = k.v_in[p];
+    const double2 sp = lattice_site(ix, iy, half_span, two_r);
+    const R s_self = ss[(iy - ty0 + 1) * rw + (ix - tx0 + 1)];
+    const uint32_t present = present_links(ix, iy, nx, ny);
+    uint32_t links = a.links[p] & present;
+    const double eps_t_p = m.per_disk_strength ? a.eps_t_disk[p] : m.breaking_strain;
+    const double eps_s_p = m.per_disk_strength ? a.eps_s_disk[p] : m.shear_breaking_strain;
+
+    // ---- unbonded contact candidates -------------------------------------
+    // Candidate lists live in global memory, slot-major (cand[e*n + p]); the
+    // owning thread is the only reader and writer of its slots.
+    int *cand = k.w.cand + p;
+    int nc = 0;
+    const int slots = a.cand_slots;
+    if (mode == MODE_SEARCH) {
+      const double cutoff = two_r + m.skin;
+      const double h = m.cell_size;
+      int cx = (int)floor((xp.x - m.grid_origin[0]) / h);
+      int cy = (int)floor((xp.y - m.grid_origin[1]) / h);
+      if (!(xp.x == xp.x) || !(xp.y == xp.y)) { cx = 0; cy = 0; }
+      cx = min(max(cx, 0), m.ncx - 1);
+      cy = min(max(cy, 0), m.ncy - 1);
+      const int *cnt = k.w.cell_count + (size_t)m.tab * a.ncells;
+      const int *tab = k.w.cell_tab + (size_t)m.tab * a.ncells * a.cell_cap;
+      for (int oy = -1; oy <= 1; ++oy) {
+        int gy = cy + oy;
+        if (gy < 0 || gy >= m.ncy) continue;
+        for (int ox = -1; ox <= 1; ++ox) {
+          int gx = cx + ox;
+          if (gx < 0 || gx >= m.ncx) continue;
+          int cell = m.cell_first + gy * m.ncx + gx;
+          int cn = min(cnt[cell], a.cell_cap);
+          for (int e = 0; e < cn; ++e) {
+            int q = tab[(size_t)cell * a.cell_cap + e];
+            if (q == p) continue;
+            int dq = q - p;
+            bool bonded = false;
+#pragma unroll
+            for (int c = 0; c < 8; ++c)
+              if (((present >> c) & 1u) && dq == dyc(c) * nx + dxc(c)) bonded = true;
+            if (bonded) continue;
+            double2 xq = k.x_in[q];
+            double dx = xq.x - xp.x, dy = xq.y - xp.y;
+            if (!(sqrt(dx * dx + dy * dy) < cutoff)) continue;
+            if (nc == slots) { status |= TDM_ST_CAND_OVERFLOW; continue; }
+            int pos = nc++;
+            while (pos > 0 && cand[(size_t)(pos - 1) * n] > q) {
+              cand[(size_t)pos * n] = cand[(size_t)(pos - 1) * n];
+              --pos;
+            }
+            cand[(size_t)pos * n] = q;
+          }
+        }
+      }
+      k.w.ncand[p] = (uint8_t)nc;
+      k.w.xb[p] = make_float2((float)(xp.x - sp.x), (float)(xp.y - sp.y));
+    } else {
+      nc = k.w.ncand[p];
+      float2 b = k.w.xb[p];
+      double ddx = (xp.x - sp.x) - (double)b.x, ddy = (xp.y - sp.y) - (double)b.y;
+      double lim = 0.5 * m.skin * 0.999;
+      if (ddx * ddx + ddy * ddy >= lim * lim) status |= TDM_ST_STALE_LIST;
+    }
+
+    // ---- bonds and contacts in index_add_ order --------------------------
+    double fx = (-m.drag) * vp.x, fy = (-m.drag) * vp.y;
+    int split = 0;
+    while (split < nc && cand[(size_t)split * n] < p) ++split;
+    for (int group = 0; group < 2; ++group) {
+      int code = group == 0 ? 4 : 0, code_end = group == 0 ? 8 : 4;
+      int ci = group == 0 ? split : 0, ci_end = group == 0 ? nc : split;
+      while (true) {
+        while (code < code_end && !((present >> code) & 1u)) ++code;
+        int qb = code < code_end ? p + dyc(code) * nx + dxc(code) : INT32_MAX;
+        int qc = ci < ci_end ? cand[(size_t)ci * n] : INT32_MAX;
+        if (qb == INT32_MAX && qc == INT32_MAX) break;
+        if (qb < qc) {
+          // ---- lattice bond p <-> q (direction `code`) ----
+          const int c = code++;
+          const bool self_lo = c >= 4;
+          const int jx = ix + dxc(c), jy = iy + dyc(c);
+          const double2 xq = tile.x(jx, jy);
+          const double2 sq = lattice_site(jx, jy, half_span, two_r);
+          const R s_q = ss[(jy - ty0 + 1) * rw + (jx - tx0 + 1)];
+          const double2 x_lo = self_lo ? xp : xq, x_hi = self_lo ? xq : xp;
+          const double2 s_lo = self_lo ? sp : sq, s_hi = self_lo ? sq : sp;
+          const R sh_lo = self_lo ? s_self : s_q, sh_hi = self_lo ? s_q : s_self;
+          const double rx = s_hi.x - s_lo.x, ry = s_hi.y - s_lo.y;
+          const double rest = sqrt(rx * rx + ry * ry);
+          double eps_t = m.breaking_strain, eps_s = m.shear_breaking_strain;
+          if (m.per_disk_strength) {
+            const int q = p + dyc(c) * nx + dxc(c);
+            eps_t = fmin(eps_t_p, a.eps_t_disk[q]);
+            eps_s = fmin(eps_s_p, a.eps_s_disk[q]);
+          }
+          bool alive = (links >> c) & 1u;
+          if (k.op == OP_ENERGY) {
+            // dem.py:603-624: each bond counted once, at its lower end
+            if (self_lo) {
+              double dx = x_hi.x - x_lo.x, dy = x_hi.y - x_lo.y;
+              double d = sqrt(dx * dx + dy * dy);
+              if (alive) { double e = d - rest; ch[E_BOND] += e * e; }
+              else { double ov = two_r - d; if (ov > 0) ch[E_CONTACT] += ov * ov; }
+            }
+            continue;
+          }
+          double Fx, Fy;
+          // exact FP64 geometry (always needed on the FP64 path, and for
+          // break decisions / records on the FP32 path)
+          auto geom64 = [&](double &d, double &nxv, double &nyv) {
+            double dx = x_hi.x - x_lo.x, dy = x_hi.y - x_lo.y;
+            d = sqrt(dx * dx + dy * dy);
+            if (d > 0) { double dd = d > kEps64 ? d : kEps64; nxv = dx / dd; nyv = dy / dd; }
+            else { nxv = rx / rest; nyv = ry / rest; }
+          };
+          if constexpr (Prec<R>::fp64) {
+            double d, nxv, nyv;
+            geom64(d, nxv, nyv);
+            double ext = d - rest;
+            if (alive) {
+              double tstrain = ext / rest;
+              double bshear = 0.5 * (sh_lo + sh_hi);
+              int sf = bshear > eps_s, tf = tstrain > eps_t;
+              if (sf || tf) {
+                alive = false;
+                links &= ~(1u << c);
+                if (self_lo) {
+                  int rank = __popc(present & ((1u << c) - 1u) & 0xF0u);
+                  int b = a.fwd_bond[p] + rank;
+                  a.alive[b] = 0;
+                  a.failure_mode[b] = (uint8_t)(tf + 2 * sf);
+                  a.failure_time_s[b] = time;
+                  a.failure_extension_m[b] = ext;
+                  double en = (0.5 * m.bond_stiffness) * (ext * ext);
+                  a.failure_energy_J[b] = en;
+                  ch[C_RELEASED] += en;
+                  double ov = two_r - d;
+                  if (ov < 0) ov = 0;
+                  ch[C_ACQUIRED] += (0.5 * m.contact_stiffness) * (ov * ov);
+                  ch[C_BREAKS] += 1.0;
+                }
+              }
+            }
+            if (alive) {
+              double mag = m.bond_stiffness * ext;
+              Fx = mag * nxv; Fy = mag * nyv;
+              if (self_lo) { fx += Fx; fy += Fy; } else { fx += -Fx; fy += -Fy; }
+            } else {
+              const int q = p + dyc(c) * nx + dxc(c);
+              const double2 vq = k.v_in[q];
+              contact_pair<double>(x_lo, x_hi, self_lo ? vp : vq, self_lo ? vq : vp,
+                                   make_double2(rx / rest, ry / rest), m, self_lo, ch, fx, fy,
+                                   self_lo);
+            }
+          } else {
+            // FP32: displacement relative to the rest vector keeps the
+            // extension free of cancellation.
+            float ux = (float)((x_hi.x - x_lo.x) - rx), uy = (float)((x_hi.y - x_lo.y) - ry);
+            float rfx = (float)rx, rfy = (float)ry, restf = (float)rest;
+            float dvx = rfx + ux, dvy = rfy + uy;
+            float d = sqrtf(dvx * dvx + dvy * dvy);
+            float ext = (2.0f * (rfx * ux + rfy * uy) + (ux * ux + uy * uy)) / (d + restf);
+            float nxv, nyv;
+            if (d > 0) { float dd = fmaxf(d, kEps32); nxv = dvx / dd; nyv = dvy / dd; }
+            else { nxv = rfx / restf; nyv = rfy / restf; }
+            if (alive) {
+              float tstrain = ext / restf;
+              float bshear = 0.5f * ((float)sh_lo + (float)sh_hi);
+              const double mg = m.fp32_margin;
+              bool near_t = fabs((double)tstrain - eps_t) <= mg * eps_t;
+              bool near_s = fabs((double)bshear - eps_s) <= mg * eps_s;
+              int tf = (double)tstrain > eps_t;
+              int sf = (double)bshear > eps_s;
+              if (near_t || near_s) {
+                double d64, n1, n2;
+                geom64(d64, n1, n2);
+                if (near_t) tf = ((d64 - rest) / rest) > eps_t;
+                if (near_s) {
+                  double slo = strain64(tile, self_lo ? ix : jx, self_lo ? iy : jy);
+                  double shi = strain64(tile, self_lo ? jx : ix, self_lo ? jy : iy);
+                  sf = 0.5 * (slo + shi) > eps_s;
+                }
+              }
+              if (sf || tf) {
+                alive = false;
+                links &= ~(1u << c);
+                if (self_lo) {
+                  double d64, n1, n2;
+                  geom64(d64, n1, n2);
+                  double ext64 = d64 - rest;
+                  int rank = __popc(present & ((1u << c) - 1u) & 0xF0u);
+                  int b = a.fwd_bond[p] + rank;
+                  a.alive[b] = 0;
+                  a.failure_mode[b] = (uint8_t)(tf + 2 * sf);
+                  a.failure_time_s[b] = time;
+                  a.failure_extension_m[b] = ext64;
+                  double en = (0.5 * m.bond_stiffness) * (ext64 * ext64);
+                  a.failure_energy_J[b] = en;
+                  ch[C_RELEASED] += en;
+                  double ov = two_r - d64;
+                  if (ov < 0) ov = 0;
+                  ch[C_ACQUIRED] += (0.5 * m.contact_stiffness) * (ov * ov);
+                  ch[C_BREAKS] += 1.0;
+                }
+              }
+            }
+            if (alive) {
+              float mag = (float)m.bond_stiffness * ext;
+              float Fxf = mag * nxv, Fyf = mag * nyv;
+              if (self_lo) { fx += (double)Fxf; fy += (double)Fyf; }
+              else { fx -= (double)Fxf; fy -= (double)Fyf; }
+            } else {
+              const int q = p + dyc(c) * nx + dxc(c);
+              const double2 vq = k.v_in[q];
+              contact_pair<float>(x_lo, x_hi, self_lo ? vp : vq, self_lo ? vq : vp,
+                                  make_double2(rx / rest, ry / rest), m, self_lo, ch, fx, fy,
+                                  self_lo);
+            }
+          }
+        } else {
+          // ---- unbonded contact candidate ----
+          const int q = qc;
+          ++ci;
+          const bool self_lo = p < q;
+          const double2 xq = k.x_in[q];
+          if (k.op == OP_ENERGY) {
+            if (self_lo) {
+              double dx = xq.x - xp.x, dy = xq.y - xp.y;
+              double ov = two_r - sqrt(dx * dx + dy * dy);
+              if (ov > 0) ch[E_CONTACT] += ov * ov;
+            }
+            continue;
+          }
+          const double2 vq = k.v_in[q];
+          const int ql = q - first, jx = ql % nx, jy = ql / nx;
+          const double2 sq = lattice_site(jx, jy, half_span, two_r);
+          const double2 s_lo = self_lo ? sp : sq, s_hi = self_lo ? sq : sp;
+          double ix0 = s_hi.x - s_lo.x, iy0 = s_hi.y - s_lo.y;
+          double id = sqrt(ix0 * ix0 + iy0 * iy0);
+          contact_pair<R>(self_lo ? xp : xq, self_lo ? xq : xp, self_lo ? vp : vq,
+                          self_lo ? vq : vp, make_double2(ix0 / id, iy0 / id), m, self_lo, ch,
+                          fx, fy, self_lo);
+        }
+      }
+    }
+    a.links[p] = (uint8_t)links;
+
+    if (k.op == OP_ENERGY) {
+      double d;
+      const double ox = m.tool_start[0] + time * m.tool_velocity[0];
+      const double oy = m.tool_start[1] + time * m.tool_velocity[1];
+      if (m.hull_m == 0) {
+        double dx = xp.x - ox, dy = xp.y - oy;
+        d = sqrt(dx * dx + dy * dy);
+      } else {
+        const double2 *hv = reinterpret_cast<const double2 *>(a.hull) + m.hull_first;
+        d = INFINITY;
+        for (int s2 = 0; s2 < m.hull_m - 1; ++s2) {
+          double sx0 = hv[s2].x + ox, sy0 = hv[s2].y + oy;
+          double gx = (hv[s2 + 1].x + ox) - sx0, gy = (hv[s2 + 1].y + oy) - sy0;
+          double fr = ((xp.x - sx0) * gx + (xp.y - sy0) * gy) / (gx * gx + gy * gy);
+          fr = fr < 0.0 ? 0.0 : (fr > 1.0 ? 1.0 : fr);
+          double ddx = xp.x - (sx0 + fr * gx), ddy = xp.y - (sy0 + fr * gy);
+          double dd = sqrt(ddx * ddx + ddy * ddy);
+          if (dd < d) d = dd;
+        }
+      }
+      double ov = m.radius + m.tool_radius - d;
+      if (ov > 0) ch[E_TOOL] += ov * ov;
+      ch[E_VSQ] += vp.x * vp.x + vp.y * vp.y;
+    } else {
+    // ---- tool or hull (dem.py:294-324, hull_contact.py:7-75) --------------
+    const double tvx = m.tool_velocity[0], tvy = m.tool_velocity[1];
+    double tfx, tfy, cfx, cfy;
+    if (m.hull_m == 0) {
+      const double tpx = m.tool_start[0] + time * m.tool_velocity[0];
+      const double tpy = m.tool_start[1] + time * m.tool_velocity[1];
+      double dx = xp.x - tpx, dy = xp.y - tpy;
+      if constexpr (Prec<R>::fp64) {
+        double d = sqrt(dx * dx + dy * dy);
+        double nxv, nyv;
+        if (d > 0) { double dd = d > kEps64 ? d : kEps64; nxv = dx / dd; nyv = dy / dd; }
+        else { nxv = 0.0; nyv = -1.0; }
+        double ov = m.radius + m.tool_radius - d;
+        if (ov < 0) ov = 0;
+        double sp2 = (vp.x - tvx) * nxv + (vp.y - tvy) * nyv;
+        double mag = m.contact_stiffness * ov - m.contact_damping * sp2;
+        if (mag < 0) mag = 0;
+        if (!(ov > 0)) mag = 0;
+        tfx = mag * nxv; tfy = mag * nyv;
+        cfx = m.contact_stiffness * ov * nxv; cfy = m.contact_stiffness * ov * nyv;
+      } else {
+        float dxf = (float)dx, dyf = (float)dy;
+        float d = sqrtf(dxf * dxf + dyf * dyf);
+        float nxv, nyv;
+        if (d > 0) { float dd = fmaxf(d, kEps32); nxv = dxf / dd; nyv = dyf / dd; }
+        else { nxv = 0.0f; nyv = -1.0f; }
+        float ov = fmaxf((float)(m.radius + m.tool_radius) - d, 0.0f);
+        float sp2 = (float)(vp.x - tvx) * nxv + (float)(vp.y - tvy) * nyv;
+        float kc = (float)m.contact_stiffness;
+        float mag = ov > 0 ? fmaxf(kc * ov - (float)m.contact_damping * sp2, 0.0f) : 0.0f;
+        tfx = (double)(mag * nxv); tfy = (double)(mag * nyv);
+        cfx = (double)(kc * ov * nxv); cfy = (double)(kc * ov * nyv);
+      }
+    } else {
+      const double ox = m.tool_start[0] + time * m.tool_velocity[0];
+      const double oy = m.tool_start[1] + time * m.tool_velocity[1];
+      const double2 *hv = reinterpret_cast<const double2 *>(a.hull) + m.hull_first;
+      int best = 0;
+      double best_d2 = INFINITY, bdx = 0, bdy = 0;
+      for (int s = 0; s < m.hull_m - 1; ++s) {
+        double sx0 = hv[s].x + ox, sy0 = hv[s].y + oy;
+        double gx = (hv[s + 1].x + ox) - sx0, gy = (hv[s + 1].y + oy) - sy0;
+        double len2 = gx * gx + gy * gy;
+        double fr = ((xp.x - sx0) * gx + (xp.y - sy0) * gy) / len2;
+        fr = fr < 0.0 ? 0.0 : (fr > 1.0 ? 1.0 : fr);
+        double ddx = xp.x - (sx0 + fr * gx), ddy = xp.y - (sy0 + fr * gy);
+        double d2 = ddx * ddx + ddy * ddy;
+        if (d2 < best_d2) { best_d2 = d2; best = s; bdx = ddx; bdy = ddy; }
+      }
+      double distance = sqrt(best_d2);
+      double gx = (hv[best + 1].x + ox) - (hv[best].x + ox);
+      double gy = (hv[best + 1].y + oy) - (hv[best].y + oy);
+      double len = sqrt(gx * gx + gy * gy);
+      double ttx = gx / len, tty = gy / len;
+      if (ttx < 0 || (ttx == 0 && tty < 0)) { ttx = -ttx; tty = -tty; }
+      double fbx = -tty, fby = ttx;
+      double rvx = vp.x - tvx, rvy = vp.y - tvy;
+      if (rvx * fbx + rvy * fby > 0) { fbx = -fbx; fby = -fby; }
+      double nxv, nyv;
+      if (distance > kEps64) { nxv = bdx / distance; nyv = bdy / distance; }
+      else { nxv = fbx; nyv = fby; }
+      double ov = m.radius + m.tool_radius - distance;
+      if (ov < 0) ov = 0;
+      double sp2 = rvx * nxv + rvy * nyv;
+      double kc = m.contact_stiffness;
+      double mag = kc * ov - m.contact_damping * sp2;
+      if (mag < 0) mag = 0;
+      if (!(ov > 0)) mag = 0;
+      tfx = mag * nxv; tfy = mag * nyv;
+      double cm = kc * ov;  // damping 0; clamp is a no-op for ov >= 0
+      if (!(ov > 0)) cm = 0;
+      cfx = cm * nxv; cfy = cm * nyv;
+    }
+    ch[C_TOOL_FX] += tfx; ch[C_TOOL_FY] += tfy;
+    ch[C_TOOL_DISS] += (tfx - cfx) * (vp.x - tvx) + (tfy - cfy) * (vp.y - tvy);
+    ch[C_VSQ] += vp.x * vp.x + vp.y * vp.y;
+    ch[C_VSUM_X] += vp.x; ch[C_VSUM_Y] += vp.y;
+    fx += tfx; fy += tfy;
+    double2 ld = make_double2(0.0, 0.0);
+    if (m.has_loads) {
+      ld = reinterpret_cast<const double2 *>(a.loads)[p];
+      fx += ld.x; fy += ld.y;
+      ch[C_LOAD_X] += ld.x; ch[C_LOAD_Y] += ld.y;
+      ch[C_MOMENT] += xp.x * ld.y - xp.y * ld.x;
+    }
+    const bool fixed = a.fixed[p] != 0;
+    if (fixed) { ch[C_SUP_X] += fx; ch[C_SUP_Y] += fy; }
+
+    if (k.op == OP_STEP) {
+      // dem.py:363-366
+      double vx = vp.x + (m.dt * fx) / m.mass, vy = vp.y + (m.dt * fy) / m.mass;
+      if (fixed) { vx = 0.0; vy = 0.0; }
+      double xn = xp.x + m.dt * vx, yn = xp.y + m.dt * vy;
+      if (fixed) { xn = sp.x; yn = sp.y; }
+      k.x_out[p] = make_double2(xn, yn);
+      k.v_out[p] = make_double2(vx, vy);
+      if (m.has_loads) ch[C_WORK] += ld.x * (xn - xp.x) + ld.y * (yn - xp.y);
+      if (!isfinite(xn) || !isfinite(yn) || !isfinite(vx) || !isfinite(vy)) status |= TDM_ST_NONFINITE;
+      // rebuild request: quarter skin since the candidate list was built
+      float2 b = mode == MODE_SEARCH ? make_float2((float)(xp.x - sp.x), (float)(xp.y - sp.y))
+                                     : k.w.xb[p];
+      double ddx = (xn - sp.x) - (double)b.x, ddy = (yn - sp.y) - (double)b.y;
+      double q4 = 0.25 * m.skin;
+      if (ddx * ddx + ddy * ddy >= q4 * q4) req = 1;
+      if (mode == MODE_BIN) {
+        const double h = m.cell_size;
+        int cx = (int)floor((xn - m.grid_origin[0]) / h);
+        int cy = (int)floor((yn - m.grid_origin[1]) / h);
+        if (!(xn == xn) || !(yn == yn)) { cx = 0; cy = 0; }
+        cx = min(max(cx, 0), m.ncx - 1);
+        cy = min(max(cy, 0), m.ncy - 1);
+        int cell = m.cell_first + cy * m.ncx + cx;
+        int t = m.tab ^ 1;
+        int slot = atomicAdd(k.w.cell_count + (size_t)t * a.ncells + cell, 1);
+        if (slot < a.cell_cap) k.w.cell_tab[((size_t)t * a.ncells + cell) * a.cell_cap + slot] = p;
+        else status |= TDM_ST_CELL_OVERFLOW;
+      }
+    } else {
+      reinterpret_cast<double2 *>(a.force)[p] = make_double2(fx, fy);
+    }
+    }  // op != OP_ENERGY
+  }
+
+  // a search step clears the other hash table for the next bin phase
+  if (mode == MODE_SEARCH) {
+    const int ncell_m = m.ncx * m.ncy;
+    int *cnt = k.w.cell_count + (size_t)(m.tab ^ 1) * a.ncells + m.cell_first;
+    for (int c = blk.w * kThreads + threadIdx.x; c < ncell_m; c += m.nblocks * kThreads) cnt[c] = 0;
+  }
+  if (status) atomicOr(&a.members[mi].status, status);
+  if (req) atomicOr(&k.w.req[mi], 1);
+
+  block_sum(ch, red);
+  if (threadIdx.x == 0) {
+    double *dst = k.w.partials + (size_t)(m.block_first + blk.w) * kCh;
+#pragma unroll
+    for (int c = 0; c < kCh; ++c) dst[c] = ch[c];
+    __threadfence();
+    unsigned done = atomicAdd(&k.w.counters[mi], 1u);
+    last_flag = done == (unsigned)(m.nblocks - 1);
+  }
+  __syncthreads();
+  if (!last_flag) return;
+  __threadfence();
+
+  // ---- last block of this member: fixed-order ledger reduction ------------
+  double tot[kCh];
+#pragma unroll
+  for (int c = 0; c < kCh; ++c) tot[c] = 0.0;
+  for (int b = threadIdx.x; b < m.nblocks; b += kThreads) {
+    const double *src = k.w.partials + (size_t)(m.block_first + b) * kCh;
+#pragma unroll
+    for (int c = 0; c < kCh; ++c) tot[c] += src[c];
+  }
+  block_sum(tot, red);
+  if (threadIdx.x != 0) return;
+  tdm_plate_member &mw = a.members[mi];
+  double *acc = a.accounting + (size_t)mi * TDM_NACC;
+  const double dt = m.dt;
+  if (k.op != OP_ENERGY) {
+    acc[TDM_ACC_RELEASE] += tot[C_RELEASED];
+    acc[TDM_ACC_ACQUIRED] += tot[C_ACQUIRED];
+  }
+  const double rx_ = -tot[C_TOOL_FX], ry_ = -tot[C_TOOL_FY];
+  if (k.op == OP_ENERGY) {
+    double *en = a.energy + (size_t)mi * 5;
+    en[0] = 0.5 * m.mass * tot[E_VSQ];
+    en[1] = 0.5 * m.bond_stiffness * tot[E_BOND];
+    en[2] = 0.5 * m.contact_stiffness * tot[E_CONTACT];
+    en[3] = 0.5 * m.contact_stiffness * tot[E_TOOL];
+    en[4] = ((en[0] + en[1]) + en[2]) + en[3];
+    atomicExch(&k.w.req[mi], 0);
+    if (mode == MODE_SEARCH) mw.mode = MODE_IDLE;
+  } else if (k.op == OP_STEP) {
+    const double supx = -tot[C_SUP_X], supy = -tot[C_SUP_Y];
+    const double extx = m.has_loads ? tot[C_LOAD_X] : 0.0, exty = m.has_loads ? tot[C_LOAD_Y] : 0.0;
+    const double dimx = (-m.drag) * tot[C_VSUM_X], dimy = (-m.drag) * tot[C_VSUM_Y];
+    acc[TDM_ACC_EXT_IMP] = acc[TDM_ACC_EXT_IMP] + dt * extx;
+    acc[TDM_ACC_EXT_IMP + 1] = acc[TDM_ACC_EXT_IMP + 1] + dt * exty;
+    acc[TDM_ACC_SUP_IMP] = acc[TDM_ACC_SUP_IMP] + dt * supx;
+    acc[TDM_ACC_SUP_IMP + 1] = acc[TDM_ACC_SUP_IMP + 1] + dt * supy;
+    acc[TDM_ACC_DRAG_IMP] = acc[TDM_ACC_DRAG_IMP] + dt * dimx;
+    acc[TDM_ACC_DRAG_IMP + 1] = acc[TDM_ACC_DRAG_IMP + 1] + dt * dimy;
+    acc[TDM_ACC_TOOL_IMP] = acc[TDM_ACC_TOOL_IMP] + dt * (-rx_);
+    acc[TDM_ACC_TOOL_IMP + 1] = acc[TDM_ACC_TOOL_IMP + 1] + dt * (-ry_);
+    const double damp = (tot[C_PAIR_DISS] + (-tot[C_TOOL_DISS])) + m.drag * tot[C_VSQ];
+    const double tpow = -(rx_ * m.tool_velocity[0] + ry_ * m.tool_velocity[1]);
+    const double work = m.has_loads ? tot[C_WORK] : 0.0;
+    acc[TDM_ACC_EXT_WORK] += work;
+    acc[TDM_ACC_TOOL_WORK] += dt * tpow;
+    acc[TDM_ACC_DAMPING] += dt * damp;
+    double *last = a.last + (size_t)mi * TDM_NLAST;
+    last[TDM_LAST_T0] = time;
+    last[TDM_LAST_T1] = time + dt;
+    last[TDM_LAST_WORK] = work;
+    last[TDM_LAST_SUPPORT] = supx;
+    last[TDM_LAST_SUPPORT + 1] = supy;
+    last[TDM_LAST_REACTION] = rx_;
+    last[TDM_LAST_REACTION + 1] = ry_;
+    last[TDM_LAST_EXTERNAL] = extx;
+    last[TDM_LAST_EXTERNAL + 1] = exty;
+    last[TDM_LAST_MOMENT] = m.has_loads ? tot[C_MOMENT] : 0.0;
+    mw.time = time + dt;
+    mw.step_count = m.step_count + 1;
+    const int rq = atomicExch(&k.w.req[mi], 0);
+    if (mode == MODE_BIN) { mw.tab = m.tab ^ 1; mw.mode = MODE_SEARCH; }
+    else mw.mode = rq ? MODE_BIN : MODE_IDLE;
+  } else {
+    a.reaction[2 * mi] = rx_;
+    a.reaction[2 * mi + 1] = ry_;
+    atomicExch(&k.w.req[mi], 0);
+    if (mode == MODE_SEARCH) mw.mode = MODE_IDLE;
+  }
+  k.w.counters[mi] = 0;
+}
+
+// ---------------------------------------------------------------------------
+// setup helpers
+// ---------------------------------------------------------------------------
+__global__ void sync_links_kernel(tdm_plate_arrays a, Workspace w)
+{
+  // one thread per disk: link bit c is alive[bond(p, c)] for every present code
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.n) return;
+  // locate the member by binary search over `first`
+  int lo = 0, hi = a.nmembers - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (a.members[mid].first <= t) lo = mid; else hi = mid - 1;
+  }
+  const tdm_plate_member &m = a.members[lo];
+  const int l = t - m.first, nx = m.nx, ix = l % nx, iy = l / nx;
+  const uint32_t present = present_links(ix, iy, nx, m.ny);
+  uint32_t links = 0;
+  for (int c = 0; c < 8; ++c) {
+    if (!((present >> c) & 1u)) continue;
+    int owner = c >= 4 ? t : t + dyc(c) * nx + dxc(c);
+    int oc = c >= 4 ? c : 7 - c;
+    int ox = owner - m.first;
+    uint32_t opres = present_links(ox % nx, ox / nx, nx, m.ny);
+    int b = a.fwd_bond[owner] + __popc(opres & ((1u << oc) - 1u) & 0xF0u);
+    if (a.alive[b]) links |= 1u << c;
+  }
+  a.links[t] = (uint8_t)links;
+}
+
+__global__ void bin_kernel(tdm_plate_arrays a, Workspace w, const double2 *x)
+{
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.n) return;
+  int lo = 0, hi = a.nmembers - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (a.members[mid].first <= t) lo = mid; else hi = mid - 1;
+  }
+  const tdm_plate_member &m = a.members[lo];
+  const double2 p = x[t];
+  int cx = (int)floor((p.x - m.grid_origin[0]) / m.cell_size);
+  int cy = (int)floor((p.y - m.grid_origin[1]) / m.cell_size);
+  if (!(p.x == p.x) || !(p.y == p.y)) { cx = 0; cy = 0; }
+  cx = min(max(cx, 0), m.ncx - 1);
+  cy = min(max(cy, 0), m.ncy - 1);
+  int cell = m.cell_first + cy * m.ncx + cx;
+  int tb = m.tab;
+  int slot = atomicAdd(w.cell_count + (size_t)tb * a.ncells + cell, 1);
+  if (slot < a.cell_cap) w.cell_tab[((size_t)tb * a.ncells + cell) * a.cell_cap + slot] = t;
+  else atomicOr(&a.members[lo].status, TDM_ST_CELL_OVERFLOW);
+}
+
+__global__ void fill_int_kernel(int *p, size_t n, int v)
+{
+  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (; t < n; t += (size_t)gridDim.x * blockDim.x) p[t] = v;
+}
+
+__global__ void set_mode_kernel(tdm_plate_member *m, int nm, int mode)
+{
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < nm) m[t].mode = mode;
+}
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg)
+{
+  g_err = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                  \
+  do {                                                                                 \
+    hipError_t e_ = (expr);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return fail(TDM_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_));        \
+  } while (0)
+
+int check_arrays(const tdm_plate_arrays *a, Workspace *w)
+{
+  if (!a || !a->members || !a->x || !a->v || !a->x_alt || !a->v_alt || !a->links ||
+      !a->fixed || !a->fwd_bond || !a->alive || !a->accounting || !a->last || !a->workspace)
+    return fail(TDM_EINVAL, "null array in tdm_plate_arrays");
+  if (a->n <= 0 || a->nmembers <= 0 || a->nblocks <= 0 || a->ncells <= 0)
+    return fail(TDM_EINVAL, "empty batch");
+  if (a->cand_slots < 0 || a->cand_slots > kMaxCand || a->cell_cap <= 0)
+    return fail(TDM_EINVAL, "cand_slots must be in [0,16] and cell_cap > 0");
+  size_t need = 0;
+  carve(a->workspace, a->n, a->nblocks, a->nmembers, a->ncells, a->cand_slots, a->cell_cap, w,
+        &need);
+  if (need > a->workspace_bytes) return fail(TDM_ESPACE, "workspace too small");
+  if (reinterpret_cast<uintptr_t>(a->workspace) & 255)
+    return fail(TDM_EINVAL, "workspace must be 256-byte aligned");
+  return TDM_OK;
+}
+
+}  // namespace
+
+struct tdm_graph {
+  hipGraph_t graph;
+  hipGraphExec_t exec;
+};
+
+extern "C" {
+
+int tdm_version(void) { return 100; }
+
+int tdm_last_error(char *buf, size_t len)
+{
+  if (!buf || len == 0) return TDM_EINVAL;
+  std::snprintf(buf, len, "%s", g_err.c_str());
+  return TDM_OK;
+}
+
+int tdm_abi_sizes(size_t *member_bytes, size_t *arrays_bytes)
+{
+  if (!member_bytes || !arrays_bytes) return fail(TDM_EINVAL, "null output");
+  *member_bytes = sizeof(tdm_plate_member);
+  *arrays_bytes = sizeof(tdm_plate_arrays);
+  return TDM_OK;
+}
+
+int tdm_plate_tiles(int nx, int ny, int tile_x)
+{
+  if (nx < 2 || ny < 2 || tile_x < 16 || tile_x > 256 || (kThreads % tile_x))
+    return fail(TDM_EINVAL, "tile_x must divide 256 and lie in [16,256]");
+  int ty = kThreads / tile_x;
+  return ((nx + tile_x - 1) / tile_x) * ((ny + ty - 1) / ty);
+}
+
+size_t tdm_plate_workspace_bytes(int n, int nblocks, int nmembers, int ncells, int cand_slots,
+                                 int cell_cap)
+{
+  Workspace w;
+  size_t total = 0;
+  carve(nullptr, n, nblocks, nmembers, ncells, cand_slots, cell_cap, &w, &total);
+  return total;
+}
+
+int tdm_plate_setup(const tdm_plate_arrays *a, void *stream)
+{
+  Workspace w;
+  int rc = check_arrays(a, &w);
+  if (rc) return rc;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  // block table: host copy of the member table gives tile counts
+  const int nm = a->nmembers;
+  tdm_plate_member *hm = new tdm_plate_member[nm];
+  hipError_t e = hipMemcpyAsync(hm, a->members, sizeof(tdm_plate_member) * nm,
+                                hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) { delete[] hm; return fail(TDM_EHIP, hipGetErrorString(e)); }
+  int total = 0;
+  for (int i = 0; i < nm; ++i) {
+    int t = tdm_plate_tiles(hm[i].nx, hm[i].ny, hm[i].tile_x);
+    if (t < 0) { delete[] hm; return t; }
+    if (hm[i].block_first != total || hm[i].nblocks != t) {
+      delete[] hm;
+      return fail(TDM_EINVAL, "member block_first/nblocks do not match tdm_plate_tiles");
+    }
+    total += t;
+  }
+  if (total != a->nblocks) { delete[] hm; return fail(TDM_EINVAL, "nblocks mismatch"); }
+  // Logical tile order keeps a member's tiles contiguous (row-major tiles) so
+  // its ledger partials sum in a fixed order.  The physical launch order is
+  // permuted so that runs of consecutive tiles share one XCD (blocks are
+  // dealt round-robin over the 8 XCDs): launch slot b runs logical tile
+  // (b % 8) * ceil(nb/8) + b / 8 when that tile exists.
+  int4 *hb = new int4[total];
+  int4 *logical = new int4[total];
+  int li = 0;
+  for (int i = 0; i < nm; ++i) {
+    int tx = hm[i].tile_x, ty = kThreads / tx;
+    int ntx = (hm[i].nx + tx - 1) / tx, nty = (hm[i].ny + ty - 1) / ty;
+    for (int by = 0; by < nty; ++by)
+      for (int bx = 0; bx < ntx; ++bx)
+        logical[li++] = make_int4(i, bx * tx, by * ty, by * ntx + bx);
+  }
+  // Within a member, launch slots are grouped by the XCD they land on
+  // ((global slot) % 8); each XCD's slots take a contiguous run of logical
+  // tiles, so neighbouring tiles share that XCD's L2.  This is a bijection
+  // between the member's slots and its tiles.
+  for (int i = 0; i < nm; ++i) {
+    int b0 = hm[i].block_first, nb = hm[i].nblocks;
+    int next = 0;
+    for (int lane = 0; lane < 8; ++lane)
+      for (int b = 0; b < nb; ++b)
+        if ((b0 + b) % 8 == lane) hb[b0 + b] = logical[b0 + next++];
+  }
+  delete[] logical;
+  e = hipMemcpyAsync(w.blocks, hb, sizeof(int4) * total, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  delete[] hb;
+  delete[] hm;
+  if (e != hipSuccess) return fail(TDM_EHIP, hipGetErrorString(e));
+  // counters, requests, hash tables, candidate counts
+  fill_int_kernel<<<64, 256, 0, s>>>(reinterpret_cast<int *>(w.counters), (size_t)a->nmembers, 0);
+  fill_int_kernel<<<64, 256, 0, s>>>(w.req, (size_t)a->nmembers, 0);
+  fill_int_kernel<<<1024, 256, 0, s>>>(w.cell_count, 2 * (size_t)a->ncells, 0);
+  (void)hipMemsetAsync(w.ncand, 0, (size_t)a->n, s);
+  HIP_TRY(hipGetLastError());
+  rc = tdm_plate_sync_links(a, stream);
+  if (rc) return rc;
+  return tdm_plate_rebin(a, stream);
+}
+
+int tdm_plate_sync_links(const tdm_plate_arrays *a, void *stream)
+{
+  Workspace w;
+  int rc = check_arrays(a, &w);
+  if (rc) return rc;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  sync_links_kernel<<<(a->n + 255) / 256, 256, 0, s>>>(*a, w);
+  HIP_TRY(hipGetLastError());
+  return TDM_OK;
+}
+
+int tdm_plate_rebin(const tdm_plate_arrays *a, void *stream)
+{
+  // bins the state held in x (parity 0) into each member's current table
+  Workspace w;
+  int rc = check_arrays(a, &w);
+  if (rc) return rc;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  fill_int_kernel<<<1024, 256, 0, s>>>(w.cell_count, 2 * (size_t)a->ncells, 0);
+  bin_kernel<<<(a->n + 255) / 256, 256, 0, s>>>(*a, w, reinterpret_cast<const double2 *>(a->x));
+  set_mode_kernel<<<(a->nmembers + 255) / 256, 256, 0, s>>>(a->members, a->nmembers, MODE_SEARCH);
+  fill_int_kernel<<<64, 256, 0, s>>>(w.req, (size_t)a->nmembers, 0);
+  HIP_TRY(hipGetLastError());
+  return TDM_OK;
+}
+
+static int launch_step(const tdm_plate_arrays *a, const Workspace &w, int parity, int fp32,
+                       int op, hipStream_t s)
+{
+  KArgs k;
+  k.a = *a;
+  k.w = w;
+  k.op = op;
+  const double2 *x = reinterpret_cast<const double2 *>(a->x);
+  const double2 *v = reinterpret_cast<const double2 *>(a->v);
+  const double2 *xa = reinterpret_cast<const double2 *>(a->x_alt);
+  const double2 *va = reinterpret_cast<const double2 *>(a->v_alt);
+  k.x_in = parity ? xa : x;
+  k.v_in = parity ? va : v;
+  k.x_out = const_cast<double2 *>(parity ? x : xa);
+  k.v_out = const_cast<double2 *>(parity ? v : va);
+  if (fp32) plate_step_kernel<float><<<a->nblocks, kThreads, 0, s>>>(k);
+  else plate_step_kernel<double><<<a->nblocks, kThreads, 0, s>>>(k);
+  HIP_TRY(hipGetLastError());
+  return TDM_OK;
+}
+
+int tdm_plate_step(const tdm_plate_arrays *a, int nsteps, int parity, int fp32, void *stream)
+{
+  Workspace w;
+  int rc = check_arrays(a, &w);
+  if (rc) return rc;
+  if (nsteps < 0 || (parity != 0 && parity != 1)) return fail(TDM_EINVAL, "bad nsteps/parity");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  for (int i = 0; i < nsteps; ++i) {
+    rc = launch_step(a, w, parity, fp32, OP_STEP, s);
+    if (rc) return rc;
+    parity ^= 1;
+  }
+  return TDM_OK;
+}
+
+int tdm_plate_forces(const tdm_plate_arrays *a, int parity, int fp32, void *stream)
+{
+  Workspace w;
+  int rc = check_arrays(a, &w);
+  if (rc) return rc;
+  if (!a->force || !a->reaction) return fail(TDM_EINVAL, "force/reaction outputs required");
+  return launch_step(a, w, parity, fp32, OP_FORCES, static_cast<hipStream_t>(stream));
+}
+
+int tdm_plate_energy(const tdm_plate_arrays *a, int parity, void *stream)
+{
+  Workspace w;
+  int rc = check_arrays(a, &w);
+  if (rc) return rc;
+  if (!a->energy) return fail(TDM_EINVAL, "energy output required");
+  return launch_step(a, w, parity, 0, OP_ENERGY, static_cast<hipStream_t>(stream));
+}
+
+int tdm_plate_graph_create(const tdm_plate_arrays *a, int pairs, int fp32, void *stream,
+                           tdm_graph **out)
+{
+  Workspace w;
+  int rc = check_arrays(a, &w);
+  if (rc) return rc;
+  if (!out || pairs <= 0) return fail(TDM_EINVAL, "bad graph request");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  for (int i = 0; i < 2 * pairs && rc == TDM_OK; ++i) rc = launch_step(a, w, i & 1, fp32, OP_STEP, s);
+  hipGraph_t g;
+  hipError_t e = hipStreamEndCapture(s, &g);
+  if (rc) return rc;
+  if (e != hipSuccess) return fail(TDM_EHIP, hipGetErrorString(e));
+  hipGraphExec_t ex;
+  e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+  if (e != hipSuccess) { (void)hipGraphDestroy(g); return fail(TDM_EHIP, hipGetErrorString(e)); }
+  *out = new tdm_graph{g, ex};
+  return TDM_OK;
+}
+
+int tdm_graph_launch(tdm_graph *g, void *stream)
+{
+  if (!g) return fail(TDM_EINVAL, "null graph");
+  HIP_TRY(hipGraphLaunch(g->exec, static_cast<hipStream_t>(stream)));
+  return TDM_OK;
+}
+
+int tdm_graph_destroy(tdm_graph *g)
+{
+  if (!g) return TDM_OK;
+  (void)hipGraphExecDestroy(g->exec);
+  (void)hipGraphDestroy(g->graph);
+  delete g;
+  return TDM_OK;
+}
+
+}  // extern "C"

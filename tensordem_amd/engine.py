@@ -1,0 +1,294 @@
+"""Device-resident batch of bonded ice plates driven by the HIP kernels.
+
+`PlateEngine` owns every device array of one or more independent plates
+(members) as torch tensors on the ROCm device, fills the `tdm_plate_arrays`
+descriptor with their addresses and calls the C ABI.  One kernel launch
+advances all members by one step of `IceDEM.step` (src/tensordem/dem.py:
+347-382); members never interact, and each member's ledger is reduced in an
+order that depends only on that member, so a member gives the same result
+alone and inside any batch.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import _lib
+from .config import DEMConfig
+from .lattice import bonds, fixed_mask, positions
+
+THREADS = 256
+
+
+@dataclass
+class PlateSpec:
+    """One plate: reference config, time step, tool or hull, optional per-disk strengths."""
+    config: DEMConfig
+    dt: float | None = None
+    hull_profile: np.ndarray | None = None
+    hull_start: tuple[float, float] = (0.0, 0.0)
+    hull_velocity: tuple[float, float] = (0.0, 0.0)
+    eps_t_disk: np.ndarray | None = None
+    eps_s_disk: np.ndarray | None = None
+    tile_x: int | None = None
+    extra: dict = field(default_factory=dict)
+
+
+def default_tile(nx: int) -> int:
+    """Tile width for a lattice row of nx disks (256 disks per tile)."""
+    if nx >= 48:
+        return 32
+    return 16
+
+
+class PlateEngine:
+    def __init__(self, specs: list[PlateSpec], *, device: str | torch.device = "cuda",
+                 fp32: bool = False, skin_factor: float = 0.5, cand_slots: int = 8,
+                 cell_cap: int = 8, grid_margin: float = 1.0, fp32_margin: float = 1e-3):
+        self.lib = _lib.load()
+        self.device = torch.device(device)
+        if self.device.type != "cuda":
+            raise ValueError("PlateEngine runs on the ROCm device (device='cuda')")
+        if not torch.cuda.is_available():
+            raise RuntimeError("no ROCm device available")
+        self.specs = list(specs)
+        self.fp32 = bool(fp32)
+        nm = len(self.specs)
+        members = (_lib.Member * nm)()
+        xs, fixed, fwd, eps_t, eps_s, hull = [], [], [], [], [], []
+        bond_i, bond_j, rest = [], [], []
+        n = nb = blocks = cells = hull_n = 0
+        per_disk = any(s.eps_t_disk is not None for s in self.specs)
+        for k, s in enumerate(self.specs):
+            c = s.config
+            pos = positions(c.nx, c.ny, c.radius)
+            bi, bj, rl, fw = bonds(c.nx, c.ny, c.radius)
+            m = members[k]
+            m.radius, m.bond_stiffness, m.contact_stiffness = c.radius, c.bond_stiffness, c.contact_stiffness
+            m.breaking_strain, m.shear_breaking_strain = c.breaking_strain, c.shear_breaking_strain
+            m.contact_damping, m.drag, m.tool_radius = c.contact_damping, c.drag, c.tool_radius
+            m.dt = float(s.dt if s.dt is not None else (c.dt if c.dt is not None else c.recommended_dt))
+            m.mass = c.mass
+            if s.hull_profile is None:
+                m.tool_start[0] = 0.0
+                m.tool_start[1] = ((c.ny - 1) * 2 * c.radius + c.radius + c.tool_radius
+                                   + c.tool_gap)
+                m.tool_velocity[0], m.tool_velocity[1] = 0.0, -c.tool_speed
+                m.hull_first, m.hull_m = 0, 0
+            else:
+                hp = np.asarray(s.hull_profile, dtype=np.float64)
+                m.tool_start[0], m.tool_start[1] = s.hull_start
+                m.tool_velocity[0], m.tool_velocity[1] = s.hull_velocity
+                m.hull_first, m.hull_m = hull_n, len(hp)
+                hull.append(hp)
+                hull_n += len(hp)
+            m.time = 0.0
+            m.skin = skin_factor * c.radius
+            m.cell_size = 2 * c.radius + m.skin
+            w = (c.nx - 1) * 2 * c.radius
+            h = (c.ny - 1) * 2 * c.radius
+            mx = grid_margin * max(w, h) + 2 * m.cell_size
+            m.grid_origin[0] = pos[:, 0].min() - mx
+            m.grid_origin[1] = -mx
+            m.ncx = int(math.ceil((w + 2 * mx) / m.cell_size)) + 1
+            m.ncy = int(math.ceil((h + 2 * mx) / m.cell_size)) + 1
+            m.cell_first = cells
+            cells += m.ncx * m.ncy
+            m.fp32_margin = fp32_margin
+            m.step_count = 0
+            m.nx, m.ny, m.first = c.nx, c.ny, n
+            m.bond_first, m.nbonds = nb, len(bi)
+            m.tile_x = s.tile_x or default_tile(c.nx)
+            nt = self.lib.tdm_plate_tiles(c.nx, c.ny, m.tile_x)
+            _lib.check(nt if nt < 0 else 0)
+            m.block_first, m.nblocks = blocks, nt
+            blocks += nt
+            m.mode, m.tab, m.status, m.has_loads = 1, 0, 0, 0
+            m.per_disk_strength = int(s.eps_t_disk is not None)
+            xs.append(pos)
+            fixed.append(fixed_mask(c.nx, c.ny, c.fix_edges))
+            fwd.append(fw + nb)
+            bond_i.append(bi + n)
+            bond_j.append(bj + n)
+            rest.append(rl)
+            if per_disk:
+                npd = c.nx * c.ny
+                eps_t.append(np.full(npd, c.breaking_strain) if s.eps_t_disk is None
+                             else np.asarray(s.eps_t_disk, np.float64))
+                eps_s.append(np.full(npd, c.shear_breaking_strain) if s.eps_s_disk is None
+                             else np.asarray(s.eps_s_disk, np.float64))
+            n += c.nx * c.ny
+            nb += len(bi)
+        self.n, self.nbonds, self.nmembers, self.nblocks, self.ncells = n, nb, nm, blocks, cells
+        self.cand_slots, self.cell_cap = cand_slots, cell_cap
+        dev = self.device
+        f64 = dict(dtype=torch.float64, device=dev)
+        x0 = np.concatenate(xs)
+        self.initial_positions = torch.tensor(x0, **f64)
+        self.x = self.initial_positions.clone()
+        self.v = torch.zeros_like(self.x)
+        self.x_alt = torch.empty_like(self.x)
+        self.v_alt = torch.empty_like(self.x)
+        self.fixed = torch.tensor(np.concatenate(fixed), dtype=torch.bool, device=dev)
+        self.links = torch.zeros(n, dtype=torch.uint8, device=dev)
+        self.fwd_bond = torch.tensor(np.concatenate(fwd), dtype=torch.int32, device=dev)
+        self.bond_i = torch.tensor(np.concatenate(bond_i), dtype=torch.int64, device=dev)
+        self.bond_j = torch.tensor(np.concatenate(bond_j), dtype=torch.int64, device=dev)
+        self.rest_lengths = torch.tensor(np.concatenate(rest), **f64)
+        self.alive = torch.ones(nb, dtype=torch.bool, device=dev)
+        self.failure_mode = torch.zeros(nb, dtype=torch.uint8, device=dev)
+        self.failure_time_s = torch.full((nb,), -1.0, **f64)
+        self.failure_extension_m = torch.zeros(nb, **f64)
+        self.failure_energy_J = torch.zeros(nb, **f64)
+        self.loads = torch.zeros((n, 2), **f64)
+        self.hull = (torch.tensor(np.concatenate(hull), **f64) if hull
+                     else torch.zeros((1, 2), **f64))
+        self.eps_t_disk = torch.tensor(np.concatenate(eps_t), **f64) if per_disk else None
+        self.eps_s_disk = torch.tensor(np.concatenate(eps_s), **f64) if per_disk else None
+        self.accounting = torch.zeros((nm, _lib.NACC), **f64)
+        self.last = torch.zeros((nm, _lib.NLAST), **f64)
+        self.force = torch.zeros((n, 2), **f64)
+        self.reaction = torch.zeros((nm, 2), **f64)
+        self.energy = torch.zeros((nm, 5), **f64)
+        self._members_host = members
+        self._has_loads = [0] * nm
+        self.members = torch.frombuffer(bytearray(bytes(members)), dtype=torch.uint8).to(dev)
+        wsb = self.lib.tdm_plate_workspace_bytes(n, blocks, nm, cells, cand_slots, cell_cap)
+        self.workspace = torch.zeros(int(wsb) + 256, dtype=torch.uint8, device=dev)
+        self._arrays = _lib.Arrays()
+        self._refresh()
+        _lib.check(self.lib.tdm_plate_setup(ctypes.byref(self._arrays), self.stream))
+        self._graphs: dict = {}
+
+    # ---- descriptor ----------------------------------------------------------
+    @property
+    def stream(self) -> int:
+        return torch.cuda.current_stream(self.device).cuda_stream
+
+    def _refresh(self, loads: bool = False) -> None:
+        a = self._arrays
+        p = lambda t: None if t is None else t.data_ptr()  # noqa: E731
+        a.x, a.v, a.x_alt, a.v_alt = p(self.x), p(self.v), p(self.x_alt), p(self.v_alt)
+        a.fixed, a.links, a.fwd_bond, a.alive = p(self.fixed), p(self.links), p(self.fwd_bond), p(self.alive)
+        a.failure_mode, a.failure_time_s = p(self.failure_mode), p(self.failure_time_s)
+        a.failure_extension_m, a.failure_energy_J = p(self.failure_extension_m), p(self.failure_energy_J)
+        a.loads, a.hull = p(self.loads), p(self.hull)
+        a.eps_t_disk, a.eps_s_disk = p(self.eps_t_disk), p(self.eps_s_disk)
+        a.accounting, a.last, a.force, a.reaction = p(self.accounting), p(self.last), p(self.force), p(self.reaction)
+        a.energy = p(self.energy)
+        a.members = p(self.members)
+        base = self.workspace.data_ptr()
+        aligned = (base + 255) & ~255
+        a.workspace = aligned
+        a.workspace_bytes = self.workspace.numel() - (aligned - base)
+        a.n, a.nbonds, a.nmembers, a.nblocks, a.ncells = self.n, self.nbonds, self.nmembers, self.nblocks, self.ncells
+        a.cand_slots, a.cell_cap = self.cand_slots, self.cell_cap
+
+    def _swap(self) -> None:
+        self.x, self.x_alt = self.x_alt, self.x
+        self.v, self.v_alt = self.v_alt, self.v
+        self._refresh()
+
+    # ---- member table ----------------------------------------------------------
+    def read_members(self):
+        raw = self.members.cpu().numpy().tobytes()
+        return (_lib.Member * self.nmembers).from_buffer_copy(raw)
+
+    def write_members(self, members) -> None:
+        self.members.copy_(torch.frombuffer(bytearray(bytes(members)), dtype=torch.uint8))
+
+    def _field_bytes(self, member: int, name: str, ctype, value) -> None:
+        off = member * ctypes.sizeof(_lib.Member) + getattr(_lib.Member, name).offset
+        raw = bytes(ctype(value))
+        self.members[off:off + len(raw)].copy_(torch.frombuffer(bytearray(raw), dtype=torch.uint8))
+
+    def set_double(self, member: int, name: str, value: float) -> None:
+        self._field_bytes(member, name, ctypes.c_double, float(value))
+
+    def set_int(self, member: int, name: str, value: int) -> None:
+        self._field_bytes(member, name, ctypes.c_int32, int(value))
+        if name == "has_loads":
+            self._has_loads[member] = int(value)
+
+    def set_int64(self, member: int, name: str, value: int) -> None:
+        self._field_bytes(member, name, ctypes.c_int64, int(value))
+
+    def has_loads(self, member: int) -> bool:
+        return bool(self._has_loads[member])
+
+    def read_status(self, member: int) -> int:
+        off = member * ctypes.sizeof(_lib.Member) + _lib.Member.status.offset
+        return int(self.members[off:off + 4].cpu().view(torch.int32).item())
+
+    def clear_status(self, member: int) -> None:
+        self.set_int(member, "status", 0)
+
+    def set_field(self, name: str, values) -> None:
+        """Set one member-table field for all members (host round trip)."""
+        ms = self.read_members()
+        for k, m in enumerate(ms):
+            val = values[k] if isinstance(values, (list, tuple, np.ndarray)) else values
+            if isinstance(getattr(m, name), ctypes.Array):
+                for t, e in enumerate(val):
+                    getattr(m, name)[t] = e
+            else:
+                setattr(m, name, val)
+        self.write_members(ms)
+
+    # ---- operations ----------------------------------------------------------
+    def step(self, nsteps: int = 1) -> None:
+        _lib.check(self.lib.tdm_plate_step(ctypes.byref(self._arrays), int(nsteps), 0,
+                                           int(self.fp32), self.stream))
+        if nsteps % 2:
+            self._swap()
+
+    def forces(self) -> None:
+        _lib.check(self.lib.tdm_plate_forces(ctypes.byref(self._arrays), 0, int(self.fp32),
+                                             self.stream))
+
+    def mechanical_energy(self) -> None:
+        _lib.check(self.lib.tdm_plate_energy(ctypes.byref(self._arrays), 0, self.stream))
+
+    def sync_links(self) -> None:
+        _lib.check(self.lib.tdm_plate_sync_links(ctypes.byref(self._arrays), self.stream))
+
+    def rebin(self) -> None:
+        _lib.check(self.lib.tdm_plate_rebin(ctypes.byref(self._arrays), self.stream))
+
+    def graph(self, pairs: int):
+        """Capture 2*pairs steps (state returns to x/v) as a replayable graph."""
+        key = (pairs, self.fp32, self.x.data_ptr())
+        if key not in self._graphs:
+            g = ctypes.c_void_p()
+            stream = torch.cuda.Stream(self.device)
+            stream.wait_stream(torch.cuda.current_stream(self.device))
+            with torch.cuda.stream(stream):
+                _lib.check(self.lib.tdm_plate_graph_create(
+                    ctypes.byref(self._arrays), int(pairs), int(self.fp32), stream.cuda_stream,
+                    ctypes.byref(g)))
+            torch.cuda.current_stream(self.device).wait_stream(stream)
+            self._graphs[key] = g
+        return self._graphs[key]
+
+    def run_graph(self, pairs: int, repeats: int = 1) -> None:
+        g = self.graph(pairs)
+        for _ in range(repeats):
+            _lib.check(self.lib.tdm_graph_launch(g, self.stream))
+
+    def status(self) -> np.ndarray:
+        return np.array([m.status for m in self.read_members()], dtype=np.int32)
+
+    def close(self) -> None:
+        for g in self._graphs.values():
+            self.lib.tdm_graph_destroy(g)
+        self._graphs.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,386 @@
+"""HIP plate step against the FP64 oracle (same seeded inputs) and the reference goldens.
+
+FP64 kernels are compiled without fused multiply-add and visit bonds and
+contacts in the reference's `index_add_` order, so positions, velocities and
+fracture records are compared bit for bit with the oracle; ledger channels
+are sums in a different order and match to 1e-12 relative.  FP32-force runs
+are held to the tolerances stated in each test.
+"""
+import csv
+import json
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from conftest import GOLDEN
+from oracle.plate import OraclePlate
+from tensordem_amd import DEMConfig
+from tensordem_amd.engine import PlateEngine, PlateSpec
+from tensordem_amd.icedem import IceDEM
+
+pytestmark = pytest.mark.gpu
+
+ENERGY = json.loads((GOLDEN / "dem2d_energy_ledger_closure.json").read_text())
+LEDGER = ("external_work_J", "fracture_release_J", "pair_contact_acquired_J", "tool_work_J",
+          "damping_dissipation_J")
+
+
+def cuda(**kw):
+    return DEMConfig(device="cuda", **kw)
+
+
+def np_(t):
+    return t.detach().cpu().numpy()
+
+
+def assert_same_state(sim: IceDEM, ref: OraclePlate, what=""):
+    x, v = np_(sim.positions), np_(sim.velocities)
+    dx = np.abs(x - ref.positions).max()
+    dv = np.abs(v - ref.velocities).max()
+    assert np.array_equal(x, ref.positions), f"{what} positions differ by {dx:.3e}"
+    assert np.array_equal(v, ref.velocities), f"{what} velocities differ by {dv:.3e}"
+    assert np.array_equal(np_(sim.alive), ref.alive.astype(bool)), f"{what} alive differs"
+
+
+def assert_ledger_close(sim, ref, rel=1e-12):
+    a, b = sim.accounting, ref.accounting
+    for k in LEDGER:
+        assert a[k] == pytest.approx(b[k], rel=rel, abs=1e-18), k
+    for k in ("external_impulse_Ns", "support_impulse_Ns", "drag_impulse_Ns",
+              "tool_impulse_on_ice_Ns"):
+        assert a[k] == pytest.approx(b[k], rel=rel, abs=1e-15), k
+
+
+def test_golden_indentation_fp64():
+    """Reference 7x3 indentation: 21 events in order, ledger, bit-exact vs oracle."""
+    c = cuda(nx=7, ny=3, tool_speed=0.2)
+    sim, ref = IceDEM(c), OraclePlate(c)
+    seen, events = set(), []
+    for _ in range(345):
+        sim.step()
+        ref.step()
+        broken = set(np.nonzero(~np_(sim.alive))[0].tolist())
+        for b in sorted(broken - seen):
+            seen.add(b)
+            i, j = np_(sim.pairs[b]).tolist()
+            events.append((i, j, int(sim.failure_mode[b]), float(sim.failure_extension_m[b])))
+    assert_same_state(sim, ref, "7x3")
+    with open(GOLDEN / "dem2d_fracture_events.csv") as fh:
+        gold = list(csv.DictReader(fh))
+    assert [(i, j, m) for i, j, m, _ in events] == [
+        (int(g["pair_i"]), int(g["pair_j"]), int(g["mode"])) for g in gold]
+    for (_, _, _, ext), g in zip(events, gold):
+        assert abs(ext - float(g["extension_m"])) < 6e-10
+    for arr in ("failure_mode", "failure_time_s", "failure_extension_m", "failure_energy_J"):
+        assert np.array_equal(np_(getattr(sim, arr)), getattr(ref, arr)), arr
+    g = next(c for c in ENERGY["cases"] if c["case"] == "C_tool_indentation" and c["dt_scale"] == 1.0)
+    acc = sim.accounting
+    for k in ("fracture_release_J", "pair_contact_acquired_J", "damping_dissipation_J", "tool_work_J"):
+        assert acc[k] == pytest.approx(g[k], rel=1e-11), k
+    assert sim.mechanical_energy()["mechanical_J"] == pytest.approx(g["e_end_J"], rel=1e-11)
+    assert_ledger_close(sim, ref)
+
+
+def test_icebreaker_c1_fp64_matches_oracle():
+    """21x5 plate, strains 0.01/0.02, 2000 steps at the reference's auto dt."""
+    c = cuda(nx=21, ny=5, breaking_strain=0.01, shear_breaking_strain=0.02)
+    sim, ref = IceDEM(c), OraclePlate(c)
+    for k in range(2000):
+        sim.step()
+        ref.step()
+        if k % 250 == 249:
+            assert_same_state(sim, ref, f"step {k + 1}")
+    assert sim.broken_bonds == ref.broken_bonds > 0
+    assert_ledger_close(sim, ref)
+    e_gpu, e_ref = sim.mechanical_energy(), ref.mechanical_energy()
+    for k in e_ref:
+        assert e_gpu[k] == pytest.approx(e_ref[k], rel=1e-12, abs=1e-18), k
+
+
+def test_icebreaker_c1_fp32_forces():
+    """FP32 forces, FP64 state: error vs FP64 stated; same broken-bond set after 2000 steps."""
+    c = cuda(nx=21, ny=5, breaking_strain=0.01, shear_breaking_strain=0.02)
+    sim, ref = IceDEM(c, precision="fp32"), OraclePlate(c)
+    span = 21 * 0.05
+    worst = 0.0
+    for k in range(2000):
+        sim.step()
+        ref.step()
+        if k % 100 == 99:
+            worst = max(worst, float(np.abs(np_(sim.positions) - ref.positions).max()))
+    print(f"C1 fp32: max |x - x_fp64| over 2000 steps = {worst:.3e} m ({worst / span:.2e} of span)")
+    assert worst < 1e-6 * span
+    assert np.array_equal(np_(sim.alive), ref.alive.astype(bool))
+    acc, racc = sim.accounting, ref.accounting
+    for k in ("fracture_release_J", "tool_work_J", "damping_dissipation_J"):
+        assert acc[k] == pytest.approx(racc[k], rel=1e-5), k
+
+
+def test_fp32_break_decisions_equal_fp64_on_same_state():
+    """Same-state parity: load the oracle's state, commit fracture in FP32, compare sets."""
+    c = cuda(nx=21, ny=5, breaking_strain=0.01, shear_breaking_strain=0.02)
+    ref = OraclePlate(c)
+    checked = 0
+    for k in range(1600):
+        ref.step()
+        if k % 20 != 19:
+            continue
+        sim = IceDEM(c, precision="fp32")
+        sim.positions = torch.tensor(ref.positions)
+        sim.velocities = torch.tensor(ref.velocities)
+        sim.alive = torch.tensor(ref.alive.astype(bool))
+        sim.time = ref.time
+        probe = OraclePlate(c)
+        probe.positions[:] = ref.positions
+        probe.velocities[:] = ref.velocities
+        probe.alive[:] = ref.alive
+        probe.time = ref.time
+        f_ref, r_ref = probe.forces()
+        f, r = sim.forces()
+        assert np.array_equal(np_(sim.alive), probe.alive.astype(bool)), f"step {k + 1}"
+        scale = np.abs(f_ref).max()
+        assert np.abs(np_(f) - f_ref).max() <= 2e-5 * scale
+        checked += 1
+    assert checked == 80
+
+
+def test_determinism_two_runs_bitwise():
+    c = cuda(nx=21, ny=5, breaking_strain=0.01, shear_breaking_strain=0.02)
+    runs = []
+    for prec in ("fp64", "fp32", "fp32"):
+        sim = IceDEM(c, precision=prec)
+        sim.run(600)
+        runs.append((np_(sim.positions).copy(), np_(sim.alive).copy(), sim.accounting))
+    assert np.array_equal(runs[1][0], runs[2][0]) and np.array_equal(runs[1][1], runs[2][1])
+    assert runs[1][2] == runs[2][2]
+
+
+def test_unbonded_contacts_and_rebuilds_match_oracle():
+    """All bonds broken, disks jostled: the cell hash, candidate lists and the
+    pipelined rebuild reproduce the oracle's all-pairs contact forces."""
+    c = cuda(nx=10, ny=8, drag=0.5, fix_edges=False, tool_gap=5.0, tool_speed=0.1)
+    rng = np.random.default_rng(7)
+    sim, ref = IceDEM(c), OraclePlate(c)
+    v0 = rng.normal(scale=0.6, size=ref.velocities.shape)
+    ref.velocities[:] = v0
+    ref.alive[:] = 0
+    sim.velocities = torch.tensor(v0)
+    sim.alive = torch.zeros_like(sim.alive)
+    for k in range(3000):
+        sim.step()
+        ref.step()
+        if k % 300 == 299:
+            assert_same_state(sim, ref, f"step {k + 1}")
+    moved = np.abs(ref.positions - ref.initial_positions).max()
+    assert moved > 0.05  # many cells crossed, so several rebuilds happened
+    assert_ledger_close(sim, ref)
+
+
+def test_loads_and_ledger_match_oracle():
+    c = cuda(nx=9, ny=4, tool_speed=0.5)
+    sim, ref = IceDEM(c), OraclePlate(c)
+    rng = np.random.default_rng(11)
+    for k in range(400):
+        load = rng.normal(scale=0.5, size=ref.positions.shape)
+        sim.step(torch.tensor(load, device="cuda"), load_time=sim.time)
+        ref.step(load)
+    assert_same_state(sim, ref, "loads")
+    assert_ledger_close(sim, ref)
+    last = sim.last_step
+    assert last["external_moment_Nm"] == pytest.approx(ref.last[9], rel=1e-12)
+
+
+def test_hull_contact_matches_oracle():
+    hull = torch.tensor([[-0.15, 0.10], [0.0, 0.0], [-0.15, -0.10]], dtype=torch.float64)
+    c = cuda(nx=12, ny=4, tool_speed=0.5)
+    start, vel = (0.45, 0.075), (-0.6, 0.0)
+    sim = IceDEM(c, hull_profile=hull, hull_start=start, hull_velocity=vel)
+    ref = OraclePlate(c, hull_profile=hull.numpy(), hull_start=start, hull_velocity=vel)
+    for _ in range(800):
+        sim.step()
+        ref.step()
+    assert sim.broken_bonds > 0
+    assert_same_state(sim, ref, "hull")
+    assert_ledger_close(sim, ref)
+
+
+def test_batch_member_equals_alone():
+    """Members of one launch never interact: each equals its solo run bit for bit."""
+    cfgs = [cuda(nx=21, ny=5, breaking_strain=0.01, shear_breaking_strain=0.02),
+            cuda(nx=40, ny=9, tool_speed=0.7),
+            cuda(nx=7, ny=3, tool_speed=0.2)]
+    dts = [c.recommended_dt for c in cfgs]
+    batch = PlateEngine([PlateSpec(c, dt=d) for c, d in zip(cfgs, dts)])
+    batch.step(700)
+    for k, (c, d) in enumerate(zip(cfgs, dts)):
+        solo = PlateEngine([PlateSpec(c, dt=d)])
+        solo.step(700)
+        lo = batch.read_members()[k].first
+        n = c.nx * c.ny
+        assert np.array_equal(np_(batch.x[lo:lo + n]), np_(solo.x))
+        assert np.array_equal(np_(batch.accounting[k]), np_(solo.accounting[0]))
+
+
+def test_graph_replay_equals_eager():
+    c = cuda(nx=21, ny=5, breaking_strain=0.01, shear_breaking_strain=0.02)
+    eager = PlateEngine([PlateSpec(c)])
+    graph = PlateEngine([PlateSpec(c)])
+    for rep in range(4):
+        eager.step(100)
+        graph.run_graph(50)
+        torch.cuda.synchronize()
+        _ = graph.x.sum().item()  # eager work between replays
+        assert np.array_equal(np_(eager.x), np_(graph.x)), f"replay {rep}"
+        assert np.array_equal(np_(eager.accounting), np_(graph.accounting))
+    assert graph.read_members()[0].step_count == 400
+
+
+# ---- reference unit tests (tests/test_dem.py) on the HIP path -----------------------
+def small(**kw):
+    return IceDEM(cuda(nx=3, ny=2, drag=0, fix_edges=False, **kw))
+
+
+def test_equilibrium_and_action_reaction():
+    sim = small()
+    force, reaction = sim.forces()
+    assert float(force.abs().max()) < 1e-10 and float(reaction.abs().max()) == 0
+    sim.positions[1, 0] += 0.0001
+    force, _ = sim.forces()
+    assert float(force.sum(0).abs().max()) < 1e-15
+
+
+def test_rigid_rotation_and_shear():
+    ang = 0.6
+    rot = torch.tensor([[math.cos(ang), -math.sin(ang)], [math.sin(ang), math.cos(ang)]],
+                       dtype=torch.float64, device="cuda")
+    sim = small()
+    sim.positions = sim.positions @ rot.T + torch.tensor([4.0, -4.0], device="cuda")
+    force, _ = sim.forces()
+    assert sim.broken_bonds == 0 and float(force.abs().max()) < 1e-10
+    sheared = small(shear_breaking_strain=0.02, breaking_strain=0.5)
+    sheared.positions[:, 0] += 0.08 * sheared.positions[:, 1]
+    sheared.forces()
+    assert sheared.broken_bonds > 0
+    sheared.positions = sheared.initial_positions.clone()
+    sheared.forces()
+    assert sheared.broken_bonds > 0
+
+
+def test_tension_breaks_irreversibly_but_compression_does_not():
+    sim = small()
+    sim.positions *= 0.9
+    sim.forces()
+    assert sim.broken_bonds == 0
+    sim.positions = sim.initial_positions * 1.02
+    force, _ = sim.forces()
+    assert sim.broken_bonds == int(sim.bonded.sum())
+    assert float(force.abs().max()) < 1e-12
+    sim.positions = sim.initial_positions * 0.9
+    force, _ = sim.forces()
+    assert not bool(sim.alive.any()) and float(force.abs().sum()) > 0
+
+
+def test_contact_repels_without_attraction_and_coincident_is_finite():
+    sim = small()
+    sim.alive[:] = False
+    sim.positions[1, 0] -= 0.005
+    force, _ = sim.forces()
+    assert float(force[0, 0]) < 0 < float(force[1, 0])
+    sim.velocities[:, 0] = sim.positions[:, 0] * 1e6
+    force, _ = sim.forces()
+    assert float(force.abs().max()) == 0
+    sim.velocities.zero_()
+    sim.positions[1] = sim.positions[0]
+    force, _ = sim.forces()
+    assert bool(torch.isfinite(force).all()) and float(force.abs().sum()) > 0
+
+
+def test_tool_reaction_fixed_edges_and_demo_fractures():
+    sim = IceDEM(cuda(nx=5, ny=3, drag=0))
+    sim.time = 0.02
+    force, reaction = sim.forces()
+    assert float(reaction[1]) > 0
+    assert torch.allclose(force.sum(0), -reaction, rtol=1e-12, atol=1e-15)
+    for _ in range(100):
+        sim.step()
+    assert torch.equal(sim.positions[sim.fixed], sim.initial_positions[sim.fixed])
+    assert not bool(sim.velocities[sim.fixed].any())
+    demo = IceDEM(cuda())
+    prev = 0
+    for _ in range(20):
+        demo.run(100)
+        assert demo.broken_bonds >= prev
+        prev = demo.broken_bonds
+    assert demo.broken_bonds > 0 and bool(torch.isfinite(demo.positions).all())
+
+
+def test_momentum_conserved_and_validation():
+    sim = small()
+    sim.positions += torch.tensor([0.0, -10.0], device="cuda")
+    sim.velocities[:] = torch.tensor([0.1, 0.05], device="cuda")
+    p0 = sim.velocities.sum(0).clone()
+    for _ in range(50):
+        sim.step()
+    assert torch.allclose(sim.velocities.sum(0), p0, rtol=1e-12, atol=1e-15)
+    with pytest.raises(ValueError):
+        IceDEM(cuda(dt=1.0))
+    with pytest.raises(ValueError):
+        IceDEM(DEMConfig(device="meta"))
+    with pytest.raises(TypeError):
+        sim.step(np.zeros((6, 2)))
+    with pytest.raises(ValueError):
+        sim.step(torch.zeros(5, 2, device="cuda"))
+    with pytest.raises(ValueError):
+        sim.step(load_time=sim.time + 1.0)
+
+
+def test_nonfinite_state_raises():
+    sim = small()
+    sim.velocities[0, 0] = float("nan")
+    with pytest.raises(FloatingPointError):
+        sim.step()
+
+
+def test_restart_round_trip_bitwise():
+    c = cuda(nx=9, ny=4, tool_speed=0.5)
+    sim = IceDEM(c)
+    sim.run(300)
+    snap = json.loads(json.dumps(sim.snapshot()))
+    restored = IceDEM.from_snapshot(snap)
+    assert restored.fracture_events() == sim.fracture_events()
+    sim.run(200)
+    restored.run(200)
+    assert torch.equal(sim.positions, restored.positions)
+    assert sim.accounting == restored.accounting
+    bad = dict(snap)
+    bad["alive"] = [True] * len(snap["alive"])
+    with pytest.raises(ValueError):
+        IceDEM.from_snapshot(bad)
+
+
+def test_fragments_match_breadth_first_search():
+    c = cuda(nx=21, ny=5, breaking_strain=0.01, shear_breaking_strain=0.02)
+    sim = IceDEM(c)
+    sim.run(2000)
+    frags = sim.fragments()
+    # host breadth-first search over the alive bonds (dem.py:462-484)
+    pairs = np_(sim.pairs)[np_(sim.alive)]
+    nb = [set() for _ in range(105)]
+    for i, j in pairs.tolist():
+        nb[i].add(j)
+        nb[j].add(i)
+    unseen, comps = set(range(105)), []
+    while unseen:
+        todo = [min(unseen)]
+        unseen.remove(todo[0])
+        comp = []
+        while todo:
+            i = todo.pop()
+            comp.append(i)
+            for j in sorted(nb[i] & unseen):
+                unseen.remove(j)
+                todo.append(j)
+        comps.append(sorted(comp))
+    assert [f["particle_ids"] for f in frags] == comps
