@@ -273,6 +273,11 @@ def main(argv=None):
         eng.close()
         del eng
     r32, r64 = results["fp32"], results["fp64"]
+    for name, r in results.items():
+        if r["status"]:
+            print(f"bench.py: {name} run ended with status bits {r['status']} (contact search "
+                  "invalid or non-finite state); its rate is not a valid measurement",
+                  file=sys.stderr)
 
     plain = None
     if not args.no_plain:

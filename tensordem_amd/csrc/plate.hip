@@ -54,10 +54,9 @@ __host__ __device__ constexpr int dxc(int c) { return c < 3 ? c - 1 : (c == 3 ? 
 __host__ __device__ constexpr int dyc(int c) { return c < 3 ? -1 : (c < 5 ? 0 : 1); }
 
 struct Workspace {
-  int4 *blocks;          // (member, tile x0, tile y0, block rank in member)
+  int4 *blocks;          // 2 per block: (member, tile x0, tile y0, rank), (nx, ny, first, tile_x)
   double *partials;      // [nblocks][kCh]
-  unsigned *counters;    // [nmembers]
-  int *req;              // [nmembers] rebuild requests
+  int *bflags;           // [nblocks] status bits | rebuild request << 16
   int *cell_count;       // [2][ncells]
   int *cell_tab;         // [2][ncells][cap]
   int *cand;             // [slots][n]
@@ -77,10 +76,9 @@ __host__ __device__ inline void carve(void *base, int n, int nblocks, int nmembe
     off += align256(bytes);
     return p;
   };
-  w->blocks = reinterpret_cast<int4 *>(take(sizeof(int4) * size_t(nblocks)));
+  w->blocks = reinterpret_cast<int4 *>(take(2 * sizeof(int4) * size_t(nblocks)));
   w->partials = reinterpret_cast<double *>(take(sizeof(double) * kCh * size_t(nblocks)));
-  w->counters = reinterpret_cast<unsigned *>(take(sizeof(unsigned) * size_t(nmembers)));
-  w->req = reinterpret_cast<int *>(take(sizeof(int) * size_t(nmembers)));
+  w->bflags = reinterpret_cast<int *>(take(sizeof(int) * size_t(nblocks)));
   w->cell_count = reinterpret_cast<int *>(take(sizeof(int) * 2 * size_t(ncells)));
   w->cell_tab = reinterpret_cast<int *>(take(sizeof(int) * 2 * size_t(ncells) * size_t(cap)));
   w->cand = reinterpret_cast<int *>(take(sizeof(int) * size_t(slots) * size_t(n)));
@@ -88,6 +86,12 @@ __host__ __device__ inline void carve(void *base, int n, int nblocks, int nmembe
   w->xb = reinterpret_cast<float2 *>(take(sizeof(float2) * size_t(n)));
   *total = off;
 }
+
+// FP32 force path: hardware reciprocal and square root (at most 1 ulp), far
+// cheaper than the correctly rounded expansions; break decisions within the
+// FP32 margin are re-done in FP64, so these never decide a fracture.
+__device__ inline float rcp32(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ inline float sqrt32(float x) { return __builtin_amdgcn_sqrtf(x); }
 
 __device__ inline double2 lattice_site(int ix, int iy, double half_span, double two_r)
 {
@@ -172,15 +176,14 @@ __device__ float strain32(const Tile &t, int ix, int iy)
 {
   float c00 = 0, c01 = 0, c11 = 0, u00 = 0, u01 = 0, u10 = 0, u11 = 0;
   const double2 xp = t.x(ix, iy);
-  const double2 sp = lattice_site(ix, iy, t.half_span, t.two_r);
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
     const int c = (k + 4) & 7;
     const int jx = ix + dxc(c), jy = iy + dyc(c);
     if (!on_lattice(jx, jy, t.nx, t.ny)) continue;
     const double2 xq = t.x(jx, jy);
-    const double2 sq = lattice_site(jx, jy, t.half_span, t.two_r);
-    double rx = sq.x - sp.x, ry = sq.y - sp.y;
+    // nominal lattice vector: differs from the site difference by ulps only
+    const double rx = dxc(c) * t.two_r, ry = dyc(c) * t.two_r;
     float ux = (float)((xq.x - xp.x) - rx), uy = (float)((xq.y - xp.y) - ry);
     float fx = (float)rx, fy = (float)ry;
     c00 += fx * fx; c01 += fx * fy; c11 += fy * fy;
@@ -189,12 +192,12 @@ __device__ float strain32(const Tile &t, int ix, int iy)
   float det = c00 * c11 - c01 * c01;
   float i00, i01, i11;
   if (det != 0.0f) {
-    i00 = c11 / det; i01 = -c01 / det; i11 = c00 / det;
+    const float r = rcp32(det);
+    i00 = c11 * r; i01 = -c01 * r; i11 = c00 * r;
   } else {
     float tr = c00 + c11;
-    float t2 = tr > 0.0f ? tr * tr : 1.0f;
-    i00 = tr > 0.0f ? c00 / t2 : 0.0f; i01 = tr > 0.0f ? c01 / t2 : 0.0f;
-    i11 = tr > 0.0f ? c11 / t2 : 0.0f;
+    float r = tr > 0.0f ? rcp32(tr * tr) : 0.0f;
+    i00 = c00 * r; i01 = c01 * r; i11 = c11 * r;
   }
   float h00 = u00 * i00 + u01 * i01;
   float h01 = u00 * i01 + u01 * i11;
@@ -204,7 +207,7 @@ __device__ float strain32(const Tile &t, int ix, int iy)
   float eyy = h11 + 0.5f * (h01 * h01 + h11 * h11);
   float exy = 0.5f * (h01 + h10) + 0.5f * (h00 * h01 + h10 * h11);
   float dxy = exx - eyy;
-  return sqrtf(dxy * dxy + 4.0f * (exy * exy));
+  return sqrt32(dxy * dxy + 4.0f * (exy * exy));
 }
 
 struct KArgs {
@@ -215,26 +218,49 @@ struct KArgs {
   int op;  // OP_FORCES, OP_STEP or OP_ENERGY
 };
 
+// Deterministic wave-wide sum of a double: a fixed DPP shift/broadcast tree
+// (row_shr 1,2,4,8 inside each 16-lane row, then row_bcast 15 and 31), both
+// 32-bit halves moved with the same control.  The total is read from lane 63
+// and returned in every lane.
+template <int CTRL, int ROWMASK>
+__device__ inline double dpp_move(double v)
+{
+  int lo = __double2loint(v), hi = __double2hiint(v);
+  lo = __builtin_amdgcn_update_dpp(0, lo, CTRL, ROWMASK, 0xF, false);
+  hi = __builtin_amdgcn_update_dpp(0, hi, CTRL, ROWMASK, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+
 __device__ inline double wave_sum(double v)
 {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-  return v;
+  v += dpp_move<0x111, 0xF>(v);
+  v += dpp_move<0x112, 0xF>(v);
+  v += dpp_move<0x114, 0xF>(v);
+  v += dpp_move<0x118, 0xF>(v);
+  v += dpp_move<0x142, 0xA>(v);
+  v += dpp_move<0x143, 0xC>(v);
+  int lo = __builtin_amdgcn_readlane(__double2loint(v), 63);
+  int hi = __builtin_amdgcn_readlane(__double2hiint(v), 63);
+  return __hiloint2double(hi, lo);
 }
 
 // Deterministic block sum of kCh channels; result valid in thread 0.
-__device__ void block_sum(double (&ch)[kCh], double (*lds)[kCh])
+template <int NCH>
+__device__ void block_sum(double (&ch)[NCH], double (*lds)[kCh])
 {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
-  for (int k = 0; k < kCh; ++k) {
-    double s = wave_sum(ch[k]);
+  for (int k = 0; k < NCH; ++k) {
+    // most channels are zero in most waves (breaks, loads, supports, tool):
+    // a wave-uniform skip keeps their DPP trees off the critical path
+    double s = 0.0;
+    if (__ballot(ch[k] != 0.0)) s = wave_sum(ch[k]);
     if (lane == 0) lds[wave][k] = s;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
 #pragma unroll
-    for (int k = 0; k < kCh; ++k) ch[k] = ((lds[0][k] + lds[1][k]) + lds[2][k]) + lds[3][k];
+    for (int k = 0; k < NCH; ++k) ch[k] = ((lds[0][k] + lds[1][k]) + lds[2][k]) + lds[3][k];
   }
 }
 
@@ -242,45 +268,47 @@ template <typename R> struct Prec;
 template <> struct Prec<double> { static constexpr bool fp64 = true; };
 template <> struct Prec<float> { static constexpr bool fp64 = false; };
 
-// One bond's or contact's contribution, written for both ends with the same
-// operand order (lo -> hi) so the two ends agree bit for bit.
+// Repulsive contact of a non-bonded or broken pair (dem.py:278-288), written
+// with the lo -> hi orientation so both ends compute identical bits.  Returns
+// false when the pair does not overlap; else (Fx, Fy) is the force on lo's
+// partner side convention of the reference (`pair_force`, applied +F to lo,
+// -F to hi) and `diss` the pair's dissipated power (dem.py:327-328).
 template <typename R>
-__device__ void contact_pair(double2 x_lo, double2 x_hi, double2 v_lo, double2 v_hi, double2 n0,
-                             const tdm_plate_member &m, bool owner, double (&ch)[kCh],
-                             double &fx, double &fy, bool self_lo)
+__device__ inline bool contact_pair(double2 x_lo, double2 x_hi, double2 v_lo, double2 v_hi,
+                                    double2 n0, double radius, double kc, double cdamp,
+                                    double &Fx, double &Fy, double &diss)
 {
-  const double two_r = 2 * m.radius;
+  const double two_r = 2 * radius;
   if constexpr (Prec<R>::fp64) {
     double dx = x_hi.x - x_lo.x, dy = x_hi.y - x_lo.y;
     double d = sqrt(dx * dx + dy * dy);
-    if (!(d < two_r)) return;
+    if (!(d < two_r)) return false;
     double nx, ny;
     if (d > 0) { double dd = d > kEps64 ? d : kEps64; nx = dx / dd; ny = dy / dd; }
     else { nx = n0.x; ny = n0.y; }
     double ov = two_r - d;
     double vn = (v_hi.x - v_lo.x) * nx + (v_hi.y - v_lo.y) * ny;
-    double rep = m.contact_stiffness * ov - m.contact_damping * vn;
+    double rep = kc * ov - cdamp * vn;
     if (rep < 0) rep = 0;
     double mag = -rep;
-    double Fx = mag * nx, Fy = mag * ny;
-    if (self_lo) { fx += Fx; fy += Fy; } else { fx += -Fx; fy += -Fy; }
-    if (owner) ch[C_PAIR_DISS] += (rep - m.contact_stiffness * ov) * (-vn);
+    Fx = mag * nx; Fy = mag * ny;
+    diss = (rep - kc * ov) * (-vn);
   } else {
     float dx = (float)(x_hi.x - x_lo.x), dy = (float)(x_hi.y - x_lo.y);
-    float d = sqrtf(dx * dx + dy * dy);
+    float d = sqrt32(dx * dx + dy * dy);
     float tr = (float)two_r;
-    if (!(d < tr)) return;
+    if (!(d < tr)) return false;
     float nx, ny;
-    if (d > 0) { float dd = fmaxf(d, kEps32); nx = dx / dd; ny = dy / dd; }
+    if (d > 0) { float r = rcp32(fmaxf(d, kEps32)); nx = dx * r; ny = dy * r; }
     else { nx = (float)n0.x; ny = (float)n0.y; }
     float ov = tr - d;
     float vn = (float)(v_hi.x - v_lo.x) * nx + (float)(v_hi.y - v_lo.y) * ny;
-    float kc = (float)m.contact_stiffness;
-    float rep = fmaxf(kc * ov - (float)m.contact_damping * vn, 0.0f);
-    float Fx = -rep * nx, Fy = -rep * ny;
-    if (self_lo) { fx += (double)Fx; fy += (double)Fy; } else { fx -= (double)Fx; fy -= (double)Fy; }
-    if (owner) ch[C_PAIR_DISS] += (double)((rep - kc * ov) * (-vn));
+    float kcf = (float)kc;
+    float rep = fmaxf(kcf * ov - (float)cdamp * vn, 0.0f);
+    Fx = (double)(-rep * nx); Fy = (double)(-rep * ny);
+    diss = (double)((rep - kcf * ov) * (-vn));
   }
+  return true;
 }
 
 template <typename R>
@@ -289,37 +317,73 @@ __global__ __launch_bounds__(kThreads) void plate_step_kernel(KArgs k)
   __shared__ double2 sx[kMaxLds];
   __shared__ R ss[kMaxRing];
   __shared__ double red[4][kCh];
-  __shared__ int last_flag;
+  __shared__ tdm_plate_member sm;
+  __shared__ int sflags[2];
 
   const tdm_plate_arrays &a = k.a;
-  const int4 blk = k.w.blocks[blockIdx.x];
-  const int mi = blk.x;
-  const tdm_plate_member &m = a.members[mi];
-  const int nx = m.nx, ny = m.ny, TX = m.tile_x, TY = kThreads / m.tile_x;
-  const int tx0 = blk.y, ty0 = blk.z;
+  // block table entry: (member, tile x0, tile y0, rank), (nx, ny, first, tile_x)
+  const int4 blk = k.w.blocks[2 * blockIdx.x];
+  const int4 geo = k.w.blocks[2 * blockIdx.x + 1];
+  const int mi = blk.x, tx0 = blk.y, ty0 = blk.z;
+  const int nx = geo.x, ny = geo.y, first = geo.z, TX = geo.w, TY = kThreads / TX;
   const int lw = TX + 2 * kHalo, lh = TY + 2 * kHalo;
+  const int n = a.n;
+
+  // per-disk inputs are issued before the first barrier so their latency
+  // overlaps the halo load
+  const int ix = tx0 + (int)threadIdx.x % TX, iy = ty0 + (int)threadIdx.x / TX;
+  const bool valid = ix < nx && iy < ny;
+  const int p = valid ? first + iy * nx + ix : first;
+  double2 vp = make_double2(0.0, 0.0);
+  uint32_t links = 0;
+  bool fixed = false;
+  int nc = 0;
+  float2 xb = make_float2(0.0f, 0.0f);
+  if (valid) {
+    vp = k.v_in[p];
+    links = a.links[p];
+    fixed = a.fixed[p] != 0;
+    nc = k.w.ncand[p];
+    xb = k.w.xb[p];
+  }
+  // the member record is read-only during the step: stage it in LDS so that
+  // global stores later in the kernel cannot force re-reads
+  static_assert(sizeof(tdm_plate_member) % 8 == 0, "member record must be 8-byte sized");
+  if (threadIdx.x < sizeof(tdm_plate_member) / 8)
+    reinterpret_cast<uint64_t *>(&sm)[threadIdx.x] =
+        reinterpret_cast<const uint64_t *>(a.members + mi)[threadIdx.x];
+  if (threadIdx.x < 2) sflags[threadIdx.x] = 0;
+  // stage positions of tile + 2-disk halo
+  // (at most kMaxLds / kThreads + 1 = 6 rounds; loads of a round are issued together)
+  for (int t0 = 0; t0 < lw * lh; t0 += 2 * kThreads) {
+    double2 v[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int t = t0 + u * kThreads + threadIdx.x;
+      const int hx = tx0 - kHalo + t % lw, hy = ty0 - kHalo + t / lw;
+      v[u] = make_double2(0.0, 0.0);
+      if (t < lw * lh && on_lattice(hx, hy, nx, ny)) v[u] = k.x_in[first + hy * nx + hx];
+    }
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int t = t0 + u * kThreads + threadIdx.x;
+      if (t < lw * lh) sx[t] = v[u];
+    }
+  }
+  __syncthreads();
+  const tdm_plate_member &m = sm;
   const double two_r = 2 * m.radius;
   const double half_span = (double)(nx - 1) / 2;
   const int mode = m.mode;
   const double time = m.time;
-  const int first = m.first;
-
-  // stage positions of tile + 2-disk halo
-  for (int t = threadIdx.x; t < lw * lh; t += kThreads) {
-    int ix = tx0 - kHalo + t % lw, iy = ty0 - kHalo + t / lw;
-    double2 v = make_double2(0.0, 0.0);
-    if (on_lattice(ix, iy, nx, ny)) v = k.x_in[first + iy * nx + ix];
-    sx[t] = v;
-  }
-  __syncthreads();
   Tile tile{sx, lw, tx0, ty0, nx, ny, half_span, two_r};
   const int rw = TX + 2;
   for (int t = threadIdx.x; t < rw * (TY + 2); t += kThreads) {
-    int ix = tx0 - 1 + t % rw, iy = ty0 - 1 + t / rw;
+    int hx = tx0 - 1 + t % rw, hy = ty0 - 1 + t / rw;
     R s = 0;
-    if (on_lattice(ix, iy, nx, ny)) {
-      if constexpr (Prec<R>::fp64) s = strain64(tile, ix, iy);
-      else s = strain32(tile, ix, iy);
+    if (on_lattice(hx, hy, nx, ny)) {
+      if constexpr (Prec<R>::fp64) s = strain64(tile, hx, hy);
+      else s = strain32(tile, hx, hy);
     }
     ss[t] = s;
   }
@@ -328,29 +392,26 @@ __global__ __launch_bounds__(kThreads) void plate_step_kernel(KArgs k)
   double ch[kCh];
 #pragma unroll
   for (int c = 0; c < kCh; ++c) ch[c] = 0.0;
-
-  const int ix = tx0 + (int)threadIdx.x % TX, iy = ty0 + (int)threadIdx.x / TX;
-  const bool valid = ix < nx && iy < ny;
-  const int n = a.n;
   int status = 0, req = 0;
   if (valid) {
-    const int p = first + iy * nx + ix;
     const double2 xp = tile.x(ix, iy);
-    const double2 vp = k.v_in[p];
     const double2 sp = lattice_site(ix, iy, half_span, two_r);
     const R s_self = ss[(iy - ty0 + 1) * rw + (ix - tx0 + 1)];
     const uint32_t present = present_links(ix, iy, nx, ny);
-    uint32_t links = a.links[p] & present;
+    links &= present;
+    const double kb = m.bond_stiffness, kc = m.contact_stiffness, cdamp = m.contact_damping;
     const double eps_t_p = m.per_disk_strength ? a.eps_t_disk[p] : m.breaking_strain;
     const double eps_s_p = m.per_disk_strength ? a.eps_s_disk[p] : m.shear_breaking_strain;
+    double2 ld = make_double2(0.0, 0.0);
+    if (m.has_loads) ld = reinterpret_cast<const double2 *>(a.loads)[p];
 
     // ---- unbonded contact candidates -------------------------------------
     // Candidate lists live in global memory, slot-major (cand[e*n + p]); the
     // owning thread is the only reader and writer of its slots.
     int *cand = k.w.cand + p;
-    int nc = 0;
     const int slots = a.cand_slots;
     if (mode == MODE_SEARCH) {
+      nc = 0;
       const double cutoff = two_r + m.skin;
       const double h = m.cell_size;
       int cx = (int)floor((xp.x - m.grid_origin[0]) / h);
@@ -360,14 +421,21 @@ __global__ __launch_bounds__(kThreads) void plate_step_kernel(KArgs k)
       cy = min(max(cy, 0), m.ncy - 1);
       const int *cnt = k.w.cell_count + (size_t)m.tab * a.ncells;
       const int *tab = k.w.cell_tab + (size_t)m.tab * a.ncells * a.cell_cap;
-      for (int oy = -1; oy <= 1; ++oy) {
-        int gy = cy + oy;
-        if (gy < 0 || gy >= m.ncy) continue;
-        for (int ox = -1; ox <= 1; ++ox) {
-          int gx = cx + ox;
-          if (gx < 0 || gx >= m.ncx) continue;
-          int cell = m.cell_first + gy * m.ncx + gx;
-          int cn = min(cnt[cell], a.cell_cap);
+      // the nine cell counts are fetched together, then each cell's entries
+      int ccount[9];
+#pragma unroll
+      for (int o = 0; o < 9; ++o) {
+        const int gx = cx + o % 3 - 1, gy = cy + o / 3 - 1;
+        const bool in = gx >= 0 && gx < m.ncx && gy >= 0 && gy < m.ncy;
+        ccount[o] = in ? min(cnt[m.cell_first + gy * m.ncx + gx], a.cell_cap) : 0;
+      }
+#pragma unroll
+      for (int o = 0; o < 9; ++o) {
+        {
+          const int gx = cx + o % 3 - 1, gy = cy + o / 3 - 1;
+          const int cn = ccount[o];
+          if (cn == 0) continue;
+          const int cell = m.cell_first + gy * m.ncx + gx;
           for (int e = 0; e < cn; ++e) {
             int q = tab[(size_t)cell * a.cell_cap + e];
             if (q == p) continue;
@@ -391,202 +459,62 @@ __global__ __launch_bounds__(kThreads) void plate_step_kernel(KArgs k)
         }
       }
       k.w.ncand[p] = (uint8_t)nc;
-      k.w.xb[p] = make_float2((float)(xp.x - sp.x), (float)(xp.y - sp.y));
+      xb = make_float2((float)(xp.x - sp.x), (float)(xp.y - sp.y));
+      k.w.xb[p] = xb;
     } else {
-      nc = k.w.ncand[p];
-      float2 b = k.w.xb[p];
-      double ddx = (xp.x - sp.x) - (double)b.x, ddy = (xp.y - sp.y) - (double)b.y;
+      double ddx = (xp.x - sp.x) - (double)xb.x, ddy = (xp.y - sp.y) - (double)xb.y;
       double lim = 0.5 * m.skin * 0.999;
       if (ddx * ddx + ddy * ddy >= lim * lim) status |= TDM_ST_STALE_LIST;
     }
-
-    // ---- bonds and contacts in index_add_ order --------------------------
-    double fx = (-m.drag) * vp.x, fy = (-m.drag) * vp.y;
     int split = 0;
     while (split < nc && cand[(size_t)split * n] < p) ++split;
-    for (int group = 0; group < 2; ++group) {
-      int code = group == 0 ? 4 : 0, code_end = group == 0 ? 8 : 4;
-      int ci = group == 0 ? split : 0, ci_end = group == 0 ? nc : split;
-      while (true) {
-        while (code < code_end && !((present >> code) & 1u)) ++code;
-        int qb = code < code_end ? p + dyc(code) * nx + dxc(code) : INT32_MAX;
-        int qc = ci < ci_end ? cand[(size_t)ci * n] : INT32_MAX;
-        if (qb == INT32_MAX && qc == INT32_MAX) break;
-        if (qb < qc) {
-          // ---- lattice bond p <-> q (direction `code`) ----
-          const int c = code++;
-          const bool self_lo = c >= 4;
-          const int jx = ix + dxc(c), jy = iy + dyc(c);
-          const double2 xq = tile.x(jx, jy);
-          const double2 sq = lattice_site(jx, jy, half_span, two_r);
-          const R s_q = ss[(jy - ty0 + 1) * rw + (jx - tx0 + 1)];
-          const double2 x_lo = self_lo ? xp : xq, x_hi = self_lo ? xq : xp;
-          const double2 s_lo = self_lo ? sp : sq, s_hi = self_lo ? sq : sp;
-          const R sh_lo = self_lo ? s_self : s_q, sh_hi = self_lo ? s_q : s_self;
-          const double rx = s_hi.x - s_lo.x, ry = s_hi.y - s_lo.y;
-          const double rest = sqrt(rx * rx + ry * ry);
-          double eps_t = m.breaking_strain, eps_s = m.shear_breaking_strain;
-          if (m.per_disk_strength) {
-            const int q = p + dyc(c) * nx + dxc(c);
-            eps_t = fmin(eps_t_p, a.eps_t_disk[q]);
-            eps_s = fmin(eps_s_p, a.eps_s_disk[q]);
-          }
-          bool alive = (links >> c) & 1u;
-          if (k.op == OP_ENERGY) {
-            // dem.py:603-624: each bond counted once, at its lower end
-            if (self_lo) {
-              double dx = x_hi.x - x_lo.x, dy = x_hi.y - x_lo.y;
-              double d = sqrt(dx * dx + dy * dy);
-              if (alive) { double e = d - rest; ch[E_BOND] += e * e; }
-              else { double ov = two_r - d; if (ov > 0) ch[E_CONTACT] += ov * ov; }
-            }
-            continue;
-          }
-          double Fx, Fy;
-          // exact FP64 geometry (always needed on the FP64 path, and for
-          // break decisions / records on the FP32 path)
-          auto geom64 = [&](double &d, double &nxv, double &nyv) {
-            double dx = x_hi.x - x_lo.x, dy = x_hi.y - x_lo.y;
-            d = sqrt(dx * dx + dy * dy);
-            if (d > 0) { double dd = d > kEps64 ? d : kEps64; nxv = dx / dd; nyv = dy / dd; }
-            else { nxv = rx / rest; nyv = ry / rest; }
-          };
-          if constexpr (Prec<R>::fp64) {
-            double d, nxv, nyv;
-            geom64(d, nxv, nyv);
-            double ext = d - rest;
-            if (alive) {
-              double tstrain = ext / rest;
-              double bshear = 0.5 * (sh_lo + sh_hi);
-              int sf = bshear > eps_s, tf = tstrain > eps_t;
-              if (sf || tf) {
-                alive = false;
-                links &= ~(1u << c);
-                if (self_lo) {
-                  int rank = __popc(present & ((1u << c) - 1u) & 0xF0u);
-                  int b = a.fwd_bond[p] + rank;
-                  a.alive[b] = 0;
-                  a.failure_mode[b] = (uint8_t)(tf + 2 * sf);
-                  a.failure_time_s[b] = time;
-                  a.failure_extension_m[b] = ext;
-                  double en = (0.5 * m.bond_stiffness) * (ext * ext);
-                  a.failure_energy_J[b] = en;
-                  ch[C_RELEASED] += en;
-                  double ov = two_r - d;
-                  if (ov < 0) ov = 0;
-                  ch[C_ACQUIRED] += (0.5 * m.contact_stiffness) * (ov * ov);
-                  ch[C_BREAKS] += 1.0;
-                }
-              }
-            }
-            if (alive) {
-              double mag = m.bond_stiffness * ext;
-              Fx = mag * nxv; Fy = mag * nyv;
-              if (self_lo) { fx += Fx; fy += Fy; } else { fx += -Fx; fy += -Fy; }
-            } else {
-              const int q = p + dyc(c) * nx + dxc(c);
-              const double2 vq = k.v_in[q];
-              contact_pair<double>(x_lo, x_hi, self_lo ? vp : vq, self_lo ? vq : vp,
-                                   make_double2(rx / rest, ry / rest), m, self_lo, ch, fx, fy,
-                                   self_lo);
-            }
-          } else {
-            // FP32: displacement relative to the rest vector keeps the
-            // extension free of cancellation.
-            float ux = (float)((x_hi.x - x_lo.x) - rx), uy = (float)((x_hi.y - x_lo.y) - ry);
-            float rfx = (float)rx, rfy = (float)ry, restf = (float)rest;
-            float dvx = rfx + ux, dvy = rfy + uy;
-            float d = sqrtf(dvx * dvx + dvy * dvy);
-            float ext = (2.0f * (rfx * ux + rfy * uy) + (ux * ux + uy * uy)) / (d + restf);
-            float nxv, nyv;
-            if (d > 0) { float dd = fmaxf(d, kEps32); nxv = dvx / dd; nyv = dvy / dd; }
-            else { nxv = rfx / restf; nyv = rfy / restf; }
-            if (alive) {
-              float tstrain = ext / restf;
-              float bshear = 0.5f * ((float)sh_lo + (float)sh_hi);
-              const double mg = m.fp32_margin;
-              bool near_t = fabs((double)tstrain - eps_t) <= mg * eps_t;
-              bool near_s = fabs((double)bshear - eps_s) <= mg * eps_s;
-              int tf = (double)tstrain > eps_t;
-              int sf = (double)bshear > eps_s;
-              if (near_t || near_s) {
-                double d64, n1, n2;
-                geom64(d64, n1, n2);
-                if (near_t) tf = ((d64 - rest) / rest) > eps_t;
-                if (near_s) {
-                  double slo = strain64(tile, self_lo ? ix : jx, self_lo ? iy : jy);
-                  double shi = strain64(tile, self_lo ? jx : ix, self_lo ? jy : iy);
-                  sf = 0.5 * (slo + shi) > eps_s;
-                }
-              }
-              if (sf || tf) {
-                alive = false;
-                links &= ~(1u << c);
-                if (self_lo) {
-                  double d64, n1, n2;
-                  geom64(d64, n1, n2);
-                  double ext64 = d64 - rest;
-                  int rank = __popc(present & ((1u << c) - 1u) & 0xF0u);
-                  int b = a.fwd_bond[p] + rank;
-                  a.alive[b] = 0;
-                  a.failure_mode[b] = (uint8_t)(tf + 2 * sf);
-                  a.failure_time_s[b] = time;
-                  a.failure_extension_m[b] = ext64;
-                  double en = (0.5 * m.bond_stiffness) * (ext64 * ext64);
-                  a.failure_energy_J[b] = en;
-                  ch[C_RELEASED] += en;
-                  double ov = two_r - d64;
-                  if (ov < 0) ov = 0;
-                  ch[C_ACQUIRED] += (0.5 * m.contact_stiffness) * (ov * ov);
-                  ch[C_BREAKS] += 1.0;
-                }
-              }
-            }
-            if (alive) {
-              float mag = (float)m.bond_stiffness * ext;
-              float Fxf = mag * nxv, Fyf = mag * nyv;
-              if (self_lo) { fx += (double)Fxf; fy += (double)Fyf; }
-              else { fx -= (double)Fxf; fy -= (double)Fyf; }
-            } else {
-              const int q = p + dyc(c) * nx + dxc(c);
-              const double2 vq = k.v_in[q];
-              contact_pair<float>(x_lo, x_hi, self_lo ? vp : vq, self_lo ? vq : vp,
-                                  make_double2(rx / rest, ry / rest), m, self_lo, ch, fx, fy,
-                                  self_lo);
-            }
-          }
-        } else {
-          // ---- unbonded contact candidate ----
-          const int q = qc;
-          ++ci;
-          const bool self_lo = p < q;
-          const double2 xq = k.x_in[q];
-          if (k.op == OP_ENERGY) {
-            if (self_lo) {
-              double dx = xq.x - xp.x, dy = xq.y - xp.y;
-              double ov = two_r - sqrt(dx * dx + dy * dy);
-              if (ov > 0) ch[E_CONTACT] += ov * ov;
-            }
-            continue;
-          }
-          const double2 vq = k.v_in[q];
-          const int ql = q - first, jx = ql % nx, jy = ql / nx;
-          const double2 sq = lattice_site(jx, jy, half_span, two_r);
-          const double2 s_lo = self_lo ? sp : sq, s_hi = self_lo ? sq : sp;
-          double ix0 = s_hi.x - s_lo.x, iy0 = s_hi.y - s_lo.y;
-          double id = sqrt(ix0 * ix0 + iy0 * iy0);
-          contact_pair<R>(self_lo ? xp : xq, self_lo ? xq : xp, self_lo ? vp : vq,
-                          self_lo ? vq : vp, make_double2(ix0 / id, iy0 / id), m, self_lo, ch,
-                          fx, fy, self_lo);
+
+    // unbonded candidate q, lo -> hi oriented; accumulates into f
+    auto unbonded = [&](int q, double &fx, double &fy, bool energy) {
+      const bool self_lo = p < q;
+      const double2 xq = k.x_in[q];
+      if (energy) {
+        if (self_lo) {
+          double dx = xq.x - xp.x, dy = xq.y - xp.y;
+          double ov = two_r - sqrt(dx * dx + dy * dy);
+          if (ov > 0) ch[E_CONTACT] += ov * ov;
         }
+        return;
       }
-    }
-    a.links[p] = (uint8_t)links;
+      const double2 vq = k.v_in[q];
+      const int ql = q - first, jx = ql % nx, jy = ql / nx;
+      const double2 sq = lattice_site(jx, jy, half_span, two_r);
+      const double2 s_lo = self_lo ? sp : sq, s_hi = self_lo ? sq : sp;
+      double ix0 = s_hi.x - s_lo.x, iy0 = s_hi.y - s_lo.y;
+      double id = sqrt(ix0 * ix0 + iy0 * iy0);
+      double Fx, Fy, diss;
+      if (contact_pair<R>(self_lo ? xp : xq, self_lo ? xq : xp, self_lo ? vp : vq,
+                          self_lo ? vq : vp, make_double2(ix0 / id, iy0 / id), m.radius, kc,
+                          cdamp, Fx, Fy, diss)) {
+        if (self_lo) { fx += Fx; fy += Fy; ch[C_PAIR_DISS] += diss; }
+        else { fx += -Fx; fy += -Fy; }
+      }
+    };
 
     if (k.op == OP_ENERGY) {
-      double d;
+      // dem.py:603-624: each bond and contact counted once, at its lower end
+#pragma unroll
+      for (int c = 4; c < 8; ++c) {
+        if (!((present >> c) & 1u)) continue;
+        const double2 xq = tile.x(ix + dxc(c), iy + dyc(c));
+        const double2 sq = lattice_site(ix + dxc(c), iy + dyc(c), half_span, two_r);
+        double rx = sq.x - sp.x, ry = sq.y - sp.y;
+        double rest = sqrt(rx * rx + ry * ry);
+        double dx = xq.x - xp.x, dy = xq.y - xp.y;
+        double d = sqrt(dx * dx + dy * dy);
+        if ((links >> c) & 1u) { double e = d - rest; ch[E_BOND] += e * e; }
+        else { double ov = two_r - d; if (ov > 0) ch[E_CONTACT] += ov * ov; }
+      }
+      double dummy_x = 0, dummy_y = 0;
+      for (int e = split; e < nc; ++e) unbonded(cand[(size_t)e * n], dummy_x, dummy_y, true);
       const double ox = m.tool_start[0] + time * m.tool_velocity[0];
       const double oy = m.tool_start[1] + time * m.tool_velocity[1];
+      double d;
       if (m.hull_m == 0) {
         double dx = xp.x - ox, dy = xp.y - oy;
         d = sqrt(dx * dx + dy * dy);
@@ -607,127 +535,311 @@ __global__ __launch_bounds__(kThreads) void plate_step_kernel(KArgs k)
       if (ov > 0) ch[E_TOOL] += ov * ov;
       ch[E_VSQ] += vp.x * vp.x + vp.y * vp.y;
     } else {
-    // ---- tool or hull (dem.py:294-324, hull_contact.py:7-75) --------------
-    const double tvx = m.tool_velocity[0], tvy = m.tool_velocity[1];
-    double tfx, tfy, cfx, cfy;
-    if (m.hull_m == 0) {
-      const double tpx = m.tool_start[0] + time * m.tool_velocity[0];
-      const double tpy = m.tool_start[1] + time * m.tool_velocity[1];
-      double dx = xp.x - tpx, dy = xp.y - tpy;
-      if constexpr (Prec<R>::fp64) {
-        double d = sqrt(dx * dx + dy * dy);
+      // ---- all present bonds, independently (8-way instruction-level
+      // parallelism); fracture commit and force per bond -------------------
+      double Fx[8], Fy[8];
+      uint32_t has = 0;
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        Fx[c] = 0.0;
+        Fy[c] = 0.0;
+        if (!((present >> c) & 1u)) continue;
+        const bool self_lo = c >= 4;
+        const int jx = ix + dxc(c), jy = iy + dyc(c);
+        const int q = p + dyc(c) * nx + dxc(c);
+        const double2 xq = tile.x(jx, jy);
+        const R s_q = ss[(jy - ty0 + 1) * rw + (jx - tx0 + 1)];
+        const double2 x_lo = self_lo ? xp : xq, x_hi = self_lo ? xq : xp;
+        const R sh_lo = self_lo ? s_self : s_q, sh_hi = self_lo ? s_q : s_self;
+        // exact rest vector from the two lattice sites (dem.py:119-129)
+        auto site_vec = [&](double &rx_, double &ry_, double &rest_) {
+          const double2 sq = lattice_site(jx, jy, half_span, two_r);
+          const double2 s_lo = self_lo ? sp : sq, s_hi = self_lo ? sq : sp;
+          rx_ = s_hi.x - s_lo.x;
+          ry_ = s_hi.y - s_lo.y;
+          rest_ = sqrt(rx_ * rx_ + ry_ * ry_);
+        };
+        double eps_t = m.breaking_strain, eps_s = m.shear_breaking_strain;
+        if (m.per_disk_strength) {
+          eps_t = fmin(eps_t_p, a.eps_t_disk[q]);
+          eps_s = fmin(eps_s_p, a.eps_s_disk[q]);
+        }
+        bool alive = (links >> c) & 1u;
+        // exact FP64 geometry of the pair (the FP32 path uses it for break
+        // decisions near a threshold and for the failure record)
+        auto geom64 = [&](double rx, double ry, double rest, double &d, double &nxv,
+                          double &nyv) {
+          double dx = x_hi.x - x_lo.x, dy = x_hi.y - x_lo.y;
+          d = sqrt(dx * dx + dy * dy);
+          if (d > 0) { double dd = d > kEps64 ? d : kEps64; nxv = dx / dd; nyv = dy / dd; }
+          else { nxv = rx / rest; nyv = ry / rest; }
+        };
+        auto record = [&](int tf, int sf, double d64, double rest) {
+          // the lower-index end owns the bond's failure record (dem.py:259-277)
+          links &= ~(1u << c);
+          if (!self_lo) return;
+          const double ext64 = d64 - rest;
+          const int b = a.fwd_bond[p] + __popc(present & ((1u << c) - 1u) & 0xF0u);
+          a.alive[b] = 0;
+          a.failure_mode[b] = (uint8_t)(tf + 2 * sf);
+          a.failure_time_s[b] = time;
+          a.failure_extension_m[b] = ext64;
+          const double en = (0.5 * kb) * (ext64 * ext64);
+          a.failure_energy_J[b] = en;
+          ch[C_RELEASED] += en;
+          double ov = two_r - d64;
+          if (ov < 0) ov = 0;
+          ch[C_ACQUIRED] += (0.5 * kc) * (ov * ov);
+          ch[C_BREAKS] += 1.0;
+        };
+        if constexpr (Prec<R>::fp64) {
+          double rx, ry, rest, d, nxv, nyv;
+          site_vec(rx, ry, rest);
+          geom64(rx, ry, rest, d, nxv, nyv);
+          double ext = d - rest;
+          if (alive) {
+            double tstrain = ext / rest;
+            double bshear = 0.5 * (sh_lo + sh_hi);
+            int sf = bshear > eps_s, tf = tstrain > eps_t;
+            if (sf || tf) { alive = false; record(tf, sf, d, rest); }
+          }
+          if (alive) {
+            double mag = kb * ext;
+            Fx[c] = mag * nxv;
+            Fy[c] = mag * nyv;
+            has |= 1u << c;
+          } else {
+            const double2 vq = k.v_in[q];
+            double diss;
+            if (contact_pair<double>(x_lo, x_hi, self_lo ? vp : vq, self_lo ? vq : vp,
+                                     make_double2(rx / rest, ry / rest), m.radius, kc, cdamp,
+                                     Fx[c], Fy[c], diss)) {
+              has |= 1u << c;
+              if (self_lo) ch[C_PAIR_DISS] += diss;
+            }
+          }
+        } else {
+          // FP32: displacement relative to the nominal lattice vector keeps
+          // the extension free of cancellation; exact FP64 site geometry is
+          // only formed for break decisions inside the margin and records.
+          const double rnx = (self_lo ? dxc(c) : -dxc(c)) * two_r;
+          const double rny = (self_lo ? dyc(c) : -dyc(c)) * two_r;
+          const float rfx = (float)rnx, rfy = (float)rny;
+          const float restf = (dxc(c) != 0 && dyc(c) != 0) ? (float)(two_r * 1.4142135623730951)
+                                                             : (float)two_r;
+          float ux = (float)((x_hi.x - x_lo.x) - rnx), uy = (float)((x_hi.y - x_lo.y) - rny);
+          float dvx = rfx + ux, dvy = rfy + uy;
+          float d = sqrt32(dvx * dvx + dvy * dvy);
+          float ext = (2.0f * (rfx * ux + rfy * uy) + (ux * ux + uy * uy)) * rcp32(d + restf);
+          const float inv_rest = rcp32(restf);
+          float nxv, nyv;
+          if (d > 0) { float r = rcp32(fmaxf(d, kEps32)); nxv = dvx * r; nyv = dvy * r; }
+          else { nxv = rfx * inv_rest; nyv = rfy * inv_rest; }
+          if (alive) {
+            const float tstrain = ext * inv_rest;
+            const float bshear = 0.5f * ((float)sh_lo + (float)sh_hi);
+            const double mg = m.fp32_margin;
+            const bool near_t = tstrain >= (float)(eps_t * (1.0 - mg)) &&
+                                tstrain <= (float)(eps_t * (1.0 + mg));
+            const bool near_s = bshear >= (float)(eps_s * (1.0 - mg)) &&
+                                bshear <= (float)(eps_s * (1.0 + mg));
+            int tf = tstrain > (float)eps_t;
+            int sf = bshear > (float)eps_s;
+            if (near_t || near_s) {
+              double rx, ry, rest, d64, n1, n2;
+              site_vec(rx, ry, rest);
+              geom64(rx, ry, rest, d64, n1, n2);
+              if (near_t) tf = ((d64 - rest) / rest) > eps_t;
+              if (near_s) {
+                double slo = strain64(tile, self_lo ? ix : jx, self_lo ? iy : jy);
+                double shi = strain64(tile, self_lo ? jx : ix, self_lo ? jy : iy);
+                sf = 0.5 * (slo + shi) > eps_s;
+              }
+            }
+            if (sf || tf) {
+              alive = false;
+              double rx, ry, rest, d64, n1, n2;
+              site_vec(rx, ry, rest);
+              geom64(rx, ry, rest, d64, n1, n2);
+              record(tf, sf, d64, rest);
+            }
+          }
+          if (alive) {
+            float mag = (float)kb * ext;
+            Fx[c] = (double)(mag * nxv);
+            Fy[c] = (double)(mag * nyv);
+            has |= 1u << c;
+          } else {
+            const double2 vq = k.v_in[q];
+            double diss, rx, ry, rest;
+            site_vec(rx, ry, rest);
+            if (contact_pair<float>(x_lo, x_hi, self_lo ? vp : vq, self_lo ? vq : vp,
+                                    make_double2(rx / rest, ry / rest), m.radius, kc, cdamp,
+                                    Fx[c], Fy[c], diss)) {
+              has |= 1u << c;
+              if (self_lo) ch[C_PAIR_DISS] += diss;
+            }
+          }
+        }
+      }
+
+      // ---- sum in index_add_ order: -drag*v, then +F of partners above
+      // (ascending), then -F of partners below (ascending); unbonded
+      // candidates are merged in by partner index --------------------------
+      double fx = (-m.drag) * vp.x, fy = (-m.drag) * vp.y;
+      int ci = split;
+#pragma unroll
+      for (int c = 4; c < 8; ++c) {
+        if (!((present >> c) & 1u)) continue;
+        const int q = p + dyc(c) * nx + dxc(c);
+        while (ci < nc) {
+          const int qc = cand[(size_t)ci * n];
+          if (qc > q) break;
+          unbonded(qc, fx, fy, false);
+          ++ci;
+        }
+        if ((has >> c) & 1u) { fx += Fx[c]; fy += Fy[c]; }
+      }
+      for (; ci < nc; ++ci) unbonded(cand[(size_t)ci * n], fx, fy, false);
+      ci = 0;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        if (!((present >> c) & 1u)) continue;
+        const int q = p + dyc(c) * nx + dxc(c);
+        while (ci < split) {
+          const int qc = cand[(size_t)ci * n];
+          if (qc > q) break;
+          unbonded(qc, fx, fy, false);
+          ++ci;
+        }
+        if ((has >> c) & 1u) { fx += -Fx[c]; fy += -Fy[c]; }
+      }
+      for (; ci < split; ++ci) unbonded(cand[(size_t)ci * n], fx, fy, false);
+      a.links[p] = (uint8_t)links;
+
+      // ---- tool or hull (dem.py:294-324, hull_contact.py:7-75) ------------
+      const double tvx = m.tool_velocity[0], tvy = m.tool_velocity[1];
+      double tfx, tfy, cfx, cfy;
+      if (m.hull_m == 0) {
+        const double tpx = m.tool_start[0] + time * m.tool_velocity[0];
+        const double tpy = m.tool_start[1] + time * m.tool_velocity[1];
+        double dx = xp.x - tpx, dy = xp.y - tpy;
+        if constexpr (Prec<R>::fp64) {
+          double d = sqrt(dx * dx + dy * dy);
+          double nxv, nyv;
+          if (d > 0) { double dd = d > kEps64 ? d : kEps64; nxv = dx / dd; nyv = dy / dd; }
+          else { nxv = 0.0; nyv = -1.0; }
+          double ov = m.radius + m.tool_radius - d;
+          if (ov < 0) ov = 0;
+          double sp2 = (vp.x - tvx) * nxv + (vp.y - tvy) * nyv;
+          double mag = kc * ov - cdamp * sp2;
+          if (mag < 0) mag = 0;
+          if (!(ov > 0)) mag = 0;
+          tfx = mag * nxv; tfy = mag * nyv;
+          cfx = kc * ov * nxv; cfy = kc * ov * nyv;
+        } else {
+          float dxf = (float)dx, dyf = (float)dy;
+          float d = sqrt32(dxf * dxf + dyf * dyf);
+          float nxv, nyv;
+          if (d > 0) { float r = rcp32(fmaxf(d, kEps32)); nxv = dxf * r; nyv = dyf * r; }
+          else { nxv = 0.0f; nyv = -1.0f; }
+          float ov = fmaxf((float)(m.radius + m.tool_radius) - d, 0.0f);
+          float sp2 = (float)(vp.x - tvx) * nxv + (float)(vp.y - tvy) * nyv;
+          float kcf = (float)kc;
+          float mag = ov > 0 ? fmaxf(kcf * ov - (float)cdamp * sp2, 0.0f) : 0.0f;
+          tfx = (double)(mag * nxv); tfy = (double)(mag * nyv);
+          cfx = (double)(kcf * ov * nxv); cfy = (double)(kcf * ov * nyv);
+        }
+      } else {
+        const double ox = m.tool_start[0] + time * m.tool_velocity[0];
+        const double oy = m.tool_start[1] + time * m.tool_velocity[1];
+        const double2 *hv = reinterpret_cast<const double2 *>(a.hull) + m.hull_first;
+        int best = 0;
+        double best_d2 = INFINITY, bdx = 0, bdy = 0;
+        for (int s2 = 0; s2 < m.hull_m - 1; ++s2) {
+          double sx0 = hv[s2].x + ox, sy0 = hv[s2].y + oy;
+          double gx = (hv[s2 + 1].x + ox) - sx0, gy = (hv[s2 + 1].y + oy) - sy0;
+          double len2 = gx * gx + gy * gy;
+          double fr = ((xp.x - sx0) * gx + (xp.y - sy0) * gy) / len2;
+          fr = fr < 0.0 ? 0.0 : (fr > 1.0 ? 1.0 : fr);
+          double ddx = xp.x - (sx0 + fr * gx), ddy = xp.y - (sy0 + fr * gy);
+          double d2 = ddx * ddx + ddy * ddy;
+          if (d2 < best_d2) { best_d2 = d2; best = s2; bdx = ddx; bdy = ddy; }
+        }
+        double distance = sqrt(best_d2);
+        double gx = (hv[best + 1].x + ox) - (hv[best].x + ox);
+        double gy = (hv[best + 1].y + oy) - (hv[best].y + oy);
+        double len = sqrt(gx * gx + gy * gy);
+        double ttx = gx / len, tty = gy / len;
+        if (ttx < 0 || (ttx == 0 && tty < 0)) { ttx = -ttx; tty = -tty; }
+        double fbx = -tty, fby = ttx;
+        double rvx = vp.x - tvx, rvy = vp.y - tvy;
+        if (rvx * fbx + rvy * fby > 0) { fbx = -fbx; fby = -fby; }
         double nxv, nyv;
-        if (d > 0) { double dd = d > kEps64 ? d : kEps64; nxv = dx / dd; nyv = dy / dd; }
-        else { nxv = 0.0; nyv = -1.0; }
-        double ov = m.radius + m.tool_radius - d;
+        if (distance > kEps64) { nxv = bdx / distance; nyv = bdy / distance; }
+        else { nxv = fbx; nyv = fby; }
+        double ov = m.radius + m.tool_radius - distance;
         if (ov < 0) ov = 0;
-        double sp2 = (vp.x - tvx) * nxv + (vp.y - tvy) * nyv;
-        double mag = m.contact_stiffness * ov - m.contact_damping * sp2;
+        double sp2 = rvx * nxv + rvy * nyv;
+        double mag = kc * ov - cdamp * sp2;
         if (mag < 0) mag = 0;
         if (!(ov > 0)) mag = 0;
         tfx = mag * nxv; tfy = mag * nyv;
-        cfx = m.contact_stiffness * ov * nxv; cfy = m.contact_stiffness * ov * nyv;
-      } else {
-        float dxf = (float)dx, dyf = (float)dy;
-        float d = sqrtf(dxf * dxf + dyf * dyf);
-        float nxv, nyv;
-        if (d > 0) { float dd = fmaxf(d, kEps32); nxv = dxf / dd; nyv = dyf / dd; }
-        else { nxv = 0.0f; nyv = -1.0f; }
-        float ov = fmaxf((float)(m.radius + m.tool_radius) - d, 0.0f);
-        float sp2 = (float)(vp.x - tvx) * nxv + (float)(vp.y - tvy) * nyv;
-        float kc = (float)m.contact_stiffness;
-        float mag = ov > 0 ? fmaxf(kc * ov - (float)m.contact_damping * sp2, 0.0f) : 0.0f;
-        tfx = (double)(mag * nxv); tfy = (double)(mag * nyv);
-        cfx = (double)(kc * ov * nxv); cfy = (double)(kc * ov * nyv);
+        double cm = kc * ov;  // damping 0; the clamp is a no-op for ov >= 0
+        if (!(ov > 0)) cm = 0;
+        cfx = cm * nxv; cfy = cm * nyv;
       }
-    } else {
-      const double ox = m.tool_start[0] + time * m.tool_velocity[0];
-      const double oy = m.tool_start[1] + time * m.tool_velocity[1];
-      const double2 *hv = reinterpret_cast<const double2 *>(a.hull) + m.hull_first;
-      int best = 0;
-      double best_d2 = INFINITY, bdx = 0, bdy = 0;
-      for (int s = 0; s < m.hull_m - 1; ++s) {
-        double sx0 = hv[s].x + ox, sy0 = hv[s].y + oy;
-        double gx = (hv[s + 1].x + ox) - sx0, gy = (hv[s + 1].y + oy) - sy0;
-        double len2 = gx * gx + gy * gy;
-        double fr = ((xp.x - sx0) * gx + (xp.y - sy0) * gy) / len2;
-        fr = fr < 0.0 ? 0.0 : (fr > 1.0 ? 1.0 : fr);
-        double ddx = xp.x - (sx0 + fr * gx), ddy = xp.y - (sy0 + fr * gy);
-        double d2 = ddx * ddx + ddy * ddy;
-        if (d2 < best_d2) { best_d2 = d2; best = s; bdx = ddx; bdy = ddy; }
+      ch[C_TOOL_FX] += tfx; ch[C_TOOL_FY] += tfy;
+      ch[C_TOOL_DISS] += (tfx - cfx) * (vp.x - tvx) + (tfy - cfy) * (vp.y - tvy);
+      ch[C_VSQ] += vp.x * vp.x + vp.y * vp.y;
+      ch[C_VSUM_X] += vp.x; ch[C_VSUM_Y] += vp.y;
+      fx += tfx; fy += tfy;
+      if (m.has_loads) {
+        fx += ld.x; fy += ld.y;
+        ch[C_LOAD_X] += ld.x; ch[C_LOAD_Y] += ld.y;
+        ch[C_MOMENT] += xp.x * ld.y - xp.y * ld.x;
       }
-      double distance = sqrt(best_d2);
-      double gx = (hv[best + 1].x + ox) - (hv[best].x + ox);
-      double gy = (hv[best + 1].y + oy) - (hv[best].y + oy);
-      double len = sqrt(gx * gx + gy * gy);
-      double ttx = gx / len, tty = gy / len;
-      if (ttx < 0 || (ttx == 0 && tty < 0)) { ttx = -ttx; tty = -tty; }
-      double fbx = -tty, fby = ttx;
-      double rvx = vp.x - tvx, rvy = vp.y - tvy;
-      if (rvx * fbx + rvy * fby > 0) { fbx = -fbx; fby = -fby; }
-      double nxv, nyv;
-      if (distance > kEps64) { nxv = bdx / distance; nyv = bdy / distance; }
-      else { nxv = fbx; nyv = fby; }
-      double ov = m.radius + m.tool_radius - distance;
-      if (ov < 0) ov = 0;
-      double sp2 = rvx * nxv + rvy * nyv;
-      double kc = m.contact_stiffness;
-      double mag = kc * ov - m.contact_damping * sp2;
-      if (mag < 0) mag = 0;
-      if (!(ov > 0)) mag = 0;
-      tfx = mag * nxv; tfy = mag * nyv;
-      double cm = kc * ov;  // damping 0; clamp is a no-op for ov >= 0
-      if (!(ov > 0)) cm = 0;
-      cfx = cm * nxv; cfy = cm * nyv;
-    }
-    ch[C_TOOL_FX] += tfx; ch[C_TOOL_FY] += tfy;
-    ch[C_TOOL_DISS] += (tfx - cfx) * (vp.x - tvx) + (tfy - cfy) * (vp.y - tvy);
-    ch[C_VSQ] += vp.x * vp.x + vp.y * vp.y;
-    ch[C_VSUM_X] += vp.x; ch[C_VSUM_Y] += vp.y;
-    fx += tfx; fy += tfy;
-    double2 ld = make_double2(0.0, 0.0);
-    if (m.has_loads) {
-      ld = reinterpret_cast<const double2 *>(a.loads)[p];
-      fx += ld.x; fy += ld.y;
-      ch[C_LOAD_X] += ld.x; ch[C_LOAD_Y] += ld.y;
-      ch[C_MOMENT] += xp.x * ld.y - xp.y * ld.x;
-    }
-    const bool fixed = a.fixed[p] != 0;
-    if (fixed) { ch[C_SUP_X] += fx; ch[C_SUP_Y] += fy; }
+      if (fixed) { ch[C_SUP_X] += fx; ch[C_SUP_Y] += fy; }
 
-    if (k.op == OP_STEP) {
-      // dem.py:363-366
-      double vx = vp.x + (m.dt * fx) / m.mass, vy = vp.y + (m.dt * fy) / m.mass;
-      if (fixed) { vx = 0.0; vy = 0.0; }
-      double xn = xp.x + m.dt * vx, yn = xp.y + m.dt * vy;
-      if (fixed) { xn = sp.x; yn = sp.y; }
-      k.x_out[p] = make_double2(xn, yn);
-      k.v_out[p] = make_double2(vx, vy);
-      if (m.has_loads) ch[C_WORK] += ld.x * (xn - xp.x) + ld.y * (yn - xp.y);
-      if (!isfinite(xn) || !isfinite(yn) || !isfinite(vx) || !isfinite(vy)) status |= TDM_ST_NONFINITE;
-      // rebuild request: quarter skin since the candidate list was built
-      float2 b = mode == MODE_SEARCH ? make_float2((float)(xp.x - sp.x), (float)(xp.y - sp.y))
-                                     : k.w.xb[p];
-      double ddx = (xn - sp.x) - (double)b.x, ddy = (yn - sp.y) - (double)b.y;
-      double q4 = 0.25 * m.skin;
-      if (ddx * ddx + ddy * ddy >= q4 * q4) req = 1;
-      if (mode == MODE_BIN) {
-        const double h = m.cell_size;
-        int cx = (int)floor((xn - m.grid_origin[0]) / h);
-        int cy = (int)floor((yn - m.grid_origin[1]) / h);
-        if (!(xn == xn) || !(yn == yn)) { cx = 0; cy = 0; }
-        cx = min(max(cx, 0), m.ncx - 1);
-        cy = min(max(cy, 0), m.ncy - 1);
-        int cell = m.cell_first + cy * m.ncx + cx;
-        int t = m.tab ^ 1;
-        int slot = atomicAdd(k.w.cell_count + (size_t)t * a.ncells + cell, 1);
-        if (slot < a.cell_cap) k.w.cell_tab[((size_t)t * a.ncells + cell) * a.cell_cap + slot] = p;
-        else status |= TDM_ST_CELL_OVERFLOW;
+      if (k.op == OP_STEP) {
+        // dem.py:363-366
+        double vx, vy;
+        if constexpr (Prec<R>::fp64) {
+          vx = vp.x + (m.dt * fx) / m.mass;  // operand order of dem.py:363
+          vy = vp.y + (m.dt * fy) / m.mass;
+        } else {
+          const double dt_m = m.dt / m.mass;
+          vx = vp.x + fx * dt_m;
+          vy = vp.y + fy * dt_m;
+        }
+        if (fixed) { vx = 0.0; vy = 0.0; }
+        double xn = xp.x + m.dt * vx, yn = xp.y + m.dt * vy;
+        if (fixed) { xn = sp.x; yn = sp.y; }
+        k.x_out[p] = make_double2(xn, yn);
+        k.v_out[p] = make_double2(vx, vy);
+        if (m.has_loads) ch[C_WORK] += ld.x * (xn - xp.x) + ld.y * (yn - xp.y);
+        if (!isfinite(xn) || !isfinite(yn) || !isfinite(vx) || !isfinite(vy)) status |= TDM_ST_NONFINITE;
+        // rebuild request: quarter skin since the candidate list was built
+        double ddx = (xn - sp.x) - (double)xb.x, ddy = (yn - sp.y) - (double)xb.y;
+        double q4 = 0.25 * m.skin;
+        if (ddx * ddx + ddy * ddy >= q4 * q4) req = 1;
+        if (mode == MODE_BIN) {
+          const double h = m.cell_size;
+          int cx = (int)floor((xn - m.grid_origin[0]) / h);
+          int cy = (int)floor((yn - m.grid_origin[1]) / h);
+          if (!(xn == xn) || !(yn == yn)) { cx = 0; cy = 0; }
+          cx = min(max(cx, 0), m.ncx - 1);
+          cy = min(max(cy, 0), m.ncy - 1);
+          int cell = m.cell_first + cy * m.ncx + cx;
+          int t = m.tab ^ 1;
+          int slot = atomicAdd(k.w.cell_count + (size_t)t * a.ncells + cell, 1);
+          if (slot < a.cell_cap) k.w.cell_tab[((size_t)t * a.ncells + cell) * a.cell_cap + slot] = p;
+          else status |= TDM_ST_CELL_OVERFLOW;
+        }
+      } else {
+        reinterpret_cast<double2 *>(a.force)[p] = make_double2(fx, fy);
       }
-    } else {
-      reinterpret_cast<double2 *>(a.force)[p] = make_double2(fx, fy);
     }
-    }  // op != OP_ENERGY
   }
 
   // a search step clears the other hash table for the next bin phase
@@ -736,36 +848,77 @@ __global__ __launch_bounds__(kThreads) void plate_step_kernel(KArgs k)
     int *cnt = k.w.cell_count + (size_t)(m.tab ^ 1) * a.ncells + m.cell_first;
     for (int c = blk.w * kThreads + threadIdx.x; c < ncell_m; c += m.nblocks * kThreads) cnt[c] = 0;
   }
-  if (status) atomicOr(&a.members[mi].status, status);
-  if (req) atomicOr(&k.w.req[mi], 1);
-
+  if (status) atomicOr(&sflags[0], status);
+  if (req) atomicOr(&sflags[1], 1);
   block_sum(ch, red);
   if (threadIdx.x == 0) {
-    double *dst = k.w.partials + (size_t)(m.block_first + blk.w) * kCh;
+    const size_t slot = (size_t)(m.block_first + blk.w);
+    double *dst = k.w.partials + slot * kCh;
 #pragma unroll
     for (int c = 0; c < kCh; ++c) dst[c] = ch[c];
-    __threadfence();
-    unsigned done = atomicAdd(&k.w.counters[mi], 1u);
-    last_flag = done == (unsigned)(m.nblocks - 1);
+    k.w.bflags[slot] = sflags[0] | (sflags[1] << 16);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Per-member ledger reduction and clock/mode update: one block per member,
+// partials summed in the member's logical tile order (independent of the
+// batch around it).  A separate launch makes every step block's partials
+// visible without device-scope fences inside the step kernel.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kThreads) void plate_finalize_kernel(KArgs k)
+{
+  __shared__ int sflag;
+  const tdm_plate_arrays &a = k.a;
+  const int mi = blockIdx.x;
+  tdm_plate_member *mp = a.members + mi;
+  const int b0 = mp->block_first, nb = mp->nblocks;
+  __shared__ double seg[16][kCh + 1];
+  __shared__ double stot[kCh];
+  if (threadIdx.x == 0) sflag = 0;
+  __syncthreads();
+  // thread t sums channel t % 16 over the member's rows t/16, t/16 + 16, ...
+  // (coalesced: 16 consecutive threads read one 128-byte row), then 16
+  // threads add the 16 segments in order; the order depends only on nb.
+  static_assert(kCh == 16, "reduction layout assumes 16 channels");
+  const int c = threadIdx.x & 15, r0 = threadIdx.x >> 4;
+  double sum = 0.0;
+  // eight independent loads in flight per round; summed in row order
+  for (int b = r0; b < nb; b += 16 * 8) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int bb = b + 16 * u;
+      v[u] = bb < nb ? k.w.partials[(size_t)(b0 + bb) * kCh + c] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) sum += v[u];
+  }
+  seg[r0][c] = sum;
+  int fl = 0;
+  for (int b = threadIdx.x; b < nb; b += kThreads) fl |= k.w.bflags[b0 + b];
+  if (fl) atomicOr(&sflag, fl);
+  __syncthreads();
+  if (threadIdx.x < kCh) {
+    double t = 0.0;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) t += seg[r][threadIdx.x];
+    stot[threadIdx.x] = t;
   }
   __syncthreads();
-  if (!last_flag) return;
-  __threadfence();
-
-  // ---- last block of this member: fixed-order ledger reduction ------------
+  if (threadIdx.x != 0) return;
   double tot[kCh];
 #pragma unroll
-  for (int c = 0; c < kCh; ++c) tot[c] = 0.0;
-  for (int b = threadIdx.x; b < m.nblocks; b += kThreads) {
-    const double *src = k.w.partials + (size_t)(m.block_first + b) * kCh;
+  for (int q = 0; q < kCh; ++q) tot[q] = stot[q];
+  const tdm_plate_member m = *mp;  // registers: no aliasing with the stores below
+  const int status = sflag & 0xFFFF, rq = sflag >> 16;
+  double acc[TDM_NACC];
+  double *accp = a.accounting + (size_t)mi * TDM_NACC;
 #pragma unroll
-    for (int c = 0; c < kCh; ++c) tot[c] += src[c];
-  }
-  block_sum(tot, red);
-  if (threadIdx.x != 0) return;
-  tdm_plate_member &mw = a.members[mi];
-  double *acc = a.accounting + (size_t)mi * TDM_NACC;
-  const double dt = m.dt;
+  for (int c = 0; c < TDM_NACC; ++c) acc[c] = accp[c];
+  const double dt = m.dt, time = m.time;
+  const int mode = m.mode;
+  int new_mode = mode, new_tab = m.tab;
   if (k.op != OP_ENERGY) {
     acc[TDM_ACC_RELEASE] += tot[C_RELEASED];
     acc[TDM_ACC_ACQUIRED] += tot[C_ACQUIRED];
@@ -773,13 +926,13 @@ __global__ __launch_bounds__(kThreads) void plate_step_kernel(KArgs k)
   const double rx_ = -tot[C_TOOL_FX], ry_ = -tot[C_TOOL_FY];
   if (k.op == OP_ENERGY) {
     double *en = a.energy + (size_t)mi * 5;
-    en[0] = 0.5 * m.mass * tot[E_VSQ];
-    en[1] = 0.5 * m.bond_stiffness * tot[E_BOND];
-    en[2] = 0.5 * m.contact_stiffness * tot[E_CONTACT];
-    en[3] = 0.5 * m.contact_stiffness * tot[E_TOOL];
-    en[4] = ((en[0] + en[1]) + en[2]) + en[3];
-    atomicExch(&k.w.req[mi], 0);
-    if (mode == MODE_SEARCH) mw.mode = MODE_IDLE;
+    double e0 = 0.5 * m.mass * tot[E_VSQ];
+    double e1 = 0.5 * m.bond_stiffness * tot[E_BOND];
+    double e2 = 0.5 * m.contact_stiffness * tot[E_CONTACT];
+    double e3 = 0.5 * m.contact_stiffness * tot[E_TOOL];
+    en[0] = e0; en[1] = e1; en[2] = e2; en[3] = e3;
+    en[4] = ((e0 + e1) + e2) + e3;
+    if (mode == MODE_SEARCH) new_mode = MODE_IDLE;
   } else if (k.op == OP_STEP) {
     const double supx = -tot[C_SUP_X], supy = -tot[C_SUP_Y];
     const double extx = m.has_loads ? tot[C_LOAD_X] : 0.0, exty = m.has_loads ? tot[C_LOAD_Y] : 0.0;
@@ -798,29 +951,25 @@ __global__ __launch_bounds__(kThreads) void plate_step_kernel(KArgs k)
     acc[TDM_ACC_EXT_WORK] += work;
     acc[TDM_ACC_TOOL_WORK] += dt * tpow;
     acc[TDM_ACC_DAMPING] += dt * damp;
-    double *last = a.last + (size_t)mi * TDM_NLAST;
-    last[TDM_LAST_T0] = time;
-    last[TDM_LAST_T1] = time + dt;
-    last[TDM_LAST_WORK] = work;
-    last[TDM_LAST_SUPPORT] = supx;
-    last[TDM_LAST_SUPPORT + 1] = supy;
-    last[TDM_LAST_REACTION] = rx_;
-    last[TDM_LAST_REACTION + 1] = ry_;
-    last[TDM_LAST_EXTERNAL] = extx;
-    last[TDM_LAST_EXTERNAL + 1] = exty;
-    last[TDM_LAST_MOMENT] = m.has_loads ? tot[C_MOMENT] : 0.0;
-    mw.time = time + dt;
-    mw.step_count = m.step_count + 1;
-    const int rq = atomicExch(&k.w.req[mi], 0);
-    if (mode == MODE_BIN) { mw.tab = m.tab ^ 1; mw.mode = MODE_SEARCH; }
-    else mw.mode = rq ? MODE_BIN : MODE_IDLE;
+    double last[TDM_NLAST] = {time, time + dt, work, supx, supy, rx_, ry_, extx, exty,
+                              m.has_loads ? tot[C_MOMENT] : 0.0};
+    double *lastp = a.last + (size_t)mi * TDM_NLAST;
+#pragma unroll
+    for (int c = 0; c < TDM_NLAST; ++c) lastp[c] = last[c];
+    mp->time = time + dt;
+    mp->step_count = m.step_count + 1;
+    if (mode == MODE_BIN) { new_tab = m.tab ^ 1; new_mode = MODE_SEARCH; }
+    else new_mode = rq ? MODE_BIN : MODE_IDLE;
   } else {
     a.reaction[2 * mi] = rx_;
     a.reaction[2 * mi + 1] = ry_;
-    atomicExch(&k.w.req[mi], 0);
-    if (mode == MODE_SEARCH) mw.mode = MODE_IDLE;
+    if (mode == MODE_SEARCH) new_mode = MODE_IDLE;
   }
-  k.w.counters[mi] = 0;
+#pragma unroll
+  for (int c = 0; c < TDM_NACC; ++c) accp[c] = acc[c];
+  mp->mode = new_mode;
+  mp->tab = new_tab;
+  mp->status = m.status | status;
 }
 
 // ---------------------------------------------------------------------------
@@ -993,7 +1142,7 @@ int tdm_plate_setup(const tdm_plate_arrays *a, void *stream)
   // permuted so that runs of consecutive tiles share one XCD (blocks are
   // dealt round-robin over the 8 XCDs): launch slot b runs logical tile
   // (b % 8) * ceil(nb/8) + b / 8 when that tile exists.
-  int4 *hb = new int4[total];
+  int4 *hb = new int4[2 * total];
   int4 *logical = new int4[total];
   int li = 0;
   for (int i = 0; i < nm; ++i) {
@@ -1012,17 +1161,19 @@ int tdm_plate_setup(const tdm_plate_arrays *a, void *stream)
     int next = 0;
     for (int lane = 0; lane < 8; ++lane)
       for (int b = 0; b < nb; ++b)
-        if ((b0 + b) % 8 == lane) hb[b0 + b] = logical[b0 + next++];
+        if ((b0 + b) % 8 == lane) {
+          hb[2 * (b0 + b)] = logical[b0 + next++];
+          hb[2 * (b0 + b) + 1] = make_int4(hm[i].nx, hm[i].ny, hm[i].first, hm[i].tile_x);
+        }
   }
   delete[] logical;
-  e = hipMemcpyAsync(w.blocks, hb, sizeof(int4) * total, hipMemcpyHostToDevice, s);
+  e = hipMemcpyAsync(w.blocks, hb, 2 * sizeof(int4) * total, hipMemcpyHostToDevice, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   delete[] hb;
   delete[] hm;
   if (e != hipSuccess) return fail(TDM_EHIP, hipGetErrorString(e));
   // counters, requests, hash tables, candidate counts
-  fill_int_kernel<<<64, 256, 0, s>>>(reinterpret_cast<int *>(w.counters), (size_t)a->nmembers, 0);
-  fill_int_kernel<<<64, 256, 0, s>>>(w.req, (size_t)a->nmembers, 0);
+  fill_int_kernel<<<64, 256, 0, s>>>(w.bflags, (size_t)a->nblocks, 0);
   fill_int_kernel<<<1024, 256, 0, s>>>(w.cell_count, 2 * (size_t)a->ncells, 0);
   (void)hipMemsetAsync(w.ncand, 0, (size_t)a->n, s);
   HIP_TRY(hipGetLastError());
@@ -1052,7 +1203,6 @@ int tdm_plate_rebin(const tdm_plate_arrays *a, void *stream)
   fill_int_kernel<<<1024, 256, 0, s>>>(w.cell_count, 2 * (size_t)a->ncells, 0);
   bin_kernel<<<(a->n + 255) / 256, 256, 0, s>>>(*a, w, reinterpret_cast<const double2 *>(a->x));
   set_mode_kernel<<<(a->nmembers + 255) / 256, 256, 0, s>>>(a->members, a->nmembers, MODE_SEARCH);
-  fill_int_kernel<<<64, 256, 0, s>>>(w.req, (size_t)a->nmembers, 0);
   HIP_TRY(hipGetLastError());
   return TDM_OK;
 }
@@ -1074,6 +1224,8 @@ static int launch_step(const tdm_plate_arrays *a, const Workspace &w, int parity
   k.v_out = const_cast<double2 *>(parity ? v : va);
   if (fp32) plate_step_kernel<float><<<a->nblocks, kThreads, 0, s>>>(k);
   else plate_step_kernel<double><<<a->nblocks, kThreads, 0, s>>>(k);
+  HIP_TRY(hipGetLastError());
+  plate_finalize_kernel<<<a->nmembers, kThreads, 0, s>>>(k);
   HIP_TRY(hipGetLastError());
   return TDM_OK;
 }

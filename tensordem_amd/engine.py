@@ -47,8 +47,8 @@ def default_tile(nx: int) -> int:
 
 class PlateEngine:
     def __init__(self, specs: list[PlateSpec], *, device: str | torch.device = "cuda",
-                 fp32: bool = False, skin_factor: float = 0.5, cand_slots: int = 8,
-                 cell_cap: int = 8, grid_margin: float = 1.0, fp32_margin: float = 1e-3):
+                 fp32: bool = False, skin_factor: float = 1.0, cand_slots: int = 16,
+                 cell_cap: int = 8, grid_margin: float = 0.25, fp32_margin: float = 1e-3):
         self.lib = _lib.load()
         self.device = torch.device(device)
         if self.device.type != "cuda":

@@ -140,8 +140,10 @@ def test_fp32_break_decisions_equal_fp64_on_same_state():
         f_ref, r_ref = probe.forces()
         f, r = sim.forces()
         assert np.array_equal(np_(sim.alive), probe.alive.astype(bool)), f"step {k + 1}"
+        # FP32 forces: 2e-5 of the largest force plus a 1e-9 N floor (at rest the
+        # FP64 force is exactly 0 and FP32 leaves ~1e-13 N of rounding)
         scale = np.abs(f_ref).max()
-        assert np.abs(np_(f) - f_ref).max() <= 2e-5 * scale
+        assert np.abs(np_(f) - f_ref).max() <= 2e-5 * scale + 1e-9
         checked += 1
     assert checked == 80
 
