@@ -50,11 +50,11 @@ class PlainFP64:
     the GPU, with a per-step sort-based cell list instead of all N^2 pairs.
     Ledger sums stay on the device (no host syncs)."""
 
-    def __init__(self, cfg, dt, device="cuda"):
+    def __init__(self, cfg, dt, device="cuda", pinv=True):
         import torch
         from tensordem_amd.lattice import bonds, fixed_mask, positions
         dev = torch.device(device)
-        self.cfg, self.dt = cfg, dt
+        self.cfg, self.dt, self.pinv = cfg, dt, pinv
         f64 = dict(dtype=torch.float64, device=dev)
         self.x0 = torch.tensor(positions(cfg.nx, cfg.ny, cfg.radius), **f64)
         self.x = self.x0.clone()
@@ -122,7 +122,13 @@ class PlainFP64:
         cov.index_add_(0, j, ro)
         crs.index_add_(0, i, cr)
         crs.index_add_(0, j, cr)
-        F = crs @ torch.linalg.pinv(cov)
+        if self.pinv:
+            inv = torch.linalg.pinv(cov)  # the reference's op (dem.py:202)
+        else:
+            det = cov[:, 0, 0] * cov[:, 1, 1] - cov[:, 0, 1] * cov[:, 1, 0]
+            inv = torch.stack((torch.stack((cov[:, 1, 1], -cov[:, 0, 1]), 1),
+                               torch.stack((-cov[:, 1, 0], cov[:, 0, 0]), 1)), 1) / det[:, None, None]
+        F = crs @ inv
         E = 0.5 * (F.transpose(1, 2) @ F - torch.eye(2, dtype=x.dtype, device=x.device))
         s = torch.sqrt((E[:, 0, 0] - E[:, 1, 1]).square() + 4.0 * E[:, 0, 1].square())
         # fracture (dem.py:238-277)
@@ -174,9 +180,9 @@ class PlainFP64:
         self.time += self.dt
 
 
-def time_plain(cfg, dt, steps, warmup):
+def time_plain(cfg, dt, steps, warmup, pinv=True):
     import torch
-    p = PlainFP64(cfg, dt)
+    p = PlainFP64(cfg, dt, pinv=pinv)
     for _ in range(warmup):
         p.step()
     torch.cuda.synchronize()
@@ -189,18 +195,34 @@ def time_plain(cfg, dt, steps, warmup):
 
 # ---------------------------------------------------------------------------
 def kernel_event_time(eng, launches):
-    """Mean duration of single step launches, bracketed by HIP events on the
-    launch stream."""
+    """Mean step-kernel duration (s): HIP events recorded by the library on the
+    launch stream around each step-kernel launch, continuing the timed run."""
+    return eng.step_timed(launches - launches % 2) * 1e-6
+
+
+def traffic_from_profile():
+    """HBM bytes per step-kernel launch measured with rocprofv3 counters
+    (FETCH_SIZE + WRITE_SIZE, calibrated on a copy kernel), if recorded."""
+    path = ROOT / "profiles" / "plate_step_c2_counters.md"
+    if not path.exists():
+        return None
+    for line in path.read_text().splitlines():
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
+def copy_calibration(mib=512):
+    """A plain device copy of known size, run under rocprofv3 --pmc to scale
+    FETCH_SIZE / WRITE_SIZE (each byte is read once and written once)."""
     import torch
-    stream = torch.cuda.current_stream()
-    starts = [torch.cuda.Event(enable_timing=True) for _ in range(launches)]
-    ends = [torch.cuda.Event(enable_timing=True) for _ in range(launches)]
-    for k in range(launches):
-        starts[k].record(stream)
-        eng.step(1)
-        ends[k].record(stream)
+    x = torch.ones(mib * 2**20 // 8, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
     torch.cuda.synchronize()
-    return sum(s.elapsed_time(e) for s, e in zip(starts, ends)) / launches * 1e-3
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": x.numel() * 8, "launches": 3}))
 
 
 def cpu_baseline(cfg, dt, budget_s=12.0):
@@ -227,11 +249,17 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--chunk", type=int, default=100, help="steps per captured graph")
     ap.add_argument("--plain-steps", type=int, default=30)
-    ap.add_argument("--event-launches", type=int, default=200)
+    ap.add_argument("--event-launches", type=int, default=0,
+                    help="steps of the kernel-timing pass (0: same as --steps)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only a 512 MiB device copy (for counter calibration)")
     args = ap.parse_args(argv)
+    if args.copy_calibration:
+        copy_calibration()
+        return 0
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: every plate and request of this workload runs on a "
               "single MI355X and no multi-GPU path is built (see DESIGN.md).", file=sys.stderr)
@@ -266,7 +294,13 @@ def main(argv=None):
         gpu_s = e0.elapsed_time(e1) * 1e-3
         st = eng.status()[0]
         broken = int((~eng.alive).sum())
-        k_s = kernel_event_time(eng, args.event_launches)
+        eng.close()
+        del eng
+        # same workload and phase again, each step kernel timed by HIP events
+        # attached to its dispatch (the ledger launch is not in the bracket)
+        eng = engine(fp32)
+        eng.step(args.warmup - args.warmup % 2)
+        k_s = kernel_event_time(eng, args.event_launches if args.event_launches else steps)
         results["fp32" if fp32 else "fp64"] = dict(
             seconds=gpu_s, wall_s=wall, ms_per_step=gpu_s / steps * 1e3,
             rate=n * steps / gpu_s, kernel_s=k_s, status=int(st), broken=broken)
@@ -281,11 +315,15 @@ def main(argv=None):
 
     plain = None
     if not args.no_plain:
-        ps = time_plain(cfg, dt, args.plain_steps, 3)
-        plain = dict(value=n / ps, unit="particle-steps/s", ms_per_step=ps * 1e3,
-                     steps=args.plain_steps,
+        ps = time_plain(cfg, dt, args.plain_steps, 3, pinv=True)
+        ps2 = time_plain(cfg, dt, args.plain_steps, 3, pinv=False)
+        plain = dict(value=n / min(ps, ps2), unit="particle-steps/s",
+                     ms_per_step=min(ps, ps2) * 1e3, steps=args.plain_steps,
+                     ms_per_step_reference_pinv=ps * 1e3,
+                     ms_per_step_closed_form_inverse=ps2 * 1e3,
                      what="eager PyTorch FP64 restatement of IceDEM.step on the same GPU "
-                          "(per-step sort-based cell list)")
+                          "(per-step sort-based cell list); the faster of the reference's "
+                          "torch.linalg.pinv and a closed-form 2x2 inverse")
     bytes_per_launch = PLATE_BYTES_PER_DISK * n
     achieved = bytes_per_launch / r32["kernel_s"] / 1e9
     line = {
@@ -299,7 +337,7 @@ def main(argv=None):
                    "nx": 1024, "ny": 128, "dt_s": dt, "graph_chunk_steps": chunk,
                    "broken_bonds_end": r32["broken"], "status_bits": r32["status"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profile(),
                      "kernel": "plate_step_kernel<float>",
                      "kernel_us": r32["kernel_s"] * 1e6,
                      "bytes_per_launch": bytes_per_launch},

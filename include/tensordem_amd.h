@@ -170,6 +170,15 @@ int tdm_plate_rebin(const tdm_plate_arrays *a, void *stream);
 int tdm_plate_step(const tdm_plate_arrays *a, int nsteps, int parity, int fp32, void *stream);
 
 /*
+ * tdm_plate_step for nsteps steps (parity 0 on entry) with a HIP event pair
+ * attached to every step-kernel dispatch on `stream` (hipExtLaunchKernelGGL;
+ * the ledger launch is not timed); writes the mean step-kernel duration in
+ * microseconds.  Measurement helper with no counterpart in the reference.
+ */
+int tdm_plate_step_timed(const tdm_plate_arrays *a, int nsteps, int fp32, void *stream,
+                         double *mean_kernel_us);
+
+/*
  * IceDEM.forces (dem.py:226-345): commits fracture, writes `force` (N,2) and
  * the ice-on-tool `reaction` (nmembers,2); positions and time do not move.
  */

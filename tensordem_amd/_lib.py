@@ -76,6 +76,8 @@ _SIGNATURES = {
     "tdm_plate_rebin": ([ctypes.POINTER(Arrays), ctypes.c_void_p], ctypes.c_int),
     "tdm_plate_step": ([ctypes.POINTER(Arrays), ctypes.c_int, ctypes.c_int, ctypes.c_int,
                         ctypes.c_void_p], ctypes.c_int),
+    "tdm_plate_step_timed": ([ctypes.POINTER(Arrays), ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                              ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
     "tdm_plate_forces": ([ctypes.POINTER(Arrays), ctypes.c_int, ctypes.c_int, ctypes.c_void_p],
                          ctypes.c_int),
     "tdm_plate_energy": ([ctypes.POINTER(Arrays), ctypes.c_int, ctypes.c_void_p], ctypes.c_int),

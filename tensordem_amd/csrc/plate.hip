@@ -17,6 +17,7 @@
 //   * the cell hash for unbonded contacts is rebuilt in two pipelined phases
 //     (bin, then search) folded into ordinary steps, triggered on device when
 //     a disk has moved a quarter of the Verlet skin.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -1208,7 +1209,7 @@ int tdm_plate_rebin(const tdm_plate_arrays *a, void *stream)
 }
 
 static int launch_step(const tdm_plate_arrays *a, const Workspace &w, int parity, int fp32,
-                       int op, hipStream_t s)
+                       int op, hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr)
 {
   KArgs k;
   k.a = *a;
@@ -1222,8 +1223,19 @@ static int launch_step(const tdm_plate_arrays *a, const Workspace &w, int parity
   k.v_in = parity ? va : v;
   k.x_out = const_cast<double2 *>(parity ? x : xa);
   k.v_out = const_cast<double2 *>(parity ? v : va);
-  if (fp32) plate_step_kernel<float><<<a->nblocks, kThreads, 0, s>>>(k);
-  else plate_step_kernel<double><<<a->nblocks, kThreads, 0, s>>>(k);
+  if (ev0) {
+    // events carried by the dispatch itself: they time exactly this kernel
+    if (fp32)
+      hipExtLaunchKernelGGL(plate_step_kernel<float>, dim3(a->nblocks), dim3(kThreads), 0, s,
+                            ev0, ev1, 0, k);
+    else
+      hipExtLaunchKernelGGL(plate_step_kernel<double>, dim3(a->nblocks), dim3(kThreads), 0, s,
+                            ev0, ev1, 0, k);
+  } else if (fp32) {
+    plate_step_kernel<float><<<a->nblocks, kThreads, 0, s>>>(k);
+  } else {
+    plate_step_kernel<double><<<a->nblocks, kThreads, 0, s>>>(k);
+  }
   HIP_TRY(hipGetLastError());
   plate_finalize_kernel<<<a->nmembers, kThreads, 0, s>>>(k);
   HIP_TRY(hipGetLastError());
@@ -1243,6 +1255,37 @@ int tdm_plate_step(const tdm_plate_arrays *a, int nsteps, int parity, int fp32, 
     parity ^= 1;
   }
   return TDM_OK;
+}
+
+int tdm_plate_step_timed(const tdm_plate_arrays *a, int nsteps, int fp32, void *stream,
+                         double *mean_kernel_us)
+{
+  Workspace w;
+  int rc = check_arrays(a, &w);
+  if (rc) return rc;
+  if (nsteps <= 0 || !mean_kernel_us) return fail(TDM_EINVAL, "bad timing request");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  hipEvent_t *ev = new hipEvent_t[2 * nsteps]();
+  int made = 0;
+  for (; made < 2 * nsteps; ++made)
+    if (hipEventCreate(&ev[made]) != hipSuccess) break;
+  if (made < 2 * nsteps) rc = fail(TDM_EHIP, "hipEventCreate failed");
+  for (int i = 0; i < nsteps && rc == TDM_OK; ++i)
+    rc = launch_step(a, w, i & 1, fp32, OP_STEP, s, ev[2 * i], ev[2 * i + 1]);
+  double total = 0.0;
+  if (rc == TDM_OK) {
+    hipError_t e = hipStreamSynchronize(s);
+    for (int i = 0; i < nsteps && e == hipSuccess; ++i) {
+      float ms = 0.0f;
+      e = hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]);
+      total += ms;
+    }
+    if (e != hipSuccess) rc = fail(TDM_EHIP, hipGetErrorString(e));
+  }
+  for (int i = 0; i < made; ++i) (void)hipEventDestroy(ev[i]);
+  delete[] ev;
+  if (rc == TDM_OK) *mean_kernel_us = total * 1e3 / nsteps;
+  return rc;
 }
 
 int tdm_plate_forces(const tdm_plate_arrays *a, int parity, int fp32, void *stream)

@@ -246,6 +246,16 @@ class PlateEngine:
         if nsteps % 2:
             self._swap()
 
+    def step_timed(self, nsteps: int) -> float:
+        """Advance nsteps (even) and return the mean step-kernel time in us,
+        from HIP events recorded around each step-kernel launch."""
+        if nsteps % 2:
+            raise ValueError("step_timed needs an even step count")
+        out = ctypes.c_double()
+        _lib.check(self.lib.tdm_plate_step_timed(ctypes.byref(self._arrays), int(nsteps),
+                                                 int(self.fp32), self.stream, ctypes.byref(out)))
+        return out.value
+
     def forces(self) -> None:
         _lib.check(self.lib.tdm_plate_forces(ctypes.byref(self._arrays), 0, int(self.fp32),
                                              self.stream))
