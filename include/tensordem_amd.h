@@ -201,6 +201,85 @@ int tdm_plate_graph_create(const tdm_plate_arrays *a, int pairs, int fp32, void 
 int tdm_graph_launch(tdm_graph *g, void *stream);
 int tdm_graph_destroy(tdm_graph *g);
 
+/* ------------------------------------------------------------------------
+ * Unbonded Hertz-Mindlin disk beds (2-D restatement of the reference's
+ * floe-bed law, src/tensordem/scale_ceiling/kernel.py:277-401): Hertz normal
+ * force with a restitution-derived viscous term, Cundall-Strack tangential
+ * spring capped by Coulomb friction with per-pair history that is dropped on
+ * separation, floor and back-wall contacts, gravity, semi-implicit Euler.
+ * Disks of a member are contiguous; any order works, spatially coherent
+ * order is fastest.
+ * ---------------------------------------------------------------------- */
+
+/* per-member bed ledger (accounting), float64 */
+#define TDM_BED_DASHPOT 0      /* energy absorbed by the pair viscous terms [J] */
+#define TDM_BED_FRICTION 1     /* work of the pair tangential springs [J] */
+#define TDM_BED_WALL_DASHPOT 2 /* ... wall viscous terms [J] */
+#define TDM_BED_WALL_FRICTION 3
+#define TDM_BED_WALL_IMPULSE 4 /* x, y: impulse of the walls on the disks [N s] */
+#define TDM_BED_CONTACTS 6     /* pair contacts during the last step */
+#define TDM_BED_WALL_CONTACTS 7
+#define TDM_BED_NACC 8
+
+typedef struct tdm_bed_member {
+  double e_star, g_star;       /* pair moduli E/(2(1-nu^2)), G/(2(2-nu)) */
+  double damping;              /* 2 sqrt(5/6) beta */
+  double friction, wall_friction;
+  double rho_pi_t;             /* mass = rho_pi_t * r^2 */
+  double gravity_y;            /* [m/s^2], e.g. -9.81 */
+  double dt;
+  double floor_y, back_wall_x;
+  double time, skin, cell_size, grid_origin[2];
+  int64_t step_count;
+  int32_t first, n;            /* disks [first, first+n) */
+  int32_t use_floor, use_back_wall;
+  int32_t ncx, ncy, cell_first;
+  int32_t block_first, nblocks;
+  int32_t mode, tab, list_parity, status, pad_;
+} tdm_bed_member;
+
+typedef struct tdm_bed_arrays {
+  double *x, *v, *x_alt, *v_alt; /* (N,2) state and second buffers */
+  const double *radius;          /* (N,) */
+  double *xi_wall;               /* (N,4) floor and back-wall spring history */
+  double *accounting;            /* (nmembers, TDM_BED_NACC) */
+  double *energy;                /* (nmembers,4): kinetic, gravity, Hertz normal, tangential */
+  tdm_bed_member *members;
+  void *workspace;
+  size_t workspace_bytes;
+  int32_t n, nmembers, nblocks, ncells, cand_slots, cell_cap;
+} tdm_bed_arrays;
+
+/* Blocks (256 disks each) a member of n disks occupies. */
+int tdm_bed_blocks(int n);
+size_t tdm_bed_workspace_bytes(int n, int nblocks, int nmembers, int ncells, int cand_slots,
+                               int cell_cap);
+/* Byte offsets inside the workspace of the candidate lists (int32
+ * [2][slots][N]), their spring histories (double2 [2][slots][N]), the
+ * contact masks (uint32 [2][N]: bit e = slot e in contact, bits 16/17 =
+ * floor/back wall) and counts (uint8 [2][N]); index 0/1 is the list parity. */
+int tdm_bed_workspace_layout(int n, int nblocks, int nmembers, int ncells, int cand_slots,
+                             int cell_cap, size_t *cand_off, size_t *xi_off, size_t *mask_off,
+                             size_t *count_off);
+/* Block table, cleared histories, first binning (members filled by caller). */
+int tdm_bed_setup(const tdm_bed_arrays *a, void *stream);
+/* Re-bins x and schedules a candidate search (after the caller moved disks). */
+int tdm_bed_rebin(const tdm_bed_arrays *a, void *stream);
+/* nsteps steps of the bed law; state alternates x/v <-> x_alt/v_alt starting
+ * from x/v (parity 0).  fp32 != 0: forces in FP32, contact detection, state,
+ * histories and ledger in FP64. */
+int tdm_bed_step(const tdm_bed_arrays *a, int nsteps, int fp32, void *stream);
+/* As tdm_bed_step (nsteps even) with HIP events attached to each step-kernel
+ * dispatch; mean step-kernel duration in microseconds.  Measurement helper. */
+int tdm_bed_step_timed(const tdm_bed_arrays *a, int nsteps, int fp32, void *stream,
+                       double *mean_kernel_us);
+/* Kinetic, gravitational (zero at y = 0), Hertz normal and tangential spring
+ * energy of the current state into `energy`. */
+int tdm_bed_energy(const tdm_bed_arrays *a, void *stream);
+/* 2*pairs bed steps captured as a graph (state ends in x/v). */
+int tdm_bed_graph_create(const tdm_bed_arrays *a, int pairs, int fp32, void *stream,
+                         tdm_graph **out);
+
 #ifdef __cplusplus
 }
 #endif

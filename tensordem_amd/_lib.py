@@ -63,6 +63,37 @@ class Arrays(ctypes.Structure):
     ]
 
 
+class BedMember(ctypes.Structure):
+    """Mirror of `tdm_bed_member`."""
+    _fields_ = [
+        ("e_star", ctypes.c_double), ("g_star", ctypes.c_double), ("damping", ctypes.c_double),
+        ("friction", ctypes.c_double), ("wall_friction", ctypes.c_double),
+        ("rho_pi_t", ctypes.c_double), ("gravity_y", ctypes.c_double), ("dt", ctypes.c_double),
+        ("floor_y", ctypes.c_double), ("back_wall_x", ctypes.c_double), ("time", ctypes.c_double),
+        ("skin", ctypes.c_double), ("cell_size", ctypes.c_double),
+        ("grid_origin", ctypes.c_double * 2), ("step_count", ctypes.c_int64),
+        ("first", ctypes.c_int32), ("n", ctypes.c_int32), ("use_floor", ctypes.c_int32),
+        ("use_back_wall", ctypes.c_int32), ("ncx", ctypes.c_int32), ("ncy", ctypes.c_int32),
+        ("cell_first", ctypes.c_int32), ("block_first", ctypes.c_int32),
+        ("nblocks", ctypes.c_int32), ("mode", ctypes.c_int32), ("tab", ctypes.c_int32),
+        ("list_parity", ctypes.c_int32), ("status", ctypes.c_int32), ("pad_", ctypes.c_int32),
+    ]
+
+
+class BedArrays(ctypes.Structure):
+    """Mirror of `tdm_bed_arrays`."""
+    _fields_ = [
+        ("x", ctypes.c_void_p), ("v", ctypes.c_void_p), ("x_alt", ctypes.c_void_p),
+        ("v_alt", ctypes.c_void_p), ("radius", ctypes.c_void_p), ("xi_wall", ctypes.c_void_p),
+        ("accounting", ctypes.c_void_p), ("energy", ctypes.c_void_p), ("members", ctypes.c_void_p),
+        ("workspace", ctypes.c_void_p), ("workspace_bytes", ctypes.c_size_t),
+        ("n", ctypes.c_int32), ("nmembers", ctypes.c_int32), ("nblocks", ctypes.c_int32),
+        ("ncells", ctypes.c_int32), ("cand_slots", ctypes.c_int32), ("cell_cap", ctypes.c_int32),
+    ]
+
+
+BED_NACC = 8
+
 # every entry point declared in include/tensordem_amd.h, with its ctypes signature
 _SIGNATURES = {
     "tdm_version": ([], ctypes.c_int),
@@ -84,6 +115,19 @@ _SIGNATURES = {
     "tdm_plate_graph_create": ([ctypes.POINTER(Arrays), ctypes.c_int, ctypes.c_int,
                                 ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)], ctypes.c_int),
     "tdm_graph_launch": ([ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
+    "tdm_bed_blocks": ([ctypes.c_int], ctypes.c_int),
+    "tdm_bed_workspace_bytes": ([ctypes.c_int] * 6, ctypes.c_size_t),
+    "tdm_bed_workspace_layout": ([ctypes.c_int] * 6 + [ctypes.POINTER(ctypes.c_size_t)] * 4,
+                                 ctypes.c_int),
+    "tdm_bed_setup": ([ctypes.POINTER(BedArrays), ctypes.c_void_p], ctypes.c_int),
+    "tdm_bed_rebin": ([ctypes.POINTER(BedArrays), ctypes.c_void_p], ctypes.c_int),
+    "tdm_bed_step": ([ctypes.POINTER(BedArrays), ctypes.c_int, ctypes.c_int, ctypes.c_void_p],
+                     ctypes.c_int),
+    "tdm_bed_step_timed": ([ctypes.POINTER(BedArrays), ctypes.c_int, ctypes.c_int,
+                            ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+    "tdm_bed_energy": ([ctypes.POINTER(BedArrays), ctypes.c_void_p], ctypes.c_int),
+    "tdm_bed_graph_create": ([ctypes.POINTER(BedArrays), ctypes.c_int, ctypes.c_int,
+                              ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)], ctypes.c_int),
     "tdm_graph_destroy": ([ctypes.c_void_p], ctypes.c_int),
 }
 

@@ -1,0 +1,217 @@
+"""Device-resident batch of Hertz-Mindlin disk beds driven by the HIP kernels.
+
+`BedEngine` owns the device arrays of one or more independent beds (members)
+and calls the bed entry points of include/tensordem_amd.h.  One launch
+advances every member by one step of the bed law (tensordem_amd/csrc/bed.hip);
+a member's ledger is reduced in an order that depends only on that member.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _lib
+from .bedconfig import BedConfig
+
+THREADS = 256
+
+
+@dataclass
+class BedSpec:
+    """One bed: law, initial positions (n,2) and radii (n,), time step."""
+    config: BedConfig
+    positions: np.ndarray
+    radii: np.ndarray
+    dt: float | None = None
+    velocities: np.ndarray | None = None
+
+
+class BedEngine:
+    def __init__(self, specs: list[BedSpec], *, device="cuda", fp32: bool = False,
+                 skin_factor: float = 0.4, cand_slots: int = 16, cell_cap: int = 8,
+                 runout: float = 1.0):
+        self.lib = _lib.load()
+        self.device = torch.device(device)
+        if self.device.type != "cuda" or not torch.cuda.is_available():
+            raise ValueError("BedEngine runs on the ROCm device (device='cuda')")
+        self.specs = list(specs)
+        self.fp32 = bool(fp32)
+        nm = len(self.specs)
+        members = (_lib.BedMember * nm)()
+        xs, vs, rs = [], [], []
+        n = blocks = cells = 0
+        for k, s in enumerate(self.specs):
+            c = s.config
+            x = np.asarray(s.positions, np.float64)
+            r = np.asarray(s.radii, np.float64)
+            if x.ndim != 2 or x.shape[1] != 2 or r.shape != (len(x),) or len(x) == 0:
+                raise ValueError("positions must be (n,2) and radii (n,)")
+            m = members[k]
+            m.e_star, m.g_star, m.damping = c.e_star, c.g_star, c.damping_coefficient
+            m.friction, m.wall_friction, m.rho_pi_t = c.friction, c.wall_friction, c.rho_pi_t
+            m.gravity_y = -c.gravity
+            m.dt = float(s.dt if s.dt is not None else
+                         (c.dt if c.dt is not None else c.stable_dt(float(r.min()))))
+            m.floor_y, m.back_wall_x = c.floor_y, c.back_wall_x
+            m.time, m.step_count = 0.0, 0
+            r0 = float(r.mean())
+            m.skin = skin_factor * r0
+            m.cell_size = 2.0 * float(r.max()) + m.skin
+            lo = x.min(0) - r.max()
+            hi = x.max(0) + r.max()
+            w, h = hi - lo
+            span = max(w, h)
+            x0 = min(lo[0], c.back_wall_x) - 2 * m.cell_size
+            y0 = min(lo[1], c.floor_y) - 2 * m.cell_size
+            m.grid_origin[0], m.grid_origin[1] = x0, y0
+            m.ncx = int(math.ceil((hi[0] + runout * span - x0) / m.cell_size)) + 1
+            m.ncy = int(math.ceil((hi[1] + 0.25 * span - y0) / m.cell_size)) + 1
+            m.cell_first = cells
+            cells += m.ncx * m.ncy
+            m.first, m.n = n, len(x)
+            m.use_floor, m.use_back_wall = int(c.floor), int(c.back_wall)
+            nb = self.lib.tdm_bed_blocks(len(x))
+            m.block_first, m.nblocks = blocks, nb
+            blocks += nb
+            m.mode, m.tab, m.list_parity, m.status = 1, 0, 0, 0
+            xs.append(x)
+            rs.append(r)
+            vs.append(np.zeros_like(x) if s.velocities is None else np.asarray(s.velocities, np.float64))
+            n += len(x)
+        self.n, self.nmembers, self.nblocks, self.ncells = n, nm, blocks, cells
+        self.cand_slots, self.cell_cap = cand_slots, cell_cap
+        dev = self.device
+        f64 = dict(dtype=torch.float64, device=dev)
+        self.x = torch.tensor(np.concatenate(xs), **f64)
+        self.v = torch.tensor(np.concatenate(vs), **f64)
+        self.x_alt = torch.empty_like(self.x)
+        self.v_alt = torch.empty_like(self.v)
+        self.radius = torch.tensor(np.concatenate(rs), **f64)
+        self.xi_wall = torch.zeros((n, 4), **f64)
+        self.accounting = torch.zeros((nm, _lib.BED_NACC), **f64)
+        self.energy = torch.zeros((nm, 4), **f64)
+        self.members = torch.frombuffer(bytearray(bytes(members)), dtype=torch.uint8).to(dev)
+        args = (n, blocks, nm, cells, cand_slots, cell_cap)
+        wsb = self.lib.tdm_bed_workspace_bytes(*args)
+        self.workspace = torch.zeros(int(wsb) + 256, dtype=torch.uint8, device=dev)
+        offs = [ctypes.c_size_t() for _ in range(4)]
+        _lib.check(self.lib.tdm_bed_workspace_layout(*args, *[ctypes.byref(o) for o in offs]))
+        self._layout = [o.value for o in offs]
+        self._arrays = _lib.BedArrays()
+        self._refresh()
+        _lib.check(self.lib.tdm_bed_setup(ctypes.byref(self._arrays), self.stream))
+        self._graphs = {}
+
+    @property
+    def stream(self) -> int:
+        return torch.cuda.current_stream(self.device).cuda_stream
+
+    def _ws_base(self) -> int:
+        base = self.workspace.data_ptr()
+        return (base + 255) & ~255
+
+    def _refresh(self) -> None:
+        a = self._arrays
+        a.x, a.v, a.x_alt, a.v_alt = (t.data_ptr() for t in (self.x, self.v, self.x_alt, self.v_alt))
+        a.radius, a.xi_wall = self.radius.data_ptr(), self.xi_wall.data_ptr()
+        a.accounting, a.energy = self.accounting.data_ptr(), self.energy.data_ptr()
+        a.members = self.members.data_ptr()
+        a.workspace = self._ws_base()
+        a.workspace_bytes = self.workspace.numel() - (a.workspace - self.workspace.data_ptr())
+        a.n, a.nmembers, a.nblocks, a.ncells = self.n, self.nmembers, self.nblocks, self.ncells
+        a.cand_slots, a.cell_cap = self.cand_slots, self.cell_cap
+
+    def _swap(self) -> None:
+        self.x, self.x_alt = self.x_alt, self.x
+        self.v, self.v_alt = self.v_alt, self.v
+        self._refresh()
+
+    # ---- operations ------------------------------------------------------------
+    def step(self, nsteps: int = 1) -> None:
+        _lib.check(self.lib.tdm_bed_step(ctypes.byref(self._arrays), int(nsteps), int(self.fp32),
+                                         self.stream))
+        if nsteps % 2:
+            self._swap()
+
+    def step_timed(self, nsteps: int) -> float:
+        out = ctypes.c_double()
+        _lib.check(self.lib.tdm_bed_step_timed(ctypes.byref(self._arrays), int(nsteps),
+                                               int(self.fp32), self.stream, ctypes.byref(out)))
+        return out.value
+
+    def rebin(self) -> None:
+        _lib.check(self.lib.tdm_bed_rebin(ctypes.byref(self._arrays), self.stream))
+
+    def energies(self) -> torch.Tensor:
+        _lib.check(self.lib.tdm_bed_energy(ctypes.byref(self._arrays), self.stream))
+        return self.energy.clone()
+
+    def graph(self, pairs: int):
+        key = (pairs, self.fp32, self.x.data_ptr())
+        if key not in self._graphs:
+            g = ctypes.c_void_p()
+            stream = torch.cuda.Stream(self.device)
+            stream.wait_stream(torch.cuda.current_stream(self.device))
+            with torch.cuda.stream(stream):
+                _lib.check(self.lib.tdm_bed_graph_create(ctypes.byref(self._arrays), int(pairs),
+                                                         int(self.fp32), stream.cuda_stream,
+                                                         ctypes.byref(g)))
+            torch.cuda.current_stream(self.device).wait_stream(stream)
+            self._graphs[key] = g
+        return self._graphs[key]
+
+    def run_graph(self, pairs: int, repeats: int = 1) -> None:
+        g = self.graph(pairs)
+        for _ in range(repeats):
+            _lib.check(self.lib.tdm_graph_launch(g, self.stream))
+
+    def read_members(self):
+        raw = self.members.cpu().numpy().tobytes()
+        return (_lib.BedMember * self.nmembers).from_buffer_copy(raw)
+
+    def status(self) -> np.ndarray:
+        return np.array([m.status for m in self.read_members()], dtype=np.int32)
+
+    def contact_pairs(self) -> np.ndarray:
+        """Pairs (i<j) in contact during the last step, from the kernels' own
+        per-slot contact masks (sorted)."""
+        ms = self.read_members()
+        S, n = self.cand_slots, self.n
+        u8 = self.workspace
+        base = self._ws_base() - self.workspace.data_ptr()
+        out = []
+        for m in ms:
+            lp = m.list_parity
+            co, xo, mo, no = self._layout
+            cand = u8[base + co: base + co + 2 * S * n * 4].view(torch.int32).view(2, S, n)[lp]
+            mask = u8[base + mo: base + mo + 2 * n * 4].view(torch.int32).view(2, n)[lp]
+            cnt = u8[base + no: base + no + 2 * n].view(2, n)[lp]
+            sl = slice(m.first, m.first + m.n)
+            c = cand[:, sl].cpu().numpy()
+            mk = mask[sl].cpu().numpy().astype(np.int64)
+            nc = cnt[sl].cpu().numpy()
+            for e in range(S):
+                sel = ((mk >> e) & 1).astype(bool) & (e < nc)
+                p = np.nonzero(sel)[0] + m.first
+                q = c[e][sel]
+                keep = q > p
+                out.append(np.stack((p[keep], q[keep]), axis=1))
+        if not out:
+            return np.zeros((0, 2), dtype=np.int64)
+        pairs = np.concatenate(out)
+        return pairs[np.lexsort((pairs[:, 1], pairs[:, 0]))]
+
+    def close(self) -> None:
+        for g in self._graphs.values():
+            self.lib.tdm_graph_destroy(g)
+        self._graphs.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,781 @@
+// One step of unbonded Hertz-Mindlin disk beds on MI355X (gfx950).
+//
+// The contact law restates, in two dimensions, the reference's floe-bed
+// kernel (src/tensordem/scale_ceiling/kernel.py:277-401): Hertz normal force
+// (2/3) s_n d with s_n = 2 E* sqrt(R* d), viscous term
+// 2 sqrt(5/6) beta sqrt(s_n m*) v_n, non-cohesive clamp; Cundall-Strack
+// tangential spring xi += v_t dt with stiffness 8 G* sqrt(R* d) scaled back
+// to the Coulomb cap; floor and back-wall contacts; gravity.  A pair's spring
+// history is dropped once it separates (granular2d/engine.py:388-396).
+//
+// Layout: every disk owns a sorted list of candidate partners (all disks
+// within r_p + r_q + skin when the list was built) and one spring history per
+// list slot.  Both disks of a pair keep a copy of the pair's history and
+// update it with identical arithmetic (lo -> hi orientation), so the copies
+// stay bit-identical and no disk ever writes another disk's data: forces are
+// gathered, not scattered, and no float atomics are used.  Lists and
+// histories are double-buffered by list parity; a rebuild writes the new
+// lists into the spare buffer and carries every active history over by
+// merging the old and new partner lists (sorted-key remap).
+//
+// Per disk the force is gravity, then the pair forces (partners above by
+// ascending index, then partners below), then floor, then back wall -- the
+// order of the FP64 oracle (oracle/bed2d.c), so FP64 runs are bit-identical
+// to it.
+#include "common.h"
+
+namespace {
+
+using namespace tdm;
+
+constexpr int kMaxSlots = 16;
+
+enum { MODE_IDLE = 0, MODE_SEARCH = 1, MODE_BIN = 2 };
+enum { OP_STEP = 1, OP_ENERGY = 2 };
+// ledger partial channels
+enum {
+  C_DASH = 0, C_FRIC, C_WDASH, C_WFRIC, C_WFX, C_WFY, C_NPAIR, C_NWALL,
+  C_KIN = 8, C_POT, C_ELN, C_ELT
+};
+constexpr uint32_t kFloorBit = 1u << 16, kBackBit = 1u << 17;
+
+struct BedWs {
+  int4 *blocks;      // (member, first disk, count, rank)
+  double *partials;  // [nblocks][kCh]
+  int *bflags;       // [nblocks]
+  int *cell_count;   // [2][ncells]
+  int *cell_tab;     // [2][ncells][cap]
+  int *cand;         // [2][slots][n]
+  double2 *xi;       // [2][slots][n]
+  uint32_t *mask;    // [2][n]
+  uint8_t *ncand;    // [2][n]
+  double2 *xb;       // [n] position when the list was built
+};
+
+inline void carve(void *base, int n, int nblocks, int ncells, int slots, int cap, BedWs *w,
+                  size_t *total)
+{
+  size_t off = 0;
+  char *b = static_cast<char *>(base);
+  auto take = [&](size_t bytes) {
+    char *p = b ? b + off : nullptr;
+    off += align256(bytes);
+    return p;
+  };
+  w->blocks = reinterpret_cast<int4 *>(take(sizeof(int4) * size_t(nblocks)));
+  w->partials = reinterpret_cast<double *>(take(sizeof(double) * kCh * size_t(nblocks)));
+  w->bflags = reinterpret_cast<int *>(take(sizeof(int) * size_t(nblocks)));
+  w->cell_count = reinterpret_cast<int *>(take(sizeof(int) * 2 * size_t(ncells)));
+  w->cell_tab = reinterpret_cast<int *>(take(sizeof(int) * 2 * size_t(ncells) * size_t(cap)));
+  w->cand = reinterpret_cast<int *>(take(sizeof(int) * 2 * size_t(slots) * size_t(n)));
+  w->xi = reinterpret_cast<double2 *>(take(sizeof(double2) * 2 * size_t(slots) * size_t(n)));
+  w->mask = reinterpret_cast<uint32_t *>(take(sizeof(uint32_t) * 2 * size_t(n)));
+  w->ncand = reinterpret_cast<uint8_t *>(take(2 * size_t(n)));
+  w->xb = reinterpret_cast<double2 *>(take(sizeof(double2) * size_t(n)));
+  *total = off;
+}
+
+struct BKArgs {
+  tdm_bed_arrays a;
+  BedWs w;
+  const double2 *x_in, *v_in;
+  double2 *x_out, *v_out;
+  int op;
+};
+
+// One pair p-q seen from p, lo -> hi oriented.  Returns false without
+// touching anything when the disks do not overlap (decided in FP64 for both
+// precisions).  On contact: F is the force on hi (lo receives -F), xi is
+// updated in place, and the owner's ledger channels are filled.
+template <typename R>
+__device__ inline bool hm_pair(const tdm_bed_member &m, double2 x_lo, double2 x_hi,
+                               double2 v_lo, double2 v_hi, double r_lo, double r_hi, double2 &xi,
+                               double &Fx, double &Fy, double &dash, double &fric)
+{
+  const double dx = x_hi.x - x_lo.x, dy = x_hi.y - x_lo.y;
+  const double dist = sqrt(dx * dx + dy * dy);
+  const double d = (r_lo + r_hi) - dist;
+  if (!(d > 0)) return false;
+  const double vrx = v_lo.x - v_hi.x, vry = v_lo.y - v_hi.y;
+  if constexpr (Prec<R>::fp64) {
+    const double dd = dist > 1e-300 ? dist : 1e-300;
+    const double nx = dx / dd, ny = dy / dd;
+    const double reff = (r_lo * r_hi) / (r_lo + r_hi);
+    const double mlo = m.rho_pi_t * (r_lo * r_lo), mhi = m.rho_pi_t * (r_hi * r_hi);
+    const double meff = (mlo * mhi) / (mlo + mhi);
+    const double sn = (2 * m.e_star) * sqrt(reff * d);
+    const double fel = ((2.0 / 3.0) * sn) * d;
+    const double vn = vrx * nx + vry * ny;
+    double fn = fel + (m.damping * sqrt(sn * meff)) * vn;
+    if (fn < 0) fn = 0;
+    const double st = (8 * m.g_star) * sqrt(reff * d);
+    const double vtx = vrx - vn * nx, vty = vry - vn * ny;
+    double ex = xi.x + vtx * m.dt, ey = xi.y + vty * m.dt;
+    const double mag = sqrt(ex * ex + ey * ey);
+    const double cap = m.friction * fn;
+    if (mag > 0) {
+      const double den = st * mag;
+      double sc = cap / (den > 1e-300 ? den : 1e-300);
+      if (sc > 1) sc = 1;
+      ex = ex * sc;
+      ey = ey * sc;
+    }
+    xi = make_double2(ex, ey);
+    Fx = fn * nx + st * ex;
+    Fy = fn * ny + st * ey;
+    dash = (fn - fel) * vn;
+    fric = st * (ex * vrx + ey * vry);
+  } else {
+    const float df = (float)d, distf = (float)dist;
+    const float inv = rcp32(fmaxf(distf, 1e-30f));
+    const float nx = (float)dx * inv, ny = (float)dy * inv;
+    const float rl = (float)r_lo, rh = (float)r_hi;
+    const float reff = (rl * rh) * rcp32(rl + rh);
+    const float k = (float)m.rho_pi_t;
+    const float mlo = k * (rl * rl), mhi = k * (rh * rh);
+    const float meff = (mlo * mhi) * rcp32(mlo + mhi);
+    const float sq = sqrt32(reff * df);
+    const float sn = (2.0f * (float)m.e_star) * sq;
+    const float fel = ((2.0f / 3.0f) * sn) * df;
+    const float vn = (float)vrx * nx + (float)vry * ny;
+    const float fn = fmaxf(fel + ((float)m.damping * sqrt32(sn * meff)) * vn, 0.0f);
+    const float st = (8.0f * (float)m.g_star) * sq;
+    const float vtx = (float)vrx - vn * nx, vty = (float)vry - vn * ny;
+    const float dtf = (float)m.dt;
+    float ex = (float)xi.x + vtx * dtf, ey = (float)xi.y + vty * dtf;
+    const float mag = sqrt32(ex * ex + ey * ey);
+    const float cap = (float)m.friction * fn;
+    if (mag > 0) {
+      const float sc = fminf(cap * rcp32(fmaxf(st * mag, 1e-30f)), 1.0f);
+      ex *= sc;
+      ey *= sc;
+    }
+    xi = make_double2((double)ex, (double)ey);
+    Fx = (double)(fn * nx + st * ex);
+    Fy = (double)(fn * ny + st * ey);
+    dash = (double)((fn - fel) * vn);
+    fric = (double)(st * (ex * (float)vrx + ey * (float)vry));
+  }
+  return true;
+}
+
+// Wall contact along the inward unit normal (nx, ny) at distance `gap`.
+template <typename R>
+__device__ inline bool hm_wall(const tdm_bed_member &m, double r, double mass, double gap,
+                               double nx, double ny, double2 v, double2 &xi, double &Fx,
+                               double &Fy, double &dash, double &fric)
+{
+  const double d = r - gap;
+  if (!(d > 0)) return false;
+  if constexpr (Prec<R>::fp64) {
+    const double sn = (2 * m.e_star) * sqrt(r * d);
+    const double fel = ((2.0 / 3.0) * sn) * d;
+    const double vn = v.x * nx + v.y * ny;
+    double fn = fel - (m.damping * sqrt(sn * mass)) * vn;
+    if (fn < 0) fn = 0;
+    const double st = (8 * m.g_star) * sqrt(r * d);
+    const double vtx = v.x - vn * nx, vty = v.y - vn * ny;
+    double ex = xi.x + vtx * m.dt, ey = xi.y + vty * m.dt;
+    const double mag = sqrt(ex * ex + ey * ey);
+    const double cap = m.wall_friction * fn;
+    if (mag > 0) {
+      const double den = st * mag;
+      double sc = cap / (den > 1e-300 ? den : 1e-300);
+      if (sc > 1) sc = 1;
+      ex = ex * sc;
+      ey = ey * sc;
+    }
+    xi = make_double2(ex, ey);
+    Fx = fn * nx - st * ex;
+    Fy = fn * ny - st * ey;
+    dash = (fn - fel) * (-vn);
+    fric = st * (ex * v.x + ey * v.y);
+  } else {
+    const float df = (float)d, rf = (float)r;
+    const float sq = sqrt32(rf * df);
+    const float sn = (2.0f * (float)m.e_star) * sq;
+    const float fel = ((2.0f / 3.0f) * sn) * df;
+    const float vn = (float)v.x * (float)nx + (float)v.y * (float)ny;
+    const float fn = fmaxf(fel - ((float)m.damping * sqrt32(sn * (float)mass)) * vn, 0.0f);
+    const float st = (8.0f * (float)m.g_star) * sq;
+    const float vtx = (float)v.x - vn * (float)nx, vty = (float)v.y - vn * (float)ny;
+    const float dtf = (float)m.dt;
+    float ex = (float)xi.x + vtx * dtf, ey = (float)xi.y + vty * dtf;
+    const float mag = sqrt32(ex * ex + ey * ey);
+    const float cap = (float)m.wall_friction * fn;
+    if (mag > 0) {
+      const float sc = fminf(cap * rcp32(fmaxf(st * mag, 1e-30f)), 1.0f);
+      ex *= sc;
+      ey *= sc;
+    }
+    xi = make_double2((double)ex, (double)ey);
+    Fx = (double)(fn * (float)nx - st * ex);
+    Fy = (double)(fn * (float)ny - st * ey);
+    dash = (double)((fn - fel) * (-vn));
+    fric = (double)(st * (ex * (float)v.x + ey * (float)v.y));
+  }
+  return true;
+}
+
+__device__ inline int cell_of(const tdm_bed_member &m, double2 x, int *cx, int *cy)
+{
+  int ix = (int)floor((x.x - m.grid_origin[0]) / m.cell_size);
+  int iy = (int)floor((x.y - m.grid_origin[1]) / m.cell_size);
+  if (!(x.x == x.x) || !(x.y == x.y)) { ix = 0; iy = 0; }
+  ix = min(max(ix, 0), m.ncx - 1);
+  iy = min(max(iy, 0), m.ncy - 1);
+  *cx = ix;
+  *cy = iy;
+  return m.cell_first + iy * m.ncx + ix;
+}
+
+template <typename R>
+__global__ __launch_bounds__(kThreads) void bed_step_kernel(BKArgs k)
+{
+  __shared__ double red[4][kCh];
+  __shared__ tdm_bed_member sm;
+  __shared__ int sflags[2];
+  const tdm_bed_arrays &a = k.a;
+  const int4 blk = k.w.blocks[blockIdx.x];
+  const int mi = blk.x, p0 = blk.y, cnt = blk.z;
+  const bool valid = (int)threadIdx.x < cnt;
+  const int p = p0 + (valid ? (int)threadIdx.x : 0);
+  const int n = a.n, S = a.cand_slots;
+  double2 xp = make_double2(0.0, 0.0), vp = make_double2(0.0, 0.0), xb = make_double2(0.0, 0.0);
+  double rp = 0.0;
+  if (valid) {
+    xp = k.x_in[p];
+    vp = k.v_in[p];
+    rp = a.radius[p];
+    xb = k.w.xb[p];
+  }
+  static_assert(sizeof(tdm_bed_member) % 8 == 0, "member record must be 8-byte sized");
+  if (threadIdx.x < sizeof(tdm_bed_member) / 8)
+    reinterpret_cast<uint64_t *>(&sm)[threadIdx.x] =
+        reinterpret_cast<const uint64_t *>(a.members + mi)[threadIdx.x];
+  if (threadIdx.x < 2) sflags[threadIdx.x] = 0;
+  __syncthreads();
+  const tdm_bed_member &m = sm;
+  const int mode = m.mode, lp = m.list_parity;
+  double ch[kCh];
+#pragma unroll
+  for (int c = 0; c < kCh; ++c) ch[c] = 0.0;
+  int status = 0, req = 0;
+
+  if (valid) {
+    const double mass = m.rho_pi_t * (rp * rp);
+    int *cand = k.w.cand + (size_t)lp * S * n + p;        // slot e at cand[e * n]
+    double2 *xis = k.w.xi + (size_t)lp * S * n + p;
+    uint32_t mask = k.w.mask[(size_t)lp * n + p];
+    int nc = k.w.ncand[(size_t)lp * n + p];
+    uint32_t *mask_out = k.w.mask + (size_t)lp * n + p;
+
+    if (mode == MODE_SEARCH) {
+      // ---- rebuild: new sorted list in the spare buffer, histories carried
+      // over by partner (sorted-key merge of old and new lists) -------------
+      int *ncand_new = k.w.cand + (size_t)(lp ^ 1) * S * n + p;
+      double2 *nxi = k.w.xi + (size_t)(lp ^ 1) * S * n + p;
+      int nn = 0;
+      int cx, cy;
+      cell_of(m, xp, &cx, &cy);
+      const int *cntab = k.w.cell_count + (size_t)m.tab * a.ncells;
+      const int *tab = k.w.cell_tab + (size_t)m.tab * a.ncells * a.cell_cap;
+      int ccount[9];
+#pragma unroll
+      for (int o = 0; o < 9; ++o) {
+        const int gx = cx + o % 3 - 1, gy = cy + o / 3 - 1;
+        const bool in = gx >= 0 && gx < m.ncx && gy >= 0 && gy < m.ncy;
+        ccount[o] = in ? min(cntab[m.cell_first + gy * m.ncx + gx], a.cell_cap) : 0;
+      }
+#pragma unroll
+      for (int o = 0; o < 9; ++o) {
+        {
+          const int cn = ccount[o];
+          if (cn == 0) continue;
+          const int cell = m.cell_first + (cy + o / 3 - 1) * m.ncx + (cx + o % 3 - 1);
+          for (int e = 0; e < cn; ++e) {
+            const int q = tab[(size_t)cell * a.cell_cap + e];
+            if (q == p) continue;
+            const double2 xq = k.x_in[q];
+            const double rq = a.radius[q];
+            const double dx = xq.x - xp.x, dy = xq.y - xp.y;
+            if (!(sqrt(dx * dx + dy * dy) < (rp + rq) + m.skin)) continue;
+            if (nn == S) { status |= TDM_ST_CAND_OVERFLOW; continue; }
+            int pos = nn++;
+            while (pos > 0 && ncand_new[(size_t)(pos - 1) * n] > q) {
+              ncand_new[(size_t)pos * n] = ncand_new[(size_t)(pos - 1) * n];
+              --pos;
+            }
+            ncand_new[(size_t)pos * n] = q;
+          }
+        }
+      }
+      // merge: both lists ascending; copy the history of every active old slot
+      uint32_t nmask = mask & (kFloorBit | kBackBit);
+      int eo = 0;
+      for (int e = 0; e < nn; ++e) {
+        const int q = ncand_new[(size_t)e * n];
+        double2 h = make_double2(0.0, 0.0);
+        while (eo < nc && cand[(size_t)eo * n] < q) ++eo;
+        if (eo < nc && cand[(size_t)eo * n] == q && ((mask >> eo) & 1u)) {
+          h = xis[(size_t)eo * n];
+          nmask |= 1u << e;
+        }
+        nxi[(size_t)e * n] = h;
+      }
+      cand = ncand_new;
+      xis = nxi;
+      mask = nmask;
+      nc = nn;
+      mask_out = k.w.mask + (size_t)(lp ^ 1) * n + p;
+      k.w.ncand[(size_t)(lp ^ 1) * n + p] = (uint8_t)nn;
+      xb = xp;
+      k.w.xb[p] = xp;
+    } else {
+      const double ddx = xp.x - xb.x, ddy = xp.y - xb.y;
+      const double lim = 0.5 * m.skin * 0.999;
+      if (ddx * ddx + ddy * ddy >= lim * lim) status |= TDM_ST_STALE_LIST;
+    }
+    int split = 0;
+    while (split < nc && cand[(size_t)split * n] < p) ++split;
+
+    if (k.op == OP_ENERGY) {
+      // kinetic, gravity, Hertz normal (2/5 f_el d) and spring (s_t |xi|^2 / 2)
+      ch[C_KIN] = 0.5 * mass * (vp.x * vp.x + vp.y * vp.y);
+      ch[C_POT] = -mass * m.gravity_y * xp.y;
+      for (int e = split; e < nc; ++e) {
+        const int q = cand[(size_t)e * n];
+        const double2 xq = k.x_in[q];
+        const double rq = a.radius[q];
+        const double dx = xq.x - xp.x, dy = xq.y - xp.y;
+        const double d = (rp + rq) - sqrt(dx * dx + dy * dy);
+        if (!(d > 0)) continue;
+        const double reff = (rp * rq) / (rp + rq);
+        const double sn = (2 * m.e_star) * sqrt(reff * d);
+        ch[C_ELN] += 0.4 * (((2.0 / 3.0) * sn) * d) * d;
+        if ((mask >> e) & 1u) {
+          const double2 h = xis[(size_t)e * n];
+          ch[C_ELT] += 0.5 * ((8 * m.g_star) * sqrt(reff * d)) * (h.x * h.x + h.y * h.y);
+        }
+      }
+      const double gaps[2] = {xp.y - m.floor_y, xp.x - m.back_wall_x};
+      const int use[2] = {m.use_floor, m.use_back_wall};
+      for (int w = 0; w < 2; ++w) {
+        const double d = rp - gaps[w];
+        if (!use[w] || !(d > 0)) continue;
+        const double sn = (2 * m.e_star) * sqrt(rp * d);
+        ch[C_ELN] += 0.4 * (((2.0 / 3.0) * sn) * d) * d;
+        if ((mask >> (16 + w)) & 1u) {
+          const double2 h = reinterpret_cast<const double2 *>(a.xi_wall)[2 * p + w];
+          ch[C_ELT] += 0.5 * ((8 * m.g_star) * sqrt(rp * d)) * (h.x * h.x + h.y * h.y);
+        }
+      }
+      *mask_out = mask;
+    } else {
+      // ---- forces: gravity, pairs above p, pairs below p, floor, back wall
+      double fx = 0.0, fy = mass * m.gravity_y;
+      uint32_t nmask = 0;
+      for (int pass = 0; pass < 2; ++pass) {
+        const int e0 = pass == 0 ? split : 0, e1 = pass == 0 ? nc : split;
+        for (int e = e0; e < e1; ++e) {
+          const int q = cand[(size_t)e * n];
+          const bool self_lo = pass == 0;  // partners above p come first
+          const double2 xq = k.x_in[q], vq = k.v_in[q];
+          const double rq = a.radius[q];
+          double2 h = ((mask >> e) & 1u) ? xis[(size_t)e * n] : make_double2(0.0, 0.0);
+          double Fx, Fy, dash, fric;
+          const bool hit = self_lo
+              ? hm_pair<R>(m, xp, xq, vp, vq, rp, rq, h, Fx, Fy, dash, fric)
+              : hm_pair<R>(m, xq, xp, vq, vp, rq, rp, h, Fx, Fy, dash, fric);
+          if (!hit) continue;
+          xis[(size_t)e * n] = h;
+          nmask |= 1u << e;
+          if (self_lo) {
+            fx += -Fx; fy += -Fy;
+            ch[C_DASH] += dash; ch[C_FRIC] += fric; ch[C_NPAIR] += 1.0;
+          } else {
+            fx += Fx; fy += Fy;
+          }
+        }
+      }
+      double2 *xw = reinterpret_cast<double2 *>(a.xi_wall) + 2 * p;
+      if (m.use_floor) {
+        double2 h = (mask & kFloorBit) ? xw[0] : make_double2(0.0, 0.0);
+        double Fx, Fy, dash, fric;
+        if (hm_wall<R>(m, rp, mass, xp.y - m.floor_y, 0.0, 1.0, vp, h, Fx, Fy, dash, fric)) {
+          xw[0] = h;
+          nmask |= kFloorBit;
+          fx += Fx; fy += Fy;
+          ch[C_WDASH] += dash; ch[C_WFRIC] += fric; ch[C_WFX] += Fx; ch[C_WFY] += Fy;
+          ch[C_NWALL] += 1.0;
+        }
+      }
+      if (m.use_back_wall) {
+        double2 h = (mask & kBackBit) ? xw[1] : make_double2(0.0, 0.0);
+        double Fx, Fy, dash, fric;
+        if (hm_wall<R>(m, rp, mass, xp.x - m.back_wall_x, 1.0, 0.0, vp, h, Fx, Fy, dash, fric)) {
+          xw[1] = h;
+          nmask |= kBackBit;
+          fx += Fx; fy += Fy;
+          ch[C_WDASH] += dash; ch[C_WFRIC] += fric; ch[C_WFX] += Fx; ch[C_WFY] += Fy;
+          ch[C_NWALL] += 1.0;
+        }
+      }
+      *mask_out = nmask;
+
+      // ---- semi-implicit Euler (dem.py:363-365) -------------------------
+      double vx, vy;
+      if constexpr (Prec<R>::fp64) {
+        vx = vp.x + (m.dt * fx) / mass;
+        vy = vp.y + (m.dt * fy) / mass;
+      } else {
+        const double dt_m = m.dt / mass;
+        vx = vp.x + fx * dt_m;
+        vy = vp.y + fy * dt_m;
+      }
+      const double xn = xp.x + m.dt * vx, yn = xp.y + m.dt * vy;
+      k.x_out[p] = make_double2(xn, yn);
+      k.v_out[p] = make_double2(vx, vy);
+      if (!isfinite(xn) || !isfinite(yn) || !isfinite(vx) || !isfinite(vy)) status |= TDM_ST_NONFINITE;
+      const double ddx = xn - xb.x, ddy = yn - xb.y;
+      const double q4 = 0.25 * m.skin;
+      if (ddx * ddx + ddy * ddy >= q4 * q4) req = 1;
+      if (mode == MODE_BIN) {
+        int cx, cy;
+        const int cell = cell_of(m, make_double2(xn, yn), &cx, &cy);
+        const int t = m.tab ^ 1;
+        const int slot = atomicAdd(k.w.cell_count + (size_t)t * a.ncells + cell, 1);
+        if (slot < a.cell_cap) k.w.cell_tab[((size_t)t * a.ncells + cell) * a.cell_cap + slot] = p;
+        else status |= TDM_ST_CELL_OVERFLOW;
+      }
+    }
+  }
+  if (mode == MODE_SEARCH) {
+    // clear the other hash table for the next bin phase
+    const int ncell_m = m.ncx * m.ncy;
+    int *cntc = k.w.cell_count + (size_t)(m.tab ^ 1) * a.ncells + m.cell_first;
+    for (int c = blk.w * kThreads + threadIdx.x; c < ncell_m; c += m.nblocks * kThreads)
+      cntc[c] = 0;
+  }
+  if (status) atomicOr(&sflags[0], status);
+  if (req) atomicOr(&sflags[1], 1);
+  block_sum(ch, red);
+  if (threadIdx.x == 0) {
+    const size_t slot = (size_t)(m.block_first + blk.w);
+    double *dst = k.w.partials + slot * kCh;
+#pragma unroll
+    for (int c = 0; c < kCh; ++c) dst[c] = ch[c];
+    k.w.bflags[slot] = sflags[0] | (sflags[1] << 16);
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void bed_finalize_kernel(BKArgs k)
+{
+  __shared__ int sflag;
+  __shared__ double seg[16][kCh + 1];
+  __shared__ double stot[kCh];
+  const tdm_bed_arrays &a = k.a;
+  const int mi = blockIdx.x;
+  tdm_bed_member *mp = a.members + mi;
+  const int b0 = mp->block_first, nb = mp->nblocks;
+  if (threadIdx.x == 0) sflag = 0;
+  __syncthreads();
+  const int c = threadIdx.x & 15, r0 = threadIdx.x >> 4;
+  double sum = 0.0;
+  for (int b = r0; b < nb; b += 16 * 8) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int bb = b + 16 * u;
+      v[u] = bb < nb ? k.w.partials[(size_t)(b0 + bb) * kCh + c] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) sum += v[u];
+  }
+  seg[r0][c] = sum;
+  int fl = 0;
+  for (int b = threadIdx.x; b < nb; b += kThreads) fl |= k.w.bflags[b0 + b];
+  if (fl) atomicOr(&sflag, fl);
+  __syncthreads();
+  if (threadIdx.x < kCh) {
+    double t = 0.0;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) t += seg[r][threadIdx.x];
+    stot[threadIdx.x] = t;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  const tdm_bed_member m = *mp;
+  const int status = sflag & 0xFFFF, rq = sflag >> 16;
+  int new_mode = m.mode, new_tab = m.tab, new_lp = m.list_parity;
+  if (m.mode == MODE_SEARCH) new_lp ^= 1;
+  if (k.op == OP_ENERGY) {
+    double *en = a.energy + (size_t)mi * 4;
+    en[0] = stot[C_KIN];
+    en[1] = stot[C_POT];
+    en[2] = stot[C_ELN];
+    en[3] = stot[C_ELT];
+    if (m.mode == MODE_SEARCH) new_mode = MODE_IDLE;
+  } else {
+    double *acc = a.accounting + (size_t)mi * TDM_BED_NACC;
+    const double dt = m.dt;
+    acc[TDM_BED_DASHPOT] += dt * stot[C_DASH];
+    acc[TDM_BED_FRICTION] += dt * stot[C_FRIC];
+    acc[TDM_BED_WALL_DASHPOT] += dt * stot[C_WDASH];
+    acc[TDM_BED_WALL_FRICTION] += dt * stot[C_WFRIC];
+    acc[TDM_BED_WALL_IMPULSE] += dt * stot[C_WFX];
+    acc[TDM_BED_WALL_IMPULSE + 1] += dt * stot[C_WFY];
+    acc[TDM_BED_CONTACTS] = stot[C_NPAIR];
+    acc[TDM_BED_WALL_CONTACTS] = stot[C_NWALL];
+    mp->time = m.time + dt;
+    mp->step_count = m.step_count + 1;
+    if (m.mode == MODE_BIN) { new_tab = m.tab ^ 1; new_mode = MODE_SEARCH; }
+    else new_mode = rq ? MODE_BIN : MODE_IDLE;
+  }
+  mp->mode = new_mode;
+  mp->tab = new_tab;
+  mp->list_parity = new_lp;
+  mp->status = m.status | status;
+}
+
+__global__ void bed_bin_kernel(tdm_bed_arrays a, BedWs w)
+{
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.n) return;
+  int lo = 0, hi = a.nmembers - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (a.members[mid].first <= t) lo = mid; else hi = mid - 1;
+  }
+  const tdm_bed_member &m = a.members[lo];
+  int cx, cy;
+  const int cell = cell_of(m, reinterpret_cast<const double2 *>(a.x)[t], &cx, &cy);
+  const int slot = atomicAdd(w.cell_count + (size_t)m.tab * a.ncells + cell, 1);
+  if (slot < a.cell_cap) w.cell_tab[((size_t)m.tab * a.ncells + cell) * a.cell_cap + slot] = t;
+  else atomicOr(&a.members[lo].status, TDM_ST_CELL_OVERFLOW);
+}
+
+__global__ void bed_set_mode_kernel(tdm_bed_member *m, int nm, int mode)
+{
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < nm) m[t].mode = mode;
+}
+
+int check_bed(const tdm_bed_arrays *a, BedWs *w)
+{
+  if (!a || !a->members || !a->x || !a->v || !a->x_alt || !a->v_alt || !a->radius ||
+      !a->xi_wall || !a->accounting || !a->workspace)
+    return fail(TDM_EINVAL, "null array in tdm_bed_arrays");
+  if (a->n <= 0 || a->nmembers <= 0 || a->nblocks <= 0 || a->ncells <= 0)
+    return fail(TDM_EINVAL, "empty bed batch");
+  if (a->cand_slots <= 0 || a->cand_slots > kMaxSlots || a->cell_cap <= 0)
+    return fail(TDM_EINVAL, "cand_slots must be in [1,16] and cell_cap > 0");
+  size_t need = 0;
+  carve(a->workspace, a->n, a->nblocks, a->ncells, a->cand_slots, a->cell_cap, w, &need);
+  if (need > a->workspace_bytes) return fail(TDM_ESPACE, "bed workspace too small");
+  if (reinterpret_cast<uintptr_t>(a->workspace) & 255)
+    return fail(TDM_EINVAL, "workspace must be 256-byte aligned");
+  return TDM_OK;
+}
+
+int launch_bed(const tdm_bed_arrays *a, const BedWs &w, int parity, int fp32, int op,
+               hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr)
+{
+  BKArgs k;
+  k.a = *a;
+  k.w = w;
+  k.op = op;
+  const double2 *x = reinterpret_cast<const double2 *>(a->x);
+  const double2 *v = reinterpret_cast<const double2 *>(a->v);
+  const double2 *xa = reinterpret_cast<const double2 *>(a->x_alt);
+  const double2 *va = reinterpret_cast<const double2 *>(a->v_alt);
+  k.x_in = parity ? xa : x;
+  k.v_in = parity ? va : v;
+  k.x_out = const_cast<double2 *>(parity ? x : xa);
+  k.v_out = const_cast<double2 *>(parity ? v : va);
+  if (ev0) {
+    if (fp32)
+      hipExtLaunchKernelGGL(bed_step_kernel<float>, dim3(a->nblocks), dim3(kThreads), 0, s, ev0,
+                            ev1, 0, k);
+    else
+      hipExtLaunchKernelGGL(bed_step_kernel<double>, dim3(a->nblocks), dim3(kThreads), 0, s, ev0,
+                            ev1, 0, k);
+  } else if (fp32) {
+    bed_step_kernel<float><<<a->nblocks, kThreads, 0, s>>>(k);
+  } else {
+    bed_step_kernel<double><<<a->nblocks, kThreads, 0, s>>>(k);
+  }
+  HIP_TRY(hipGetLastError());
+  bed_finalize_kernel<<<a->nmembers, kThreads, 0, s>>>(k);
+  HIP_TRY(hipGetLastError());
+  return TDM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tdm_bed_blocks(int n)
+{
+  if (n <= 0) return fail(TDM_EINVAL, "bed member needs n > 0");
+  return (n + kThreads - 1) / kThreads;
+}
+
+size_t tdm_bed_workspace_bytes(int n, int nblocks, int nmembers, int ncells, int cand_slots,
+                               int cell_cap)
+{
+  BedWs w;
+  size_t total = 0;
+  (void)nmembers;
+  carve(nullptr, n, nblocks, ncells, cand_slots, cell_cap, &w, &total);
+  return total;
+}
+
+int tdm_bed_workspace_layout(int n, int nblocks, int nmembers, int ncells, int cand_slots,
+                             int cell_cap, size_t *cand_off, size_t *xi_off, size_t *mask_off,
+                             size_t *count_off)
+{
+  if (!cand_off || !xi_off || !mask_off || !count_off) return fail(TDM_EINVAL, "null output");
+  BedWs w;
+  size_t total = 0;
+  (void)nmembers;
+  carve(nullptr, n, nblocks, ncells, cand_slots, cell_cap, &w, &total);
+  *cand_off = reinterpret_cast<size_t>(w.cand);
+  *xi_off = reinterpret_cast<size_t>(w.xi);
+  *mask_off = reinterpret_cast<size_t>(w.mask);
+  *count_off = reinterpret_cast<size_t>(w.ncand);
+  return TDM_OK;
+}
+
+int tdm_bed_setup(const tdm_bed_arrays *a, void *stream)
+{
+  BedWs w;
+  int rc = check_bed(a, &w);
+  if (rc) return rc;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int nm = a->nmembers;
+  tdm_bed_member *hm = new tdm_bed_member[nm];
+  hipError_t e = hipMemcpyAsync(hm, a->members, sizeof(tdm_bed_member) * nm,
+                                hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) { delete[] hm; return fail(TDM_EHIP, hipGetErrorString(e)); }
+  int total = 0;
+  for (int i = 0; i < nm; ++i) {
+    if (hm[i].block_first != total || hm[i].nblocks != tdm_bed_blocks(hm[i].n)) {
+      delete[] hm;
+      return fail(TDM_EINVAL, "bed member block_first/nblocks do not match tdm_bed_blocks");
+    }
+    total += hm[i].nblocks;
+  }
+  if (total != a->nblocks) { delete[] hm; return fail(TDM_EINVAL, "bed nblocks mismatch"); }
+  int4 *hb = new int4[total];
+  int *order = new int[total];
+  for (int i = 0; i < nm; ++i) {
+    const int b0 = hm[i].block_first, nb = hm[i].nblocks;
+    xcd_order(b0, nb, order);
+    for (int b = 0; b < nb; ++b) {
+      const int item = order[b];
+      const int first = hm[i].first + item * kThreads;
+      hb[b0 + b] = make_int4(i, first, min(kThreads, hm[i].first + hm[i].n - first), item);
+    }
+  }
+  e = hipMemcpyAsync(w.blocks, hb, sizeof(int4) * total, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  delete[] hb;
+  delete[] order;
+  delete[] hm;
+  if (e != hipSuccess) return fail(TDM_EHIP, hipGetErrorString(e));
+  fill_kernel<int><<<64, 256, 0, s>>>(w.bflags, (size_t)a->nblocks, 0);
+  fill_kernel<uint32_t><<<1024, 256, 0, s>>>(w.mask, 2 * (size_t)a->n, 0u);
+  fill_kernel<double><<<1024, 256, 0, s>>>(a->xi_wall, 4 * (size_t)a->n, 0.0);
+  HIP_TRY(hipGetLastError());
+  return tdm_bed_rebin(a, stream);
+}
+
+int tdm_bed_rebin(const tdm_bed_arrays *a, void *stream)
+{
+  BedWs w;
+  int rc = check_bed(a, &w);
+  if (rc) return rc;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  fill_kernel<int><<<1024, 256, 0, s>>>(w.cell_count, 2 * (size_t)a->ncells, 0);
+  bed_bin_kernel<<<(a->n + 255) / 256, 256, 0, s>>>(*a, w);
+  bed_set_mode_kernel<<<(a->nmembers + 255) / 256, 256, 0, s>>>(a->members, a->nmembers,
+                                                                 MODE_SEARCH);
+  HIP_TRY(hipGetLastError());
+  return TDM_OK;
+}
+
+int tdm_bed_step(const tdm_bed_arrays *a, int nsteps, int fp32, void *stream)
+{
+  BedWs w;
+  int rc = check_bed(a, &w);
+  if (rc) return rc;
+  if (nsteps < 0) return fail(TDM_EINVAL, "bad nsteps");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  for (int i = 0; i < nsteps && rc == TDM_OK; ++i) rc = launch_bed(a, w, i & 1, fp32, OP_STEP, s);
+  return rc;
+}
+
+int tdm_bed_step_timed(const tdm_bed_arrays *a, int nsteps, int fp32, void *stream,
+                       double *mean_kernel_us)
+{
+  BedWs w;
+  int rc = check_bed(a, &w);
+  if (rc) return rc;
+  if (nsteps <= 0 || (nsteps & 1) || !mean_kernel_us) return fail(TDM_EINVAL, "bad timing request");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  hipEvent_t *ev = new hipEvent_t[2 * nsteps]();
+  int made = 0;
+  for (; made < 2 * nsteps; ++made)
+    if (hipEventCreate(&ev[made]) != hipSuccess) break;
+  if (made < 2 * nsteps) rc = fail(TDM_EHIP, "hipEventCreate failed");
+  for (int i = 0; i < nsteps && rc == TDM_OK; ++i)
+    rc = launch_bed(a, w, i & 1, fp32, OP_STEP, s, ev[2 * i], ev[2 * i + 1]);
+  double total = 0.0;
+  if (rc == TDM_OK) {
+    hipError_t e = hipStreamSynchronize(s);
+    for (int i = 0; i < nsteps && e == hipSuccess; ++i) {
+      float ms = 0.0f;
+      e = hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]);
+      total += ms;
+    }
+    if (e != hipSuccess) rc = fail(TDM_EHIP, hipGetErrorString(e));
+  }
+  for (int i = 0; i < made; ++i) (void)hipEventDestroy(ev[i]);
+  delete[] ev;
+  if (rc == TDM_OK) *mean_kernel_us = total * 1e3 / nsteps;
+  return rc;
+}
+
+int tdm_bed_energy(const tdm_bed_arrays *a, void *stream)
+{
+  BedWs w;
+  int rc = check_bed(a, &w);
+  if (rc) return rc;
+  if (!a->energy) return fail(TDM_EINVAL, "energy output required");
+  return launch_bed(a, w, 0, 0, OP_ENERGY, static_cast<hipStream_t>(stream));
+}
+
+int tdm_bed_graph_create(const tdm_bed_arrays *a, int pairs, int fp32, void *stream,
+                         tdm_graph **out)
+{
+  BedWs w;
+  int rc = check_bed(a, &w);
+  if (rc) return rc;
+  if (!out || pairs <= 0) return fail(TDM_EINVAL, "bad graph request");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  for (int i = 0; i < 2 * pairs && rc == TDM_OK; ++i) rc = launch_bed(a, w, i & 1, fp32, OP_STEP, s);
+  hipGraph_t g;
+  hipError_t e = hipStreamEndCapture(s, &g);
+  if (rc) return rc;
+  if (e != hipSuccess) return fail(TDM_EHIP, hipGetErrorString(e));
+  hipGraphExec_t ex;
+  e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+  if (e != hipSuccess) { (void)hipGraphDestroy(g); return fail(TDM_EHIP, hipGetErrorString(e)); }
+  *out = new tdm_graph{g, ex};
+  return TDM_OK;
+}
+
+}  // extern "C"

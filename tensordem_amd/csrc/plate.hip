@@ -17,27 +17,16 @@
 //   * the cell hash for unbonded contacts is rebuilt in two pipelined phases
 //     (bin, then search) folded into ordinary steps, triggered on device when
 //     a disk has moved a quarter of the Verlet skin.
-#include <hip/hip_ext.h>
-#include <hip/hip_runtime.h>
-
-#include <cmath>
-#include <cstdint>
-#include <cstdio>
-#include <cstring>
-#include <string>
-
-#include "../../include/tensordem_amd.h"
+#include "common.h"
 
 namespace {
 
-constexpr int kThreads = 256;
+using namespace tdm;
+
 constexpr int kHalo = 2;
 constexpr int kMaxLds = 1300;   // (TX+4)*(TY+4) for TX in 16..256, TY = 256/TX
 constexpr int kMaxRing = 800;   // (TX+2)*(TY+2)
-constexpr int kCh = 16;         // ledger partial channels
 constexpr int kMaxCand = 16;    // upper bound for cand_slots
-constexpr double kEps64 = 2.220446049250313e-16;  // torch.finfo(float64).eps
-constexpr float kEps32 = 1.1920928955078125e-07f;
 
 // ledger partial channels
 enum {
@@ -65,7 +54,6 @@ struct Workspace {
   float2 *xb;            // [n] position - lattice site when the list was built
 };
 
-__host__ __device__ inline size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
 
 __host__ __device__ inline void carve(void *base, int n, int nblocks, int nmembers, int ncells,
                                       int slots, int cap, Workspace *w, size_t *total)
@@ -88,11 +76,6 @@ __host__ __device__ inline void carve(void *base, int n, int nblocks, int nmembe
   *total = off;
 }
 
-// FP32 force path: hardware reciprocal and square root (at most 1 ulp), far
-// cheaper than the correctly rounded expansions; break decisions within the
-// FP32 margin are re-done in FP64, so these never decide a fracture.
-__device__ inline float rcp32(float x) { return __builtin_amdgcn_rcpf(x); }
-__device__ inline float sqrt32(float x) { return __builtin_amdgcn_sqrtf(x); }
 
 __device__ inline double2 lattice_site(int ix, int iy, double half_span, double two_r)
 {
@@ -219,55 +202,6 @@ struct KArgs {
   int op;  // OP_FORCES, OP_STEP or OP_ENERGY
 };
 
-// Deterministic wave-wide sum of a double: a fixed DPP shift/broadcast tree
-// (row_shr 1,2,4,8 inside each 16-lane row, then row_bcast 15 and 31), both
-// 32-bit halves moved with the same control.  The total is read from lane 63
-// and returned in every lane.
-template <int CTRL, int ROWMASK>
-__device__ inline double dpp_move(double v)
-{
-  int lo = __double2loint(v), hi = __double2hiint(v);
-  lo = __builtin_amdgcn_update_dpp(0, lo, CTRL, ROWMASK, 0xF, false);
-  hi = __builtin_amdgcn_update_dpp(0, hi, CTRL, ROWMASK, 0xF, false);
-  return __hiloint2double(hi, lo);
-}
-
-__device__ inline double wave_sum(double v)
-{
-  v += dpp_move<0x111, 0xF>(v);
-  v += dpp_move<0x112, 0xF>(v);
-  v += dpp_move<0x114, 0xF>(v);
-  v += dpp_move<0x118, 0xF>(v);
-  v += dpp_move<0x142, 0xA>(v);
-  v += dpp_move<0x143, 0xC>(v);
-  int lo = __builtin_amdgcn_readlane(__double2loint(v), 63);
-  int hi = __builtin_amdgcn_readlane(__double2hiint(v), 63);
-  return __hiloint2double(hi, lo);
-}
-
-// Deterministic block sum of kCh channels; result valid in thread 0.
-template <int NCH>
-__device__ void block_sum(double (&ch)[NCH], double (*lds)[kCh])
-{
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll
-  for (int k = 0; k < NCH; ++k) {
-    // most channels are zero in most waves (breaks, loads, supports, tool):
-    // a wave-uniform skip keeps their DPP trees off the critical path
-    double s = 0.0;
-    if (__ballot(ch[k] != 0.0)) s = wave_sum(ch[k]);
-    if (lane == 0) lds[wave][k] = s;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int k = 0; k < NCH; ++k) ch[k] = ((lds[0][k] + lds[1][k]) + lds[2][k]) + lds[3][k];
-  }
-}
-
-template <typename R> struct Prec;
-template <> struct Prec<double> { static constexpr bool fp64 = true; };
-template <> struct Prec<float> { static constexpr bool fp64 = false; };
 
 // Repulsive contact of a non-bonded or broken pair (dem.py:278-288), written
 // with the lo -> hi orientation so both ends compute identical bits.  Returns
@@ -1026,32 +960,12 @@ __global__ void bin_kernel(tdm_plate_arrays a, Workspace w, const double2 *x)
   else atomicOr(&a.members[lo].status, TDM_ST_CELL_OVERFLOW);
 }
 
-__global__ void fill_int_kernel(int *p, size_t n, int v)
-{
-  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  for (; t < n; t += (size_t)gridDim.x * blockDim.x) p[t] = v;
-}
 
 __global__ void set_mode_kernel(tdm_plate_member *m, int nm, int mode)
 {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t < nm) m[t].mode = mode;
 }
-
-thread_local std::string g_err;
-
-int fail(int code, const std::string &msg)
-{
-  g_err = msg;
-  return code;
-}
-
-#define HIP_TRY(expr)                                                                  \
-  do {                                                                                 \
-    hipError_t e_ = (expr);                                                            \
-    if (e_ != hipSuccess)                                                              \
-      return fail(TDM_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_));        \
-  } while (0)
 
 int check_arrays(const tdm_plate_arrays *a, Workspace *w)
 {
@@ -1073,10 +987,6 @@ int check_arrays(const tdm_plate_arrays *a, Workspace *w)
 
 }  // namespace
 
-struct tdm_graph {
-  hipGraph_t graph;
-  hipGraphExec_t exec;
-};
 
 extern "C" {
 
@@ -1085,7 +995,7 @@ int tdm_version(void) { return 100; }
 int tdm_last_error(char *buf, size_t len)
 {
   if (!buf || len == 0) return TDM_EINVAL;
-  std::snprintf(buf, len, "%s", g_err.c_str());
+  std::snprintf(buf, len, "%s", tdm::last_error().c_str());
   return TDM_OK;
 }
 
@@ -1174,8 +1084,8 @@ int tdm_plate_setup(const tdm_plate_arrays *a, void *stream)
   delete[] hm;
   if (e != hipSuccess) return fail(TDM_EHIP, hipGetErrorString(e));
   // counters, requests, hash tables, candidate counts
-  fill_int_kernel<<<64, 256, 0, s>>>(w.bflags, (size_t)a->nblocks, 0);
-  fill_int_kernel<<<1024, 256, 0, s>>>(w.cell_count, 2 * (size_t)a->ncells, 0);
+  fill_kernel<int><<<64, 256, 0, s>>>(w.bflags, (size_t)a->nblocks, 0);
+  fill_kernel<int><<<1024, 256, 0, s>>>(w.cell_count, 2 * (size_t)a->ncells, 0);
   (void)hipMemsetAsync(w.ncand, 0, (size_t)a->n, s);
   HIP_TRY(hipGetLastError());
   rc = tdm_plate_sync_links(a, stream);
@@ -1201,7 +1111,7 @@ int tdm_plate_rebin(const tdm_plate_arrays *a, void *stream)
   int rc = check_arrays(a, &w);
   if (rc) return rc;
   hipStream_t s = static_cast<hipStream_t>(stream);
-  fill_int_kernel<<<1024, 256, 0, s>>>(w.cell_count, 2 * (size_t)a->ncells, 0);
+  fill_kernel<int><<<1024, 256, 0, s>>>(w.cell_count, 2 * (size_t)a->ncells, 0);
   bin_kernel<<<(a->n + 255) / 256, 256, 0, s>>>(*a, w, reinterpret_cast<const double2 *>(a->x));
   set_mode_kernel<<<(a->nmembers + 255) / 256, 256, 0, s>>>(a->members, a->nmembers, MODE_SEARCH);
   HIP_TRY(hipGetLastError());

@@ -1,0 +1,115 @@
+"""HIP Hertz-Mindlin bed against the FP64 bed oracle on the same seeded columns.
+
+FP64: positions, velocities and the contact-pair set bit-identical to the
+oracle at every checkpoint (same operand order, no fused multiply-add);
+ledger sums to 1e-12 relative.  FP32 forces: the contact-pair set is decided
+in FP64 and must equal the oracle's set at the same positions.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle.bed import OracleBed
+from tensordem_amd.bed import BedEngine, BedSpec
+from tensordem_amd.bedconfig import BedConfig, column_packing
+
+pytestmark = pytest.mark.gpu
+
+R0 = 0.01
+
+
+def column(n, seed, **kw):
+    x, r = column_packing(n, R0, seed=seed)
+    return BedConfig(**kw), x, r
+
+
+def oracle_pairs(ref: OracleBed) -> np.ndarray:
+    n = len(ref.radii)
+    k = ref.hist_keys
+    return np.stack((k // n, k % n), axis=1) if len(k) else np.zeros((0, 2), dtype=np.int64)
+
+
+def shake(n, seed, scale=0.3):
+    return np.random.default_rng(seed).normal(scale=scale, size=(n, 2))
+
+
+def test_column_fp64_bit_exact_with_contact_sets():
+    cfg, x, r = column(600, seed=3)
+    v0 = shake(len(r), 3)
+    ref = OracleBed(cfg, x, r)
+    ref.velocities[:] = v0
+    eng = BedEngine([BedSpec(cfg, x, r, dt=ref.dt, velocities=v0)])
+    rebuild_seen = False
+    for k in range(1, 3001):
+        eng.step(1)
+        ref.step()
+        if k % 250 == 0:
+            xg = eng.x.cpu().numpy()
+            assert np.array_equal(xg, ref.positions), f"step {k}: |dx| {np.abs(xg - ref.positions).max():.3e}"
+            assert np.array_equal(eng.v.cpu().numpy(), ref.velocities)
+            got = eng.contact_pairs() - eng.read_members()[0].first
+            assert np.array_equal(got, oracle_pairs(ref)), f"step {k} contact sets differ"
+            rebuild_seen |= eng.read_members()[0].list_parity == 1
+    assert len(ref.hist_keys) > 300
+    assert rebuild_seen
+    assert eng.status()[0] == 0
+    acc = eng.accounting[0].cpu().numpy()
+    for i in range(6):
+        assert acc[i] == pytest.approx(ref.acc[i], rel=1e-11, abs=1e-15), i
+    e = eng.energies()[0].cpu().numpy()
+    oe = ref.energies()
+    assert e[0] == pytest.approx(oe["kinetic_J"], rel=1e-12)
+    assert e[1] == pytest.approx(oe["potential_J"], rel=1e-12)
+    assert e[2] == pytest.approx(oe["elastic_normal_J"], rel=1e-10)
+
+
+def test_fp32_contact_sets_equal_fp64_decisions():
+    cfg, x, r = column(800, seed=5)
+    v0 = shake(len(r), 5)
+    eng = BedEngine([BedSpec(cfg, x, r, velocities=v0)], fp32=True)
+    probe = OracleBed(cfg, x, r)
+    for k in range(1, 2001):
+        before = eng.x.cpu().numpy().copy() if k % 200 == 0 else None
+        eng.step(1)
+        if before is not None:
+            # pairs in contact during step k were decided at the positions
+            # entering step k: compare with the oracle's FP64 set there
+            probe.positions[:] = before
+            exp = probe.contacts().astype(np.int64)
+            assert np.array_equal(eng.contact_pairs(), exp), f"step {k}"
+    # FP32 vs FP64 trajectories: stated divergence after 2000 steps
+    ref = OracleBed(cfg, x, r)
+    ref.velocities[:] = v0
+    for _ in range(2000):
+        ref.step()
+    err = np.abs(eng.x.cpu().numpy() - ref.positions).max()
+    print(f"bed fp32: max |x - x_fp64| after 2000 steps = {err:.3e} m")
+    assert err < 1e-3 * R0
+
+
+def test_bed_batch_members_equal_alone():
+    specs = []
+    for seed, n in ((11, 300), (12, 700), (13, 257)):
+        cfg, x, r = column(n, seed, restitution=0.3 + 0.1 * (seed - 11))
+        specs.append(BedSpec(cfg, x, r, velocities=shake(n, seed)))
+    batch = BedEngine(specs)
+    batch.step(900)
+    for k, s in enumerate(specs):
+        solo = BedEngine([s])
+        solo.step(900)
+        m = batch.read_members()[k]
+        assert np.array_equal(batch.x[m.first:m.first + m.n].cpu().numpy(), solo.x.cpu().numpy())
+        assert np.array_equal(batch.accounting[k].cpu().numpy(), solo.accounting[0].cpu().numpy())
+
+
+def test_bed_graph_replay_equals_eager():
+    cfg, x, r = column(500, seed=21)
+    v0 = shake(len(r), 21)
+    eager = BedEngine([BedSpec(cfg, x, r, velocities=v0)])
+    graph = BedEngine([BedSpec(cfg, x, r, velocities=v0)])
+    for rep in range(3):
+        eager.step(200)
+        graph.run_graph(100)
+        torch.cuda.synchronize()
+        assert np.array_equal(eager.x.cpu().numpy(), graph.x.cpu().numpy()), rep
+    assert graph.read_members()[0].step_count == 600

@@ -1,0 +1,171 @@
+"""The FP64 bed oracle against closed forms of its contact law.
+
+No reference output exists for a 2-D Hertz-Mindlin bed, so the law is pinned
+by textbook results: Hertz force and potential, Mindlin stick stiffness,
+Coulomb sliding, restitution, momentum conservation, brute-force contacts.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle.bed import OracleBed, BS_NPAIRS
+from oracle.plate import pairs_within
+from tensordem_amd.bedconfig import BedConfig, column_packing
+
+R = 0.01
+
+
+def free(**kw):
+    base = dict(gravity=0.0, floor=False, back_wall=False)
+    base.update(kw)
+    return BedConfig(**base)
+
+
+def test_static_hertz_force_and_potential():
+    cfg = free()
+    for delta in (1e-6, 3e-5, 2e-4):
+        x = np.array([[0.0, 0.0], [2 * R - delta, 0.0]])
+        b = OracleBed(cfg, x, np.array([R, R]), dt=1e-7)
+        b.step(integrate=False)
+        r_eff = R / 2
+        expect = (4.0 / 3.0) * cfg.e_star * math.sqrt(r_eff) * delta ** 1.5
+        assert b.force[1, 0] == pytest.approx(expect, rel=1e-12)
+        assert b.force[0, 0] == -b.force[1, 0]
+        e = b.energies()["elastic_normal_J"]
+        assert e == pytest.approx((8.0 / 15.0) * cfg.e_star * math.sqrt(r_eff) * delta ** 2.5,
+                                  rel=1e-12)
+
+
+def test_hertz_exponent_from_two_overlaps():
+    """log F vs log delta has slope 3/2 (a dropped or squared term would not)."""
+    cfg = free()
+    f = []
+    for delta in (1e-5, 4e-5):
+        b = OracleBed(cfg, np.array([[0.0, 0.0], [2 * R - delta, 0.0]]), np.array([R, R]), dt=1e-7)
+        b.step(integrate=False)
+        f.append(b.force[1, 0])
+    assert math.log(f[1] / f[0]) / math.log(4.0) == pytest.approx(1.5, rel=1e-12)
+
+
+def test_mindlin_stick_stiffness():
+    """A disk pressed on the floor and moved tangentially for one step stores
+    xi = v dt; the tangential force is s_t xi with s_t = 8 G* sqrt(r d)."""
+    cfg = BedConfig(gravity=0.0, back_wall=False, friction=0.5, wall_friction=10.0)
+    delta, vt, dt = 1e-4, 1e-3, 1e-6
+    b = OracleBed(cfg, np.array([[0.0, R - delta]]), np.array([R]), dt=dt)
+    b.velocities[0] = [vt, 0.0]
+    b.step(integrate=False)
+    s_t = 8 * cfg.g_star * math.sqrt(R * delta)
+    assert b.xi_floor[0, 0] == pytest.approx(vt * dt, rel=1e-15)
+    assert b.force[0, 0] == pytest.approx(-s_t * vt * dt, rel=1e-12)
+    assert 2 * cfg.g_star * 4 / (2 * cfg.e_star) == pytest.approx(
+        2 * (1 - cfg.poisson) / (2 - cfg.poisson), rel=1e-12)
+
+
+def test_coulomb_cap_while_sliding():
+    cfg = BedConfig(back_wall=False, wall_friction=0.3)
+    m = cfg.rho_pi_t * R * R
+    # resting overlap under gravity: (4/3) E* sqrt(r) d^1.5 = m g
+    d = (m * cfg.gravity / ((4.0 / 3.0) * cfg.e_star * math.sqrt(R))) ** (2.0 / 3.0)
+    b = OracleBed(cfg, np.array([[0.0, R - d]]), np.array([R]), dt=2e-6)
+    b.velocities[0] = [0.5, 0.0]
+    for _ in range(200):
+        b.step()
+    b.step(integrate=False)
+    fn = b.force[0, 1] + m * cfg.gravity
+    ft = abs(b.force[0, 0])
+    assert b.velocities[0, 0] > 0.1  # still sliding
+    assert ft == pytest.approx(0.3 * fn, rel=1e-9)
+
+
+def _impact_restitution_rk4(cfg, v0, r, dt=1e-8):
+    """Independent 1-DOF impact: m d'' = -max(K d^1.5 + C sqrt(2 E* sqrt(r d) m) d', 0)
+    with K = (4/3) E* sqrt(r), C = 2 sqrt(5/6) beta, integrated by RK4."""
+    m = cfg.rho_pi_t * r * r
+    K = (4.0 / 3.0) * cfg.e_star * math.sqrt(r)
+    C = 2.0 * math.sqrt(5.0 / 6.0) * cfg.beta
+
+    def acc(d, dd):
+        if d <= 0:
+            return 0.0
+        sn = 2.0 * cfg.e_star * math.sqrt(r * d)
+        f = K * d ** 1.5 + C * math.sqrt(sn * m) * dd
+        return -max(f, 0.0) / m
+
+    d, dd = 0.0, v0
+    while True:
+        k1 = (dd, acc(d, dd))
+        k2 = (dd + 0.5 * dt * k1[1], acc(d + 0.5 * dt * k1[0], dd + 0.5 * dt * k1[1]))
+        k3 = (dd + 0.5 * dt * k2[1], acc(d + 0.5 * dt * k2[0], dd + 0.5 * dt * k2[1]))
+        k4 = (dd + dt * k3[1], acc(d + dt * k3[0], dd + dt * k3[1]))
+        d += dt * (k1[0] + 2 * k2[0] + 2 * k3[0] + k4[0]) / 6
+        dd += dt * (k1[1] + 2 * k2[1] + 2 * k3[1] + k4[1]) / 6
+        if d <= 0 and dd < 0:
+            return -dd / v0
+
+
+def test_restitution_of_a_floor_impact():
+    """The restitution realised by the viscous term (with the non-cohesive clamp,
+    which raises it above the nominal e) equals an independent RK4 integration
+    of the same impact to 1 %, and is within 0.06 of the nominal value."""
+    for e in (0.5, 0.8):
+        cfg = BedConfig(gravity=0.0, back_wall=False, restitution=e, wall_friction=0.0)
+        b = OracleBed(cfg, np.array([[0.0, R + 1e-5]]), np.array([R]), dt=2e-7)
+        b.velocities[0] = [0.0, -0.2]
+        for _ in range(20000):
+            b.step()
+            if b.positions[0, 1] > R + 2e-5 and b.velocities[0, 1] > 0:
+                break
+        e_meas = b.velocities[0, 1] / 0.2
+        e_ode = _impact_restitution_rk4(cfg, 0.2, R)
+        assert e_meas == pytest.approx(e_ode, rel=0.01), (e, e_meas, e_ode)
+        assert abs(e_meas - e) < 0.06
+
+
+def test_momentum_conserved_in_a_free_cluster():
+    rng = np.random.default_rng(4)
+    x, r = column_packing(300, R, seed=4, gap=-0.05)
+    cfg = free()
+    b = OracleBed(cfg, x, r)
+    b.velocities[:] = rng.normal(scale=0.05, size=b.velocities.shape)
+    m = b.masses[:, None]
+    p0 = (m * b.velocities).sum(0)
+    for _ in range(300):
+        b.step()
+    assert b.last[BS_NPAIRS] > 0
+    assert np.abs((m * b.velocities).sum(0) - p0).max() < 1e-13 * np.abs(m * b.velocities).sum()
+
+
+def test_contact_set_equals_brute_force():
+    x, r = column_packing(400, R, seed=9, gap=-0.08)
+    b = OracleBed(free(), x, r)
+    got = {tuple(p) for p in b.contacts().tolist()}
+    # brute force with the per-pair cutoff r_i + r_j
+    cand = pairs_within(x, 2 * r.max())
+    exp = {(int(i), int(j)) for i, j in cand
+           if (r[i] + r[j]) - math.sqrt(((x[j] - x[i]) ** 2).sum()) > 0}
+    assert got == exp and len(got) > 50
+
+
+def test_history_resets_on_separation():
+    cfg = free(friction=0.5)
+    x = np.array([[0.0, 0.0], [2 * R - 1e-5, 0.0]])
+    b = OracleBed(cfg, x, np.array([R, R]), dt=1e-6)
+    b.velocities[1] = [0.0, 0.01]
+    b.step()
+    assert len(b.hist_keys) == 1 and np.abs(b.hist_xi).max() > 0
+    b.positions[1, 0] = 2 * R + 1e-4  # pulled apart
+    b.step()
+    assert len(b.hist_keys) == 0
+
+
+def test_column_packing_is_overlap_free_and_tall():
+    x, r = column_packing(2000, R, seed=1)
+    assert (x[:, 1] - r).min() > 0 and (x[:, 0] - r).min() > 0
+    d = pairs_within(x, 2.4 * R * 1.2)
+    for i, j in d:
+        assert math.sqrt(((x[j] - x[i]) ** 2).sum()) >= r[i] + r[j]
+    h = x[:, 1].max() - x[:, 1].min()
+    w = x[:, 0].max() - x[:, 0].min()
+    assert h / w == pytest.approx(2.0, rel=0.1)
