@@ -52,13 +52,15 @@ struct BedWs {
   double2 *xb;       // [n] position when the list was built
 };
 
+// Carves the workspace at `base`; with base == nullptr only the byte offsets
+// are wanted and the returned pointers hold them (used by the layout query).
 inline void carve(void *base, int n, int nblocks, int ncells, int slots, int cap, BedWs *w,
                   size_t *total)
 {
   size_t off = 0;
-  char *b = static_cast<char *>(base);
+  const uintptr_t b = reinterpret_cast<uintptr_t>(base);
   auto take = [&](size_t bytes) {
-    char *p = b ? b + off : nullptr;
+    char *p = reinterpret_cast<char *>(b + off);
     off += align256(bytes);
     return p;
   };
