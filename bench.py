@@ -1,4 +1,5 @@
-"""Benchmark: particle-steps/s of the FP32-force / FP64-state HIP plate step.
+"""Benchmark: particle-steps/s of the FP32-force / FP64-state HIP plate step,
+and (key "c3_bed") of the Hertz-Mindlin bed step on a 1M-disk column collapse.
 
 Workload (default): the large single ice plate, a 1024 x 128 reference lattice
 (131 072 disks, 520 834 bonds), both ends clamped, circular indenter of
@@ -194,6 +195,149 @@ def time_plain(cfg, dt, steps, warmup, pinv=True):
 
 
 # ---------------------------------------------------------------------------
+# plain FP64 eager PyTorch bed step (the C3 speedup denominator)
+# ---------------------------------------------------------------------------
+class PlainBedFP64:
+    """The Hertz-Mindlin bed step as eager FP64 PyTorch ops on the GPU, the way
+    the reference's floe bed runs it (src/tensordem/scale_ceiling/kernel.py:
+    250-260, 277-401; search.py:60-162): a Verlet list of pairs within
+    r_i + r_j + skin from a sort-based cell list (stable argsort, bincount
+    starts, five half-stencil offsets), rebuilt when some disk has moved more
+    than skin/4 (infinity norm) since the last build; spring history carried
+    over by searchsorted on sorted pair keys i*n+j (granular2d/engine.py:
+    388-396); forces scattered with index_add_; semi-implicit Euler.  Same
+    law, constants and wall handling as the HIP kernel.  Ledger sums stay on
+    the device; the rebuild test is the one host sync per step."""
+
+    def __init__(self, cfg, x, r, dt, skin, v=None, device="cuda"):
+        import torch
+        dev = torch.device(device)
+        f64 = dict(dtype=torch.float64, device=dev)
+        self.cfg, self.dt, self.skin = cfg, float(dt), float(skin)
+        self.x = torch.as_tensor(x, **f64).clone()
+        self.v = torch.zeros_like(self.x) if v is None else torch.as_tensor(v, **f64).clone()
+        self.r = torch.as_tensor(r, **f64)
+        self.n = len(self.r)
+        self.m = cfg.rho_pi_t * (self.r * self.r)
+        self.cell = 2.0 * float(self.r.max()) + self.skin
+        self.keys = torch.zeros(0, dtype=torch.int64, device=dev)
+        self.xi = torch.zeros((0, 2), **f64)
+        self.xi_wall = torch.zeros((2, self.n, 2), **f64)
+        self.acc = torch.zeros(6, **f64)
+        self.rebuilds = 0
+        self._build()
+
+    def _build(self):
+        import torch
+        x, r, n, h = self.x, self.r, self.n, self.cell
+        origin = x.min(0).values - h
+        ij = torch.floor((x - origin) / h).long()
+        ncx = int(ij[:, 0].max()) + 2
+        cid = ij[:, 1] * ncx + ij[:, 0]
+        ncell = (int(ij[:, 1].max()) + 2) * ncx
+        order = torch.argsort(cid, stable=True)
+        counts = torch.bincount(cid, minlength=ncell)
+        starts = torch.cumsum(counts, 0) - counts
+        kmax = int(counts.max())
+        ar = torch.arange(n, device=x.device)
+        pi, pj = [], []
+        for ox, oy in ((0, 0), (1, 0), (-1, 1), (0, 1), (1, 1)):
+            nc = cid + ox + oy * ncx
+            cnt, st = counts[nc], starts[nc]
+            for k in range(kmax):
+                ok = k < cnt
+                j = order[(st + k).clamp(max=n - 1)]
+                if ox == 0 and oy == 0:
+                    ok &= j > ar
+                pi.append(ar[ok])
+                pj.append(j[ok])
+        i, j = torch.cat(pi), torch.cat(pj)
+        lo, hi = torch.minimum(i, j), torch.maximum(i, j)
+        d = torch.linalg.vector_norm(x[hi] - x[lo], dim=1)
+        keep = d < (r[lo] + r[hi]) + self.skin
+        key = torch.sort(lo[keep] * n + hi[keep]).values
+        if len(self.keys):
+            pos = torch.searchsorted(self.keys, key).clamp(max=len(self.keys) - 1)
+            hit = self.keys[pos] == key
+            self.xi = torch.where(hit[:, None], self.xi[pos], torch.zeros_like(self.xi[pos]))
+        else:
+            self.xi = torch.zeros((len(key), 2), dtype=x.dtype, device=x.device)
+        self.keys, self.i, self.j = key, key // n, key % n
+        self.xb = x.clone()
+        self.rebuilds += 1
+
+    def _wall(self, w, gap, nrm):
+        import torch
+        c, r, v, dt = self.cfg, self.r, self.v, self.dt
+        d = r - gap
+        act = d > 0
+        dp = d.clamp(min=0.0)
+        sn = (2 * c.e_star) * torch.sqrt(r * dp)
+        fel = ((2.0 / 3.0) * sn) * dp
+        vn = v @ nrm
+        fn = (fel - (c.damping_coefficient * torch.sqrt(sn * self.m)) * vn).clamp(min=0.0)
+        st = (8 * c.g_star) * torch.sqrt(r * dp)
+        e = self.xi_wall[w] + (v - vn[:, None] * nrm) * dt
+        mag = torch.linalg.vector_norm(e, dim=1)
+        sc = ((c.wall_friction * fn) / (st * mag).clamp(min=1e-300)).clamp(max=1.0)
+        e = torch.where(act[:, None], e * torch.where(mag > 0, sc, 1.0)[:, None], 0.0)
+        self.xi_wall[w] = e
+        f = torch.where(act[:, None], fn[:, None] * nrm - st[:, None] * e, 0.0)
+        self.acc[2] += dt * torch.where(act, (fn - fel) * (-vn), 0.0).sum()
+        self.acc[3] += dt * torch.where(act, st * (e * v).sum(1), 0.0).sum()
+        self.acc[4:6] += dt * f.sum(0)
+        return f
+
+    def step(self):
+        import torch
+        c, dt = self.cfg, self.dt
+        if float((self.x - self.xb).abs().max()) > 0.25 * self.skin:
+            self._build()
+        x, v, r, i, j = self.x, self.v, self.r, self.i, self.j
+        dv = x[j] - x[i]
+        dist = torch.linalg.vector_norm(dv, dim=1)
+        d = (r[i] + r[j]) - dist
+        act = d > 0
+        dp = d.clamp(min=0.0)
+        nrm = dv / dist.clamp(min=1e-300)[:, None]
+        reff = (r[i] * r[j]) / (r[i] + r[j])
+        meff = (self.m[i] * self.m[j]) / (self.m[i] + self.m[j])
+        sn = (2 * c.e_star) * torch.sqrt(reff * dp)
+        fel = ((2.0 / 3.0) * sn) * dp
+        vr = v[i] - v[j]
+        vn = (vr * nrm).sum(1)
+        fn = (fel + (c.damping_coefficient * torch.sqrt(sn * meff)) * vn).clamp(min=0.0)
+        st = (8 * c.g_star) * torch.sqrt(reff * dp)
+        e = self.xi + (vr - vn[:, None] * nrm) * dt
+        mag = torch.linalg.vector_norm(e, dim=1)
+        sc = ((c.friction * fn) / (st * mag).clamp(min=1e-300)).clamp(max=1.0)
+        e = torch.where(act[:, None], e * torch.where(mag > 0, sc, 1.0)[:, None], 0.0)
+        self.xi = e
+        f = torch.where(act[:, None], fn[:, None] * nrm + st[:, None] * e, 0.0)
+        force = torch.zeros_like(x)
+        force[:, 1] = self.m * (-c.gravity)
+        force.index_add_(0, j, f)
+        force.index_add_(0, i, -f)
+        self.acc[0] += dt * torch.where(act, (fn - fel) * vn, 0.0).sum()
+        self.acc[1] += dt * torch.where(act, st * (e * vr).sum(1), 0.0).sum()
+        if c.floor:
+            force += self._wall(0, x[:, 1] - c.floor_y, torch.tensor([0.0, 1.0], dtype=x.dtype,
+                                                                      device=x.device))
+        if c.back_wall:
+            force += self._wall(1, x[:, 0] - c.back_wall_x, torch.tensor([1.0, 0.0], dtype=x.dtype,
+                                                                         device=x.device))
+        self.v = v + (dt * force) / self.m[:, None]
+        self.x = x + dt * self.v
+
+    def contact_pairs(self):
+        import torch
+        x, r, i, j = self.x, self.r, self.i, self.j
+        d = (r[i] + r[j]) - torch.linalg.vector_norm(x[j] - x[i], dim=1)
+        k = self.keys[d > 0]
+        return torch.stack((k // self.n, k % self.n), 1)
+
+
+# ---------------------------------------------------------------------------
 def kernel_event_time(eng, launches):
     """Mean step-kernel duration (s): HIP events recorded by the library on the
     launch stream around each step-kernel launch, continuing the timed run."""
@@ -225,6 +369,145 @@ def copy_calibration(mib=512):
     print(json.dumps({"copy_calibration_bytes": x.numel() * 8, "launches": 3}))
 
 
+# ---------------------------------------------------------------------------
+# C3: unbonded granular column collapse, 1 048 576 disks
+# ---------------------------------------------------------------------------
+BED_N = 1 << 20
+BED_R0 = 0.01
+# compulsory bytes per disk per step of bed_step_kernel, before the list- and
+# contact-dependent terms: x, v read (32) and written (32), radius (8), list
+# origin xb (16), contact mask read + written (8), candidate count (1)
+BED_BYTES_FIXED = 97
+
+
+def bed_bytes_per_disk(cand_mean, contacts_per_disk, wall_per_disk):
+    """+ 4 B per candidate index, + 32 B per pair-history copy (read and
+    written), + 32 B per wall history (read and written)."""
+    return BED_BYTES_FIXED + 4.0 * cand_mean + 32.0 * contacts_per_disk + 32.0 * wall_per_disk
+
+
+def c3_inputs():
+    from tensordem_amd.bedconfig import BedConfig, column_packing
+    x, r = column_packing(BED_N, BED_R0, seed=0)
+    return BedConfig(), x, r
+
+
+def bench_bed(args):
+    """The 1M-disk column collapse: FP32 and FP64 forces over the full step
+    count from release, kernel events with list statistics on a second pass,
+    the plain FP64 path from the half-way state, the oracle on a CPU sample."""
+    import torch
+    from tensordem_amd import _lib
+    from tensordem_amd.bed import BedEngine, BedSpec
+
+    cfg, x, r = c3_inputs()
+    n = len(r)
+    chunk = 1000
+    steps = max(chunk, args.bed_steps - args.bed_steps % chunk)
+    warm = BedEngine([BedSpec(cfg, x, r)], fp32=True)
+    warm.run_graph(chunk // 2)
+    torch.cuda.synchronize()
+    dt = warm.read_members()[0].dt
+    skin = warm.read_members()[0].skin
+    warm.close()
+    del warm
+    out = {}
+    for fp32 in (True, False):
+        eng = BedEngine([BedSpec(cfg, x, r)], fp32=fp32)
+        eng.graph(chunk // 2)
+        stream = torch.cuda.current_stream()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record(stream)
+        eng.run_graph(chunk // 2, repeats=steps // chunk)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        gpu_s = e0.elapsed_time(e1) * 1e-3
+        acc = eng.accounting[0].cpu().numpy()
+        out["fp32" if fp32 else "fp64"] = dict(
+            seconds=gpu_s, ms_per_step=gpu_s / steps * 1e3, rate=n * steps / gpu_s,
+            status=int(eng.status()[0]), contacts_end=int(acc[_lib.BED_CONTACTS]),
+            stats_end=eng.list_stats())
+        eng.close()
+        del eng
+    # kernel-event pass over the same run, sampling the lists for the byte count
+    eng = BedEngine([BedSpec(cfg, x, r)], fp32=True)
+    kus, byts, half = [], [], None
+    for k in range(steps // chunk):
+        kus.append(eng.step_timed(chunk))
+        if k % 5 == 4:
+            st = eng.list_stats()
+            acc = eng.accounting[0].cpu().numpy()
+            byts.append(bed_bytes_per_disk(st["cand_mean"], st["contacts_per_disk"],
+                                           acc[_lib.BED_WALL_CONTACTS] / n))
+        if k == steps // chunk // 2 - 1:
+            half = (eng.x.cpu().numpy(), eng.v.cpu().numpy())
+    kernel_s = sum(kus) / len(kus) * 1e-6
+    eng.close()
+    del eng
+    bpd = sum(byts) / len(byts)
+    achieved = bpd * n / kernel_s / 1e9
+    r32, r64 = out["fp32"], out["fp64"]
+    res = {
+        "workload": f"granular column collapse, {n} disks, radii U[0.8,1.2]x{BED_R0} m, "
+                    "E=1 GPa, nu=0.3, mu=0.5, e=0.5 (viscous), rho=917, t=0.02 m, "
+                    f"aspect 2 loose column (seed 0), floor + back wall, gravity, "
+                    f"dt={dt:.6g} s, {steps} steps from release",
+        "value": r32["rate"], "unit": "particle-steps/s", "steps": steps,
+        "ms_per_step": r32["ms_per_step"], "dtype": "f32-force/f64-state",
+        "skin_m": skin, "graph_chunk_steps": chunk,
+        "status_bits": r32["status"], "contact_pairs_end": r32["contacts_end"],
+        "lists_end": r32["stats_end"],
+        "fp64_kernel": {"value": r64["rate"], "ms_per_step": r64["ms_per_step"],
+                        "status_bits": r64["status"], "contact_pairs_end": r64["contacts_end"]},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "bed_step_kernel<float>", "kernel_us": kernel_s * 1e6,
+                     "bytes_per_disk": bpd, "bytes_per_launch": bpd * n},
+    }
+    for name, rr in out.items():
+        if rr["status"]:
+            print(f"bench.py: bed {name} run ended with status bits {rr['status']}; its rate "
+                  "is not a valid measurement", file=sys.stderr)
+    if not args.no_plain:
+        p = PlainBedFP64(cfg, half[0], r, dt, skin, v=half[1])
+        for _ in range(3):
+            p.step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.bed_plain_steps):
+            p.step()
+        torch.cuda.synchronize()
+        ps = (time.perf_counter() - t0) / args.bed_plain_steps
+        res["plain_fp64_gpu"] = dict(
+            value=n / ps, unit="particle-steps/s", ms_per_step=ps * 1e3,
+            steps=args.bed_plain_steps, list_rebuilds=p.rebuilds - 1,
+            what="eager PyTorch FP64 bed step on the same GPU (sort-based cell list, Verlet "
+                 "list rebuilt at |dx|inf > skin/4, searchsorted history remap, index_add_), "
+                 f"from the state at step {steps // 2}")
+        res["speedup_vs_plain_fp64"] = r32["rate"] * ps / n
+        del p
+    if not args.no_cpu:
+        res["cpu_baseline"] = cpu_baseline_bed(cfg, x, r, dt, args.cpu_seconds)
+    return res
+
+
+def cpu_baseline_bed(cfg, x, r, dt, budget_s):
+    """C oracle (FP64, one core) on the same 1M-disk column from release."""
+    from oracle.bed import OracleBed
+    sim = OracleBed(cfg, x, r, dt=dt)
+    sim.step()
+    t0 = time.perf_counter()
+    steps = 0
+    while time.perf_counter() - t0 < budget_s:
+        sim.step()
+        steps += 1
+    el = time.perf_counter() - t0
+    return dict(value=len(r) * steps / el, unit="particle-steps/s", cores=1, kind="port",
+                sample=f"{steps} steps of the {len(r)}-disk column from release (FP64 C oracle, "
+                       f"cell grid rebuilt every step), {el:.1f} s")
+
+
 def cpu_baseline(cfg, dt, budget_s=12.0):
     """C oracle (FP64, one core) on the same plate, run for ~budget_s."""
     from oracle.plate import OraclePlate
@@ -254,6 +537,10 @@ def main(argv=None):
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--workload", choices=("all", "plate", "bed"), default="all",
+                    help="plate: the 1024x128 plate only; bed: the 1M-disk column only")
+    ap.add_argument("--bed-steps", type=int, default=50000)
+    ap.add_argument("--bed-plain-steps", type=int, default=60)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a 512 MiB device copy (for counter calibration)")
     args = ap.parse_args(argv)
@@ -264,6 +551,27 @@ def main(argv=None):
         print("bench.py measures one GPU: every plate and request of this workload runs on a "
               "single MI355X and no multi-GPU path is built (see DESIGN.md).", file=sys.stderr)
         return 2
+    if args.workload == "bed":
+        bed = bench_bed(args)
+        line = {"metric": METRIC, "value": bed.pop("value"), "unit": "particle-steps/s",
+                "n_gpus": 1, "steps": bed["steps"], "warmup": 1000,
+                "ms_per_step": bed["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+                "vs_baseline": None, "dtype": bed.pop("dtype"),
+                "data": "synthetic: seeded loose column packing",
+                "config": {"workload": bed.pop("workload")}}
+        line.update(bed)
+        print(json.dumps(line))
+        return 0
+    line = bench_plate(args)
+    if args.workload == "all":
+        line["c3_bed"] = bench_bed(args)
+    print(json.dumps(line))
+    return 0
+
+
+def bench_plate(args):
+    """The 1024 x 128 plate: FP32 and FP64 forces over graph-captured steps,
+    kernel events on a second pass, the plain FP64 path, the CPU oracle."""
     import torch
     from tensordem_amd.engine import PlateEngine, PlateSpec
 
@@ -349,8 +657,7 @@ def main(argv=None):
         line["speedup_vs_plain_fp64"] = r32["rate"] / plain["value"]
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(c2_config("cpu"), dt, args.cpu_seconds)
-    print(json.dumps(line))
-    return 0
+    return line
 
 
 if __name__ == "__main__":

@@ -19,8 +19,10 @@
  *     column_collapse.py:236-266: no gate wall once released), gravity;
  *   - semi-implicit Euler v += dt F / m, x += dt v as IceDEM.step
  *     (dem.py:363-365).
- * Per disk the force is gravity, then pair forces (partners above by
- * ascending index, then partners below), then floor, then back wall.
+ * Per disk the force is gravity, then pair forces by ascending partner
+ * index, then floor, then back wall.  (The reference's scatter order,
+ * torch index_add_, is not specified; this order is the one the GPU kernel
+ * follows so that FP64 runs compare bit for bit.)
  * Build with -ffp-contract=off.
  */
 #include <math.h>
@@ -108,12 +110,11 @@ static bpair *bed_pairs(int n, const double *x, const double *r, int *count)
   return out;
 }
 
-typedef struct { int32_t partner, group; double fx, fy; } bcontrib;
+typedef struct { int32_t partner; double fx, fy; } bcontrib;
 
 static int cmp_bcontrib(const void *a, const void *b)
 {
   const bcontrib *u = (const bcontrib *)a, *v = (const bcontrib *)b;
-  if (u->group != v->group) return u->group - v->group;
   return u->partner - v->partner;
 }
 
@@ -214,9 +215,9 @@ int orb_step(const double *prm, int n, double *x, double *v, const double *r, in
   memcpy(fill, deg, sizeof(int32_t) * (size_t)n);
   for (int t = 0; t < np; ++t) {
     bcontrib *a = &cb[fill[pr[t].i]++];
-    a->partner = pr[t].j; a->group = 0; a->fx = -pfx[t]; a->fy = -pfy[t];
+    a->partner = pr[t].j; a->fx = -pfx[t]; a->fy = -pfy[t];
     bcontrib *b = &cb[fill[pr[t].j]++];
-    b->partner = pr[t].i; b->group = 1; b->fx = pfx[t]; b->fy = pfy[t];
+    b->partner = pr[t].i; b->fx = pfx[t]; b->fy = pfy[t];
   }
   int bad = 0;
   for (int p = 0; p < n; ++p) {

@@ -92,7 +92,8 @@ class BedArrays(ctypes.Structure):
     ]
 
 
-BED_NACC = 8
+BED_DASHPOT, BED_FRICTION, BED_WALL_DASHPOT, BED_WALL_FRICTION, BED_WALL_IMPULSE = 0, 1, 2, 3, 4
+BED_CONTACTS, BED_WALL_CONTACTS, BED_NACC = 6, 7, 8
 
 # every entry point declared in include/tensordem_amd.h, with its ctypes signature
 _SIGNATURES = {

@@ -176,6 +176,24 @@ class BedEngine:
     def status(self) -> np.ndarray:
         return np.array([m.status for m in self.read_members()], dtype=np.int32)
 
+    def _ws_view(self, which: int, dtype, shape):
+        off = self._ws_base() - self.workspace.data_ptr() + self._layout[which]
+        nbytes = int(np.prod(shape)) * torch.empty((), dtype=dtype).element_size()
+        return self.workspace[off: off + nbytes].view(dtype).view(*shape)
+
+    def list_stats(self) -> dict:
+        """Mean / max candidate-list length and mean pair contacts per disk
+        (both copies of a pair counted) of the current lists, over the batch."""
+        ms = self.read_members()
+        lp = torch.tensor([m.list_parity for m in ms], device=self.device)
+        sel = torch.repeat_interleave(lp, torch.tensor([m.n for m in ms], device=self.device))
+        idx = torch.arange(self.n, device=self.device)
+        nc = self._ws_view(3, torch.uint8, (2, self.n))[sel, idx].float()
+        mk = self._ws_view(2, torch.int32, (2, self.n))[sel, idx]
+        pc = sum(((mk >> b) & 1).sum() for b in range(self.cand_slots)).item()
+        return dict(cand_mean=nc.mean().item(), cand_max=int(nc.max().item()),
+                    contacts_per_disk=pc / self.n)
+
     def contact_pairs(self) -> np.ndarray:
         """Pairs (i<j) in contact during the last step, from the kernels' own
         per-slot contact masks (sorted)."""

@@ -18,10 +18,12 @@
 // lists into the spare buffer and carries every active history over by
 // merging the old and new partner lists (sorted-key remap).
 //
-// Per disk the force is gravity, then the pair forces (partners above by
-// ascending index, then partners below), then floor, then back wall -- the
-// order of the FP64 oracle (oracle/bed2d.c), so FP64 runs are bit-identical
-// to it.
+// Per disk the force is gravity, then the pair forces by ascending partner
+// index, then floor, then back wall -- the order of the FP64 oracle
+// (oracle/bed2d.c), so FP64 runs are bit-identical to it.  A disk's
+// candidate indices are prefetched into registers and its partners' state is
+// loaded kChunk at a time, so one wave waits on one gather per chunk rather
+// than one per partner.
 #include "common.h"
 
 namespace {
@@ -29,6 +31,9 @@ namespace {
 using namespace tdm;
 
 constexpr int kMaxSlots = 16;
+constexpr int kChunk = 2;        // partners gathered per round
+constexpr int kFinThreads = 1024;
+constexpr int kNch = 12;         // ledger channels written per block
 
 enum { MODE_IDLE = 0, MODE_SEARCH = 1, MODE_BIN = 2 };
 enum { OP_STEP = 1, OP_ENERGY = 2 };
@@ -41,7 +46,7 @@ constexpr uint32_t kFloorBit = 1u << 16, kBackBit = 1u << 17;
 
 struct BedWs {
   int4 *blocks;      // (member, first disk, count, rank)
-  double *partials;  // [nblocks][kCh]
+  double *partials;  // [kNch][nblocks], channel-major
   int *bflags;       // [nblocks]
   int *cell_count;   // [2][ncells]
   int *cell_tab;     // [2][ncells][cap]
@@ -65,7 +70,7 @@ inline void carve(void *base, int n, int nblocks, int ncells, int slots, int cap
     return p;
   };
   w->blocks = reinterpret_cast<int4 *>(take(sizeof(int4) * size_t(nblocks)));
-  w->partials = reinterpret_cast<double *>(take(sizeof(double) * kCh * size_t(nblocks)));
+  w->partials = reinterpret_cast<double *>(take(sizeof(double) * kNch * size_t(nblocks)));
   w->bflags = reinterpret_cast<int *>(take(sizeof(int) * size_t(nblocks)));
   w->cell_count = reinterpret_cast<int *>(take(sizeof(int) * 2 * size_t(ncells)));
   w->cell_tab = reinterpret_cast<int *>(take(sizeof(int) * 2 * size_t(ncells) * size_t(cap)));
@@ -235,6 +240,7 @@ template <typename R>
 __global__ __launch_bounds__(kThreads) void bed_step_kernel(BKArgs k)
 {
   __shared__ double red[4][kCh];
+  static_assert(kNch <= kCh, "block_sum stages at most kCh channels");
   __shared__ tdm_bed_member sm;
   __shared__ int sflags[2];
   const tdm_bed_arrays &a = k.a;
@@ -259,9 +265,9 @@ __global__ __launch_bounds__(kThreads) void bed_step_kernel(BKArgs k)
   __syncthreads();
   const tdm_bed_member &m = sm;
   const int mode = m.mode, lp = m.list_parity;
-  double ch[kCh];
+  double ch[kNch];
 #pragma unroll
-  for (int c = 0; c < kCh; ++c) ch[c] = 0.0;
+  for (int c = 0; c < kNch; ++c) ch[c] = 0.0;
   int status = 0, req = 0;
 
   if (valid) {
@@ -338,15 +344,19 @@ __global__ __launch_bounds__(kThreads) void bed_step_kernel(BKArgs k)
       const double lim = 0.5 * m.skin * 0.999;
       if (ddx * ddx + ddy * ddy >= lim * lim) status |= TDM_ST_STALE_LIST;
     }
-    int split = 0;
-    while (split < nc && cand[(size_t)split * n] < p) ++split;
+    // candidate indices in registers (slots past nc are not fetched)
+    int cq[kMaxSlots];
+#pragma unroll
+    for (int e = 0; e < kMaxSlots; ++e) cq[e] = (e < nc) ? cand[(size_t)e * n] : p;
 
     if (k.op == OP_ENERGY) {
       // kinetic, gravity, Hertz normal (2/5 f_el d) and spring (s_t |xi|^2 / 2)
       ch[C_KIN] = 0.5 * mass * (vp.x * vp.x + vp.y * vp.y);
       ch[C_POT] = -mass * m.gravity_y * xp.y;
-      for (int e = split; e < nc; ++e) {
-        const int q = cand[(size_t)e * n];
+#pragma unroll
+      for (int e = 0; e < kMaxSlots; ++e) {
+        const int q = cq[e];
+        if (e >= nc || q < p) continue;
         const double2 xq = k.x_in[q];
         const double rq = a.radius[q];
         const double dx = xq.x - xp.x, dy = xq.y - xp.y;
@@ -374,22 +384,38 @@ __global__ __launch_bounds__(kThreads) void bed_step_kernel(BKArgs k)
       }
       *mask_out = mask;
     } else {
-      // ---- forces: gravity, pairs above p, pairs below p, floor, back wall
+      // ---- forces: gravity, pairs by ascending partner, floor, back wall
       double fx = 0.0, fy = mass * m.gravity_y;
       uint32_t nmask = 0;
-      for (int pass = 0; pass < 2; ++pass) {
-        const int e0 = pass == 0 ? split : 0, e1 = pass == 0 ? nc : split;
-        for (int e = e0; e < e1; ++e) {
-          const int q = cand[(size_t)e * n];
-          const bool self_lo = pass == 0;  // partners above p come first
-          const double2 xq = k.x_in[q], vq = k.v_in[q];
-          const double rq = a.radius[q];
-          double2 h = ((mask >> e) & 1u) ? xis[(size_t)e * n] : make_double2(0.0, 0.0);
+#pragma unroll
+      for (int c0 = 0; c0 < kMaxSlots; c0 += kChunk) {
+        if (!__any(c0 < nc)) break;
+        double2 xq[kChunk], vq[kChunk], hq[kChunk];
+        double rq[kChunk];
+#pragma unroll
+        for (int j = 0; j < kChunk; ++j) {
+          const int e = c0 + j;
+          xq[j] = vq[j] = hq[j] = make_double2(0.0, 0.0);
+          rq[j] = 1.0;
+          if (e < nc) {
+            const int q = cq[e];
+            xq[j] = k.x_in[q];
+            vq[j] = k.v_in[q];
+            rq[j] = a.radius[q];
+            if ((mask >> e) & 1u) hq[j] = xis[(size_t)e * n];
+          }
+        }
+#pragma unroll
+        for (int j = 0; j < kChunk; ++j) {
+          const int e = c0 + j;
+          if (e >= nc) continue;
+          const bool self_lo = cq[e] > p;  // the pair law is evaluated lo -> hi
+          const double2 xl = self_lo ? xp : xq[j], xh = self_lo ? xq[j] : xp;
+          const double2 vl = self_lo ? vp : vq[j], vh = self_lo ? vq[j] : vp;
+          const double rl = self_lo ? rp : rq[j], rh = self_lo ? rq[j] : rp;
+          double2 h = hq[j];
           double Fx, Fy, dash, fric;
-          const bool hit = self_lo
-              ? hm_pair<R>(m, xp, xq, vp, vq, rp, rq, h, Fx, Fy, dash, fric)
-              : hm_pair<R>(m, xq, xp, vq, vp, rq, rp, h, Fx, Fy, dash, fric);
-          if (!hit) continue;
+          if (!hm_pair<R>(m, xl, xh, vl, vh, rl, rh, h, Fx, Fy, dash, fric)) continue;
           xis[(size_t)e * n] = h;
           nmask |= 1u << e;
           if (self_lo) {
@@ -464,45 +490,61 @@ __global__ __launch_bounds__(kThreads) void bed_step_kernel(BKArgs k)
   block_sum(ch, red);
   if (threadIdx.x == 0) {
     const size_t slot = (size_t)(m.block_first + blk.w);
-    double *dst = k.w.partials + slot * kCh;
 #pragma unroll
-    for (int c = 0; c < kCh; ++c) dst[c] = ch[c];
+    for (int c = 0; c < kNch; ++c) k.w.partials[(size_t)c * a.nblocks + slot] = ch[c];
     k.w.bflags[slot] = sflags[0] | (sflags[1] << 16);
   }
 }
 
-__global__ __launch_bounds__(kThreads) void bed_finalize_kernel(BKArgs k)
+// One workgroup per member: every thread loads its share of the member's
+// block partials (up to 4 blocks x 8 channels) in one round, then a fixed
+// DPP tree per wave and the 16 wave totals in order; the order depends only
+// on the member's block count.
+__global__ __launch_bounds__(kFinThreads) void bed_finalize_kernel(BKArgs k)
 {
+  constexpr int kU = 4, kC = 8;
   __shared__ int sflag;
-  __shared__ double seg[16][kCh + 1];
-  __shared__ double stot[kCh];
+  __shared__ double wsum[kFinThreads / 64][kC];
+  __shared__ double stot[kC];
   const tdm_bed_arrays &a = k.a;
   const int mi = blockIdx.x;
   tdm_bed_member *mp = a.members + mi;
   const int b0 = mp->block_first, nb = mp->nblocks;
+  const int c_lo = k.op == OP_ENERGY ? C_KIN : 0, c_n = k.op == OP_ENERGY ? 4 : 8;
   if (threadIdx.x == 0) sflag = 0;
   __syncthreads();
-  const int c = threadIdx.x & 15, r0 = threadIdx.x >> 4;
-  double sum = 0.0;
-  for (int b = r0; b < nb; b += 16 * 8) {
-    double v[8];
+  double acc[kC];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      const int bb = b + 16 * u;
-      v[u] = bb < nb ? k.w.partials[(size_t)(b0 + bb) * kCh + c] : 0.0;
+  for (int c = 0; c < kC; ++c) acc[c] = 0.0;
+  int fl = 0;
+  for (int base = 0; base < nb; base += kU * kFinThreads) {
+    double v[kU][kC];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const int b = base + u * kFinThreads + (int)threadIdx.x;
+#pragma unroll
+      for (int c = 0; c < kC; ++c)
+        v[u][c] = (b < nb && c < c_n) ? k.w.partials[(size_t)(c_lo + c) * a.nblocks + b0 + b] : 0.0;
+      if (b < nb) fl |= k.w.bflags[b0 + b];
     }
 #pragma unroll
-    for (int u = 0; u < 8; ++u) sum += v[u];
+    for (int u = 0; u < kU; ++u)
+#pragma unroll
+      for (int c = 0; c < kC; ++c) acc[c] += v[u][c];
   }
-  seg[r0][c] = sum;
-  int fl = 0;
-  for (int b = threadIdx.x; b < nb; b += kThreads) fl |= k.w.bflags[b0 + b];
   if (fl) atomicOr(&sflag, fl);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int c = 0; c < kC; ++c) {
+    double t = 0.0;
+    if (__ballot(acc[c] != 0.0)) t = wave_sum(acc[c]);
+    if (lane == 0) wsum[wave][c] = t;
+  }
   __syncthreads();
-  if (threadIdx.x < kCh) {
+  if (threadIdx.x < kC) {
     double t = 0.0;
 #pragma unroll
-    for (int r = 0; r < 16; ++r) t += seg[r][threadIdx.x];
+    for (int w = 0; w < kFinThreads / 64; ++w) t += wsum[w][threadIdx.x];
     stot[threadIdx.x] = t;
   }
   __syncthreads();
@@ -513,10 +555,10 @@ __global__ __launch_bounds__(kThreads) void bed_finalize_kernel(BKArgs k)
   if (m.mode == MODE_SEARCH) new_lp ^= 1;
   if (k.op == OP_ENERGY) {
     double *en = a.energy + (size_t)mi * 4;
-    en[0] = stot[C_KIN];
-    en[1] = stot[C_POT];
-    en[2] = stot[C_ELN];
-    en[3] = stot[C_ELT];
+    en[0] = stot[C_KIN - C_KIN];
+    en[1] = stot[C_POT - C_KIN];
+    en[2] = stot[C_ELN - C_KIN];
+    en[3] = stot[C_ELT - C_KIN];
     if (m.mode == MODE_SEARCH) new_mode = MODE_IDLE;
   } else {
     double *acc = a.accounting + (size_t)mi * TDM_BED_NACC;
@@ -608,7 +650,7 @@ int launch_bed(const tdm_bed_arrays *a, const BedWs &w, int parity, int fp32, in
     bed_step_kernel<double><<<a->nblocks, kThreads, 0, s>>>(k);
   }
   HIP_TRY(hipGetLastError());
-  bed_finalize_kernel<<<a->nmembers, kThreads, 0, s>>>(k);
+  bed_finalize_kernel<<<a->nmembers, kFinThreads, 0, s>>>(k);
   HIP_TRY(hipGetLastError());
   return TDM_OK;
 }
