@@ -427,7 +427,7 @@ def bench_bed(args):
         out["fp32" if fp32 else "fp64"] = dict(
             seconds=gpu_s, ms_per_step=gpu_s / steps * 1e3, rate=n * steps / gpu_s,
             status=int(eng.status()[0]), contacts_end=int(acc[_lib.BED_CONTACTS]),
-            stats_end=eng.list_stats())
+            stats_end=eng.list_stats(), rebuilds=int(eng.read_members()[0].rebuilds))
         eng.close()
         del eng
     # kernel-event pass over the same run, sampling the lists for the byte count
@@ -457,7 +457,7 @@ def bench_bed(args):
         "ms_per_step": r32["ms_per_step"], "dtype": "f32-force/f64-state",
         "skin_m": skin, "graph_chunk_steps": chunk,
         "status_bits": r32["status"], "contact_pairs_end": r32["contacts_end"],
-        "lists_end": r32["stats_end"],
+        "lists_end": r32["stats_end"], "list_rebuilds": r32["rebuilds"],
         "fp64_kernel": {"value": r64["rate"], "ms_per_step": r64["ms_per_step"],
                         "status_bits": r64["status"], "contact_pairs_end": r64["contacts_end"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

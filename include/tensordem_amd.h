@@ -235,7 +235,8 @@ typedef struct tdm_bed_member {
   int32_t use_floor, use_back_wall;
   int32_t ncx, ncy, cell_first;
   int32_t block_first, nblocks;
-  int32_t mode, tab, list_parity, status, pad_;
+  int32_t mode, tab, list_parity, status;
+  int32_t rebuilds;             /* candidate-list rebuilds so far */
 } tdm_bed_member;
 
 typedef struct tdm_bed_arrays {

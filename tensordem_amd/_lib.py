@@ -76,7 +76,7 @@ class BedMember(ctypes.Structure):
         ("use_back_wall", ctypes.c_int32), ("ncx", ctypes.c_int32), ("ncy", ctypes.c_int32),
         ("cell_first", ctypes.c_int32), ("block_first", ctypes.c_int32),
         ("nblocks", ctypes.c_int32), ("mode", ctypes.c_int32), ("tab", ctypes.c_int32),
-        ("list_parity", ctypes.c_int32), ("status", ctypes.c_int32), ("pad_", ctypes.c_int32),
+        ("list_parity", ctypes.c_int32), ("status", ctypes.c_int32), ("rebuilds", ctypes.c_int32),
     ]
 
 

@@ -32,11 +32,13 @@ using namespace tdm;
 
 constexpr int kMaxSlots = 16;
 constexpr int kChunk = 2;        // partners gathered per round
+constexpr int kPre = 4;          // candidate slots fetched before the list length is known
 constexpr int kFinThreads = 1024;
 constexpr int kNch = 12;         // ledger channels written per block
 
 enum { MODE_IDLE = 0, MODE_SEARCH = 1, MODE_BIN = 2 };
 enum { OP_STEP = 1, OP_ENERGY = 2 };
+constexpr int kLpBit = 1 << 16;  // list parity, mirrored into each block-table entry
 // ledger partial channels
 enum {
   C_DASH = 0, C_FRIC, C_WDASH, C_WFRIC, C_WFX, C_WFY, C_NPAIR, C_NWALL,
@@ -45,7 +47,7 @@ enum {
 constexpr uint32_t kFloorBit = 1u << 16, kBackBit = 1u << 17;
 
 struct BedWs {
-  int4 *blocks;      // (member, first disk, count, rank)
+  int4 *blocks;      // (member, first disk, count | list parity << 16, rank)
   double *partials;  // [kNch][nblocks], channel-major
   int *bflags;       // [nblocks]
   int *cell_count;   // [2][ncells]
@@ -90,12 +92,56 @@ struct BKArgs {
   int op;
 };
 
+// The member constants one step reads, moved to scalar registers (every lane
+// of a workgroup belongs to the same member).
+struct BedConst {
+  double e_star, g_star, damping, friction, wall_friction, rho_pi_t, gravity_y, dt;
+  double floor_y, back_wall_x, skin, cell_size, ox, oy;
+  int use_floor, use_back_wall, ncx, ncy, cell_first, block_first, nblocks, mode, tab;
+};
+
+__device__ inline double uniform(double v)
+{
+  return __hiloint2double(__builtin_amdgcn_readfirstlane(__double2hiint(v)),
+                          __builtin_amdgcn_readfirstlane(__double2loint(v)));
+}
+__device__ inline int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+__device__ inline BedConst load_const(const tdm_bed_member *mp)
+{
+  BedConst c;
+  c.e_star = uniform(mp->e_star);
+  c.g_star = uniform(mp->g_star);
+  c.damping = uniform(mp->damping);
+  c.friction = uniform(mp->friction);
+  c.wall_friction = uniform(mp->wall_friction);
+  c.rho_pi_t = uniform(mp->rho_pi_t);
+  c.gravity_y = uniform(mp->gravity_y);
+  c.dt = uniform(mp->dt);
+  c.floor_y = uniform(mp->floor_y);
+  c.back_wall_x = uniform(mp->back_wall_x);
+  c.skin = uniform(mp->skin);
+  c.cell_size = uniform(mp->cell_size);
+  c.ox = uniform(mp->grid_origin[0]);
+  c.oy = uniform(mp->grid_origin[1]);
+  c.use_floor = uniform(mp->use_floor);
+  c.use_back_wall = uniform(mp->use_back_wall);
+  c.ncx = uniform(mp->ncx);
+  c.ncy = uniform(mp->ncy);
+  c.cell_first = uniform(mp->cell_first);
+  c.block_first = uniform(mp->block_first);
+  c.nblocks = uniform(mp->nblocks);
+  c.mode = uniform(mp->mode);
+  c.tab = uniform(mp->tab);
+  return c;
+}
+
 // One pair p-q seen from p, lo -> hi oriented.  Returns false without
 // touching anything when the disks do not overlap (decided in FP64 for both
 // precisions).  On contact: F is the force on hi (lo receives -F), xi is
 // updated in place, and the owner's ledger channels are filled.
 template <typename R>
-__device__ inline bool hm_pair(const tdm_bed_member &m, double2 x_lo, double2 x_hi,
+__device__ inline bool hm_pair(const BedConst &m, double2 x_lo, double2 x_hi,
                                double2 v_lo, double2 v_hi, double r_lo, double r_hi, double2 &xi,
                                double &Fx, double &Fy, double &dash, double &fric)
 {
@@ -168,7 +214,7 @@ __device__ inline bool hm_pair(const tdm_bed_member &m, double2 x_lo, double2 x_
 
 // Wall contact along the inward unit normal (nx, ny) at distance `gap`.
 template <typename R>
-__device__ inline bool hm_wall(const tdm_bed_member &m, double r, double mass, double gap,
+__device__ inline bool hm_wall(const BedConst &m, double r, double mass, double gap,
                                double nx, double ny, double2 v, double2 &xi, double &Fx,
                                double &Fy, double &dash, double &fric)
 {
@@ -224,275 +270,320 @@ __device__ inline bool hm_wall(const tdm_bed_member &m, double r, double mass, d
   return true;
 }
 
-__device__ inline int cell_of(const tdm_bed_member &m, double2 x, int *cx, int *cy)
+// Cell of a position, clamped into the member's grid (non-finite -> cell 0).
+__device__ inline int cell_of(double ox, double oy, double h, int ncx, int ncy, int first,
+                              double2 x, int *cx, int *cy)
 {
-  int ix = (int)floor((x.x - m.grid_origin[0]) / m.cell_size);
-  int iy = (int)floor((x.y - m.grid_origin[1]) / m.cell_size);
+  int ix = (int)floor((x.x - ox) / h);
+  int iy = (int)floor((x.y - oy) / h);
   if (!(x.x == x.x) || !(x.y == x.y)) { ix = 0; iy = 0; }
-  ix = min(max(ix, 0), m.ncx - 1);
-  iy = min(max(iy, 0), m.ncy - 1);
+  ix = min(max(ix, 0), ncx - 1);
+  iy = min(max(iy, 0), ncy - 1);
   *cx = ix;
   *cy = iy;
-  return m.cell_first + iy * m.ncx + ix;
+  return first + iy * ncx + ix;
+}
+
+// Rebuild of disk p's candidate list (the step after the hash was filled):
+// all disks within r_p + r_q + skin in the 3x3 cells around p, ascending,
+// written to list buffer lp^1, with the spring history of every pair that
+// was in contact carried over by a sorted merge of the old and new lists.
+// Returns the new length and mask; *status gets CAND_OVERFLOW if S is short.
+__device__ inline int rebuild_list(const BKArgs &k, const BedConst &m, int p, double2 xp,
+                                   double rp, int lp, int nc, uint32_t mask, uint32_t *nmask_out,
+                                   int *status)
+{
+  const tdm_bed_arrays &a = k.a;
+  const int n = a.n, S = a.cand_slots;
+  const size_t sn = (size_t)S * n;
+  const int *cand = k.w.cand + lp * sn + p;
+  const double2 *xis = k.w.xi + lp * sn + p;
+  int *ncl = k.w.cand + (lp ^ 1) * sn + p;
+  double2 *nxi = k.w.xi + (lp ^ 1) * sn + p;
+  int nn = 0;
+  int cx, cy;
+  cell_of(m.ox, m.oy, m.cell_size, m.ncx, m.ncy, m.cell_first, xp, &cx, &cy);
+  const int *cntab = k.w.cell_count + (size_t)m.tab * a.ncells;
+  const int *tab = k.w.cell_tab + (size_t)m.tab * a.ncells * a.cell_cap;
+  int ccount[9];
+#pragma unroll
+  for (int o = 0; o < 9; ++o) {
+    const int gx = cx + o % 3 - 1, gy = cy + o / 3 - 1;
+    const bool in = gx >= 0 && gx < m.ncx && gy >= 0 && gy < m.ncy;
+    ccount[o] = in ? min(cntab[m.cell_first + gy * m.ncx + gx], a.cell_cap) : 0;
+  }
+#pragma unroll
+  for (int o = 0; o < 9; ++o) {
+    const int cn = ccount[o];
+    if (cn == 0) continue;
+    const int cell = m.cell_first + (cy + o / 3 - 1) * m.ncx + (cx + o % 3 - 1);
+    for (int e = 0; e < cn; ++e) {
+      const int q = tab[(size_t)cell * a.cell_cap + e];
+      if (q == p) continue;
+      const double2 xq = k.x_in[q];
+      const double rq = a.radius[q];
+      const double dx = xq.x - xp.x, dy = xq.y - xp.y;
+      if (!(sqrt(dx * dx + dy * dy) < (rp + rq) + m.skin)) continue;
+      if (nn == S) { *status |= TDM_ST_CAND_OVERFLOW; continue; }
+      int pos = nn++;
+      while (pos > 0 && ncl[(size_t)(pos - 1) * n] > q) {
+        ncl[(size_t)pos * n] = ncl[(size_t)(pos - 1) * n];
+        --pos;
+      }
+      ncl[(size_t)pos * n] = q;
+    }
+  }
+  uint32_t nmask = mask & (kFloorBit | kBackBit);
+  int eo = 0;
+  for (int e = 0; e < nn; ++e) {
+    const int q = ncl[(size_t)e * n];
+    double2 h = make_double2(0.0, 0.0);
+    while (eo < nc && cand[(size_t)eo * n] < q) ++eo;
+    if (eo < nc && cand[(size_t)eo * n] == q && ((mask >> eo) & 1u)) {
+      h = xis[(size_t)eo * n];
+      nmask |= 1u << e;
+    }
+    nxi[(size_t)e * n] = h;
+  }
+  k.w.ncand[(size_t)(lp ^ 1) * n + p] = (uint8_t)nn;
+  *nmask_out = nmask;
+  return nn;
 }
 
 template <typename R>
 __global__ __launch_bounds__(kThreads) void bed_step_kernel(BKArgs k)
 {
   __shared__ double red[4][kCh];
-  static_assert(kNch <= kCh, "block_sum stages at most kCh channels");
-  __shared__ tdm_bed_member sm;
   __shared__ int sflags[2];
   const tdm_bed_arrays &a = k.a;
   const int4 blk = k.w.blocks[blockIdx.x];
-  const int mi = blk.x, p0 = blk.y, cnt = blk.z;
+  const int mi = blk.x, p0 = blk.y, cnt = blk.z & 0xFFFF, lp0 = (blk.z & kLpBit) ? 1 : 0;
   const bool valid = (int)threadIdx.x < cnt;
   const int p = p0 + (valid ? (int)threadIdx.x : 0);
   const int n = a.n, S = a.cand_slots;
-  double2 xp = make_double2(0.0, 0.0), vp = make_double2(0.0, 0.0), xb = make_double2(0.0, 0.0);
-  double rp = 0.0;
-  if (valid) {
-    xp = k.x_in[p];
-    vp = k.v_in[p];
-    rp = a.radius[p];
-    xb = k.w.xb[p];
-  }
-  static_assert(sizeof(tdm_bed_member) % 8 == 0, "member record must be 8-byte sized");
-  if (threadIdx.x < sizeof(tdm_bed_member) / 8)
-    reinterpret_cast<uint64_t *>(&sm)[threadIdx.x] =
-        reinterpret_cast<const uint64_t *>(a.members + mi)[threadIdx.x];
-  if (threadIdx.x < 2) sflags[threadIdx.x] = 0;
-  __syncthreads();
-  const tdm_bed_member &m = sm;
-  const int mode = m.mode, lp = m.list_parity;
-  double ch[kNch];
+  const size_t sn = (size_t)S * n;
+  // own state, list header and the first kPre candidate slots in one round
+  const double2 xp = k.x_in[p], vp = k.v_in[p];
+  const double rp = a.radius[p];
+  double2 xb = k.w.xb[p];
+  uint32_t mask = k.w.mask[(size_t)lp0 * n + p];
+  int nc = k.w.ncand[(size_t)lp0 * n + p];
+  int cq[kMaxSlots];
 #pragma unroll
-  for (int c = 0; c < kNch; ++c) ch[c] = 0.0;
+  for (int e = 0; e < kPre; ++e) cq[e] = e < S ? k.w.cand[lp0 * sn + (size_t)e * n + p] : p;
+  const BedConst m = load_const(a.members + mi);
+  if (threadIdx.x < 2) sflags[threadIdx.x] = 0;
+  double ch[8];
+#pragma unroll
+  for (int c = 0; c < 8; ++c) ch[c] = 0.0;
   int status = 0, req = 0;
+  int lp = lp0;
 
   if (valid) {
     const double mass = m.rho_pi_t * (rp * rp);
-    int *cand = k.w.cand + (size_t)lp * S * n + p;        // slot e at cand[e * n]
-    double2 *xis = k.w.xi + (size_t)lp * S * n + p;
-    uint32_t mask = k.w.mask[(size_t)lp * n + p];
-    int nc = k.w.ncand[(size_t)lp * n + p];
-    uint32_t *mask_out = k.w.mask + (size_t)lp * n + p;
-
-    if (mode == MODE_SEARCH) {
-      // ---- rebuild: new sorted list in the spare buffer, histories carried
-      // over by partner (sorted-key merge of old and new lists) -------------
-      int *ncand_new = k.w.cand + (size_t)(lp ^ 1) * S * n + p;
-      double2 *nxi = k.w.xi + (size_t)(lp ^ 1) * S * n + p;
-      int nn = 0;
-      int cx, cy;
-      cell_of(m, xp, &cx, &cy);
-      const int *cntab = k.w.cell_count + (size_t)m.tab * a.ncells;
-      const int *tab = k.w.cell_tab + (size_t)m.tab * a.ncells * a.cell_cap;
-      int ccount[9];
-#pragma unroll
-      for (int o = 0; o < 9; ++o) {
-        const int gx = cx + o % 3 - 1, gy = cy + o / 3 - 1;
-        const bool in = gx >= 0 && gx < m.ncx && gy >= 0 && gy < m.ncy;
-        ccount[o] = in ? min(cntab[m.cell_first + gy * m.ncx + gx], a.cell_cap) : 0;
-      }
-#pragma unroll
-      for (int o = 0; o < 9; ++o) {
-        {
-          const int cn = ccount[o];
-          if (cn == 0) continue;
-          const int cell = m.cell_first + (cy + o / 3 - 1) * m.ncx + (cx + o % 3 - 1);
-          for (int e = 0; e < cn; ++e) {
-            const int q = tab[(size_t)cell * a.cell_cap + e];
-            if (q == p) continue;
-            const double2 xq = k.x_in[q];
-            const double rq = a.radius[q];
-            const double dx = xq.x - xp.x, dy = xq.y - xp.y;
-            if (!(sqrt(dx * dx + dy * dy) < (rp + rq) + m.skin)) continue;
-            if (nn == S) { status |= TDM_ST_CAND_OVERFLOW; continue; }
-            int pos = nn++;
-            while (pos > 0 && ncand_new[(size_t)(pos - 1) * n] > q) {
-              ncand_new[(size_t)pos * n] = ncand_new[(size_t)(pos - 1) * n];
-              --pos;
-            }
-            ncand_new[(size_t)pos * n] = q;
-          }
-        }
-      }
-      // merge: both lists ascending; copy the history of every active old slot
-      uint32_t nmask = mask & (kFloorBit | kBackBit);
-      int eo = 0;
-      for (int e = 0; e < nn; ++e) {
-        const int q = ncand_new[(size_t)e * n];
-        double2 h = make_double2(0.0, 0.0);
-        while (eo < nc && cand[(size_t)eo * n] < q) ++eo;
-        if (eo < nc && cand[(size_t)eo * n] == q && ((mask >> eo) & 1u)) {
-          h = xis[(size_t)eo * n];
-          nmask |= 1u << e;
-        }
-        nxi[(size_t)e * n] = h;
-      }
-      cand = ncand_new;
-      xis = nxi;
+    if (m.mode == MODE_SEARCH) {
+      uint32_t nmask;
+      nc = rebuild_list(k, m, p, xp, rp, lp0, nc, mask, &nmask, &status);
       mask = nmask;
-      nc = nn;
-      mask_out = k.w.mask + (size_t)(lp ^ 1) * n + p;
-      k.w.ncand[(size_t)(lp ^ 1) * n + p] = (uint8_t)nn;
+      lp = lp0 ^ 1;
       xb = xp;
       k.w.xb[p] = xp;
+      const int *ncl = k.w.cand + lp * sn + p;
+#pragma unroll
+      for (int e = 0; e < kMaxSlots; ++e) cq[e] = e < nc ? ncl[(size_t)e * n] : p;
     } else {
+#pragma unroll
+      for (int e = kPre; e < kMaxSlots; ++e)
+        cq[e] = e < nc ? k.w.cand[lp0 * sn + (size_t)e * n + p] : p;
       const double ddx = xp.x - xb.x, ddy = xp.y - xb.y;
       const double lim = 0.5 * m.skin * 0.999;
       if (ddx * ddx + ddy * ddy >= lim * lim) status |= TDM_ST_STALE_LIST;
     }
-    // candidate indices in registers (slots past nc are not fetched)
-    int cq[kMaxSlots];
-#pragma unroll
-    for (int e = 0; e < kMaxSlots; ++e) cq[e] = (e < nc) ? cand[(size_t)e * n] : p;
+    double2 *xis = k.w.xi + lp * sn + p;
 
-    if (k.op == OP_ENERGY) {
-      // kinetic, gravity, Hertz normal (2/5 f_el d) and spring (s_t |xi|^2 / 2)
-      ch[C_KIN] = 0.5 * mass * (vp.x * vp.x + vp.y * vp.y);
-      ch[C_POT] = -mass * m.gravity_y * xp.y;
+    // ---- forces: gravity, pairs by ascending partner, floor, back wall
+    double fx = 0.0, fy = mass * m.gravity_y;
+    uint32_t nmask = 0;
 #pragma unroll
-      for (int e = 0; e < kMaxSlots; ++e) {
-        const int q = cq[e];
-        if (e >= nc || q < p) continue;
-        const double2 xq = k.x_in[q];
-        const double rq = a.radius[q];
-        const double dx = xq.x - xp.x, dy = xq.y - xp.y;
-        const double d = (rp + rq) - sqrt(dx * dx + dy * dy);
-        if (!(d > 0)) continue;
-        const double reff = (rp * rq) / (rp + rq);
-        const double sn = (2 * m.e_star) * sqrt(reff * d);
-        ch[C_ELN] += 0.4 * (((2.0 / 3.0) * sn) * d) * d;
-        if ((mask >> e) & 1u) {
-          const double2 h = xis[(size_t)e * n];
-          ch[C_ELT] += 0.5 * ((8 * m.g_star) * sqrt(reff * d)) * (h.x * h.x + h.y * h.y);
+    for (int c0 = 0; c0 < kMaxSlots; c0 += kChunk) {
+      if (!__any(c0 < nc)) break;
+      double2 xq[kChunk], vq[kChunk], hq[kChunk];
+      double rq[kChunk];
+#pragma unroll
+      for (int j = 0; j < kChunk; ++j) {
+        const int e = c0 + j;
+        xq[j] = vq[j] = hq[j] = make_double2(0.0, 0.0);
+        rq[j] = 1.0;
+        if (e < nc) {
+          const int q = cq[e];
+          xq[j] = k.x_in[q];
+          vq[j] = k.v_in[q];
+          rq[j] = a.radius[q];
+          if ((mask >> e) & 1u) hq[j] = xis[(size_t)e * n];
         }
       }
-      const double gaps[2] = {xp.y - m.floor_y, xp.x - m.back_wall_x};
-      const int use[2] = {m.use_floor, m.use_back_wall};
-      for (int w = 0; w < 2; ++w) {
-        const double d = rp - gaps[w];
-        if (!use[w] || !(d > 0)) continue;
-        const double sn = (2 * m.e_star) * sqrt(rp * d);
-        ch[C_ELN] += 0.4 * (((2.0 / 3.0) * sn) * d) * d;
-        if ((mask >> (16 + w)) & 1u) {
-          const double2 h = reinterpret_cast<const double2 *>(a.xi_wall)[2 * p + w];
-          ch[C_ELT] += 0.5 * ((8 * m.g_star) * sqrt(rp * d)) * (h.x * h.x + h.y * h.y);
-        }
-      }
-      *mask_out = mask;
-    } else {
-      // ---- forces: gravity, pairs by ascending partner, floor, back wall
-      double fx = 0.0, fy = mass * m.gravity_y;
-      uint32_t nmask = 0;
 #pragma unroll
-      for (int c0 = 0; c0 < kMaxSlots; c0 += kChunk) {
-        if (!__any(c0 < nc)) break;
-        double2 xq[kChunk], vq[kChunk], hq[kChunk];
-        double rq[kChunk];
-#pragma unroll
-        for (int j = 0; j < kChunk; ++j) {
-          const int e = c0 + j;
-          xq[j] = vq[j] = hq[j] = make_double2(0.0, 0.0);
-          rq[j] = 1.0;
-          if (e < nc) {
-            const int q = cq[e];
-            xq[j] = k.x_in[q];
-            vq[j] = k.v_in[q];
-            rq[j] = a.radius[q];
-            if ((mask >> e) & 1u) hq[j] = xis[(size_t)e * n];
-          }
-        }
-#pragma unroll
-        for (int j = 0; j < kChunk; ++j) {
-          const int e = c0 + j;
-          if (e >= nc) continue;
-          const bool self_lo = cq[e] > p;  // the pair law is evaluated lo -> hi
-          const double2 xl = self_lo ? xp : xq[j], xh = self_lo ? xq[j] : xp;
-          const double2 vl = self_lo ? vp : vq[j], vh = self_lo ? vq[j] : vp;
-          const double rl = self_lo ? rp : rq[j], rh = self_lo ? rq[j] : rp;
-          double2 h = hq[j];
-          double Fx, Fy, dash, fric;
-          if (!hm_pair<R>(m, xl, xh, vl, vh, rl, rh, h, Fx, Fy, dash, fric)) continue;
-          xis[(size_t)e * n] = h;
-          nmask |= 1u << e;
-          if (self_lo) {
-            fx += -Fx; fy += -Fy;
-            ch[C_DASH] += dash; ch[C_FRIC] += fric; ch[C_NPAIR] += 1.0;
-          } else {
-            fx += Fx; fy += Fy;
-          }
-        }
-      }
-      double2 *xw = reinterpret_cast<double2 *>(a.xi_wall) + 2 * p;
-      if (m.use_floor) {
-        double2 h = (mask & kFloorBit) ? xw[0] : make_double2(0.0, 0.0);
+      for (int j = 0; j < kChunk; ++j) {
+        const int e = c0 + j;
+        if (e >= nc) continue;
+        const bool self_lo = cq[e] > p;  // the pair law is evaluated lo -> hi
+        const double2 xl = self_lo ? xp : xq[j], xh = self_lo ? xq[j] : xp;
+        const double2 vl = self_lo ? vp : vq[j], vh = self_lo ? vq[j] : vp;
+        const double rl = self_lo ? rp : rq[j], rh = self_lo ? rq[j] : rp;
+        double2 h = hq[j];
         double Fx, Fy, dash, fric;
-        if (hm_wall<R>(m, rp, mass, xp.y - m.floor_y, 0.0, 1.0, vp, h, Fx, Fy, dash, fric)) {
-          xw[0] = h;
-          nmask |= kFloorBit;
+        if (!hm_pair<R>(m, xl, xh, vl, vh, rl, rh, h, Fx, Fy, dash, fric)) continue;
+        xis[(size_t)e * n] = h;
+        nmask |= 1u << e;
+        if (self_lo) {
+          fx += -Fx; fy += -Fy;
+          ch[C_DASH] += dash; ch[C_FRIC] += fric; ch[C_NPAIR] += 1.0;
+        } else {
           fx += Fx; fy += Fy;
-          ch[C_WDASH] += dash; ch[C_WFRIC] += fric; ch[C_WFX] += Fx; ch[C_WFY] += Fy;
-          ch[C_NWALL] += 1.0;
         }
-      }
-      if (m.use_back_wall) {
-        double2 h = (mask & kBackBit) ? xw[1] : make_double2(0.0, 0.0);
-        double Fx, Fy, dash, fric;
-        if (hm_wall<R>(m, rp, mass, xp.x - m.back_wall_x, 1.0, 0.0, vp, h, Fx, Fy, dash, fric)) {
-          xw[1] = h;
-          nmask |= kBackBit;
-          fx += Fx; fy += Fy;
-          ch[C_WDASH] += dash; ch[C_WFRIC] += fric; ch[C_WFX] += Fx; ch[C_WFY] += Fy;
-          ch[C_NWALL] += 1.0;
-        }
-      }
-      *mask_out = nmask;
-
-      // ---- semi-implicit Euler (dem.py:363-365) -------------------------
-      double vx, vy;
-      if constexpr (Prec<R>::fp64) {
-        vx = vp.x + (m.dt * fx) / mass;
-        vy = vp.y + (m.dt * fy) / mass;
-      } else {
-        const double dt_m = m.dt / mass;
-        vx = vp.x + fx * dt_m;
-        vy = vp.y + fy * dt_m;
-      }
-      const double xn = xp.x + m.dt * vx, yn = xp.y + m.dt * vy;
-      k.x_out[p] = make_double2(xn, yn);
-      k.v_out[p] = make_double2(vx, vy);
-      if (!isfinite(xn) || !isfinite(yn) || !isfinite(vx) || !isfinite(vy)) status |= TDM_ST_NONFINITE;
-      const double ddx = xn - xb.x, ddy = yn - xb.y;
-      const double q4 = 0.25 * m.skin;
-      if (ddx * ddx + ddy * ddy >= q4 * q4) req = 1;
-      if (mode == MODE_BIN) {
-        int cx, cy;
-        const int cell = cell_of(m, make_double2(xn, yn), &cx, &cy);
-        const int t = m.tab ^ 1;
-        const int slot = atomicAdd(k.w.cell_count + (size_t)t * a.ncells + cell, 1);
-        if (slot < a.cell_cap) k.w.cell_tab[((size_t)t * a.ncells + cell) * a.cell_cap + slot] = p;
-        else status |= TDM_ST_CELL_OVERFLOW;
       }
     }
+    double2 *xw = reinterpret_cast<double2 *>(a.xi_wall) + 2 * p;
+    if (m.use_floor) {
+      double2 h = (mask & kFloorBit) ? xw[0] : make_double2(0.0, 0.0);
+      double Fx, Fy, dash, fric;
+      if (hm_wall<R>(m, rp, mass, xp.y - m.floor_y, 0.0, 1.0, vp, h, Fx, Fy, dash, fric)) {
+        xw[0] = h;
+        nmask |= kFloorBit;
+        fx += Fx; fy += Fy;
+        ch[C_WDASH] += dash; ch[C_WFRIC] += fric; ch[C_WFX] += Fx; ch[C_WFY] += Fy;
+        ch[C_NWALL] += 1.0;
+      }
+    }
+    if (m.use_back_wall) {
+      double2 h = (mask & kBackBit) ? xw[1] : make_double2(0.0, 0.0);
+      double Fx, Fy, dash, fric;
+      if (hm_wall<R>(m, rp, mass, xp.x - m.back_wall_x, 1.0, 0.0, vp, h, Fx, Fy, dash, fric)) {
+        xw[1] = h;
+        nmask |= kBackBit;
+        fx += Fx; fy += Fy;
+        ch[C_WDASH] += dash; ch[C_WFRIC] += fric; ch[C_WFX] += Fx; ch[C_WFY] += Fy;
+        ch[C_NWALL] += 1.0;
+      }
+    }
+    k.w.mask[(size_t)lp * n + p] = nmask;
+
+    // ---- semi-implicit Euler (dem.py:363-365) -------------------------
+    double vx, vy;
+    if constexpr (Prec<R>::fp64) {
+      vx = vp.x + (m.dt * fx) / mass;
+      vy = vp.y + (m.dt * fy) / mass;
+    } else {
+      const double dt_m = m.dt / mass;
+      vx = vp.x + fx * dt_m;
+      vy = vp.y + fy * dt_m;
+    }
+    const double xn = xp.x + m.dt * vx, yn = xp.y + m.dt * vy;
+    k.x_out[p] = make_double2(xn, yn);
+    k.v_out[p] = make_double2(vx, vy);
+    if (!isfinite(xn) || !isfinite(yn) || !isfinite(vx) || !isfinite(vy)) status |= TDM_ST_NONFINITE;
+    const double ddx = xn - xb.x, ddy = yn - xb.y;
+    const double q4 = 0.25 * m.skin;
+    if (ddx * ddx + ddy * ddy >= q4 * q4) req = 1;
+    if (m.mode == MODE_BIN) {
+      int cx, cy;
+      const int cell = cell_of(m.ox, m.oy, m.cell_size, m.ncx, m.ncy, m.cell_first,
+                               make_double2(xn, yn), &cx, &cy);
+      const int t = m.tab ^ 1;
+      const int slot = atomicAdd(k.w.cell_count + (size_t)t * a.ncells + cell, 1);
+      if (slot < a.cell_cap) k.w.cell_tab[((size_t)t * a.ncells + cell) * a.cell_cap + slot] = p;
+      else status |= TDM_ST_CELL_OVERFLOW;
+    }
   }
-  if (mode == MODE_SEARCH) {
+  if (m.mode == MODE_SEARCH) {
     // clear the other hash table for the next bin phase
     const int ncell_m = m.ncx * m.ncy;
     int *cntc = k.w.cell_count + (size_t)(m.tab ^ 1) * a.ncells + m.cell_first;
     for (int c = blk.w * kThreads + threadIdx.x; c < ncell_m; c += m.nblocks * kThreads)
       cntc[c] = 0;
   }
+  __syncthreads();
   if (status) atomicOr(&sflags[0], status);
   if (req) atomicOr(&sflags[1], 1);
   block_sum(ch, red);
   if (threadIdx.x == 0) {
+    // partials by the member's logical block index: the reduction order does
+    // not depend on where the member sits in the launch
     const size_t slot = (size_t)(m.block_first + blk.w);
 #pragma unroll
-    for (int c = 0; c < kNch; ++c) k.w.partials[(size_t)c * a.nblocks + slot] = ch[c];
+    for (int c = 0; c < 8; ++c) k.w.partials[(size_t)c * a.nblocks + slot] = ch[c];
     k.w.bflags[slot] = sflags[0] | (sflags[1] << 16);
+  }
+}
+
+// Energies of the current state on the current lists (no rebuild, no
+// integration): kinetic, gravity, Hertz normal (2/5 f_el d) and tangential
+// spring (s_t |xi|^2 / 2), in FP64.
+__global__ __launch_bounds__(kThreads) void bed_energy_kernel(BKArgs k)
+{
+  __shared__ double red[4][kCh];
+  __shared__ int sflags[2];
+  const tdm_bed_arrays &a = k.a;
+  const int4 blk = k.w.blocks[blockIdx.x];
+  const int mi = blk.x, p0 = blk.y, cnt = blk.z & 0xFFFF, lp = (blk.z & kLpBit) ? 1 : 0;
+  const bool valid = (int)threadIdx.x < cnt;
+  const int p = p0 + (valid ? (int)threadIdx.x : 0);
+  const int n = a.n;
+  const size_t sn = (size_t)a.cand_slots * n;
+  const BedConst m = load_const(a.members + mi);
+  if (threadIdx.x < 2) sflags[threadIdx.x] = 0;
+  double ch[4] = {0.0, 0.0, 0.0, 0.0};
+  int status = 0;
+  if (valid) {
+    const double2 xp = k.x_in[p], vp = k.v_in[p], xb = k.w.xb[p];
+    const double rp = a.radius[p];
+    const double mass = m.rho_pi_t * (rp * rp);
+    const uint32_t mask = k.w.mask[(size_t)lp * n + p];
+    const int nc = k.w.ncand[(size_t)lp * n + p];
+    const int *cand = k.w.cand + lp * sn + p;
+    const double2 *xis = k.w.xi + lp * sn + p;
+    const double ddx = xp.x - xb.x, ddy = xp.y - xb.y;
+    const double lim = 0.5 * m.skin * 0.999;
+    if (ddx * ddx + ddy * ddy >= lim * lim) status |= TDM_ST_STALE_LIST;
+    ch[0] = 0.5 * mass * (vp.x * vp.x + vp.y * vp.y);
+    ch[1] = -mass * m.gravity_y * xp.y;
+    for (int e = 0; e < nc; ++e) {
+      const int q = cand[(size_t)e * n];
+      if (q < p) continue;
+      const double2 xq = k.x_in[q];
+      const double rq = a.radius[q];
+      const double dx = xq.x - xp.x, dy = xq.y - xp.y;
+      const double d = (rp + rq) - sqrt(dx * dx + dy * dy);
+      if (!(d > 0)) continue;
+      const double reff = (rp * rq) / (rp + rq);
+      const double sn_ = (2 * m.e_star) * sqrt(reff * d);
+      ch[2] += 0.4 * (((2.0 / 3.0) * sn_) * d) * d;
+      if ((mask >> e) & 1u) {
+        const double2 h = xis[(size_t)e * n];
+        ch[3] += 0.5 * ((8 * m.g_star) * sqrt(reff * d)) * (h.x * h.x + h.y * h.y);
+      }
+    }
+    const double gaps[2] = {xp.y - m.floor_y, xp.x - m.back_wall_x};
+    const int use[2] = {m.use_floor, m.use_back_wall};
+    for (int w = 0; w < 2; ++w) {
+      const double d = rp - gaps[w];
+      if (!use[w] || !(d > 0)) continue;
+      const double sn_ = (2 * m.e_star) * sqrt(rp * d);
+      ch[2] += 0.4 * (((2.0 / 3.0) * sn_) * d) * d;
+      if ((mask >> (16 + w)) & 1u) {
+        const double2 h = reinterpret_cast<const double2 *>(a.xi_wall)[2 * p + w];
+        ch[3] += 0.5 * ((8 * m.g_star) * sqrt(rp * d)) * (h.x * h.x + h.y * h.y);
+      }
+    }
+  }
+  __syncthreads();
+  if (status) atomicOr(&sflags[0], status);
+  block_sum(ch, red);
+  if (threadIdx.x == 0) {
+    const size_t slot = (size_t)(m.block_first + blk.w);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) k.w.partials[(size_t)(C_KIN + c) * a.nblocks + slot] = ch[c];
+    k.w.bflags[slot] = sflags[0];
   }
 }
 
@@ -510,9 +601,18 @@ __global__ __launch_bounds__(kFinThreads) void bed_finalize_kernel(BKArgs k)
   const int mi = blockIdx.x;
   tdm_bed_member *mp = a.members + mi;
   const int b0 = mp->block_first, nb = mp->nblocks;
+  const int mode0 = mp->mode, lp0 = mp->list_parity;
   const int c_lo = k.op == OP_ENERGY ? C_KIN : 0, c_n = k.op == OP_ENERGY ? 4 : 8;
   if (threadIdx.x == 0) sflag = 0;
   __syncthreads();
+  if (k.op == OP_STEP && mode0 == MODE_SEARCH) {
+    // the step just rebuilt the lists into buffer lp0^1: mirror the new
+    // parity into the member's block-table entries
+    for (int b = threadIdx.x; b < nb; b += kFinThreads) {
+      const int z = k.w.blocks[b0 + b].z;
+      k.w.blocks[b0 + b].z = (lp0 ^ 1) ? (z | kLpBit) : (z & ~kLpBit);
+    }
+  }
   double acc[kC];
 #pragma unroll
   for (int c = 0; c < kC; ++c) acc[c] = 0.0;
@@ -552,15 +652,17 @@ __global__ __launch_bounds__(kFinThreads) void bed_finalize_kernel(BKArgs k)
   const tdm_bed_member m = *mp;
   const int status = sflag & 0xFFFF, rq = sflag >> 16;
   int new_mode = m.mode, new_tab = m.tab, new_lp = m.list_parity;
-  if (m.mode == MODE_SEARCH) new_lp ^= 1;
   if (k.op == OP_ENERGY) {
     double *en = a.energy + (size_t)mi * 4;
     en[0] = stot[C_KIN - C_KIN];
     en[1] = stot[C_POT - C_KIN];
     en[2] = stot[C_ELN - C_KIN];
     en[3] = stot[C_ELT - C_KIN];
-    if (m.mode == MODE_SEARCH) new_mode = MODE_IDLE;
   } else {
+    if (m.mode == MODE_SEARCH) {
+      new_lp ^= 1;
+      mp->rebuilds = m.rebuilds + 1;
+    }
     double *acc = a.accounting + (size_t)mi * TDM_BED_NACC;
     const double dt = m.dt;
     acc[TDM_BED_DASHPOT] += dt * stot[C_DASH];
@@ -593,7 +695,8 @@ __global__ void bed_bin_kernel(tdm_bed_arrays a, BedWs w)
   }
   const tdm_bed_member &m = a.members[lo];
   int cx, cy;
-  const int cell = cell_of(m, reinterpret_cast<const double2 *>(a.x)[t], &cx, &cy);
+  const int cell = cell_of(m.grid_origin[0], m.grid_origin[1], m.cell_size, m.ncx, m.ncy,
+                           m.cell_first, reinterpret_cast<const double2 *>(a.x)[t], &cx, &cy);
   const int slot = atomicAdd(w.cell_count + (size_t)m.tab * a.ncells + cell, 1);
   if (slot < a.cell_cap) w.cell_tab[((size_t)m.tab * a.ncells + cell) * a.cell_cap + slot] = t;
   else atomicOr(&a.members[lo].status, TDM_ST_CELL_OVERFLOW);
@@ -637,7 +740,9 @@ int launch_bed(const tdm_bed_arrays *a, const BedWs &w, int parity, int fp32, in
   k.v_in = parity ? va : v;
   k.x_out = const_cast<double2 *>(parity ? x : xa);
   k.v_out = const_cast<double2 *>(parity ? v : va);
-  if (ev0) {
+  if (op == OP_ENERGY) {
+    bed_energy_kernel<<<a->nblocks, kThreads, 0, s>>>(k);
+  } else if (ev0) {
     if (fp32)
       hipExtLaunchKernelGGL(bed_step_kernel<float>, dim3(a->nblocks), dim3(kThreads), 0, s, ev0,
                             ev1, 0, k);
@@ -720,7 +825,8 @@ int tdm_bed_setup(const tdm_bed_arrays *a, void *stream)
     for (int b = 0; b < nb; ++b) {
       const int item = order[b];
       const int first = hm[i].first + item * kThreads;
-      hb[b0 + b] = make_int4(i, first, min(kThreads, hm[i].first + hm[i].n - first), item);
+      const int count = min(kThreads, hm[i].first + hm[i].n - first);
+      hb[b0 + b] = make_int4(i, first, count | (hm[i].list_parity ? kLpBit : 0), item);
     }
   }
   e = hipMemcpyAsync(w.blocks, hb, sizeof(int4) * total, hipMemcpyHostToDevice, s);
