@@ -603,7 +603,18 @@ __global__ __launch_bounds__(kFinThreads) void bed_finalize_kernel(BKArgs k)
   const int b0 = mp->block_first, nb = mp->nblocks;
   const int mode0 = mp->mode, lp0 = mp->list_parity;
   const int c_lo = k.op == OP_ENERGY ? C_KIN : 0, c_n = k.op == OP_ENERGY ? 4 : 8;
-  if (threadIdx.x == 0) sflag = 0;
+  // thread 0's tail inputs (member record, ledger) are fetched with the
+  // partials, not after them
+  tdm_bed_member m;
+  double led[8];
+  if (threadIdx.x == 0) {
+    sflag = 0;
+    m = *mp;
+    const double *accp = k.op == OP_ENERGY ? a.energy + (size_t)mi * 4
+                                           : a.accounting + (size_t)mi * TDM_BED_NACC;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) led[c] = (k.op == OP_ENERGY && c >= 4) ? 0.0 : accp[c];
+  }
   __syncthreads();
   if (k.op == OP_STEP && mode0 == MODE_SEARCH) {
     // the step just rebuilt the lists into buffer lp0^1: mirror the new
@@ -649,7 +660,6 @@ __global__ __launch_bounds__(kFinThreads) void bed_finalize_kernel(BKArgs k)
   }
   __syncthreads();
   if (threadIdx.x != 0) return;
-  const tdm_bed_member m = *mp;
   const int status = sflag & 0xFFFF, rq = sflag >> 16;
   int new_mode = m.mode, new_tab = m.tab, new_lp = m.list_parity;
   if (k.op == OP_ENERGY) {
@@ -665,12 +675,12 @@ __global__ __launch_bounds__(kFinThreads) void bed_finalize_kernel(BKArgs k)
     }
     double *acc = a.accounting + (size_t)mi * TDM_BED_NACC;
     const double dt = m.dt;
-    acc[TDM_BED_DASHPOT] += dt * stot[C_DASH];
-    acc[TDM_BED_FRICTION] += dt * stot[C_FRIC];
-    acc[TDM_BED_WALL_DASHPOT] += dt * stot[C_WDASH];
-    acc[TDM_BED_WALL_FRICTION] += dt * stot[C_WFRIC];
-    acc[TDM_BED_WALL_IMPULSE] += dt * stot[C_WFX];
-    acc[TDM_BED_WALL_IMPULSE + 1] += dt * stot[C_WFY];
+    acc[TDM_BED_DASHPOT] = led[TDM_BED_DASHPOT] + dt * stot[C_DASH];
+    acc[TDM_BED_FRICTION] = led[TDM_BED_FRICTION] + dt * stot[C_FRIC];
+    acc[TDM_BED_WALL_DASHPOT] = led[TDM_BED_WALL_DASHPOT] + dt * stot[C_WDASH];
+    acc[TDM_BED_WALL_FRICTION] = led[TDM_BED_WALL_FRICTION] + dt * stot[C_WFRIC];
+    acc[TDM_BED_WALL_IMPULSE] = led[TDM_BED_WALL_IMPULSE] + dt * stot[C_WFX];
+    acc[TDM_BED_WALL_IMPULSE + 1] = led[TDM_BED_WALL_IMPULSE + 1] + dt * stot[C_WFY];
     acc[TDM_BED_CONTACTS] = stot[C_NPAIR];
     acc[TDM_BED_WALL_CONTACTS] = stot[C_NWALL];
     mp->time = m.time + dt;
