@@ -26,6 +26,8 @@ import sys
 import time
 from pathlib import Path
 
+import numpy as np
+
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
@@ -51,7 +53,7 @@ class PlainFP64:
     the GPU, with a per-step sort-based cell list instead of all N^2 pairs.
     Ledger sums stay on the device (no host syncs)."""
 
-    def __init__(self, cfg, dt, device="cuda", pinv=True):
+    def __init__(self, cfg, dt, device="cuda", pinv=True, eps_t_disk=None, eps_s_disk=None):
         import torch
         from tensordem_amd.lattice import bonds, fixed_mask, positions
         dev = torch.device(device)
@@ -66,6 +68,14 @@ class PlainFP64:
         self.j = torch.tensor(bj, device=dev)
         self.rest = torch.tensor(rest, **f64)
         self.alive = torch.ones(len(bi), dtype=torch.bool, device=dev)
+        # per-disk strengths: a bond breaks at its weaker disk
+        self.eps_t, self.eps_s = cfg.breaking_strain, cfg.shear_breaking_strain
+        if eps_t_disk is not None:
+            et = torch.tensor(eps_t_disk, **f64)
+            self.eps_t = torch.minimum(et[self.i], et[self.j])
+        if eps_s_disk is not None:
+            es = torch.tensor(eps_s_disk, **f64)
+            self.eps_s = torch.minimum(es[self.i], es[self.j])
         self.refv = self.x0[self.j] - self.x0[self.i]
         n = len(self.x)
         self.n = n
@@ -136,7 +146,7 @@ class PlainFP64:
         dist = torch.linalg.vector_norm(cur, dim=1)
         nrm = cur / dist.clamp_min(torch.finfo(x.dtype).eps)[:, None]
         ext = dist - self.rest
-        fail = ((ext / self.rest) > c.breaking_strain) | (0.5 * (s[i] + s[j]) > c.shear_breaking_strain)
+        fail = ((ext / self.rest) > self.eps_t) | (0.5 * (s[i] + s[j]) > self.eps_s)
         newly = self.alive & fail
         self.acc[0] += (0.5 * c.bond_stiffness * ext[newly].square()).sum()
         self.alive &= ~fail
@@ -508,6 +518,107 @@ def cpu_baseline_bed(cfg, x, r, dt, budget_s):
                        f"cell grid rebuilt every step), {el:.1f} s")
 
 
+# ---------------------------------------------------------------------------
+# C4: 512 independent plates with Weibull strengths, one launch per step
+# ---------------------------------------------------------------------------
+def bench_ensemble(args):
+    """512 plates (nx log-uniform 16..256, ny 4..32, tool 0.1..1 m/s, per-disk
+    Weibull strains) in one batch: autotuned launch shape, FP32 and FP64 forces
+    over graph-captured steps, kernel events, plain FP64 per member, CPU oracle."""
+    import torch
+    from tensordem_amd.autotune import autotune_plates
+    from tensordem_amd.engine import PlateEngine
+    from tensordem_amd.ensemble import random_ensemble, total_disks
+
+    specs = random_ensemble(512, seed=0)
+    n = total_disks(specs)
+    tune = autotune_plates(specs, fp32=True, steps=200, warmup=40)
+    tuned = tune.apply(specs)
+    chunk = 100
+    steps = max(chunk, args.ens_steps - args.ens_steps % chunk)
+    out = {}
+    for fp32 in (True, False):
+        eng = PlateEngine(tuned, fp32=fp32, **tune.engine_kwargs())
+        eng.graph(chunk // 2)
+        stream = torch.cuda.current_stream()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record(stream)
+        eng.run_graph(chunk // 2, repeats=steps // chunk)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        gpu_s = e0.elapsed_time(e1) * 1e-3
+        out["fp32" if fp32 else "fp64"] = dict(
+            rate=n * steps / gpu_s, ms_per_step=gpu_s / steps * 1e3,
+            broken=int((~eng.alive).sum()), status=int(np.bitwise_or.reduce(eng.status())),
+            alive=eng.alive.cpu().numpy())
+        eng.close()
+        del eng
+    eng = PlateEngine(tuned, fp32=True, **tune.engine_kwargs())
+    kernel_s = eng.step_timed(steps) * 1e-6
+    eng.close()
+    del eng
+    r32, r64 = out["fp32"], out["fp64"]
+    achieved = PLATE_BYTES_PER_DISK * n / kernel_s / 1e9
+    res = {
+        "workload": f"512 independent plates ({n} disks): nx log-uniform [16,256], ny uniform "
+                    "[4,32], tool speed U[0.1,1] m/s, per-disk Weibull(5) strains mean "
+                    "0.01 axial / 0.02 shear (bond = weaker disk), degree-bounded dt per member, "
+                    f"{steps} steps, one launch per step for all members (seed 0)",
+        "value": r32["rate"], "unit": "particle-steps/s", "steps": steps, "disks": n,
+        "dtype": "f32-force/f64-state",
+        "ms_per_step": r32["ms_per_step"], "graph_chunk_steps": chunk,
+        "broken_bonds_end": r32["broken"], "status_bits": r32["status"],
+        "broken_set_equal_fp64": bool(np.array_equal(r32["alive"], r64["alive"])),
+        "autotune": {"tile_policy": tune.tile_policy, "skin_factor": tune.skin_factor,
+                     "seconds": tune.seconds, "table": tune.table},
+        "fp64_kernel": {"value": r64["rate"], "ms_per_step": r64["ms_per_step"],
+                        "broken_bonds_end": r64["broken"], "status_bits": r64["status"]},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "plate_step_kernel<float>", "kernel_us": kernel_s * 1e6,
+                     "bytes_per_launch": PLATE_BYTES_PER_DISK * n},
+    }
+    if not args.no_plain:
+        # every 4th member, each from rest: the plain path is launch-bound, so
+        # its time per step hardly depends on the member's size
+        tot_n = tot_s = 0.0
+        for s in specs[::4]:
+            p = PlainFP64(s.config, s.dt, pinv=False, eps_t_disk=s.eps_t_disk,
+                          eps_s_disk=s.eps_s_disk)
+            p.step()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(3):
+                p.step()
+            torch.cuda.synchronize()
+            tot_s += (time.perf_counter() - t0) / 3
+            tot_n += s.config.nx * s.config.ny
+            del p
+        res["plain_fp64_gpu"] = dict(
+            value=tot_n / tot_s, unit="particle-steps/s", members_sampled=len(specs[::4]),
+            what="eager PyTorch FP64 plate step run member after member (every 4th member, "
+                 "3 steps each, closed-form 2x2 inverse), particle-steps over summed step time")
+        res["speedup_vs_plain_fp64"] = r32["rate"] / (tot_n / tot_s)
+    if not args.no_cpu:
+        from oracle.plate import OraclePlate
+        t0 = time.perf_counter()
+        done_ps, k = 0, 0
+        while time.perf_counter() - t0 < args.cpu_seconds and k < len(specs):
+            s = specs[k]
+            sim = OraclePlate(s.config, dt=s.dt, eps_t_disk=s.eps_t_disk, eps_s_disk=s.eps_s_disk)
+            for _ in range(10):
+                sim.step()
+            done_ps += 10 * s.config.nx * s.config.ny
+            k += 1
+        el = time.perf_counter() - t0
+        res["cpu_baseline"] = dict(value=done_ps / el, unit="particle-steps/s", cores=1,
+                                   kind="port",
+                                   sample=f"first {k} members, 10 steps each from rest (FP64 C "
+                                          f"oracle), {el:.1f} s")
+    return res
+
+
 def cpu_baseline(cfg, dt, budget_s=12.0):
     """C oracle (FP64, one core) on the same plate, run for ~budget_s."""
     from oracle.plate import OraclePlate
@@ -537,8 +648,10 @@ def main(argv=None):
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--workload", choices=("all", "plate", "bed"), default="all",
-                    help="plate: the 1024x128 plate only; bed: the 1M-disk column only")
+    ap.add_argument("--workload", choices=("all", "plate", "bed", "ensemble"), default="all",
+                    help="plate: the 1024x128 plate only; bed: the 1M-disk column only; "
+                         "ensemble: the 512-plate batch only")
+    ap.add_argument("--ens-steps", type=int, default=10000)
     ap.add_argument("--bed-steps", type=int, default=50000)
     ap.add_argument("--bed-plain-steps", type=int, default=60)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -551,13 +664,14 @@ def main(argv=None):
         print("bench.py measures one GPU: every plate and request of this workload runs on a "
               "single MI355X and no multi-GPU path is built (see DESIGN.md).", file=sys.stderr)
         return 2
-    if args.workload == "bed":
-        bed = bench_bed(args)
+    if args.workload in ("bed", "ensemble"):
+        bed = bench_bed(args) if args.workload == "bed" else bench_ensemble(args)
         line = {"metric": METRIC, "value": bed.pop("value"), "unit": "particle-steps/s",
                 "n_gpus": 1, "steps": bed["steps"], "warmup": 1000,
                 "ms_per_step": bed["ms_per_step"], "higher_is_better": True, "scaling": "weak",
                 "vs_baseline": None, "dtype": bed.pop("dtype"),
-                "data": "synthetic: seeded loose column packing",
+                "data": "synthetic: " + ("seeded loose column packing" if args.workload == "bed"
+                                         else "seeded parameter sweep of plates"),
                 "config": {"workload": bed.pop("workload")}}
         line.update(bed)
         print(json.dumps(line))
@@ -565,6 +679,7 @@ def main(argv=None):
     line = bench_plate(args)
     if args.workload == "all":
         line["c3_bed"] = bench_bed(args)
+        line["c4_ensemble"] = bench_ensemble(args)
     print(json.dumps(line))
     return 0
 

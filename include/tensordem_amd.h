@@ -258,10 +258,11 @@ size_t tdm_bed_workspace_bytes(int n, int nblocks, int nmembers, int ncells, int
 /* Byte offsets inside the workspace of the candidate lists (int32
  * [2][slots][N]), their spring histories (double2 [2][slots][N]), the
  * contact masks (uint32 [2][N]: bit e = slot e in contact, bits 16/17 =
- * floor/back wall) and counts (uint8 [2][N]); index 0/1 is the list parity. */
+ * floor/back wall) and counts (uint8 [2][N]), index 0/1 being the list
+ * parity; and of each disk's position when its list was built (double2 [N]). */
 int tdm_bed_workspace_layout(int n, int nblocks, int nmembers, int ncells, int cand_slots,
                              int cell_cap, size_t *cand_off, size_t *xi_off, size_t *mask_off,
-                             size_t *count_off);
+                             size_t *count_off, size_t *xb_off);
 /* Block table, cleared histories, first binning (members filled by caller). */
 int tdm_bed_setup(const tdm_bed_arrays *a, void *stream);
 /* Re-bins x and schedules a candidate search (after the caller moved disks). */

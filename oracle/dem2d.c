@@ -287,11 +287,12 @@ static void hull_particle(const double *p, const double *v, const double *vert, 
  *   hull != NULL: polyline vertices (already in global position), hull
  *                 velocity in P_TOOL_V*.
  */
-void orc_plate_forces(const double *prm, int n, const double *x, const double *v,
-                      const double *x0, int nb, const int32_t *bi, const int32_t *bj,
-                      const double *rest, uint8_t *alive, uint8_t *fmode, double *ftime,
-                      double *fext, double *fenergy, const double *loads, const double *hull,
-                      int hull_m, double *force, double *reaction, double *scal)
+static void plate_forces(const double *prm, int n, const double *x, const double *v,
+                         const double *x0, int nb, const int32_t *bi, const int32_t *bj,
+                         const double *rest, uint8_t *alive, uint8_t *fmode, double *ftime,
+                         double *fext, double *fenergy, const double *loads, const double *hull,
+                         int hull_m, double *force, double *reaction, double *scal,
+                         const double *eps_t_b, const double *eps_s_b)
 {
   const double r = prm[P_RADIUS], kb = prm[P_KB], kc = prm[P_KC];
   const double cdamp = prm[P_CDAMP], drag = prm[P_DRAG];
@@ -322,8 +323,9 @@ void orc_plate_forces(const double *prm, int n, const double *x, const double *v
     double ext = d - rest[b];
     double tstrain = ext / rest[b];
     double bshear = 0.5 * (s[i] + s[j]);
-    int sf = bshear > prm[P_EPS_S];
-    int tf = tstrain > prm[P_EPS_T];
+    /* per-bond thresholds when given (strength scatter), else the config's */
+    int sf = bshear > (eps_s_b ? eps_s_b[b] : prm[P_EPS_S]);
+    int tf = tstrain > (eps_t_b ? eps_t_b[b] : prm[P_EPS_T]);
     if (alive[b] && (sf || tf)) {
       alive[b] = 0;
       fmode[b] = (uint8_t)(tf + 2 * sf);
@@ -466,24 +468,47 @@ void orc_plate_forces(const double *prm, int n, const double *x, const double *v
   free(pf); free(deg); free(cb); free(fill);
 }
 
+void orc_plate_forces(const double *prm, int n, const double *x, const double *v,
+                      const double *x0, int nb, const int32_t *bi, const int32_t *bj,
+                      const double *rest, uint8_t *alive, uint8_t *fmode, double *ftime,
+                      double *fext, double *fenergy, const double *loads, const double *hull,
+                      int hull_m, double *force, double *reaction, double *scal)
+{
+  plate_forces(prm, n, x, v, x0, nb, bi, bj, rest, alive, fmode, ftime, fext, fenergy, loads,
+               hull, hull_m, force, reaction, scal, NULL, NULL);
+}
+
+/* As orc_plate_forces with per-bond axial / shear thresholds (nb each). */
+void orc_plate_forces_thr(const double *prm, int n, const double *x, const double *v,
+                          const double *x0, int nb, const int32_t *bi, const int32_t *bj,
+                          const double *rest, uint8_t *alive, uint8_t *fmode, double *ftime,
+                          double *fext, double *fenergy, const double *loads,
+                          const double *hull, int hull_m, double *force, double *reaction,
+                          double *scal, const double *eps_t_b, const double *eps_s_b)
+{
+  plate_forces(prm, n, x, v, x0, nb, bi, bj, rest, alive, fmode, ftime, fext, fenergy, loads,
+               hull, hull_m, force, reaction, scal, eps_t_b, eps_s_b);
+}
+
 /*
  * IceDEM.step (dem.py:347-382): forces, impulse ledger, semi-implicit Euler,
  * clamp of fixed particles, work ledger.  Returns 1 if the new state is
  * non-finite (the caller raises FloatingPointError), else 0.
  */
-int orc_plate_step(const double *prm, int n, double *x, double *v, const double *x0,
-                   const uint8_t *fixed, int nb, const int32_t *bi, const int32_t *bj,
-                   const double *rest, uint8_t *alive, uint8_t *fmode, double *ftime,
-                   double *fext, double *fenergy, const double *loads, const double *hull,
-                   int hull_m, double *acc, double *last, double *force_out)
+int orc_plate_step_thr(const double *prm, int n, double *x, double *v, const double *x0,
+                       const uint8_t *fixed, int nb, const int32_t *bi, const int32_t *bj,
+                       const double *rest, uint8_t *alive, uint8_t *fmode, double *ftime,
+                       double *fext, double *fenergy, const double *loads, const double *hull,
+                       int hull_m, double *acc, double *last, double *force_out,
+                       const double *eps_t_b, const double *eps_s_b)
 {
   const double dt = prm[P_DT], m = prm[P_MASS], drag = prm[P_DRAG];
   double *force = force_out;
   double reaction[2], scal[S_NSCAL];
   double *xold = (double *)malloc(sizeof(double) * (size_t)(2 * n > 0 ? 2 * n : 1));
   memcpy(xold, x, sizeof(double) * (size_t)(2 * n));
-  orc_plate_forces(prm, n, x, v, x0, nb, bi, bj, rest, alive, fmode, ftime, fext, fenergy,
-                   loads, hull, hull_m, force, reaction, scal);
+  plate_forces(prm, n, x, v, x0, nb, bi, bj, rest, alive, fmode, ftime, fext, fenergy, loads,
+               hull, hull_m, force, reaction, scal, eps_t_b, eps_s_b);
   acc[A_RELEASE] += scal[S_RELEASED];
   acc[A_ACQUIRED] += scal[S_ACQUIRED];
   double sup[2] = {0.0, 0.0}, ext[2] = {0.0, 0.0}, vs[2] = {0.0, 0.0};
@@ -533,6 +558,16 @@ int orc_plate_step(const double *prm, int n, double *x, double *v, const double 
   last[L_MOMENT] = moment;
   free(xold);
   return bad;
+}
+
+int orc_plate_step(const double *prm, int n, double *x, double *v, const double *x0,
+                   const uint8_t *fixed, int nb, const int32_t *bi, const int32_t *bj,
+                   const double *rest, uint8_t *alive, uint8_t *fmode, double *ftime,
+                   double *fext, double *fenergy, const double *loads, const double *hull,
+                   int hull_m, double *acc, double *last, double *force_out)
+{
+  return orc_plate_step_thr(prm, n, x, v, x0, fixed, nb, bi, bj, rest, alive, fmode, ftime,
+                            fext, fenergy, loads, hull, hull_m, acc, last, force_out, NULL, NULL);
 }
 
 /* mechanical_energy (dem.py:603-624) for the circular tool or the hull. */

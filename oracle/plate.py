@@ -51,6 +51,10 @@ def library() -> ctypes.CDLL:
     lib.orc_plate_step.argtypes = [d, c_int, d, d, d, u8, c_int, i32, i32, d, u8, u8, d, d, d,
                                    d, d, c_int, d, d, d]
     lib.orc_plate_step.restype = c_int
+    lib.orc_plate_step_thr.argtypes = lib.orc_plate_step.argtypes + [d, d]
+    lib.orc_plate_step_thr.restype = c_int
+    lib.orc_plate_forces_thr.argtypes = lib.orc_plate_forces.argtypes + [d, d]
+    lib.orc_plate_forces_thr.restype = None
     lib.orc_mech_energy.argtypes = [d, c_int, d, d, c_int, i32, i32, d, u8, d, c_int, d]
     lib.orc_mech_energy.restype = None
     lib.orc_pairs_within.argtypes = [c_int, d, ctypes.c_double, i32, c_int]
@@ -109,7 +113,7 @@ class OraclePlate:
     """
 
     def __init__(self, config, *, hull_profile=None, hull_start=(0.0, 0.0),
-                 hull_velocity=(0.0, 0.0), dt=None):
+                 hull_velocity=(0.0, 0.0), dt=None, eps_t_disk=None, eps_s_disk=None):
         self.config = config
         self.lib = library()
         c = config
@@ -137,6 +141,15 @@ class OraclePlate:
         self.hull_start = np.asarray(hull_start, np.float64)
         self.hull_velocity = np.asarray(hull_velocity, np.float64)
         self.force = np.zeros_like(pos)
+        # per-disk strengths: a bond breaks at the weaker of its two disks
+        self.eps_t_bond = self.eps_s_bond = None
+        if eps_t_disk is not None or eps_s_disk is not None:
+            et = np.full(len(pos), c.breaking_strain) if eps_t_disk is None else \
+                np.asarray(eps_t_disk, np.float64)
+            es = np.full(len(pos), c.shear_breaking_strain) if eps_s_disk is None else \
+                np.asarray(eps_s_disk, np.float64)
+            self.eps_t_bond = np.ascontiguousarray(np.minimum(et[bi], et[bj]))
+            self.eps_s_bond = np.ascontiguousarray(np.minimum(es[bi], es[bj]))
 
     # -- helpers -------------------------------------------------------------
     @property
@@ -189,13 +202,14 @@ class OraclePlate:
         reaction = np.zeros(2)
         scal = np.zeros(5)
         d, i32, u8 = ctypes.c_double, ctypes.c_int32, ctypes.c_uint8
-        self.lib.orc_plate_forces(
+        self.lib.orc_plate_forces_thr(
             _p(self._params(), d), len(self.positions), _p(self.positions, d),
             _p(self.velocities, d), _p(self.initial_positions, d), len(self.bi),
             _p(self.bi, i32), _p(self.bj, i32), _p(self.rest, d), _p(self.alive, u8),
             _p(self.failure_mode, u8), _p(self.failure_time_s, d),
             _p(self.failure_extension_m, d), _p(self.failure_energy_J, d), _p(loads, d),
-            _p(hull, d), m, _p(force, d), _p(reaction, d), _p(scal, d))
+            _p(hull, d), m, _p(force, d), _p(reaction, d), _p(scal, d),
+            _p(self.eps_t_bond, d), _p(self.eps_s_bond, d))
         self.acc[1] += scal[0]
         self.acc[2] += scal[1]
         self._damping_power, self._tool_power = scal[2], scal[3]
@@ -205,13 +219,14 @@ class OraclePlate:
         loads = self._loads(external_forces)
         hull, m = self._hull()
         d, i32, u8 = ctypes.c_double, ctypes.c_int32, ctypes.c_uint8
-        bad = self.lib.orc_plate_step(
+        bad = self.lib.orc_plate_step_thr(
             _p(self._params(), d), len(self.positions), _p(self.positions, d),
             _p(self.velocities, d), _p(self.initial_positions, d), _p(self.fixed, u8),
             len(self.bi), _p(self.bi, i32), _p(self.bj, i32), _p(self.rest, d),
             _p(self.alive, u8), _p(self.failure_mode, u8), _p(self.failure_time_s, d),
             _p(self.failure_extension_m, d), _p(self.failure_energy_J, d), _p(loads, d),
-            _p(hull, d), m, _p(self.acc, d), _p(self.last, d), _p(self.force, d))
+            _p(hull, d), m, _p(self.acc, d), _p(self.last, d), _p(self.force, d),
+            _p(self.eps_t_bond, d), _p(self.eps_s_bond, d))
         self.time += self.dt
         self.step_count += 1
         if bad:

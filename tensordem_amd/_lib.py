@@ -118,7 +118,7 @@ _SIGNATURES = {
     "tdm_graph_launch": ([ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
     "tdm_bed_blocks": ([ctypes.c_int], ctypes.c_int),
     "tdm_bed_workspace_bytes": ([ctypes.c_int] * 6, ctypes.c_size_t),
-    "tdm_bed_workspace_layout": ([ctypes.c_int] * 6 + [ctypes.POINTER(ctypes.c_size_t)] * 4,
+    "tdm_bed_workspace_layout": ([ctypes.c_int] * 6 + [ctypes.POINTER(ctypes.c_size_t)] * 5,
                                  ctypes.c_int),
     "tdm_bed_setup": ([ctypes.POINTER(BedArrays), ctypes.c_void_p], ctypes.c_int),
     "tdm_bed_rebin": ([ctypes.POINTER(BedArrays), ctypes.c_void_p], ctypes.c_int),

@@ -98,7 +98,7 @@ class BedEngine:
         args = (n, blocks, nm, cells, cand_slots, cell_cap)
         wsb = self.lib.tdm_bed_workspace_bytes(*args)
         self.workspace = torch.zeros(int(wsb) + 256, dtype=torch.uint8, device=dev)
-        offs = [ctypes.c_size_t() for _ in range(4)]
+        offs = [ctypes.c_size_t() for _ in range(5)]
         _lib.check(self.lib.tdm_bed_workspace_layout(*args, *[ctypes.byref(o) for o in offs]))
         self._layout = [o.value for o in offs]
         self._arrays = _lib.BedArrays()
@@ -173,6 +173,9 @@ class BedEngine:
         raw = self.members.cpu().numpy().tobytes()
         return (_lib.BedMember * self.nmembers).from_buffer_copy(raw)
 
+    def write_members(self, members) -> None:
+        self.members.copy_(torch.frombuffer(bytearray(bytes(members)), dtype=torch.uint8))
+
     def status(self) -> np.ndarray:
         return np.array([m.status for m in self.read_members()], dtype=np.int32)
 
@@ -204,7 +207,7 @@ class BedEngine:
         out = []
         for m in ms:
             lp = m.list_parity
-            co, xo, mo, no = self._layout
+            co, xo, mo, no, _ = self._layout
             cand = u8[base + co: base + co + 2 * S * n * 4].view(torch.int32).view(2, S, n)[lp]
             mask = u8[base + mo: base + mo + 2 * n * 4].view(torch.int32).view(2, n)[lp]
             cnt = u8[base + no: base + no + 2 * n].view(2, n)[lp]

@@ -792,9 +792,10 @@ size_t tdm_bed_workspace_bytes(int n, int nblocks, int nmembers, int ncells, int
 
 int tdm_bed_workspace_layout(int n, int nblocks, int nmembers, int ncells, int cand_slots,
                              int cell_cap, size_t *cand_off, size_t *xi_off, size_t *mask_off,
-                             size_t *count_off)
+                             size_t *count_off, size_t *xb_off)
 {
-  if (!cand_off || !xi_off || !mask_off || !count_off) return fail(TDM_EINVAL, "null output");
+  if (!cand_off || !xi_off || !mask_off || !count_off || !xb_off)
+    return fail(TDM_EINVAL, "null output");
   BedWs w;
   size_t total = 0;
   (void)nmembers;
@@ -803,6 +804,7 @@ int tdm_bed_workspace_layout(int n, int nblocks, int nmembers, int ncells, int c
   *xi_off = reinterpret_cast<size_t>(w.xi);
   *mask_off = reinterpret_cast<size_t>(w.mask);
   *count_off = reinterpret_cast<size_t>(w.ncand);
+  *xb_off = reinterpret_cast<size_t>(w.xb);
   return TDM_OK;
 }
 

@@ -208,10 +208,10 @@ struct KArgs {
 // false when the pair does not overlap; else (Fx, Fy) is the force on lo's
 // partner side convention of the reference (`pair_force`, applied +F to lo,
 // -F to hi) and `diss` the pair's dissipated power (dem.py:327-328).
-template <typename R>
+template <typename R, typename F>
 __device__ inline bool contact_pair(double2 x_lo, double2 x_hi, double2 v_lo, double2 v_hi,
                                     double2 n0, double radius, double kc, double cdamp,
-                                    double &Fx, double &Fy, double &diss)
+                                    F &Fx, F &Fy, double &diss)
 {
   const double two_r = 2 * radius;
   if constexpr (Prec<R>::fp64) {
@@ -240,7 +240,7 @@ __device__ inline bool contact_pair(double2 x_lo, double2 x_hi, double2 v_lo, do
     float vn = (float)(v_hi.x - v_lo.x) * nx + (float)(v_hi.y - v_lo.y) * ny;
     float kcf = (float)kc;
     float rep = fmaxf(kcf * ov - (float)cdamp * vn, 0.0f);
-    Fx = (double)(-rep * nx); Fy = (double)(-rep * ny);
+    Fx = (F)(-rep * nx); Fy = (F)(-rep * ny);
     diss = (double)((rep - kcf * ov) * (-vn));
   }
   return true;
@@ -472,7 +472,9 @@ __global__ __launch_bounds__(kThreads) void plate_step_kernel(KArgs k)
     } else {
       // ---- all present bonds, independently (8-way instruction-level
       // parallelism); fracture commit and force per bond -------------------
-      double Fx[8], Fy[8];
+      // per-bond forces in the force precision (FP32 values widen exactly
+      // when they are summed into the FP64 total)
+      R Fx[8], Fy[8];
       uint32_t has = 0;
 #pragma unroll
       for (int c = 0; c < 8; ++c) {
@@ -601,8 +603,8 @@ __global__ __launch_bounds__(kThreads) void plate_step_kernel(KArgs k)
           }
           if (alive) {
             float mag = (float)kb * ext;
-            Fx[c] = (double)(mag * nxv);
-            Fy[c] = (double)(mag * nyv);
+            Fx[c] = mag * nxv;
+            Fy[c] = mag * nyv;
             has |= 1u << c;
           } else {
             const double2 vq = k.v_in[q];
@@ -633,7 +635,7 @@ __global__ __launch_bounds__(kThreads) void plate_step_kernel(KArgs k)
           unbonded(qc, fx, fy, false);
           ++ci;
         }
-        if ((has >> c) & 1u) { fx += Fx[c]; fy += Fy[c]; }
+        if ((has >> c) & 1u) { fx += (double)Fx[c]; fy += (double)Fy[c]; }
       }
       for (; ci < nc; ++ci) unbonded(cand[(size_t)ci * n], fx, fy, false);
       ci = 0;
@@ -647,7 +649,7 @@ __global__ __launch_bounds__(kThreads) void plate_step_kernel(KArgs k)
           unbonded(qc, fx, fy, false);
           ++ci;
         }
-        if ((has >> c) & 1u) { fx += -Fx[c]; fy += -Fy[c]; }
+        if ((has >> c) & 1u) { fx += -(double)Fx[c]; fy += -(double)Fy[c]; }
       }
       for (; ci < split; ++ci) unbonded(cand[(size_t)ci * n], fx, fy, false);
       a.links[p] = (uint8_t)links;

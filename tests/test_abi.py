@@ -58,12 +58,13 @@ def test_bed_host_only_entry_points():
     assert lib.tdm_bed_blocks(0) == _lib.TDM_EINVAL
     args = (1000, 4, 1, 300, 16, 8)
     total = lib.tdm_bed_workspace_bytes(*args)
-    offs = [ctypes.c_size_t() for _ in range(4)]
+    offs = [ctypes.c_size_t() for _ in range(5)]
     assert lib.tdm_bed_workspace_layout(*args, *[ctypes.byref(o) for o in offs]) == 0
-    cand, xi, mask, count = (o.value for o in offs)
+    cand, xi, mask, count, xb = (o.value for o in offs)
     # cand [2][16][1000] int32, then xi [2][16][1000] double2, mask [2][1000], ncand [2][1000]
     assert 0 < cand and cand % 256 == 0
     assert xi - cand >= 2 * 16 * 1000 * 4
     assert mask - xi >= 2 * 16 * 1000 * 16
     assert count - mask >= 2 * 1000 * 4
-    assert total >= count + 2 * 1000
+    assert xb - count >= 2 * 1000 and xb % 256 == 0
+    assert total >= xb + 16 * 1000

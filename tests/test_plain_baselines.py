@@ -56,3 +56,21 @@ def test_plain_bed_follows_the_oracle():
     assert np.abs(p.x.numpy() - ref.positions).max() < 1e-9
     assert p.acc[0].item() == pytest.approx(ref.acc[0], rel=1e-6)
     assert p.acc[5].item() == pytest.approx(ref.acc[5], rel=1e-6)
+
+
+def test_plain_plate_with_weibull_strengths_matches_the_oracle():
+    from oracle.plate import OraclePlate
+    from tensordem_amd.strength import weibull_strengths
+    cfg = DEMConfig(nx=12, ny=5, tool_speed=0.6, breaking_strain=0.01, shear_breaking_strain=0.02,
+                    device="cpu")
+    dt = cfg.degree_bound_dt()
+    et = weibull_strengths(60, 5.0, seed=1, mean=0.01)
+    es = weibull_strengths(60, 5.0, seed=2, mean=0.02)
+    ref = OraclePlate(cfg, dt=dt, eps_t_disk=et, eps_s_disk=es)
+    p = PlainFP64(cfg, dt, device="cpu", pinv=False, eps_t_disk=et, eps_s_disk=es)
+    for _ in range(1500):
+        ref.step()
+        p.step()
+    assert ref.broken_bonds > 5
+    assert np.array_equal(~p.alive.numpy(), ref.alive == 0)
+    assert np.abs(p.x.numpy() - ref.positions).max() < 1e-9
