@@ -619,6 +619,108 @@ def bench_ensemble(args):
     return res
 
 
+# ---------------------------------------------------------------------------
+# C5: serving, Poisson arrivals, continuous batching
+# ---------------------------------------------------------------------------
+def _plain_step_seconds(req, steps=3):
+    """Seconds per step of the plain FP64 eager path for one request, alone."""
+    import torch
+    if req.kind == "plate":
+        p = PlainFP64(req.spec.config, req.spec.dt, pinv=False)
+    else:
+        c = req.spec.config
+        r = req.spec.radii
+        dt = c.dt if c.dt is not None else c.stable_dt(float(r.min()))
+        p = PlainBedFP64(c, req.spec.positions, r, dt, 0.4 * float(r.mean()))
+    p.step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        p.step()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps
+
+
+def bench_serve(args):
+    """Poisson arrivals (50/s over the window), half plates and half beds,
+    1k-200k disks log-uniform, 5000 steps each, served by continuous batching
+    in real time; p50/p99 completion latency and aggregate particle-steps/s,
+    beside the same requests run one after another on the plain FP64 path
+    (estimated from sampled requests) and on the CPU oracle."""
+    from tensordem_amd.serve import Server, latency_summary, sweep_requests
+
+    reqs = sweep_requests(rate=50.0, window=args.serve_window, steps=5000, seed=0)
+    # warm the kernels, graphs and allocator on a few small requests
+    warm = sweep_requests(rate=50.0, window=0.2, steps=500, n_range=(1000, 3000), seed=99)
+    Server(chunk=args.serve_chunk).run(warm, realtime=False)
+    srv = Server(chunk=args.serve_chunk, max_active=args.serve_slots)
+    t0 = time.perf_counter()
+    done = srv.run(reqs, realtime=True)
+    wall = time.perf_counter() - t0
+    total_ps = float(sum(r.n * r.steps for r in done))
+    last = max(r.finished for r in done)
+    lat = latency_summary(done)
+    kinds = {k: [r for r in done if r.kind == k] for k in ("plate", "bed")}
+    res = {
+        "workload": f"Poisson arrivals 50/s over {args.serve_window:g} s (seed 0), half bonded "
+                    "plates (indenter, swept speed and strains) and half Hertz-Mindlin beds "
+                    "(swept friction and restitution), 1k-200k disks log-uniform, 5000 steps "
+                    f"each; continuous batching with {args.serve_chunk}-step rounds, at most "
+                    f"{args.serve_slots} members per batch, 288 GB HBM admission budget",
+        "value": total_ps / last, "unit": "particle-steps/s", "requests": len(done),
+        "dtype": "f32-force/f64-state", "steps": 5000, "ms_per_step": None,
+        "latency_s": lat,
+        "latency_s_by_kind": {k: latency_summary(v) for k, v in kinds.items()},
+        "offered_particle_steps_per_s": total_ps / args.serve_window,
+        "served_particle_steps": total_ps, "last_completion_s": last, "wall_s": wall,
+        "rounds": srv.rounds, "host_issue_seconds": srv.host_seconds,
+        "max_active": [max(p for _, p, _, _ in srv.log), max(b for _, _, b, _ in srv.log)],
+        "max_waiting": max(w for *_, w in srv.log),
+        "status_nonzero": sum(1 for r in done if r.result["status"]),
+    }
+    if not args.no_plain:
+        # sequential plain FP64: time per step = a + b n per kind, fitted on a
+        # sample spanning the size range
+        fits = {}
+        for k, rs in kinds.items():
+            rs = sorted(rs, key=lambda r: r.n)
+            pick = [rs[int(i)] for i in np.linspace(0, len(rs) - 1, 6)]
+            ns = np.array([r.n for r in pick], dtype=np.float64)
+            ts = np.array([_plain_step_seconds(r) for r in pick])
+            b, a = np.polyfit(ns, ts, 1)
+            fits[k] = (max(a, 0.0), max(b, 0.0), ns.tolist(), ts.tolist())
+        seq_s = sum(r.steps * (fits[r.kind][0] + fits[r.kind][1] * r.n) for r in done)
+        res["plain_fp64_gpu"] = dict(
+            value=total_ps / seq_s, unit="particle-steps/s", sequential_seconds=seq_s,
+            fit_seconds_per_step={k: {"a": f[0], "b_per_disk": f[1], "n": f[2], "t": f[3]}
+                                  for k, f in fits.items()},
+            what="the same requests one after another on the eager PyTorch FP64 paths "
+                 "(time per step fitted as a + b n per kind on 6 sampled requests each)")
+        res["speedup_vs_plain_fp64"] = res["value"] / res["plain_fp64_gpu"]["value"]
+    if not args.no_cpu:
+        from oracle.bed import OracleBed
+        from oracle.plate import OraclePlate
+        t_c = time.perf_counter()
+        ps, k = 0, 0
+        small = sorted(done, key=lambda r: r.n)
+        while time.perf_counter() - t_c < args.cpu_seconds and k < len(small):
+            r = small[k * max(1, len(small) // 40)] if k * max(1, len(small) // 40) < len(small) \
+                else small[-1]
+            if r.kind == "plate":
+                sim = OraclePlate(r.spec.config, dt=r.spec.dt)
+            else:
+                sim = OracleBed(r.spec.config, r.spec.positions, r.spec.radii)
+            for _ in range(5):
+                sim.step()
+            ps += 5 * r.n
+            k += 1
+        el = time.perf_counter() - t_c
+        res["cpu_baseline"] = dict(value=ps / el, unit="particle-steps/s", cores=1, kind="port",
+                                   sample=f"{k} requests across the size range, 5 steps each "
+                                          f"(FP64 C oracles), {el:.1f} s")
+    return res
+
+
 def cpu_baseline(cfg, dt, budget_s=12.0):
     """C oracle (FP64, one core) on the same plate, run for ~budget_s."""
     from oracle.plate import OraclePlate
@@ -648,9 +750,13 @@ def main(argv=None):
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--workload", choices=("all", "plate", "bed", "ensemble"), default="all",
+    ap.add_argument("--workload", choices=("all", "plate", "bed", "ensemble", "serve"),
+                    default="all",
                     help="plate: the 1024x128 plate only; bed: the 1M-disk column only; "
-                         "ensemble: the 512-plate batch only")
+                         "ensemble: the 512-plate batch only; serve: the request loop only")
+    ap.add_argument("--serve-window", type=float, default=60.0)
+    ap.add_argument("--serve-chunk", type=int, default=250)
+    ap.add_argument("--serve-slots", type=int, default=64)
     ap.add_argument("--ens-steps", type=int, default=10000)
     ap.add_argument("--bed-steps", type=int, default=50000)
     ap.add_argument("--bed-plain-steps", type=int, default=60)
@@ -664,14 +770,16 @@ def main(argv=None):
         print("bench.py measures one GPU: every plate and request of this workload runs on a "
               "single MI355X and no multi-GPU path is built (see DESIGN.md).", file=sys.stderr)
         return 2
-    if args.workload in ("bed", "ensemble"):
-        bed = bench_bed(args) if args.workload == "bed" else bench_ensemble(args)
+    if args.workload in ("bed", "ensemble", "serve"):
+        bed = {"bed": bench_bed, "ensemble": bench_ensemble, "serve": bench_serve}[
+            args.workload](args)
         line = {"metric": METRIC, "value": bed.pop("value"), "unit": "particle-steps/s",
                 "n_gpus": 1, "steps": bed["steps"], "warmup": 1000,
                 "ms_per_step": bed["ms_per_step"], "higher_is_better": True, "scaling": "weak",
                 "vs_baseline": None, "dtype": bed.pop("dtype"),
-                "data": "synthetic: " + ("seeded loose column packing" if args.workload == "bed"
-                                         else "seeded parameter sweep of plates"),
+                "data": "synthetic: " + {"bed": "seeded loose column packing",
+                                         "ensemble": "seeded parameter sweep of plates",
+                                         "serve": "seeded Poisson request stream"}[args.workload],
                 "config": {"workload": bed.pop("workload")}}
         line.update(bed)
         print(json.dumps(line))
@@ -680,6 +788,7 @@ def main(argv=None):
     if args.workload == "all":
         line["c3_bed"] = bench_bed(args)
         line["c4_ensemble"] = bench_ensemble(args)
+        line["c5_serve"] = bench_serve(args)
     print(json.dumps(line))
     return 0
 

@@ -30,70 +30,99 @@ class BedSpec:
     velocities: np.ndarray | None = None
 
 
+@dataclass
+class BedParts:
+    """One bed's static data on the device and its member-record template
+    (batch-relative offsets left at zero); a batch is assembled from parts by
+    concatenation on the device."""
+    member: object
+    n: int
+    nblocks: int
+    ncells: int
+    x0: torch.Tensor
+    v0: torch.Tensor
+    radius: torch.Tensor
+
+
+def bed_parts(s: BedSpec, *, device="cuda", skin_factor: float = 0.4,
+              runout: float = 1.0) -> BedParts:
+    lib = _lib.load()
+    dev = torch.device(device)
+    c = s.config
+    x = np.asarray(s.positions, np.float64)
+    r = np.asarray(s.radii, np.float64)
+    if x.ndim != 2 or x.shape[1] != 2 or r.shape != (len(x),) or len(x) == 0:
+        raise ValueError("positions must be (n,2) and radii (n,)")
+    m = _lib.BedMember()
+    m.e_star, m.g_star, m.damping = c.e_star, c.g_star, c.damping_coefficient
+    m.friction, m.wall_friction, m.rho_pi_t = c.friction, c.wall_friction, c.rho_pi_t
+    m.gravity_y = -c.gravity
+    m.dt = float(s.dt if s.dt is not None else
+                 (c.dt if c.dt is not None else c.stable_dt(float(r.min()))))
+    m.floor_y, m.back_wall_x = c.floor_y, c.back_wall_x
+    m.time, m.step_count = 0.0, 0
+    r0 = float(r.mean())
+    m.skin = skin_factor * r0
+    m.cell_size = 2.0 * float(r.max()) + m.skin
+    lo = x.min(0) - r.max()
+    hi = x.max(0) + r.max()
+    w, h = hi - lo
+    span = max(w, h)
+    x0 = min(lo[0], c.back_wall_x) - 2 * m.cell_size
+    y0 = min(lo[1], c.floor_y) - 2 * m.cell_size
+    m.grid_origin[0], m.grid_origin[1] = x0, y0
+    m.ncx = int(math.ceil((hi[0] + runout * span - x0) / m.cell_size)) + 1
+    m.ncy = int(math.ceil((hi[1] + 0.25 * span - y0) / m.cell_size)) + 1
+    m.n = len(x)
+    m.use_floor, m.use_back_wall = int(c.floor), int(c.back_wall)
+    m.nblocks = lib.tdm_bed_blocks(len(x))
+    m.mode, m.tab, m.list_parity, m.status = 1, 0, 0, 0
+    f64 = dict(dtype=torch.float64, device=dev)
+    v = np.zeros_like(x) if s.velocities is None else np.asarray(s.velocities, np.float64)
+    return BedParts(member=m, n=len(x), nblocks=m.nblocks, ncells=m.ncx * m.ncy,
+                    x0=torch.tensor(x, **f64), v0=torch.tensor(v, **f64),
+                    radius=torch.tensor(r, **f64))
+
+
 class BedEngine:
     def __init__(self, specs: list[BedSpec], *, device="cuda", fp32: bool = False,
                  skin_factor: float = 0.4, cand_slots: int = 16, cell_cap: int = 8,
-                 runout: float = 1.0):
+                 runout: float = 1.0, parts: list[BedParts] | None = None):
         self.lib = _lib.load()
         self.device = torch.device(device)
         if self.device.type != "cuda" or not torch.cuda.is_available():
             raise ValueError("BedEngine runs on the ROCm device (device='cuda')")
         self.specs = list(specs)
         self.fp32 = bool(fp32)
-        nm = len(self.specs)
+        if parts is None:
+            parts = [bed_parts(s, device=self.device, skin_factor=skin_factor, runout=runout)
+                     for s in self.specs]
+        if len(parts) != len(self.specs):
+            raise ValueError("one BedParts per spec")
+        nm = len(parts)
         members = (_lib.BedMember * nm)()
-        xs, vs, rs = [], [], []
         n = blocks = cells = 0
-        for k, s in enumerate(self.specs):
-            c = s.config
-            x = np.asarray(s.positions, np.float64)
-            r = np.asarray(s.radii, np.float64)
-            if x.ndim != 2 or x.shape[1] != 2 or r.shape != (len(x),) or len(x) == 0:
-                raise ValueError("positions must be (n,2) and radii (n,)")
+        for k, pt in enumerate(parts):
+            ctypes.memmove(ctypes.byref(members[k]), ctypes.byref(pt.member),
+                           ctypes.sizeof(_lib.BedMember))
             m = members[k]
-            m.e_star, m.g_star, m.damping = c.e_star, c.g_star, c.damping_coefficient
-            m.friction, m.wall_friction, m.rho_pi_t = c.friction, c.wall_friction, c.rho_pi_t
-            m.gravity_y = -c.gravity
-            m.dt = float(s.dt if s.dt is not None else
-                         (c.dt if c.dt is not None else c.stable_dt(float(r.min()))))
-            m.floor_y, m.back_wall_x = c.floor_y, c.back_wall_x
-            m.time, m.step_count = 0.0, 0
-            r0 = float(r.mean())
-            m.skin = skin_factor * r0
-            m.cell_size = 2.0 * float(r.max()) + m.skin
-            lo = x.min(0) - r.max()
-            hi = x.max(0) + r.max()
-            w, h = hi - lo
-            span = max(w, h)
-            x0 = min(lo[0], c.back_wall_x) - 2 * m.cell_size
-            y0 = min(lo[1], c.floor_y) - 2 * m.cell_size
-            m.grid_origin[0], m.grid_origin[1] = x0, y0
-            m.ncx = int(math.ceil((hi[0] + runout * span - x0) / m.cell_size)) + 1
-            m.ncy = int(math.ceil((hi[1] + 0.25 * span - y0) / m.cell_size)) + 1
-            m.cell_first = cells
-            cells += m.ncx * m.ncy
-            m.first, m.n = n, len(x)
-            m.use_floor, m.use_back_wall = int(c.floor), int(c.back_wall)
-            nb = self.lib.tdm_bed_blocks(len(x))
-            m.block_first, m.nblocks = blocks, nb
-            blocks += nb
-            m.mode, m.tab, m.list_parity, m.status = 1, 0, 0, 0
-            xs.append(x)
-            rs.append(r)
-            vs.append(np.zeros_like(x) if s.velocities is None else np.asarray(s.velocities, np.float64))
-            n += len(x)
+            m.first, m.block_first, m.cell_first = n, blocks, cells
+            n += pt.n
+            blocks += pt.nblocks
+            cells += pt.ncells
         self.n, self.nmembers, self.nblocks, self.ncells = n, nm, blocks, cells
         self.cand_slots, self.cell_cap = cand_slots, cell_cap
         dev = self.device
         f64 = dict(dtype=torch.float64, device=dev)
-        self.x = torch.tensor(np.concatenate(xs), **f64)
-        self.v = torch.tensor(np.concatenate(vs), **f64)
+        self.x = torch.cat([pt.x0 for pt in parts])
+        self.v = torch.cat([pt.v0 for pt in parts])
         self.x_alt = torch.empty_like(self.x)
         self.v_alt = torch.empty_like(self.v)
-        self.radius = torch.tensor(np.concatenate(rs), **f64)
+        self.radius = torch.cat([pt.radius for pt in parts])
         self.xi_wall = torch.zeros((n, 4), **f64)
         self.accounting = torch.zeros((nm, _lib.BED_NACC), **f64)
         self.energy = torch.zeros((nm, 4), **f64)
+        self._members_host = members
         self.members = torch.frombuffer(bytearray(bytes(members)), dtype=torch.uint8).to(dev)
         args = (n, blocks, nm, cells, cand_slots, cell_cap)
         wsb = self.lib.tdm_bed_workspace_bytes(*args)

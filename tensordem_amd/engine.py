@@ -45,10 +45,96 @@ def default_tile(nx: int) -> int:
     return 16
 
 
+@dataclass
+class PlateParts:
+    """One member's static data on the device and its member-record template
+    (batch-relative offsets left at zero).  Built once per plate; a batch is
+    assembled from parts by concatenation on the device."""
+    member: object
+    n: int
+    nbonds: int
+    ntiles: int
+    ncells: int
+    x0: torch.Tensor
+    fixed: torch.Tensor
+    fwd: torch.Tensor
+    bi: torch.Tensor
+    bj: torch.Tensor
+    rest: torch.Tensor
+    eps_t: torch.Tensor | None
+    eps_s: torch.Tensor | None
+    hull: torch.Tensor | None
+
+
+def plate_parts(s: PlateSpec, *, device="cuda", skin_factor: float = 1.0,
+                grid_margin: float = 0.25, fp32_margin: float = 1e-3) -> PlateParts:
+    lib = _lib.load()
+    dev = torch.device(device)
+    c = s.config
+    pos = positions(c.nx, c.ny, c.radius)
+    bi, bj, rl, fw = bonds(c.nx, c.ny, c.radius)
+    m = _lib.Member()
+    m.radius, m.bond_stiffness, m.contact_stiffness = c.radius, c.bond_stiffness, c.contact_stiffness
+    m.breaking_strain, m.shear_breaking_strain = c.breaking_strain, c.shear_breaking_strain
+    m.contact_damping, m.drag, m.tool_radius = c.contact_damping, c.drag, c.tool_radius
+    m.dt = float(s.dt if s.dt is not None else (c.dt if c.dt is not None else c.recommended_dt))
+    m.mass = c.mass
+    hull = None
+    if s.hull_profile is None:
+        m.tool_start[0] = 0.0
+        m.tool_start[1] = (c.ny - 1) * 2 * c.radius + c.radius + c.tool_radius + c.tool_gap
+        m.tool_velocity[0], m.tool_velocity[1] = 0.0, -c.tool_speed
+        m.hull_m = 0
+    else:
+        hp = np.asarray(s.hull_profile, dtype=np.float64)
+        m.tool_start[0], m.tool_start[1] = s.hull_start
+        m.tool_velocity[0], m.tool_velocity[1] = s.hull_velocity
+        m.hull_m = len(hp)
+        hull = torch.tensor(hp, dtype=torch.float64, device=dev)
+    m.time = 0.0
+    m.skin = skin_factor * c.radius
+    m.cell_size = 2 * c.radius + m.skin
+    w = (c.nx - 1) * 2 * c.radius
+    h = (c.ny - 1) * 2 * c.radius
+    mx = grid_margin * max(w, h) + 2 * m.cell_size
+    m.grid_origin[0] = pos[:, 0].min() - mx
+    m.grid_origin[1] = -mx
+    m.ncx = int(math.ceil((w + 2 * mx) / m.cell_size)) + 1
+    m.ncy = int(math.ceil((h + 2 * mx) / m.cell_size)) + 1
+    m.fp32_margin = fp32_margin
+    m.step_count = 0
+    m.nx, m.ny = c.nx, c.ny
+    m.nbonds = len(bi)
+    m.tile_x = s.tile_x or default_tile(c.nx)
+    nt = lib.tdm_plate_tiles(c.nx, c.ny, m.tile_x)
+    _lib.check(nt if nt < 0 else 0)
+    m.nblocks = nt
+    m.mode, m.tab, m.status, m.has_loads = 1, 0, 0, 0
+    m.per_disk_strength = int(s.eps_t_disk is not None)
+    f64 = dict(dtype=torch.float64, device=dev)
+    n = c.nx * c.ny
+    et = es = None
+    if s.eps_t_disk is not None or s.eps_s_disk is not None:
+        et = torch.tensor(np.full(n, c.breaking_strain) if s.eps_t_disk is None
+                          else np.asarray(s.eps_t_disk, np.float64), **f64)
+        es = torch.tensor(np.full(n, c.shear_breaking_strain) if s.eps_s_disk is None
+                          else np.asarray(s.eps_s_disk, np.float64), **f64)
+        m.per_disk_strength = 1
+    return PlateParts(
+        member=m, n=n, nbonds=len(bi), ntiles=nt, ncells=m.ncx * m.ncy,
+        x0=torch.tensor(pos, **f64),
+        fixed=torch.tensor(fixed_mask(c.nx, c.ny, c.fix_edges), dtype=torch.bool, device=dev),
+        fwd=torch.tensor(fw, dtype=torch.int32, device=dev),
+        bi=torch.tensor(bi, dtype=torch.int64, device=dev),
+        bj=torch.tensor(bj, dtype=torch.int64, device=dev),
+        rest=torch.tensor(rl, **f64), eps_t=et, eps_s=es, hull=hull)
+
+
 class PlateEngine:
     def __init__(self, specs: list[PlateSpec], *, device: str | torch.device = "cuda",
                  fp32: bool = False, skin_factor: float = 1.0, cand_slots: int = 16,
-                 cell_cap: int = 8, grid_margin: float = 0.25, fp32_margin: float = 1e-3):
+                 cell_cap: int = 8, grid_margin: float = 0.25, fp32_margin: float = 1e-3,
+                 parts: list[PlateParts] | None = None):
         self.lib = _lib.load()
         self.device = torch.device(device)
         if self.device.type != "cuda":
@@ -57,98 +143,65 @@ class PlateEngine:
             raise RuntimeError("no ROCm device available")
         self.specs = list(specs)
         self.fp32 = bool(fp32)
-        nm = len(self.specs)
+        if parts is None:
+            parts = [plate_parts(s, device=self.device, skin_factor=skin_factor,
+                                 grid_margin=grid_margin, fp32_margin=fp32_margin)
+                     for s in self.specs]
+        if len(parts) != len(self.specs):
+            raise ValueError("one PlateParts per spec")
+        nm = len(parts)
         members = (_lib.Member * nm)()
-        xs, fixed, fwd, eps_t, eps_s, hull = [], [], [], [], [], []
-        bond_i, bond_j, rest = [], [], []
         n = nb = blocks = cells = hull_n = 0
-        per_disk = any(s.eps_t_disk is not None for s in self.specs)
-        for k, s in enumerate(self.specs):
-            c = s.config
-            pos = positions(c.nx, c.ny, c.radius)
-            bi, bj, rl, fw = bonds(c.nx, c.ny, c.radius)
+        per_disk = any(pt.eps_t is not None for pt in parts)
+        fwd, bond_i, bond_j, eps_t, eps_s, hull = [], [], [], [], [], []
+        for k, pt in enumerate(parts):
+            ctypes.memmove(ctypes.byref(members[k]), ctypes.byref(pt.member),
+                           ctypes.sizeof(_lib.Member))
             m = members[k]
-            m.radius, m.bond_stiffness, m.contact_stiffness = c.radius, c.bond_stiffness, c.contact_stiffness
-            m.breaking_strain, m.shear_breaking_strain = c.breaking_strain, c.shear_breaking_strain
-            m.contact_damping, m.drag, m.tool_radius = c.contact_damping, c.drag, c.tool_radius
-            m.dt = float(s.dt if s.dt is not None else (c.dt if c.dt is not None else c.recommended_dt))
-            m.mass = c.mass
-            if s.hull_profile is None:
-                m.tool_start[0] = 0.0
-                m.tool_start[1] = ((c.ny - 1) * 2 * c.radius + c.radius + c.tool_radius
-                                   + c.tool_gap)
-                m.tool_velocity[0], m.tool_velocity[1] = 0.0, -c.tool_speed
-                m.hull_first, m.hull_m = 0, 0
-            else:
-                hp = np.asarray(s.hull_profile, dtype=np.float64)
-                m.tool_start[0], m.tool_start[1] = s.hull_start
-                m.tool_velocity[0], m.tool_velocity[1] = s.hull_velocity
-                m.hull_first, m.hull_m = hull_n, len(hp)
-                hull.append(hp)
-                hull_n += len(hp)
-            m.time = 0.0
-            m.skin = skin_factor * c.radius
-            m.cell_size = 2 * c.radius + m.skin
-            w = (c.nx - 1) * 2 * c.radius
-            h = (c.ny - 1) * 2 * c.radius
-            mx = grid_margin * max(w, h) + 2 * m.cell_size
-            m.grid_origin[0] = pos[:, 0].min() - mx
-            m.grid_origin[1] = -mx
-            m.ncx = int(math.ceil((w + 2 * mx) / m.cell_size)) + 1
-            m.ncy = int(math.ceil((h + 2 * mx) / m.cell_size)) + 1
-            m.cell_first = cells
-            cells += m.ncx * m.ncy
-            m.fp32_margin = fp32_margin
-            m.step_count = 0
-            m.nx, m.ny, m.first = c.nx, c.ny, n
-            m.bond_first, m.nbonds = nb, len(bi)
-            m.tile_x = s.tile_x or default_tile(c.nx)
-            nt = self.lib.tdm_plate_tiles(c.nx, c.ny, m.tile_x)
-            _lib.check(nt if nt < 0 else 0)
-            m.block_first, m.nblocks = blocks, nt
-            blocks += nt
-            m.mode, m.tab, m.status, m.has_loads = 1, 0, 0, 0
-            m.per_disk_strength = int(s.eps_t_disk is not None)
-            xs.append(pos)
-            fixed.append(fixed_mask(c.nx, c.ny, c.fix_edges))
-            fwd.append(fw + nb)
-            bond_i.append(bi + n)
-            bond_j.append(bj + n)
-            rest.append(rl)
+            m.first, m.bond_first, m.block_first, m.cell_first = n, nb, blocks, cells
+            m.hull_first = hull_n
+            if pt.hull is not None:
+                hull.append(pt.hull)
+                hull_n += len(pt.hull)
+            fwd.append(pt.fwd + nb)
+            bond_i.append(pt.bi + n)
+            bond_j.append(pt.bj + n)
             if per_disk:
-                npd = c.nx * c.ny
-                eps_t.append(np.full(npd, c.breaking_strain) if s.eps_t_disk is None
-                             else np.asarray(s.eps_t_disk, np.float64))
-                eps_s.append(np.full(npd, c.shear_breaking_strain) if s.eps_s_disk is None
-                             else np.asarray(s.eps_s_disk, np.float64))
-            n += c.nx * c.ny
-            nb += len(bi)
+                c = self.specs[k].config
+                eps_t.append(pt.eps_t if pt.eps_t is not None else
+                             torch.full((pt.n,), c.breaking_strain, dtype=torch.float64,
+                                        device=self.device))
+                eps_s.append(pt.eps_s if pt.eps_s is not None else
+                             torch.full((pt.n,), c.shear_breaking_strain, dtype=torch.float64,
+                                        device=self.device))
+            n += pt.n
+            nb += pt.nbonds
+            blocks += pt.ntiles
+            cells += pt.ncells
         self.n, self.nbonds, self.nmembers, self.nblocks, self.ncells = n, nb, nm, blocks, cells
         self.cand_slots, self.cell_cap = cand_slots, cell_cap
         dev = self.device
         f64 = dict(dtype=torch.float64, device=dev)
-        x0 = np.concatenate(xs)
-        self.initial_positions = torch.tensor(x0, **f64)
+        self.initial_positions = torch.cat([pt.x0 for pt in parts])
         self.x = self.initial_positions.clone()
         self.v = torch.zeros_like(self.x)
         self.x_alt = torch.empty_like(self.x)
         self.v_alt = torch.empty_like(self.x)
-        self.fixed = torch.tensor(np.concatenate(fixed), dtype=torch.bool, device=dev)
+        self.fixed = torch.cat([pt.fixed for pt in parts])
         self.links = torch.zeros(n, dtype=torch.uint8, device=dev)
-        self.fwd_bond = torch.tensor(np.concatenate(fwd), dtype=torch.int32, device=dev)
-        self.bond_i = torch.tensor(np.concatenate(bond_i), dtype=torch.int64, device=dev)
-        self.bond_j = torch.tensor(np.concatenate(bond_j), dtype=torch.int64, device=dev)
-        self.rest_lengths = torch.tensor(np.concatenate(rest), **f64)
+        self.fwd_bond = torch.cat(fwd)
+        self.bond_i = torch.cat(bond_i)
+        self.bond_j = torch.cat(bond_j)
+        self.rest_lengths = torch.cat([pt.rest for pt in parts])
         self.alive = torch.ones(nb, dtype=torch.bool, device=dev)
         self.failure_mode = torch.zeros(nb, dtype=torch.uint8, device=dev)
         self.failure_time_s = torch.full((nb,), -1.0, **f64)
         self.failure_extension_m = torch.zeros(nb, **f64)
         self.failure_energy_J = torch.zeros(nb, **f64)
         self.loads = torch.zeros((n, 2), **f64)
-        self.hull = (torch.tensor(np.concatenate(hull), **f64) if hull
-                     else torch.zeros((1, 2), **f64))
-        self.eps_t_disk = torch.tensor(np.concatenate(eps_t), **f64) if per_disk else None
-        self.eps_s_disk = torch.tensor(np.concatenate(eps_s), **f64) if per_disk else None
+        self.hull = torch.cat(hull) if hull else torch.zeros((1, 2), **f64)
+        self.eps_t_disk = torch.cat(eps_t) if per_disk else None
+        self.eps_s_disk = torch.cat(eps_s) if per_disk else None
         self.accounting = torch.zeros((nm, _lib.NACC), **f64)
         self.last = torch.zeros((nm, _lib.NLAST), **f64)
         self.force = torch.zeros((n, 2), **f64)

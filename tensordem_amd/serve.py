@@ -24,6 +24,7 @@ minus its arrival time.
 """
 from __future__ import annotations
 
+import ctypes
 import math
 import time
 from dataclasses import dataclass, replace
@@ -32,10 +33,10 @@ import numpy as np
 import torch
 
 from . import _lib
-from .bed import BedEngine, BedSpec
+from .bed import BedEngine, BedSpec, bed_parts
 from .bedconfig import BedConfig, column_packing
 from .config import DEMConfig
-from .engine import PlateEngine, PlateSpec
+from .engine import PlateEngine, PlateSpec, plate_parts
 
 
 @dataclass
@@ -130,16 +131,110 @@ def _ranges(firsts, counts, device):
     return torch.cat([torch.arange(f, f + c, device=device) for f, c in zip(firsts, counts)])
 
 
+def _field(struct, name):
+    f = getattr(struct, name)
+    return f.offset, f.size
+
+
+def _copy_fields(dst_members, src_members, dst_idx, src_idx, struct, names):
+    """Device-to-device copy of member-record fields (no host round trip)."""
+    sz = ctypes.sizeof(struct)
+    d = dst_members.view(-1, sz)
+    s_ = src_members.view(-1, sz)
+    di = torch.tensor(dst_idx, device=d.device)
+    si = torch.tensor(src_idx, device=d.device)
+    for nm in names:
+        off, size = _field(struct, nm)
+        d[di, off:off + size] = s_[si, off:off + size]
+
+
+def _zero_fields(members, idx, struct, names):
+    sz = ctypes.sizeof(struct)
+    d = members.view(-1, sz)
+    di = torch.tensor(idx, device=d.device)
+    for nm in names:
+        off, size = _field(struct, nm)
+        d[di, off:off + size] = 0
+
+
+def _carry_plate(old: PlateEngine, old_members, eng: PlateEngine, members, keep) -> None:
+    """Queue the continuing members' state copies (device only)."""
+    om, nm = old._members_host, eng._members_host
+    dev = eng.x.device
+    si = [old_members.index(r) for r in keep]
+    di = [members.index(r) for r in keep]
+    cnt_d = [om[i].nx * om[i].ny for i in si]
+    sd = _ranges([om[i].first for i in si], cnt_d, dev)
+    dd = _ranges([nm[j].first for j in di], cnt_d, dev)
+    cnt_b = [om[i].nbonds for i in si]
+    sb = _ranges([om[i].bond_first for i in si], cnt_b, dev)
+    db = _ranges([nm[j].bond_first for j in di], cnt_b, dev)
+    eng.x[dd] = old.x[sd]
+    eng.v[dd] = old.v[sd]
+    for name in ("alive", "failure_mode", "failure_time_s", "failure_extension_m",
+                 "failure_energy_J"):
+        getattr(eng, name)[db] = getattr(old, name)[sb]
+    ti, tj = torch.tensor(si, device=dev), torch.tensor(di, device=dev)
+    eng.accounting[tj] = old.accounting[ti]
+    eng.last[tj] = old.last[ti]
+    _copy_fields(eng.members, old.members, di, si, _lib.Member, ("time", "step_count", "status"))
+    eng.sync_links()
+    eng.rebin()
+
+
+def _carry_bed(old: BedEngine, old_members, eng: BedEngine, members, keep) -> None:
+    """Queue the continuing members' state copies (device only): state, wall
+    springs, ledger, clocks, and the current candidate lists with their spring
+    histories and list origins (partner indices shifted to the new place)."""
+    om, nm = old._members_host, eng._members_host
+    dev = eng.x.device
+    S = old.cand_slots
+    si = [old_members.index(r) for r in keep]
+    di = [members.index(r) for r in keep]
+    cnt = [om[i].n for i in si]
+    sd = _ranges([om[i].first for i in si], cnt, dev)
+    dd = _ranges([nm[j].first for j in di], cnt, dev)
+    cnt_t = torch.tensor(cnt, device=dev)
+    shift = torch.tensor([nm[j].first - om[i].first for i, j in zip(si, di)], device=dev)
+    sh = torch.repeat_interleave(shift, cnt_t).to(torch.int32)
+    off, _ = _field(_lib.BedMember, "list_parity")
+    sz = ctypes.sizeof(_lib.BedMember)
+    lp_m = old.members.view(-1, sz)[:, off:off + 4].contiguous().view(torch.int32).view(-1)
+    lp = torch.repeat_interleave(lp_m[torch.tensor(si, device=dev)].long(), cnt_t)
+    eng.x[dd] = old.x[sd]
+    eng.v[dd] = old.v[sd]
+    eng.xi_wall[dd] = old.xi_wall[sd]
+    n_old, n_new = old.n, eng.n
+    oc = old._ws_view(0, torch.int32, (2, S, n_old))
+    eng._ws_view(0, torch.int32, (2, S, n_new))[0][:, dd] = oc.permute(1, 0, 2)[:, lp, sd] + sh
+    ox = old._ws_view(1, torch.float64, (2, S, n_old, 2))
+    eng._ws_view(1, torch.float64, (2, S, n_new, 2))[0][:, dd] = ox.permute(1, 0, 2, 3)[:, lp, sd]
+    eng._ws_view(2, torch.int32, (2, n_new))[0][dd] = old._ws_view(2, torch.int32, (2, n_old))[lp, sd]
+    eng._ws_view(3, torch.uint8, (2, n_new))[0][dd] = old._ws_view(3, torch.uint8, (2, n_old))[lp, sd]
+    eng._ws_view(4, torch.float64, (n_new, 2))[dd] = old._ws_view(4, torch.float64, (n_old, 2))[sd]
+    ti, tj = torch.tensor(si, device=dev), torch.tensor(di, device=dev)
+    eng.accounting[tj] = old.accounting[ti]
+    _copy_fields(eng.members, old.members, di, si, _lib.BedMember,
+                 ("time", "step_count", "status", "rebuilds"))
+    # the carried lists are current: no rebuild is pending for these members
+    _zero_fields(eng.members, di, _lib.BedMember, ("mode", "tab", "list_parity"))
+
+
 class _Batch:
     """One kind's device batch: the engine, its members (requests) and the
-    steps each has done."""
+    steps each has done.  All device work of a round is queued on the batch
+    stream; a new engine is built on a side stream meanwhile."""
 
-    def __init__(self, kind: str, fp32: bool, device):
+    def __init__(self, kind: str, fp32: bool, device, keep_state: bool):
         self.kind, self.fp32, self.device = kind, fp32, torch.device(device)
+        self.keep_state = keep_state
         self.engine = None
         self.members: list[Request] = []
         self.done: dict[int, int] = {}
-        self._stream = None
+        self.parts: dict[int, object] = {}
+        self.retired: list = []     # (engine, event): closed once the event has passed
+        self.pending: list = []     # (requests, buffers, event): results in flight
+        self._stream = self._prep = None
 
     @property
     def stream(self):
@@ -147,131 +242,138 @@ class _Batch:
             self._stream = torch.cuda.Stream(self.device)
         return self._stream
 
-    # -- repack ----------------------------------------------------------------
-    def repack(self, keep: list[Request], join: list[Request]) -> None:
+    @property
+    def prep(self):
+        if self._prep is None:
+            self._prep = torch.cuda.Stream(self.device)
+        return self._prep
+
+    def _parts_of(self, r: Request):
+        if r.rid not in self.parts:
+            with torch.cuda.stream(self.prep):
+                self.parts[r.rid] = (plate_parts(r.spec, device=self.device)
+                                     if self.kind == "plate"
+                                     else bed_parts(r.spec, device=self.device))
+        return self.parts[r.rid]
+
+    # -- one scheduling round ----------------------------------------------------
+    def round(self, join: list[Request], chunk: int, now: float):
+        fin = [r for r in self.members if self.done[r.rid] >= r.steps]
+        keep = [r for r in self.members if self.done[r.rid] < r.steps]
+        if fin:
+            self._extract(fin, now)
+        if fin or join:
+            self._repack(keep, join, chunk)
+        if self.engine is not None:
+            with torch.cuda.stream(self.stream):
+                self.engine.run_graph(chunk // 2)
+            for r in self.members:
+                self.done[r.rid] = self.done.get(r.rid, 0) + chunk
+        ev = torch.cuda.Event(enable_timing=True)
+        ev.record(self.stream)
+        self._close_retired()
+        return ev
+
+    def _extract(self, fin: list[Request], now: float) -> None:
+        eng = self.engine
+        hm = eng._members_host
+        sz = ctypes.sizeof(type(hm[0]))
+        bufs = []
+        with torch.cuda.stream(self.stream):
+            for r in fin:
+                k = self.members.index(r)
+                m = hm[k]
+                n = m.nx * m.ny if self.kind == "plate" else m.n
+                x = eng.x[m.first:m.first + n]
+                vals = [x.sum().reshape(1), eng.accounting[k]]
+                extra = {}
+                if self.kind == "plate":
+                    alive = eng.alive[m.bond_first:m.bond_first + m.nbonds]
+                    vals.insert(1, (~alive).sum().to(torch.float64).reshape(1))
+                    if self.keep_state:
+                        extra["alive"] = alive
+                if self.keep_state:
+                    extra["x"], extra["v"] = x, eng.v[m.first:m.first + n]
+                dev_parts = {"scalars": torch.cat(vals),
+                             "member": eng.members.view(-1, sz)[k].clone(), **extra}
+                host = {key: torch.empty(t.shape, dtype=t.dtype, pin_memory=True)
+                        for key, t in dev_parts.items()}
+                for key, t in dev_parts.items():
+                    host[key].copy_(t, non_blocking=True)
+                bufs.append((r, host, dev_parts))
+        ev = torch.cuda.Event(enable_timing=True)
+        ev.record(self.stream)
+        for r in fin:
+            del self.done[r.rid]
+            self.parts.pop(r.rid, None)
+        self.pending.append((bufs, ev, now))
+
+    def collect(self, to_time) -> list[Request]:
+        """Results whose copies have landed (to_time maps the event to the
+        request's completion time)."""
+        out, keep = [], []
+        for bufs, ev, now in self.pending:
+            if not ev.query():
+                keep.append((bufs, ev, now))
+                continue
+            t = to_time(ev, now)
+            for r, host, _ in bufs:
+                st = _lib.Member if self.kind == "plate" else _lib.BedMember
+                m = st.from_buffer_copy(host["member"].numpy().tobytes())
+                sc = host["scalars"].numpy()
+                res = {"x_sum": float(sc[0]), "time": m.time, "step_count": int(m.step_count),
+                       "status": int(m.status)}
+                if self.kind == "plate":
+                    res["broken_bonds"] = int(sc[1])
+                    res["accounting"] = sc[2:].copy()
+                else:
+                    res["rebuilds"] = int(m.rebuilds)
+                    res["accounting"] = sc[1:].copy()
+                for key in ("x", "v", "alive"):
+                    if key in host:
+                        res[key] = host[key].numpy().copy()
+                r.result, r.finished = res, t
+                out.append(r)
+        self.pending = keep
+        return out
+
+    def _repack(self, keep: list[Request], join: list[Request], chunk: int) -> None:
         old, old_members = self.engine, self.members
         members = keep + join
-        if not members:
-            if old is not None:
-                old.close()
-            self.engine, self.members = None, []
-            return
-        with torch.cuda.stream(self.stream):
-            if self.kind == "plate":
-                eng = PlateEngine([r.spec for r in members], device=self.device, fp32=self.fp32)
-                if old is not None and keep:
-                    _carry_plate(old, old_members, eng, members, keep)
-            else:
-                eng = BedEngine([r.spec for r in members], device=self.device, fp32=self.fp32)
-                if old is not None and keep:
-                    _carry_bed(old, old_members, eng, members, keep)
+        eng = None
+        if members:
+            parts = [self._parts_of(r) for r in members]
+            with torch.cuda.stream(self.prep):
+                if self.kind == "plate":
+                    eng = PlateEngine([r.spec for r in members], device=self.device,
+                                      fp32=self.fp32, parts=parts)
+                else:
+                    eng = BedEngine([r.spec for r in members], device=self.device,
+                                    fp32=self.fp32, parts=parts)
+                eng.graph(chunk // 2)
+            self.stream.wait_stream(self.prep)
+            if old is not None and keep:
+                with torch.cuda.stream(self.stream):
+                    if self.kind == "plate":
+                        _carry_plate(old, old_members, eng, members, keep)
+                    else:
+                        _carry_bed(old, old_members, eng, members, keep)
         if old is not None:
-            self.stream.synchronize()
-            old.close()
+            ev = torch.cuda.Event()
+            ev.record(self.stream)
+            self.retired.append((old, ev))
         self.engine, self.members = eng, members
 
-    def launch(self, steps: int) -> None:
-        if self.engine is None:
-            return
-        with torch.cuda.stream(self.stream):
-            self.engine.run_graph(steps // 2)
-        for r in self.members:
-            self.done[r.rid] = self.done.get(r.rid, 0) + steps
-
-    def harvest(self, now: float, keep_state: bool) -> list[Request]:
-        """Requests that reached their step count: record results."""
-        fin = [r for r in self.members if self.done[r.rid] >= r.steps]
-        if not fin:
-            return []
-        eng = self.engine
-        ms = eng.read_members()
-        for r in fin:
-            k = self.members.index(r)
-            m = ms[k]
-            if self.kind == "plate":
-                n = m.nx * m.ny
-                x = eng.x[m.first:m.first + n]
-                alive = eng.alive[m.bond_first:m.bond_first + m.nbonds]
-                res = {"broken_bonds": int((~alive).sum()), "time": m.time,
-                       "step_count": int(m.step_count), "status": int(m.status),
-                       "accounting": eng.accounting[k].cpu().numpy().copy()}
-                if keep_state:
-                    res["alive"] = alive.cpu().numpy().copy()
+    def _close_retired(self, wait: bool = False) -> None:
+        keep = []
+        for eng, ev in self.retired:
+            if wait:
+                ev.synchronize()
+            if ev.query():
+                eng.close()
             else:
-                x = eng.x[m.first:m.first + m.n]
-                res = {"time": m.time, "step_count": int(m.step_count), "status": int(m.status),
-                       "rebuilds": int(m.rebuilds),
-                       "accounting": eng.accounting[k].cpu().numpy().copy()}
-            xs = x.cpu().numpy()
-            res["x_sum"] = float(xs.sum())
-            if keep_state:
-                res["x"] = xs.copy()
-                res["v"] = (eng.v[m.first:m.first + len(xs)]).cpu().numpy().copy()
-            r.result, r.finished = res, now
-            del self.done[r.rid]
-        return fin
-
-
-def _carry_plate(old: PlateEngine, old_members, eng: PlateEngine, members, keep) -> None:
-    om, nm = old.read_members(), eng.read_members()
-    dev = eng.x.device
-    src_d, dst_d, cnt_d, src_b, dst_b, cnt_b = [], [], [], [], [], []
-    for r in keep:
-        i, j = old_members.index(r), members.index(r)
-        n = om[i].nx * om[i].ny
-        src_d.append(om[i].first); dst_d.append(nm[j].first); cnt_d.append(n)
-        src_b.append(om[i].bond_first); dst_b.append(nm[j].bond_first); cnt_b.append(om[i].nbonds)
-        nm[j].time, nm[j].step_count, nm[j].status = om[i].time, om[i].step_count, om[i].status
-        eng.accounting[j].copy_(old.accounting[i])
-        eng.last[j].copy_(old.last[i])
-    sd, dd = _ranges(src_d, cnt_d, dev), _ranges(dst_d, cnt_d, dev)
-    sb, db = _ranges(src_b, cnt_b, dev), _ranges(dst_b, cnt_b, dev)
-    eng.x[dd] = old.x[sd]
-    eng.v[dd] = old.v[sd]
-    for name in ("alive", "failure_mode", "failure_time_s", "failure_extension_m",
-                 "failure_energy_J"):
-        getattr(eng, name)[db] = getattr(old, name)[sb]
-    eng.write_members(nm)
-    eng.sync_links()
-    eng.rebin()
-
-
-def _carry_bed(old: BedEngine, old_members, eng: BedEngine, members, keep) -> None:
-    om, nm = old.read_members(), eng.read_members()
-    dev = eng.x.device
-    S = old.cand_slots
-    src, dst, cnt, shift, lps = [], [], [], [], []
-    for r in keep:
-        i, j = old_members.index(r), members.index(r)
-        src.append(om[i].first); dst.append(nm[j].first); cnt.append(om[i].n)
-        shift.append(nm[j].first - om[i].first)
-        lps.append(om[i].list_parity)
-        m = nm[j]
-        m.time, m.step_count, m.status, m.rebuilds = (om[i].time, om[i].step_count, om[i].status,
-                                                      om[i].rebuilds)
-        # continuing lists are valid: no rebuild is pending for this member
-        m.mode, m.tab, m.list_parity = 0, 0, 0
-        eng.accounting[j].copy_(old.accounting[i])
-    sd, dd = _ranges(src, cnt, dev), _ranges(dst, cnt, dev)
-    sh = torch.repeat_interleave(torch.tensor(shift, device=dev), torch.tensor(cnt, device=dev))
-    lp = torch.repeat_interleave(torch.tensor(lps, device=dev), torch.tensor(cnt, device=dev))
-    eng.x[dd] = old.x[sd]
-    eng.v[dd] = old.v[sd]
-    eng.xi_wall[dd] = old.xi_wall[sd]
-    n_old, n_new = old.n, eng.n
-    oc = old._ws_view(0, torch.int32, (2, S, n_old))
-    nc_ = eng._ws_view(0, torch.int32, (2, S, n_new))
-    nc_[0][:, dd] = oc.permute(1, 0, 2)[:, lp, sd] + sh.to(torch.int32)
-    ox = old._ws_view(1, torch.float64, (2, S, n_old, 2))
-    nx_ = eng._ws_view(1, torch.float64, (2, S, n_new, 2))
-    nx_[0][:, dd] = ox.permute(1, 0, 2, 3)[:, lp, sd]
-    om_ = old._ws_view(2, torch.int32, (2, n_old))
-    eng._ws_view(2, torch.int32, (2, n_new))[0][dd] = om_[lp, sd]
-    on_ = old._ws_view(3, torch.uint8, (2, n_old))
-    eng._ws_view(3, torch.uint8, (2, n_new))[0][dd] = on_[lp, sd]
-    ob = old._ws_view(4, torch.float64, (n_old, 2))
-    eng._ws_view(4, torch.float64, (n_new, 2))[dd] = ob[sd]
-    eng.write_members(nm)
+                keep.append((eng, ev))
+        self.retired = keep
 
 
 # ---------------------------------------------------------------------------
@@ -282,25 +384,28 @@ class Server:
 
     hbm_budget: bytes the admitted requests' estimated footprints may use;
     max_active: members per batch; chunk: steps between scheduling points
-    (every request's step count must be a multiple of it)."""
+    (every request's step count must be a multiple of it).  The host runs at
+    most one round ahead of the device."""
 
     def __init__(self, *, hbm_budget: float = 288e9, max_active: int = 64, chunk: int = 250,
                  fp32: bool = True, device="cuda", keep_state: bool = False):
         if chunk <= 0 or chunk % 2:
             raise ValueError("chunk must be a positive even number of steps")
         self.hbm_budget, self.max_active, self.chunk = float(hbm_budget), int(max_active), chunk
-        self.keep_state = keep_state
-        self.batches = {k: _Batch(k, fp32, device) for k in ("plate", "bed")}
+        self.batches = {k: _Batch(k, fp32, device, keep_state) for k in ("plate", "bed")}
         self.used = 0.0
         self.footprint: dict[int, int] = {}
         self.log: list[tuple] = []   # (time, active plates, active beds, waiting)
+        self.host_seconds = 0.0      # host time spent issuing rounds (builds, copies, launches)
+        self.rounds = 0
 
     def _admit(self, waiting: list[Request], now: float) -> dict:
         join = {"plate": [], "bed": []}
         while waiting:
             r = waiting[0]
             b = self.batches[r.kind]
-            active = len(b.members) + len(join[r.kind])
+            active = len([m for m in b.members if b.done.get(m.rid, 0) < m.steps]) + \
+                len(join[r.kind])
             need = device_bytes(r)
             if need > self.hbm_budget:
                 raise ValueError(f"request {r.rid} needs {need} B, more than the HBM budget")
@@ -315,8 +420,8 @@ class Server:
 
     def run(self, requests: list[Request], *, realtime: bool = True) -> list[Request]:
         """Serve `requests` until all are done.  realtime: arrivals are in
-        seconds of wall time; otherwise they count scheduling rounds (one
-        chunk each) and latencies are in rounds."""
+        seconds of wall time and completion times come from device events;
+        otherwise arrivals and completions count scheduling rounds."""
         for r in requests:
             if r.steps <= 0 or r.steps % self.chunk:
                 raise ValueError(f"request {r.rid}: steps must be a positive multiple of "
@@ -324,40 +429,65 @@ class Server:
         pending = sorted(requests, key=lambda r: r.arrival)
         waiting: list[Request] = []
         done: list[Request] = []
+        batches = list(self.batches.values())
         t0 = time.perf_counter()
+        refs = {}
+        for b in batches:
+            ev = torch.cuda.Event(enable_timing=True)
+            ev.record(b.stream)
+            refs[id(b)] = ev
+        torch.cuda.synchronize()
+        t_ref = time.perf_counter() - t0
         rounds = 0
-        clock = (lambda: time.perf_counter() - t0) if realtime else (lambda: float(rounds))
-        while pending or waiting or any(b.members for b in self.batches.values()):
+
+        def clock():
+            return time.perf_counter() - t0 if realtime else float(rounds)
+
+        def to_time_of(b):
+            if not realtime:
+                return lambda ev, now: now
+            ref = refs[id(b)]
+            return lambda ev, now: t_ref + ref.elapsed_time(ev) * 1e-3
+
+        def busy():
+            return any(b.members or b.pending for b in batches)
+
+        prev = []
+        while pending or waiting or busy():
             now = clock()
             while pending and pending[0].arrival <= now:
                 waiting.append(pending.pop(0))
-            idle = not any(b.members for b in self.batches.values())
-            if idle and not waiting and pending:
-                if realtime:
-                    time.sleep(max(0.0, pending[0].arrival - clock()))
-                else:
-                    rounds = int(math.ceil(pending[0].arrival))
-                continue
+            if not waiting and not any(b.members for b in batches):
+                if any(b.pending for b in batches):
+                    for ev in prev:
+                        ev.synchronize()
+                elif pending:
+                    if realtime:
+                        time.sleep(max(0.0, pending[0].arrival - clock()))
+                    else:
+                        rounds = int(math.ceil(pending[0].arrival))
+                    continue
             join = self._admit(waiting, now)
-            for kind, b in self.batches.items():
-                if join[kind] or len(b.members) != len(b.done):
-                    keep = [r for r in b.members if r.rid in b.done]
-                    b.repack(keep, join[kind])
-            for b in self.batches.values():
-                b.launch(self.chunk)
-            for b in self.batches.values():
-                if b.engine is not None:
-                    b.stream.synchronize()
+            t_r = time.perf_counter()
+            evs = [b.round(join[b.kind], self.chunk, now) for b in batches]
+            self.host_seconds += time.perf_counter() - t_r
+            for ev in prev:          # the host stays at most one round ahead
+                ev.synchronize()
+            prev = evs
             rounds += 1
-            now = clock()
-            for b in self.batches.values():
-                for r in b.harvest(now, self.keep_state):
+            for b in batches:
+                for r in b.collect(to_time_of(b)):
                     self.used -= self.footprint.pop(r.rid)
                     done.append(r)
-            self.log.append((now, len(self.batches["plate"].members),
-                             len(self.batches["bed"].members), len(waiting)))
-        for b in self.batches.values():
-            b.repack([], [])
+            self.log.append((clock(), len(self.batches["plate"].done),
+                             len(self.batches["bed"].done), len(waiting)))
+        torch.cuda.synchronize()
+        for b in batches:
+            for r in b.collect(to_time_of(b)):
+                self.used -= self.footprint.pop(r.rid)
+                done.append(r)
+            b._close_retired(wait=True)
+        self.rounds = rounds
         return done
 
 
