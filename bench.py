@@ -354,10 +354,10 @@ def kernel_event_time(eng, launches):
     return eng.step_timed(launches - launches % 2) * 1e-6
 
 
-def traffic_from_profile():
+def traffic_from_profile(name="plate_step_c2_counters.md"):
     """HBM bytes per step-kernel launch measured with rocprofv3 counters
     (FETCH_SIZE + WRITE_SIZE, calibrated on a copy kernel), if recorded."""
-    path = ROOT / "profiles" / "plate_step_c2_counters.md"
+    path = ROOT / "profiles" / name
     if not path.exists():
         return None
     for line in path.read_text().splitlines():
@@ -443,14 +443,16 @@ def bench_bed(args):
     # kernel-event pass over the same run, sampling the lists for the byte count
     eng = BedEngine([BedSpec(cfg, x, r)], fp32=True)
     kus, byts, half = [], [], None
-    for k in range(steps // chunk):
+    nchunks = steps // chunk
+    stride = max(1, nchunks // 10)
+    for k in range(nchunks):
         kus.append(eng.step_timed(chunk))
-        if k % 5 == 4:
+        if k % stride == stride - 1 or k == nchunks - 1:
             st = eng.list_stats()
             acc = eng.accounting[0].cpu().numpy()
             byts.append(bed_bytes_per_disk(st["cand_mean"], st["contacts_per_disk"],
                                            acc[_lib.BED_WALL_CONTACTS] / n))
-        if k == steps // chunk // 2 - 1:
+        if k == max(0, nchunks // 2 - 1):
             half = (eng.x.cpu().numpy(), eng.v.cpu().numpy())
     kernel_s = sum(kus) / len(kus) * 1e-6
     eng.close()
@@ -471,7 +473,8 @@ def bench_bed(args):
         "fp64_kernel": {"value": r64["rate"], "ms_per_step": r64["ms_per_step"],
                         "status_bits": r64["status"], "contact_pairs_end": r64["contacts_end"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic_from_profile("bed_step_c3_counters.md"),
                      "kernel": "bed_step_kernel<float>", "kernel_us": kernel_s * 1e6,
                      "bytes_per_disk": bpd, "bytes_per_launch": bpd * n},
     }
@@ -559,7 +562,9 @@ def bench_ensemble(args):
     eng.close()
     del eng
     r32, r64 = out["fp32"], out["fp64"]
-    achieved = PLATE_BYTES_PER_DISK * n / kernel_s / 1e9
+    # + the two per-disk strength thresholds read per step
+    bytes_per_disk = PLATE_BYTES_PER_DISK + 16
+    achieved = bytes_per_disk * n / kernel_s / 1e9
     res = {
         "workload": f"512 independent plates ({n} disks): nx log-uniform [16,256], ny uniform "
                     "[4,32], tool speed U[0.1,1] m/s, per-disk Weibull(5) strains mean "
@@ -575,9 +580,10 @@ def bench_ensemble(args):
         "fp64_kernel": {"value": r64["rate"], "ms_per_step": r64["ms_per_step"],
                         "broken_bonds_end": r64["broken"], "status_bits": r64["status"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic_from_profile("plate_step_c4_counters.md"),
                      "kernel": "plate_step_kernel<float>", "kernel_us": kernel_s * 1e6,
-                     "bytes_per_launch": PLATE_BYTES_PER_DISK * n},
+                     "bytes_per_disk": bytes_per_disk, "bytes_per_launch": bytes_per_disk * n},
     }
     if not args.no_plain:
         # every 4th member, each from rest: the plain path is launch-bound, so
