@@ -263,20 +263,27 @@ size_t tdm_bed_workspace_bytes(int n, int nblocks, int nmembers, int ncells, int
 int tdm_bed_workspace_layout(int n, int nblocks, int nmembers, int ncells, int cand_slots,
                              int cell_cap, size_t *cand_off, size_t *xi_off, size_t *mask_off,
                              size_t *count_off, size_t *xb_off);
-/* Block table, cleared histories, first binning (members filled by caller). */
+/* Block table, cleared histories, first binning (members filled by caller).
+ * Replaces FloeBed.__init__ / DiskSystem.__init__ state setup
+ * (scale_ceiling/kernel.py:158-216, granular2d/engine.py:147-212). */
 int tdm_bed_setup(const tdm_bed_arrays *a, void *stream);
-/* Re-bins x and schedules a candidate search (after the caller moved disks). */
+/* Re-bins x and schedules a candidate search (after the caller moved disks);
+ * replaces FloeBed.build_neighbours (scale_ceiling/kernel.py:230-266). */
 int tdm_bed_rebin(const tdm_bed_arrays *a, void *stream);
 /* nsteps steps of the bed law; state alternates x/v <-> x_alt/v_alt starting
  * from x/v (parity 0).  fp32 != 0: forces in FP32, contact detection, state,
- * histories and ledger in FP64. */
+ * histories and ledger in FP64.  Replaces FloeBed.step / forces
+ * (scale_ceiling/kernel.py:277-452) and DiskSystem.step
+ * (granular2d/engine.py:399-627); list rebuilds with history carry-over
+ * happen inside (scale_ceiling/kernel.py:250-275). */
 int tdm_bed_step(const tdm_bed_arrays *a, int nsteps, int fp32, void *stream);
 /* As tdm_bed_step (nsteps even) with HIP events attached to each step-kernel
  * dispatch; mean step-kernel duration in microseconds.  Measurement helper. */
 int tdm_bed_step_timed(const tdm_bed_arrays *a, int nsteps, int fp32, void *stream,
                        double *mean_kernel_us);
 /* Kinetic, gravitational (zero at y = 0), Hertz normal and tangential spring
- * energy of the current state into `energy`. */
+ * energy of the current state into `energy`; replaces FloeBed.kinetic_energy /
+ * gravitational_energy / contact_energy (scale_ceiling/kernel.py:455-482). */
 int tdm_bed_energy(const tdm_bed_arrays *a, void *stream);
 /* 2*pairs bed steps captured as a graph (state ends in x/v). */
 int tdm_bed_graph_create(const tdm_bed_arrays *a, int pairs, int fp32, void *stream,
