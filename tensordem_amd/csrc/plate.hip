@@ -153,47 +153,6 @@ __device__ double strain64(const Tile &t, int ix, int iy)
   return sqrt(dxy * dxy + 4.0 * (exy * exy));
 }
 
-// FP32 form of the same invariant.  Writing F = I + H with H = U C^-1,
-// U = sum (q - r) r^T, keeps the small displacement u = q - r free of
-// cancellation: E = (H + H^T + H^T H) / 2.
-__device__ float strain32(const Tile &t, int ix, int iy)
-{
-  float c00 = 0, c01 = 0, c11 = 0, u00 = 0, u01 = 0, u10 = 0, u11 = 0;
-  const double2 xp = t.x(ix, iy);
-#pragma unroll
-  for (int k = 0; k < 8; ++k) {
-    const int c = (k + 4) & 7;
-    const int jx = ix + dxc(c), jy = iy + dyc(c);
-    if (!on_lattice(jx, jy, t.nx, t.ny)) continue;
-    const double2 xq = t.x(jx, jy);
-    // nominal lattice vector: differs from the site difference by ulps only
-    const double rx = dxc(c) * t.two_r, ry = dyc(c) * t.two_r;
-    float ux = (float)((xq.x - xp.x) - rx), uy = (float)((xq.y - xp.y) - ry);
-    float fx = (float)rx, fy = (float)ry;
-    c00 += fx * fx; c01 += fx * fy; c11 += fy * fy;
-    u00 += ux * fx; u01 += ux * fy; u10 += uy * fx; u11 += uy * fy;
-  }
-  float det = c00 * c11 - c01 * c01;
-  float i00, i01, i11;
-  if (det != 0.0f) {
-    const float r = rcp32(det);
-    i00 = c11 * r; i01 = -c01 * r; i11 = c00 * r;
-  } else {
-    float tr = c00 + c11;
-    float r = tr > 0.0f ? rcp32(tr * tr) : 0.0f;
-    i00 = c00 * r; i01 = c01 * r; i11 = c11 * r;
-  }
-  float h00 = u00 * i00 + u01 * i01;
-  float h01 = u00 * i01 + u01 * i11;
-  float h10 = u10 * i00 + u11 * i01;
-  float h11 = u10 * i01 + u11 * i11;
-  float exx = h00 + 0.5f * (h00 * h00 + h10 * h10);
-  float eyy = h11 + 0.5f * (h01 * h01 + h11 * h11);
-  float exy = 0.5f * (h01 + h10) + 0.5f * (h00 * h01 + h10 * h11);
-  float dxy = exx - eyy;
-  return sqrt32(dxy * dxy + 4.0f * (exy * exy));
-}
-
 struct KArgs {
   tdm_plate_arrays a;
   Workspace w;
@@ -247,10 +206,10 @@ __device__ inline bool contact_pair(double2 x_lo, double2 x_hi, double2 v_lo, do
 }
 
 template <typename R>
-__global__ __launch_bounds__(kThreads) void plate_step_kernel(KArgs k)
+__device__ __forceinline__ void plate_step_body(const KArgs &k)
 {
   __shared__ double2 sx[kMaxLds];
-  __shared__ R ss[kMaxRing];
+  __shared__ double ss[kMaxRing];
   __shared__ double red[4][kCh];
   __shared__ tdm_plate_member sm;
   __shared__ int sflags[2];
@@ -315,11 +274,10 @@ __global__ __launch_bounds__(kThreads) void plate_step_kernel(KArgs k)
   const int rw = TX + 2;
   for (int t = threadIdx.x; t < rw * (TY + 2); t += kThreads) {
     int hx = tx0 - 1 + t % rw, hy = ty0 - 1 + t / rw;
-    R s = 0;
-    if (on_lattice(hx, hy, nx, ny)) {
-      if constexpr (Prec<R>::fp64) s = strain64(tile, hx, hy);
-      else s = strain32(tile, hx, hy);
-    }
+    // FP64 in both precisions: the strain stage is a small part of the
+    // step, and exact strains make every shear-break decision the FP64 one
+    double s = 0;
+    if (on_lattice(hx, hy, nx, ny)) s = strain64(tile, hx, hy);
     ss[t] = s;
   }
   __syncthreads();
@@ -331,7 +289,7 @@ __global__ __launch_bounds__(kThreads) void plate_step_kernel(KArgs k)
   if (valid) {
     const double2 xp = tile.x(ix, iy);
     const double2 sp = lattice_site(ix, iy, half_span, two_r);
-    const R s_self = ss[(iy - ty0 + 1) * rw + (ix - tx0 + 1)];
+    const double s_self = ss[(iy - ty0 + 1) * rw + (ix - tx0 + 1)];
     const uint32_t present = present_links(ix, iy, nx, ny);
     links &= present;
     const double kb = m.bond_stiffness, kc = m.contact_stiffness, cdamp = m.contact_damping;
@@ -470,24 +428,24 @@ __global__ __launch_bounds__(kThreads) void plate_step_kernel(KArgs k)
       if (ov > 0) ch[E_TOOL] += ov * ov;
       ch[E_VSQ] += vp.x * vp.x + vp.y * vp.y;
     } else {
-      // ---- all present bonds, independently (8-way instruction-level
-      // parallelism); fracture commit and force per bond -------------------
+      // ---- all present bonds, independently (4-way instruction-level
+      // parallelism per group); fracture commit and force per bond --------
       // per-bond forces in the force precision (FP32 values widen exactly
       // when they are summed into the FP64 total)
       R Fx[8], Fy[8];
       uint32_t has = 0;
-#pragma unroll
-      for (int c = 0; c < 8; ++c) {
+      // one bond: fracture commit and force (c is a constant after unrolling)
+      auto bond = [&](const int c) {
         Fx[c] = 0.0;
         Fy[c] = 0.0;
-        if (!((present >> c) & 1u)) continue;
+        if (!((present >> c) & 1u)) return;
         const bool self_lo = c >= 4;
         const int jx = ix + dxc(c), jy = iy + dyc(c);
         const int q = p + dyc(c) * nx + dxc(c);
         const double2 xq = tile.x(jx, jy);
-        const R s_q = ss[(jy - ty0 + 1) * rw + (jx - tx0 + 1)];
+        const double s_q = ss[(jy - ty0 + 1) * rw + (jx - tx0 + 1)];
         const double2 x_lo = self_lo ? xp : xq, x_hi = self_lo ? xq : xp;
-        const R sh_lo = self_lo ? s_self : s_q, sh_hi = self_lo ? s_q : s_self;
+        const double sh_lo = self_lo ? s_self : s_q, sh_hi = self_lo ? s_q : s_self;
         // exact rest vector from the two lattice sites (dem.py:119-129)
         auto site_vec = [&](double &rx_, double &ry_, double &rest_) {
           const double2 sq = lattice_site(jx, jy, half_span, two_r);
@@ -573,25 +531,19 @@ __global__ __launch_bounds__(kThreads) void plate_step_kernel(KArgs k)
           if (d > 0) { float r = rcp32(fmaxf(d, kEps32)); nxv = dvx * r; nyv = dvy * r; }
           else { nxv = rfx * inv_rest; nyv = rfy * inv_rest; }
           if (alive) {
+            // the shear test is the FP64 one (strains are FP64); the tensile
+            // test is redone in FP64 inside the margin of its threshold
             const float tstrain = ext * inv_rest;
-            const float bshear = 0.5f * ((float)sh_lo + (float)sh_hi);
             const double mg = m.fp32_margin;
             const bool near_t = tstrain >= (float)(eps_t * (1.0 - mg)) &&
                                 tstrain <= (float)(eps_t * (1.0 + mg));
-            const bool near_s = bshear >= (float)(eps_s * (1.0 - mg)) &&
-                                bshear <= (float)(eps_s * (1.0 + mg));
             int tf = tstrain > (float)eps_t;
-            int sf = bshear > (float)eps_s;
-            if (near_t || near_s) {
+            const int sf = 0.5 * (sh_lo + sh_hi) > eps_s;
+            if (near_t) {
               double rx, ry, rest, d64, n1, n2;
               site_vec(rx, ry, rest);
               geom64(rx, ry, rest, d64, n1, n2);
-              if (near_t) tf = ((d64 - rest) / rest) > eps_t;
-              if (near_s) {
-                double slo = strain64(tile, self_lo ? ix : jx, self_lo ? iy : jy);
-                double shi = strain64(tile, self_lo ? jx : ix, self_lo ? jy : iy);
-                sf = 0.5 * (slo + shi) > eps_s;
-              }
+              tf = ((d64 - rest) / rest) > eps_t;
             }
             if (sf || tf) {
               alive = false;
@@ -618,13 +570,17 @@ __global__ __launch_bounds__(kThreads) void plate_step_kernel(KArgs k)
             }
           }
         }
-      }
+      };
 
       // ---- sum in index_add_ order: -drag*v, then +F of partners above
       // (ascending), then -F of partners below (ascending); unbonded
       // candidates are merged in by partner index --------------------------
+      // bonds are evaluated in two groups of four (partners above, then
+      // below), each summed before the next is formed: fewer live registers
       double fx = (-m.drag) * vp.x, fy = (-m.drag) * vp.y;
       int ci = split;
+#pragma unroll
+      for (int c = 4; c < 8; ++c) bond(c);
 #pragma unroll
       for (int c = 4; c < 8; ++c) {
         if (!((present >> c) & 1u)) continue;
@@ -639,6 +595,8 @@ __global__ __launch_bounds__(kThreads) void plate_step_kernel(KArgs k)
       }
       for (; ci < nc; ++ci) unbonded(cand[(size_t)ci * n], fx, fy, false);
       ci = 0;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) bond(c);
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
         if (!((present >> c) & 1u)) continue;
@@ -802,6 +760,17 @@ __global__ __launch_bounds__(kThreads) void plate_step_kernel(KArgs k)
 // partials summed in the member's logical tile order (independent of the
 // batch around it).  A separate launch makes every step block's partials
 // visible without device-scope fences inside the step kernel.
+
+// 4 waves per SIMD: both precisions fit 128 VGPRs without spills (the
+// FP32 build keeps its FP64 fallback to the tensile test only, and strains
+// are FP64 in both), so a grid of several rounds of blocks hides latency.
+template <typename R>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4)))
+void plate_step_kernel(KArgs k)
+{
+  plate_step_body<R>(k);
+}
+
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kThreads) void plate_finalize_kernel(KArgs k)
 {
@@ -1135,18 +1104,12 @@ static int launch_step(const tdm_plate_arrays *a, const Workspace &w, int parity
   k.v_in = parity ? va : v;
   k.x_out = const_cast<double2 *>(parity ? x : xa);
   k.v_out = const_cast<double2 *>(parity ? v : va);
+  void (*kern)(KArgs) = fp32 ? plate_step_kernel<float> : plate_step_kernel<double>;
   if (ev0) {
     // events carried by the dispatch itself: they time exactly this kernel
-    if (fp32)
-      hipExtLaunchKernelGGL(plate_step_kernel<float>, dim3(a->nblocks), dim3(kThreads), 0, s,
-                            ev0, ev1, 0, k);
-    else
-      hipExtLaunchKernelGGL(plate_step_kernel<double>, dim3(a->nblocks), dim3(kThreads), 0, s,
-                            ev0, ev1, 0, k);
-  } else if (fp32) {
-    plate_step_kernel<float><<<a->nblocks, kThreads, 0, s>>>(k);
+    hipExtLaunchKernelGGL(kern, dim3(a->nblocks), dim3(kThreads), 0, s, ev0, ev1, 0, k);
   } else {
-    plate_step_kernel<double><<<a->nblocks, kThreads, 0, s>>>(k);
+    kern<<<a->nblocks, kThreads, 0, s>>>(k);
   }
   HIP_TRY(hipGetLastError());
   plate_finalize_kernel<<<a->nmembers, kThreads, 0, s>>>(k);
