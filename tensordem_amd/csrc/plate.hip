@@ -12,7 +12,7 @@
 //   * fracture is committed by both ends of a bond with identical arithmetic
 //     (the lower-index end writes the bond's failure record);
 //   * tool or hull contact, loads, semi-implicit Euler, fixed clamp and the
-//     per-block ledger partials are fused in; the last block of each member
+//     per-block ledger partials are fused in; a per-member finalize launch
 //     sums that member's partials in a fixed order and advances its clock;
 //   * the cell hash for unbonded contacts is rebuilt in two pipelined phases
 //     (bin, then search) folded into ordinary steps, triggered on device when
@@ -45,7 +45,7 @@ __host__ __device__ constexpr int dyc(int c) { return c < 3 ? -1 : (c < 5 ? 0 : 
 
 struct Workspace {
   int4 *blocks;          // 2 per block: (member, tile x0, tile y0, rank), (nx, ny, first, tile_x)
-  double *partials;      // [nblocks][kCh]
+  double *partials;      // [kCh][nblocks], channel-major
   int *bflags;           // [nblocks] status bits | rebuild request << 16
   int *cell_count;       // [2][ncells]
   int *cell_tab;         // [2][ncells][cap]
@@ -748,18 +748,11 @@ __device__ __forceinline__ void plate_step_body(const KArgs &k)
   block_sum(ch, red);
   if (threadIdx.x == 0) {
     const size_t slot = (size_t)(m.block_first + blk.w);
-    double *dst = k.w.partials + slot * kCh;
 #pragma unroll
-    for (int c = 0; c < kCh; ++c) dst[c] = ch[c];
+    for (int c = 0; c < kCh; ++c) k.w.partials[(size_t)c * a.nblocks + slot] = ch[c];
     k.w.bflags[slot] = sflags[0] | (sflags[1] << 16);
   }
 }
-
-// ---------------------------------------------------------------------------
-// Per-member ledger reduction and clock/mode update: one block per member,
-// partials summed in the member's logical tile order (independent of the
-// batch around it).  A separate launch makes every step block's partials
-// visible without device-scope fences inside the step kernel.
 
 // 4 waves per SIMD: both precisions fit 128 VGPRs without spills (the
 // FP32 build keeps its FP64 fallback to the tensile test only, and strains
@@ -772,43 +765,60 @@ void plate_step_kernel(KArgs k)
 }
 
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kThreads) void plate_finalize_kernel(KArgs k)
+// ---------------------------------------------------------------------------
+// Per-member ledger reduction and clock/mode update: one 1 024-thread block
+// per member.  Every thread loads its blocks' 16 channels in one round, then
+// a fixed DPP tree per wave and the 16 wave totals in order: the order
+// depends only on the member's block count (independent of the batch around
+// it).  The member record and ledger are fetched alongside the partials.  A
+// separate launch makes every step block's partials visible without
+// device-scope fences inside the step kernel.
+constexpr int kFinThreads = 1024;
+
+__global__ __launch_bounds__(kFinThreads) void plate_finalize_kernel(KArgs k)
 {
   __shared__ int sflag;
+  __shared__ double wsum[kFinThreads / 64][kCh];
+  __shared__ double stot[kCh];
   const tdm_plate_arrays &a = k.a;
   const int mi = blockIdx.x;
   tdm_plate_member *mp = a.members + mi;
   const int b0 = mp->block_first, nb = mp->nblocks;
-  __shared__ double seg[16][kCh + 1];
-  __shared__ double stot[kCh];
-  if (threadIdx.x == 0) sflag = 0;
-  __syncthreads();
-  // thread t sums channel t % 16 over the member's rows t/16, t/16 + 16, ...
-  // (coalesced: 16 consecutive threads read one 128-byte row), then 16
-  // threads add the 16 segments in order; the order depends only on nb.
-  static_assert(kCh == 16, "reduction layout assumes 16 channels");
-  const int c = threadIdx.x & 15, r0 = threadIdx.x >> 4;
-  double sum = 0.0;
-  // eight independent loads in flight per round; summed in row order
-  for (int b = r0; b < nb; b += 16 * 8) {
-    double v[8];
+  tdm_plate_member m;
+  double acc[TDM_NACC];
+  double *accp = a.accounting + (size_t)mi * TDM_NACC;
+  if (threadIdx.x == 0) {
+    sflag = 0;
+    m = *mp;  // registers: no aliasing with the stores below
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      const int bb = b + 16 * u;
-      v[u] = bb < nb ? k.w.partials[(size_t)(b0 + bb) * kCh + c] : 0.0;
-    }
-#pragma unroll
-    for (int u = 0; u < 8; ++u) sum += v[u];
+    for (int c = 0; c < TDM_NACC; ++c) acc[c] = accp[c];
   }
-  seg[r0][c] = sum;
+  __syncthreads();
+  double sum[kCh];
+#pragma unroll
+  for (int c = 0; c < kCh; ++c) sum[c] = 0.0;
   int fl = 0;
-  for (int b = threadIdx.x; b < nb; b += kThreads) fl |= k.w.bflags[b0 + b];
+  for (int b = threadIdx.x; b < nb; b += kFinThreads) {
+    double v[kCh];
+#pragma unroll
+    for (int c = 0; c < kCh; ++c) v[c] = k.w.partials[(size_t)c * a.nblocks + b0 + b];
+    fl |= k.w.bflags[b0 + b];
+#pragma unroll
+    for (int c = 0; c < kCh; ++c) sum[c] += v[c];
+  }
   if (fl) atomicOr(&sflag, fl);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int c = 0; c < kCh; ++c) {
+    double t = 0.0;
+    if (__ballot(sum[c] != 0.0)) t = wave_sum(sum[c]);
+    if (lane == 0) wsum[wave][c] = t;
+  }
   __syncthreads();
   if (threadIdx.x < kCh) {
     double t = 0.0;
 #pragma unroll
-    for (int r = 0; r < 16; ++r) t += seg[r][threadIdx.x];
+    for (int w = 0; w < kFinThreads / 64; ++w) t += wsum[w][threadIdx.x];
     stot[threadIdx.x] = t;
   }
   __syncthreads();
@@ -816,12 +826,7 @@ __global__ __launch_bounds__(kThreads) void plate_finalize_kernel(KArgs k)
   double tot[kCh];
 #pragma unroll
   for (int q = 0; q < kCh; ++q) tot[q] = stot[q];
-  const tdm_plate_member m = *mp;  // registers: no aliasing with the stores below
   const int status = sflag & 0xFFFF, rq = sflag >> 16;
-  double acc[TDM_NACC];
-  double *accp = a.accounting + (size_t)mi * TDM_NACC;
-#pragma unroll
-  for (int c = 0; c < TDM_NACC; ++c) acc[c] = accp[c];
   const double dt = m.dt, time = m.time;
   const int mode = m.mode;
   int new_mode = mode, new_tab = m.tab;
@@ -1112,7 +1117,7 @@ static int launch_step(const tdm_plate_arrays *a, const Workspace &w, int parity
     kern<<<a->nblocks, kThreads, 0, s>>>(k);
   }
   HIP_TRY(hipGetLastError());
-  plate_finalize_kernel<<<a->nmembers, kThreads, 0, s>>>(k);
+  plate_finalize_kernel<<<a->nmembers, kFinThreads, 0, s>>>(k);
   HIP_TRY(hipGetLastError());
   return TDM_OK;
 }
