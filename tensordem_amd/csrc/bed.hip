@@ -146,8 +146,13 @@ __device__ inline bool hm_pair(const BedConst &m, double2 x_lo, double2 x_hi,
                                double &Fx, double &Fy, double &dash, double &fric)
 {
   const double dx = x_hi.x - x_lo.x, dy = x_hi.y - x_lo.y;
-  const double dist = sqrt(dx * dx + dy * dy);
-  const double d = (r_lo + r_hi) - dist;
+  const double s2 = dx * dx + dy * dy;
+  const double rs = r_lo + r_hi;
+  // exact reject without the square root: above (rs^2)(1 + 1e-15) the
+  // correctly rounded sqrt(s2) is >= rs, so (rs - dist) > 0 is false
+  if (s2 > (rs * rs) * (1.0 + 1e-15)) return false;
+  const double dist = sqrt(s2);
+  const double d = rs - dist;
   if (!(d > 0)) return false;
   const double vrx = v_lo.x - v_hi.x, vry = v_lo.y - v_hi.y;
   if constexpr (Prec<R>::fp64) {
@@ -323,7 +328,10 @@ __device__ inline int rebuild_list(const BKArgs &k, const BedConst &m, int p, do
       const double2 xq = k.x_in[q];
       const double rq = a.radius[q];
       const double dx = xq.x - xp.x, dy = xq.y - xp.y;
-      if (!(sqrt(dx * dx + dy * dy) < (rp + rq) + m.skin)) continue;
+      // candidates: within r_p + r_q + skin (the slightly inclusive squared
+      // form only ever adds pairs, which the contact test then rejects)
+      const double cut = (rp + rq) + m.skin;
+      if (!(dx * dx + dy * dy < (cut * cut) * (1.0 + 1e-15))) continue;
       if (nn == S) { *status |= TDM_ST_CAND_OVERFLOW; continue; }
       int pos = nn++;
       while (pos > 0 && ncl[(size_t)(pos - 1) * n] > q) {
