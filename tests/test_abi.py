@@ -68,3 +68,41 @@ def test_bed_host_only_entry_points():
     assert count - mask >= 2 * 1000 * 4
     assert xb - count >= 2 * 1000 and xb % 256 == 0
     assert total >= xb + 16 * 1000
+
+
+def _bed_arrays(**kw):
+    a = _lib.BedArrays()
+    a.n, a.nmembers, a.nblocks, a.ncells, a.cand_slots, a.cell_cap = 1000, 1, 4, 100, 16, 8
+    for f in ("x", "v", "x_alt", "v_alt", "radius", "xi_wall", "accounting", "members",
+              "workspace"):
+        setattr(a, f, 256)          # never dereferenced: validation fails first
+    a.workspace_bytes = 1
+    for k, v in kw.items():
+        setattr(a, k, v)
+    return a
+
+
+@pytest.mark.parametrize("field,value,msg", [
+    ("x", 0, b"null array"),
+    ("n", 0, b"empty bed batch"),
+    ("cand_slots", 17, b"cand_slots"),
+    ("cand_slots", 0, b"cand_slots"),
+    ("cell_cap", 0, b"cell_cap"),
+])
+def test_bed_entry_points_reject_bad_descriptors(field, value, msg):
+    lib = _lib.load()
+    a = _bed_arrays(**{field: value})
+    for call in (lambda: lib.tdm_bed_step(ctypes.byref(a), 1, 0, None),
+                 lambda: lib.tdm_bed_setup(ctypes.byref(a), None),
+                 lambda: lib.tdm_bed_energy(ctypes.byref(a), None)):
+        assert call() == _lib.TDM_EINVAL
+        buf = ctypes.create_string_buffer(256)
+        lib.tdm_last_error(buf, 256)
+        assert msg in buf.value
+
+
+def test_bed_workspace_too_small_is_reported():
+    lib = _lib.load()
+    a = _bed_arrays()          # 1-byte workspace for 1000 disks
+    assert lib.tdm_bed_step(ctypes.byref(a), 1, 0, None) == _lib.TDM_ESPACE
+    assert lib.tdm_bed_step(None, 1, 0, None) == _lib.TDM_EINVAL

@@ -127,3 +127,48 @@ def test_bed_graph_replay_equals_eager():
         torch.cuda.synchronize()
         assert np.array_equal(eager.x.cpu().numpy(), graph.x.cpu().numpy()), rep
     assert graph.read_members()[0].step_count == 600
+
+
+def test_nonfinite_state_sets_the_status_bit():
+    """A NaN velocity propagates to NaN positions: the disk is binned into the
+    grid's first cell, never compares as in contact, and the member's status
+    carries TDM_ST_NONFINITE (the facade's FloatingPointError)."""
+    cfg, x, r = column(300, seed=31)
+    v = np.zeros_like(x)
+    v[17] = [np.nan, 0.0]
+    eng = BedEngine([BedSpec(cfg, x, r, velocities=v)])
+    eng.step(4)
+    assert eng.status()[0] & 1
+    assert np.isnan(eng.x.cpu().numpy()[17]).any()
+    assert np.isfinite(np.delete(eng.x.cpu().numpy(), 17, axis=0)).all()
+
+
+def test_list_and_cell_overflow_are_flagged_not_written_past():
+    cfg, x, r, v = collapsing(600, seed=33, steps=20000)
+    small_lists = BedEngine([BedSpec(cfg, x, r, velocities=v)], cand_slots=2)
+    small_lists.step(50)
+    assert small_lists.status()[0] & 4          # TDM_ST_CAND_OVERFLOW
+    small_cells = BedEngine([BedSpec(cfg, x, r, velocities=v)], cell_cap=1)
+    small_cells.step(50)
+    assert small_cells.status()[0] & 8          # TDM_ST_CELL_OVERFLOW
+    ok = BedEngine([BedSpec(cfg, x, r, velocities=v)])
+    ok.step(50)
+    assert ok.status()[0] == 0
+
+
+def test_members_of_very_different_sizes_equal_alone():
+    """Ragged batch: a 1-disk bed, a 255-disk and a 3000-disk column."""
+    specs = [BedSpec(BedConfig(), np.array([[0.05, 0.02]]), np.array([0.01]),
+                     velocities=np.array([[0.3, -0.2]]))]
+    for seed, n in ((41, 255), (42, 3000)):
+        cfg, x, r = column(n, seed)
+        specs.append(BedSpec(cfg, x, r, velocities=shake(n, seed)))
+    batch = BedEngine(specs)
+    batch.step(1200)
+    for k, s in enumerate(specs):
+        solo = BedEngine([s])
+        solo.step(1200)
+        m = batch.read_members()[k]
+        assert np.array_equal(batch.x[m.first:m.first + m.n].cpu().numpy(), solo.x.cpu().numpy())
+        assert np.array_equal(batch.v[m.first:m.first + m.n].cpu().numpy(), solo.v.cpu().numpy())
+    assert not any(batch.status())
