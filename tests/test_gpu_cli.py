@@ -1,0 +1,30 @@
+"""The dem-2d command writes the reference's three outputs from the HIP path."""
+import json
+
+import pytest
+import torch
+
+from oracle.plate import OraclePlate
+from tensordem_amd import DEMConfig
+from tensordem_amd.cli import main
+
+pytestmark = pytest.mark.gpu
+
+
+def test_dem_2d_outputs(tmp_path):
+    main(["dem-2d", "--nx", "7", "--ny", "3", "--speed", "0.2", "--steps", "345",
+          "--save-every", "50", "--breaking-strain", "0.015", "--shear-breaking-strain", "0.03",
+          "--output", str(tmp_path)])
+    rows = (tmp_path / "history.csv").read_text().strip().splitlines()
+    assert len(rows) == 1 + 8              # header; steps 0, 50, ..., 300 and the final 345
+    traj = torch.load(tmp_path / "trajectory.pt", weights_only=True)
+    assert traj["positions"].shape == (8, 21, 2) and traj["bond_alive"].shape[0] == 8
+    acc = json.loads((tmp_path / "accounting.json").read_text())
+    assert acc["steps"] == 345 and acc["total_bonds"] == len(traj["bond_pairs"])
+    # the same run on the FP64 oracle: identical final positions and breaks
+    ref = OraclePlate(DEMConfig(nx=7, ny=3, tool_speed=0.2))
+    for _ in range(345):
+        ref.step()
+    assert torch.equal(traj["positions"][-1], torch.tensor(ref.positions))
+    assert acc["broken_bonds"] == ref.broken_bonds == 21
+    assert acc["fracture_release_J"] == pytest.approx(ref.accounting["fracture_release_J"], rel=1e-12)
