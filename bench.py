@@ -405,25 +405,46 @@ def c3_inputs():
 def bench_bed(args):
     """The 1M-disk column collapse: FP32 and FP64 forces over the full step
     count from release, kernel events with list statistics on a second pass,
-    the plain FP64 path from the half-way state, the oracle on a CPU sample."""
+    the plain FP64 path from the half-way state, the oracle on a CPU sample.
+    With --bed-from S (or --bed-start FILE) every engine starts instead from
+    the state S steps into the collapse, carried over with its spring
+    histories by the bed restart."""
     import torch
     from tensordem_amd import _lib
-    from tensordem_amd.bed import BedEngine, BedSpec
+    from tensordem_amd.bed import BedEngine, BedSpec, load_bed_restart, save_bed_restart
 
     cfg, x, r = c3_inputs()
     n = len(r)
     chunk = 1000
     steps = max(chunk, args.bed_steps - args.bed_steps % chunk)
-    warm = BedEngine([BedSpec(cfg, x, r)], fp32=True)
+    state = None
+    if args.bed_start:
+        state = load_bed_restart(args.bed_start)
+    elif args.bed_from:
+        adv = BedEngine([BedSpec(cfg, x, r)], fp32=True)
+        adv.run_graph(chunk // 2, repeats=max(1, args.bed_from // chunk))
+        state = adv.snapshot()
+        adv.close()
+        del adv
+        if args.bed_save_start:
+            save_bed_restart(state, args.bed_save_start)
+
+    def engine(fp32):
+        if state is not None:
+            return BedEngine.from_snapshot(state, fp32=fp32)
+        return BedEngine([BedSpec(cfg, x, r)], fp32=fp32)
+
+    warm = engine(True)
     warm.run_graph(chunk // 2)
     torch.cuda.synchronize()
     dt = warm.read_members()[0].dt
     skin = warm.read_members()[0].skin
     warm.close()
     del warm
+    start = 0 if state is None else int(state["step_count"])
     out = {}
     for fp32 in (True, False):
-        eng = BedEngine([BedSpec(cfg, x, r)], fp32=fp32)
+        eng = engine(fp32)
         eng.graph(chunk // 2)
         stream = torch.cuda.current_stream()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -441,7 +462,7 @@ def bench_bed(args):
         eng.close()
         del eng
     # kernel-event pass over the same run, sampling the lists for the byte count
-    eng = BedEngine([BedSpec(cfg, x, r)], fp32=True)
+    eng = engine(True)
     kus, byts, half = [], [], None
     nchunks = steps // chunk
     stride = max(1, nchunks // 10)
@@ -464,17 +485,19 @@ def bench_bed(args):
         "workload": f"granular column collapse, {n} disks, radii U[0.8,1.2]x{BED_R0} m, "
                     "E=1 GPa, nu=0.3, mu=0.5, e=0.5 (viscous), rho=917, t=0.02 m, "
                     f"aspect 2 loose column (seed 0), floor + back wall, gravity, "
-                    f"dt={dt:.6g} s, {steps} steps from release",
+                    f"dt={dt:.6g} s, {steps} steps from "
+                    + ("release" if not start else f"step {start} of the collapse"),
         "value": r32["rate"], "unit": "particle-steps/s", "steps": steps,
         "ms_per_step": r32["ms_per_step"], "dtype": "f32-force/f64-state",
-        "skin_m": skin, "graph_chunk_steps": chunk,
+        "skin_m": skin, "graph_chunk_steps": chunk, "start_step": start,
         "status_bits": r32["status"], "contact_pairs_end": r32["contacts_end"],
         "lists_end": r32["stats_end"], "list_rebuilds": r32["rebuilds"],
         "fp64_kernel": {"value": r64["rate"], "ms_per_step": r64["ms_per_step"],
                         "status_bits": r64["status"], "contact_pairs_end": r64["contacts_end"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": traffic_from_profile("bed_step_c3_counters.md"),
+                     "traffic": traffic_from_profile("bed_step_c3_counters.md" if not start
+                                                     else "bed_step_c3_dense_counters.md"),
                      "kernel": "bed_step_kernel<float>", "kernel_us": kernel_s * 1e6,
                      "bytes_per_disk": bpd, "bytes_per_launch": bpd * n},
     }
@@ -497,7 +520,7 @@ def bench_bed(args):
             steps=args.bed_plain_steps, list_rebuilds=p.rebuilds - 1,
             what="eager PyTorch FP64 bed step on the same GPU (sort-based cell list, Verlet "
                  "list rebuilt at |dx|inf > skin/4, searchsorted history remap, index_add_), "
-                 f"from the state at step {steps // 2}")
+                 f"from the state at step {start + steps // 2}")
         res["speedup_vs_plain_fp64"] = r32["rate"] * ps / n
         del p
     if not args.no_cpu:
@@ -766,6 +789,12 @@ def main(argv=None):
     ap.add_argument("--ens-steps", type=int, default=10000)
     ap.add_argument("--bed-steps", type=int, default=50000)
     ap.add_argument("--bed-plain-steps", type=int, default=60)
+    ap.add_argument("--bed-from", type=int, default=0,
+                    help="start the bed timing this many steps into the collapse")
+    ap.add_argument("--bed-start", default=None,
+                    help="start the bed timing from a bed restart file")
+    ap.add_argument("--bed-save-start", default=None,
+                    help="write the --bed-from start state to this restart file")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a 512 MiB device copy (for counter calibration)")
     args = ap.parse_args(argv)
@@ -774,7 +803,7 @@ def main(argv=None):
         return 0
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: every plate and request of this workload runs on a "
-              "single MI355X and no multi-GPU path is built (see DESIGN.md).", file=sys.stderr)
+              "single MI355X and no multi-GPU path is built.", file=sys.stderr)
         return 2
     if args.workload in ("bed", "ensemble", "serve"):
         bed = {"bed": bench_bed, "ensemble": bench_ensemble, "serve": bench_serve}[

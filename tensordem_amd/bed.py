@@ -226,6 +226,90 @@ class BedEngine:
         return dict(cand_mean=nc.mean().item(), cand_max=int(nc.max().item()),
                     contacts_per_disk=pc / self.n)
 
+    # ---- restart ----------------------------------------------------------------
+    RESTART_SCHEMA = "tensordem_amd.bed_restart/1"
+
+    def snapshot(self, member: int = 0) -> dict:
+        """Complete state of one member, on the host: positions, velocities,
+        radii, law, clock, ledger, every active pair spring history (pairs
+        i < j with their xi, in the member's local indices) and the wall
+        springs.  Restoring it continues the run bit for bit."""
+        from dataclasses import asdict
+        ms = self.read_members()
+        m = ms[member]
+        sl = slice(m.first, m.first + m.n)
+        S = self.cand_slots
+        lp = m.list_parity
+        cand = self._ws_view(0, torch.int32, (2, S, self.n))[lp][:, sl].cpu().numpy()
+        xi = self._ws_view(1, torch.float64, (2, S, self.n, 2))[lp][:, sl].cpu().numpy()
+        mask = self._ws_view(2, torch.int32, (2, self.n))[lp][sl].cpu().numpy().astype(np.int64)
+        nc = self._ws_view(3, torch.uint8, (2, self.n))[lp][sl].cpu().numpy()
+        p_loc, q_loc, h = [], [], []
+        for e in range(S):
+            act = (((mask >> e) & 1) == 1) & (e < nc)
+            p = np.nonzero(act)[0]
+            q = cand[e][act] - m.first
+            keep = q > p
+            p_loc.append(p[keep]); q_loc.append(q[keep]); h.append(xi[e][act][keep])
+        pi, qi, hx = np.concatenate(p_loc), np.concatenate(q_loc), np.concatenate(h)
+        order = np.lexsort((qi, pi))
+        spec = self.specs[member]
+        return {
+            "schema": self.RESTART_SCHEMA,
+            "config": asdict(spec.config), "dt": m.dt, "time": m.time,
+            "step_count": int(m.step_count),
+            "positions": self.x[sl].cpu().numpy().copy(),
+            "velocities": self.v[sl].cpu().numpy().copy(),
+            "radii": self.radius[sl].cpu().numpy().copy(),
+            "accounting": self.accounting[member].cpu().numpy().copy(),
+            "pairs": np.stack((pi[order], qi[order]), 1).astype(np.int64),
+            "pair_xi": hx[order].copy(),
+            "wall_xi": self.xi_wall[sl].cpu().numpy().copy(),
+            "wall_active": ((mask >> 16) & 3).astype(np.uint8),
+        }
+
+    @classmethod
+    def from_snapshot(cls, state: dict, **kw) -> "BedEngine":
+        """A one-member engine continuing `state` (from `snapshot`).  The
+        spring histories are written into the current candidate lists, so the
+        first step's list rebuild carries them over like any other."""
+        validate_bed_restart(state)
+        cfg = BedConfig(**state["config"])
+        eng = cls([BedSpec(cfg, state["positions"], state["radii"], dt=state["dt"],
+                           velocities=state["velocities"])], **kw)
+        n, S, dev = eng.n, eng.cand_slots, eng.device
+        pairs = np.asarray(state["pairs"], np.int64).reshape(-1, 2)
+        hxi = np.asarray(state["pair_xi"], np.float64).reshape(-1, 2)
+        disk = np.concatenate([pairs[:, 0], pairs[:, 1]])
+        part = np.concatenate([pairs[:, 1], pairs[:, 0]])
+        hh = np.concatenate([hxi, hxi])
+        order = np.lexsort((part, disk))
+        disk, part, hh = disk[order], part[order], hh[order]
+        counts = np.bincount(disk, minlength=n)
+        if counts.max(initial=0) > S:
+            raise ValueError("restart holds more contacts per disk than candidate slots")
+        slot = np.arange(len(disk)) - np.repeat(np.cumsum(counts) - counts, counts)
+        cand = np.zeros((S, n), np.int32)
+        xi = np.zeros((S, n, 2))
+        mask = np.zeros(n, np.int64)
+        cand[slot, disk] = part
+        xi[slot, disk] = hh
+        np.add.at(mask, disk, (1 << slot).astype(np.int64))
+        mask |= np.asarray(state["wall_active"], np.int64) << 16
+        eng._ws_view(0, torch.int32, (2, S, n))[0].copy_(torch.from_numpy(cand).to(dev))
+        eng._ws_view(1, torch.float64, (2, S, n, 2))[0].copy_(torch.from_numpy(xi).to(dev))
+        eng._ws_view(2, torch.int32, (2, n))[0].copy_(
+            torch.from_numpy(mask.astype(np.uint32).view(np.int32)).to(dev))
+        eng._ws_view(3, torch.uint8, (2, n))[0].copy_(
+            torch.from_numpy(counts.astype(np.uint8)).to(dev))
+        eng._ws_view(4, torch.float64, (n, 2)).copy_(eng.x)
+        eng.xi_wall.copy_(torch.from_numpy(np.asarray(state["wall_xi"], np.float64)).to(dev))
+        eng.accounting[0].copy_(torch.from_numpy(np.asarray(state["accounting"])).to(dev))
+        ms = eng.read_members()
+        ms[0].time, ms[0].step_count = float(state["time"]), int(state["step_count"])
+        eng.write_members(ms)
+        return eng
+
     def contact_pairs(self) -> np.ndarray:
         """Pairs (i<j) in contact during the last step, from the kernels' own
         per-slot contact masks (sorted)."""
@@ -265,3 +349,69 @@ class BedEngine:
             self.close()
         except Exception:
             pass
+
+
+def validate_bed_restart(state: dict) -> None:
+    """Raise ValueError unless `state` is a complete, self-consistent bed
+    restart: shapes agree, values are finite, pairs are i < j in range with
+    no repeats, the clock is non-negative."""
+    if not isinstance(state, dict) or state.get("schema") != BedEngine.RESTART_SCHEMA:
+        raise ValueError("not a tensordem_amd bed restart state")
+    need = {"schema", "config", "dt", "time", "step_count", "positions", "velocities", "radii",
+            "accounting", "pairs", "pair_xi", "wall_xi", "wall_active"}
+    if set(state) != need:
+        raise ValueError("bed restart fields missing or unknown")
+    x = np.asarray(state["positions"], np.float64)
+    n = len(x)
+    shapes = {"positions": (n, 2), "velocities": (n, 2), "radii": (n,),
+              "accounting": (_lib.BED_NACC,), "wall_xi": (n, 4), "wall_active": (n,)}
+    for k, shp in shapes.items():
+        a = np.asarray(state[k])
+        if a.shape != shp:
+            raise ValueError(f"invalid bed restart {k} shape")
+        if not np.isfinite(a.astype(np.float64)).all():
+            raise ValueError(f"nonfinite bed restart {k}")
+    if n == 0 or (np.asarray(state["radii"]) <= 0).any():
+        raise ValueError("bed restart needs disks with positive radii")
+    pairs = np.asarray(state["pairs"])
+    xi = np.asarray(state["pair_xi"], np.float64)
+    if pairs.ndim != 2 or pairs.shape[1] != 2 or xi.shape != (len(pairs), 2):
+        raise ValueError("invalid bed restart pairs / pair_xi shape")
+    if not np.isfinite(xi).all():
+        raise ValueError("nonfinite bed restart pair_xi")
+    if len(pairs):
+        if pairs.min() < 0 or pairs.max() >= n or (pairs[:, 0] >= pairs[:, 1]).any():
+            raise ValueError("bed restart pairs must be i < j within the bed")
+        if len(np.unique(pairs[:, 0] * n + pairs[:, 1])) != len(pairs):
+            raise ValueError("repeated bed restart pair")
+    wa = np.asarray(state["wall_active"])
+    if wa.dtype.kind not in "iu" or (wa.astype(np.int64) & ~3).any():
+        raise ValueError("invalid bed restart wall_active")
+    dt, t, sc = state["dt"], state["time"], state["step_count"]
+    if not (math.isfinite(dt) and dt > 0 and math.isfinite(t) and t >= 0):
+        raise ValueError("invalid bed restart time or timestep")
+    if isinstance(sc, bool) or int(sc) != sc or sc < 0:
+        raise ValueError("invalid bed restart step_count")
+
+
+def save_bed_restart(state: dict, path) -> None:
+    """Write a `BedEngine.snapshot` to an .npz file (arrays) with the scalars
+    and the law as JSON inside it; read back by `load_bed_restart` without
+    unpickling anything."""
+    import json
+    meta = {k: state[k] for k in ("schema", "config", "dt", "time", "step_count")}
+    arrays = {k: np.asarray(v) for k, v in state.items() if k not in meta}
+    np.savez(path, meta=np.frombuffer(json.dumps(meta).encode(), dtype=np.uint8), **arrays)
+
+
+def load_bed_restart(path) -> dict:
+    import json
+    with np.load(path, allow_pickle=False) as z:
+        state = json.loads(bytes(z["meta"]).decode())
+        for k in z.files:
+            if k != "meta":
+                state[k] = z[k]
+    if state.get("schema") != BedEngine.RESTART_SCHEMA:
+        raise ValueError(f"{path}: not a tensordem_amd bed restart file")
+    validate_bed_restart(state)
+    return state
