@@ -6,7 +6,9 @@ three files: `history.csv` (one `IceDEM.diagnostics()` row per saved frame),
 `trajectory.pt` (frames of positions, velocities, bond states and tool
 positions) and `accounting.json` (the ledger plus bond and mass totals).
 Between saved frames the plate advances with `IceDEM.run`, which does not
-synchronise with the host per step.  The run needs the ROCm device
+synchronise with the host per step, and frames reach the host through a ring
+of pinned buffers (`FrameStream`), so saving a frame does not wait for the
+device either.  The run needs the ROCm device
 (`--device cuda`, the default here); `--precision fp32` and
 `--dt-policy degree` are additions.
 """
@@ -66,6 +68,59 @@ def _make_sim(args):
                   hull_velocity=(args.hull_velocity_x, args.hull_velocity_y), **kw)
 
 
+class FrameStream:
+    """Saved frames of an IceDEM run, copied device -> host without a host
+    wait per frame.
+
+    `record()` enqueues the diagnostics reductions and copies of positions,
+    velocities and bond states into a ring of pinned host slots on the
+    current stream and returns; the host waits for a slot only when the ring
+    comes round to it again, `ring` frames later, and then hands the frame to
+    `sink(row, positions, velocities, bond_alive, tool_position)`.  Status bits
+    carried with each frame raise as `IceDEM` would (non-finite state,
+    invalid contact search).  `close()` drains the ring in order."""
+
+    def __init__(self, sim, sink, ring: int = 4):
+        self.sim, self.sink = sim, sink
+        n, nb = len(sim.positions), int(sim.bonded.sum().item())
+        pin = dict(dtype=torch.float64, pin_memory=True)
+        self.slots = [dict(vec=torch.empty(sim._DIAG_LEN, **pin),
+                           pos=torch.empty((n, 2), **pin), vel=torch.empty((n, 2), **pin),
+                           alive=torch.empty(nb, dtype=torch.bool, pin_memory=True),
+                           event=torch.cuda.Event(), time=None) for _ in range(max(1, ring))]
+        self.count = 0
+
+    def record(self) -> None:
+        slot = self.slots[self.count % len(self.slots)]
+        if slot["time"] is not None:
+            self._drain(slot)
+        sim = self.sim
+        vec = sim._diagnostics_device()
+        slot["vec"].copy_(vec, non_blocking=True)
+        slot["pos"].copy_(sim.positions, non_blocking=True)
+        slot["vel"].copy_(sim.velocities, non_blocking=True)
+        slot["alive"].copy_(sim.alive[sim.bonded], non_blocking=True)
+        slot["event"].record()
+        slot["time"] = sim.time
+        self.count += 1
+
+    def _drain(self, slot) -> None:
+        slot["event"].synchronize()
+        v = slot["vec"].tolist()
+        if int(v[8]):
+            self.sim._raise_status()
+        self.sink(self.sim._diagnostics_row(v, slot["time"]), slot["pos"].clone(),
+                  slot["vel"].clone(), slot["alive"].clone(), slot["vec"][:2].clone())
+        slot["time"] = None
+
+    def close(self) -> None:
+        k = len(self.slots)
+        for i in range(max(0, self.count - k), self.count):
+            slot = self.slots[i % k]
+            if slot["time"] is not None:
+                self._drain(slot)
+
+
 def dem_2d(args: argparse.Namespace) -> dict:
     if args.steps < 1 or args.save_every < 1:
         sys.exit("--steps and --save-every must be positive")
@@ -78,25 +133,29 @@ def dem_2d(args: argparse.Namespace) -> dict:
     frames = {"positions": [], "velocities": [], "bond_alive": [], "tool_positions": [],
               "times": []}
 
-    def save_frame(writer):
-        writer.writerow(sim.diagnostics())
-        frames["positions"].append(sim.positions.detach().cpu().clone())
-        frames["velocities"].append(sim.velocities.detach().cpu().clone())
-        frames["bond_alive"].append(sim.alive[sim.bonded].detach().cpu().clone())
-        frames["tool_positions"].append(sim.tool_position.detach().cpu().clone())
-        frames["times"].append(sim.time)
-
     with (out / "history.csv").open("w", newline="", encoding="utf-8") as fh:
-        writer = csv.DictWriter(fh, fieldnames=list(sim.diagnostics()))
+        writer = csv.DictWriter(fh, fieldnames=list(sim._diagnostics_row(
+            [0.0] * sim._DIAG_LEN, 0.0)))
         writer.writeheader()
-        save_frame(writer)
+
+        def sink(row, pos, vel, alive, tool):
+            writer.writerow(row)
+            frames["positions"].append(pos)
+            frames["velocities"].append(vel)
+            frames["bond_alive"].append(alive)
+            frames["tool_positions"].append(tool)
+            frames["times"].append(row["time"])
+
+        stream = FrameStream(sim, sink)
+        stream.record()
         done = 0
         while done < args.steps:
             n = min(args.save_every - done % args.save_every, args.steps - done)
             sim.run(n)
             done += n
             if done % args.save_every == 0 or done == args.steps:
-                save_frame(writer)
+                stream.record()
+        stream.close()
     torch.save({
         "config": asdict(sim.config),
         "dt": sim.dt,

@@ -304,33 +304,59 @@ class IceDEM:
 
     @torch.no_grad()
     def diagnostics(self, external_forces: torch.Tensor | None = None) -> dict:
-        force, reaction = self.forces(external_forces=external_forces)
-        boundary_reaction = -force[self.fixed].sum(dim=0)
-        acc = self.accounting
-        tp = self.tool_position
+        vec = self._diagnostics_device(self._validate_external_forces(external_forces))
+        self._raise_status()
+        return self._diagnostics_row(vec.tolist(), self.time)
+
+    # Device vector behind diagnostics(): tool position, reaction, boundary
+    # reaction, broken-bond count, kinetic energy, member status, ledger row.
+    _DIAG_LEN = 9 + _lib.NACC
+
+    @torch.no_grad()
+    def _diagnostics_device(self, loads=None) -> torch.Tensor:
+        """Enqueue the force evaluation and the reductions of diagnostics() on
+        the current stream and return their values as one FP64 device vector,
+        without waiting for them (the member status rides along, unchecked)."""
+        self._sync_inputs()
+        self._set_loads(loads)
+        e = self._eng
+        e.forces()
+        boundary = -e.force[self.fixed].sum(dim=0)
+        kinetic = 0.5 * self.config.mass * self.velocities.square().sum()
+        broken = (~e.alive).sum().to(torch.float64)
+        off = _lib.Member.status.offset
+        status = e.members[off:off + 4].view(torch.int32).to(torch.float64)
+        return torch.cat([self.tool_position, e.reaction[0], boundary, broken[None],
+                          kinetic[None], status, e.accounting[0]])
+
+    @staticmethod
+    def _diagnostics_row(v: list, time: float) -> dict:
+        acc = v[9:]
+        vec = dict(_ACC_VECTORS)
+        sc = dict(_ACC_SCALARS)
         return {
-            "time": self.time,
-            "tool_x": float(tp[0].item()),
-            "tool_y": float(tp[1].item()),
-            "reaction_x": float(reaction[0].item()),
-            "reaction_y": float(reaction[1].item()),
-            "boundary_reaction_x": float(boundary_reaction[0].item()),
-            "boundary_reaction_y": float(boundary_reaction[1].item()),
-            "broken_bonds": self.broken_bonds,
-            "kinetic_energy": float((0.5 * self.config.mass * self.velocities.square().sum()).item()),
-            "tool_impulse_on_ice_x_Ns": float(acc["tool_impulse_on_ice_Ns"][0]),
-            "tool_impulse_on_ice_y_Ns": float(acc["tool_impulse_on_ice_Ns"][1]),
-            "support_impulse_x_Ns": float(acc["support_impulse_Ns"][0]),
-            "support_impulse_y_Ns": float(acc["support_impulse_Ns"][1]),
-            "drag_impulse_x_Ns": float(acc["drag_impulse_Ns"][0]),
-            "drag_impulse_y_Ns": float(acc["drag_impulse_Ns"][1]),
-            "external_impulse_x_Ns": float(acc["external_impulse_Ns"][0]),
-            "external_impulse_y_Ns": float(acc["external_impulse_Ns"][1]),
-            "fracture_release_J": float(acc["fracture_release_J"]),
-            "pair_contact_acquired_J": float(acc["pair_contact_acquired_J"]),
-            "tool_work_J": float(acc["tool_work_J"]),
-            "damping_dissipation_J": float(acc["damping_dissipation_J"]),
-            "external_work_J": float(acc["external_work_J"]),
+            "time": time,
+            "tool_x": v[0],
+            "tool_y": v[1],
+            "reaction_x": v[2],
+            "reaction_y": v[3],
+            "boundary_reaction_x": v[4],
+            "boundary_reaction_y": v[5],
+            "broken_bonds": int(v[6]),
+            "kinetic_energy": v[7],
+            "tool_impulse_on_ice_x_Ns": acc[vec["tool_impulse_on_ice_Ns"]],
+            "tool_impulse_on_ice_y_Ns": acc[vec["tool_impulse_on_ice_Ns"] + 1],
+            "support_impulse_x_Ns": acc[vec["support_impulse_Ns"]],
+            "support_impulse_y_Ns": acc[vec["support_impulse_Ns"] + 1],
+            "drag_impulse_x_Ns": acc[vec["drag_impulse_Ns"]],
+            "drag_impulse_y_Ns": acc[vec["drag_impulse_Ns"] + 1],
+            "external_impulse_x_Ns": acc[vec["external_impulse_Ns"]],
+            "external_impulse_y_Ns": acc[vec["external_impulse_Ns"] + 1],
+            "fracture_release_J": acc[sc["fracture_release_J"]],
+            "pair_contact_acquired_J": acc[sc["pair_contact_acquired_J"]],
+            "tool_work_J": acc[sc["tool_work_J"]],
+            "damping_dissipation_J": acc[sc["damping_dissipation_J"]],
+            "external_work_J": acc[sc["external_work_J"]],
         }
 
     def mechanical_energy(self) -> dict:

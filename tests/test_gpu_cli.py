@@ -28,3 +28,12 @@ def test_dem_2d_outputs(tmp_path):
     assert torch.equal(traj["positions"][-1], torch.tensor(ref.positions))
     assert acc["broken_bonds"] == ref.broken_bonds == 21
     assert acc["fracture_release_J"] == pytest.approx(ref.accounting["fracture_release_J"], rel=1e-12)
+    # the last history row (streamed through the pinned ring) against a direct
+    # diagnostics() call on an identical run
+    from tensordem_amd.icedem import IceDEM
+    sim = IceDEM(DEMConfig(nx=7, ny=3, tool_speed=0.2, breaking_strain=0.015,
+                           shear_breaking_strain=0.03, device="cuda"))
+    sim.run(345)
+    last = dict(zip(rows[0].split(","), rows[-1].split(",")))
+    for k, v in sim.diagnostics().items():
+        assert float(last[k]) == float(v), k
