@@ -43,15 +43,26 @@ enum { E_BOND = 0, E_CONTACT, E_TOOL, E_VSQ };
 __host__ __device__ constexpr int dxc(int c) { return c < 3 ? c - 1 : (c == 3 ? -1 : (c == 4 ? 1 : c - 6)); }
 __host__ __device__ constexpr int dyc(int c) { return c < 3 ? -1 : (c < 5 ? 0 : 1); }
 
+// Clock and contact-search phase of a member at the start of a step, carried
+// from launch to launch inside one multi-step call (double-buffered by the
+// launch parity); the member record is brought up to date when the call ends.
+struct MemberDyn {
+  double time;
+  long long step;
+  int mode, tab, fresh, pad;
+};
+
 struct Workspace {
   int4 *blocks;          // 2 per block: (member, tile x0, tile y0, rank), (nx, ny, first, tile_x)
-  double *partials;      // [kCh][nblocks], channel-major
+  double *partials;      // [2][kCh][nblocks], channel-major, by step parity
   int *bflags;           // [nblocks] status bits | rebuild request << 16
   int *cell_count;       // [2][ncells]
   int *cell_tab;         // [2][ncells][cap]
   int *cand;             // [slots][n]
   uint8_t *ncand;        // [n]
   float2 *xb;            // [n] position - lattice site when the list was built
+  MemberDyn *dyn;        // [nmembers][2]
+  int *mflags;           // [nmembers][4]: status | request << 16 of steps s % 3
 };
 
 
@@ -66,13 +77,15 @@ __host__ __device__ inline void carve(void *base, int n, int nblocks, int nmembe
     return p;
   };
   w->blocks = reinterpret_cast<int4 *>(take(2 * sizeof(int4) * size_t(nblocks)));
-  w->partials = reinterpret_cast<double *>(take(sizeof(double) * kCh * size_t(nblocks)));
+  w->partials = reinterpret_cast<double *>(take(sizeof(double) * 2 * kCh * size_t(nblocks)));
   w->bflags = reinterpret_cast<int *>(take(sizeof(int) * size_t(nblocks)));
   w->cell_count = reinterpret_cast<int *>(take(sizeof(int) * 2 * size_t(ncells)));
   w->cell_tab = reinterpret_cast<int *>(take(sizeof(int) * 2 * size_t(ncells) * size_t(cap)));
   w->cand = reinterpret_cast<int *>(take(sizeof(int) * size_t(slots) * size_t(n)));
   w->ncand = reinterpret_cast<uint8_t *>(take(size_t(n)));
   w->xb = reinterpret_cast<float2 *>(take(sizeof(float2) * size_t(n)));
+  w->dyn = reinterpret_cast<MemberDyn *>(take(sizeof(MemberDyn) * 2 * size_t(nmembers)));
+  w->mflags = reinterpret_cast<int *>(take(sizeof(int) * 4 * size_t(nmembers)));
   *total = off;
 }
 
@@ -158,7 +171,9 @@ struct KArgs {
   Workspace w;
   const double2 *x_in, *v_in;
   double2 *x_out, *v_out;
-  int op;  // OP_FORCES, OP_STEP or OP_ENERGY
+  int op;        // OP_FORCES, OP_STEP or OP_ENERGY
+  int parity;    // 0: x, v -> x_alt, v_alt; 1: the reverse
+  int deferred;  // OP_STEP inside a multi-step call: clock from `dyn`, ledger one launch later
 };
 
 
@@ -205,7 +220,10 @@ __device__ inline bool contact_pair(double2 x_lo, double2 x_hi, double2 v_lo, do
   return true;
 }
 
-template <typename R>
+__device__ void member_ledger(const KArgs &k, int mi, double time, int flags, int pslot,
+                              double (*wsum)[kCh]);
+
+template <typename R, bool kDeferred>
 __device__ __forceinline__ void plate_step_body(const KArgs &k)
 {
   __shared__ double2 sx[kMaxLds];
@@ -213,6 +231,8 @@ __device__ __forceinline__ void plate_step_body(const KArgs &k)
   __shared__ double red[4][kCh];
   __shared__ tdm_plate_member sm;
   __shared__ int sflags[2];
+  __shared__ MemberDyn sdyn;
+  __shared__ int4 sprevf;
 
   const tdm_plate_arrays &a = k.a;
   // block table entry: (member, tile x0, tile y0, rank), (nx, ny, first, tile_x)
@@ -247,6 +267,15 @@ __device__ __forceinline__ void plate_step_body(const KArgs &k)
     reinterpret_cast<uint64_t *>(&sm)[threadIdx.x] =
         reinterpret_cast<const uint64_t *>(a.members + mi)[threadIdx.x];
   if (threadIdx.x < 2) sflags[threadIdx.x] = 0;
+  if (kDeferred && threadIdx.x >= 64 && threadIdx.x < 68) {
+    // the member's state at the start of the previous step and the flags
+    // that step raised (written by the previous launch)
+    const int t = threadIdx.x - 64;
+    if (t < 4)
+      reinterpret_cast<uint64_t *>(&sdyn)[t] =
+          reinterpret_cast<const uint64_t *>(k.w.dyn + 2 * mi + (k.parity ^ 1))[t];
+    if (t == 0) sprevf = reinterpret_cast<const int4 *>(k.w.mflags)[mi];
+  }
   // stage positions of tile + 2-disk halo
   // (at most kMaxLds / kThreads + 1 = 6 rounds; loads of a round are issued together)
   for (int t0 = 0; t0 < lw * lh; t0 += 2 * kThreads) {
@@ -268,8 +297,23 @@ __device__ __forceinline__ void plate_step_body(const KArgs &k)
   const tdm_plate_member &m = sm;
   const double two_r = 2 * m.radius;
   const double half_span = (double)(nx - 1) / 2;
-  const int mode = m.mode;
-  const double time = m.time;
+  // clock and search phase of this step: from the member record, or (inside
+  // a multi-step call) advanced from the previous step's state exactly as
+  // the finalize advances the record
+  int mode = m.mode, tab = m.tab;
+  double time = m.time;
+  long long step = m.step_count;
+  if (kDeferred) {
+    const MemberDyn pd = sdyn;
+    mode = pd.mode; tab = pd.tab; time = pd.time; step = pd.step;
+    if (!pd.fresh) {
+      const int pf = reinterpret_cast<const int *>(&sprevf)[pd.step % 3];
+      time = pd.time + m.dt;
+      step = pd.step + 1;
+      if (pd.mode == MODE_BIN) { tab = pd.tab ^ 1; mode = MODE_SEARCH; }
+      else mode = (pf >> 16) ? MODE_BIN : MODE_IDLE;
+    }
+  }
   Tile tile{sx, lw, tx0, ty0, nx, ny, half_span, two_r};
   const int rw = TX + 2;
   for (int t = threadIdx.x; t < rw * (TY + 2); t += kThreads) {
@@ -312,8 +356,8 @@ __device__ __forceinline__ void plate_step_body(const KArgs &k)
       if (!(xp.x == xp.x) || !(xp.y == xp.y)) { cx = 0; cy = 0; }
       cx = min(max(cx, 0), m.ncx - 1);
       cy = min(max(cy, 0), m.ncy - 1);
-      const int *cnt = k.w.cell_count + (size_t)m.tab * a.ncells;
-      const int *tab = k.w.cell_tab + (size_t)m.tab * a.ncells * a.cell_cap;
+      const int *cnt = k.w.cell_count + (size_t)tab * a.ncells;
+      const int *ctab = k.w.cell_tab + (size_t)tab * a.ncells * a.cell_cap;
       // the nine cell counts are fetched together, then each cell's entries
       int ccount[9];
 #pragma unroll
@@ -330,7 +374,7 @@ __device__ __forceinline__ void plate_step_body(const KArgs &k)
           if (cn == 0) continue;
           const int cell = m.cell_first + gy * m.ncx + gx;
           for (int e = 0; e < cn; ++e) {
-            int q = tab[(size_t)cell * a.cell_cap + e];
+            int q = ctab[(size_t)cell * a.cell_cap + e];
             if (q == p) continue;
             int dq = q - p;
             bool bonded = false;
@@ -726,7 +770,7 @@ __device__ __forceinline__ void plate_step_body(const KArgs &k)
           cx = min(max(cx, 0), m.ncx - 1);
           cy = min(max(cy, 0), m.ncy - 1);
           int cell = m.cell_first + cy * m.ncx + cx;
-          int t = m.tab ^ 1;
+          int t = tab ^ 1;
           int slot = atomicAdd(k.w.cell_count + (size_t)t * a.ncells + cell, 1);
           if (slot < a.cell_cap) k.w.cell_tab[((size_t)t * a.ncells + cell) * a.cell_cap + slot] = p;
           else status |= TDM_ST_CELL_OVERFLOW;
@@ -740,73 +784,76 @@ __device__ __forceinline__ void plate_step_body(const KArgs &k)
   // a search step clears the other hash table for the next bin phase
   if (mode == MODE_SEARCH) {
     const int ncell_m = m.ncx * m.ncy;
-    int *cnt = k.w.cell_count + (size_t)(m.tab ^ 1) * a.ncells + m.cell_first;
+    int *cnt = k.w.cell_count + (size_t)(tab ^ 1) * a.ncells + m.cell_first;
     for (int c = blk.w * kThreads + threadIdx.x; c < ncell_m; c += m.nblocks * kThreads) cnt[c] = 0;
   }
   if (status) atomicOr(&sflags[0], status);
   if (req) atomicOr(&sflags[1], 1);
   block_sum(ch, red);
+  const int pslot = kDeferred ? (int)(step & 1) : 0;
   if (threadIdx.x == 0) {
     const size_t slot = (size_t)(m.block_first + blk.w);
 #pragma unroll
-    for (int c = 0; c < kCh; ++c) k.w.partials[(size_t)c * a.nblocks + slot] = ch[c];
-    k.w.bflags[slot] = sflags[0] | (sflags[1] << 16);
+    for (int c = 0; c < kCh; ++c)
+      k.w.partials[((size_t)pslot * kCh + c) * a.nblocks + slot] = ch[c];
+    const int f = sflags[0] | (sflags[1] << 16);
+    if (!kDeferred) k.w.bflags[slot] = f;
+    else if (f) atomicOr(k.w.mflags + 4 * mi + (int)(step % 3), f);
+  }
+  if (kDeferred && blk.w == 0) {
+    // the member's first tile also carries the member forward: this step's
+    // state for the next launch, and the ledger of the previous step (whose
+    // partials and flags are complete now)
+    if (threadIdx.x == 0) {
+      MemberDyn nd;
+      nd.time = time; nd.step = step; nd.mode = mode; nd.tab = tab; nd.fresh = 0; nd.pad = 0;
+      k.w.dyn[2 * mi + k.parity] = nd;
+      k.w.mflags[4 * mi + (int)((step + 1) % 3)] = 0;
+    }
+    const MemberDyn pd = sdyn;
+    if (!pd.fresh)
+      member_ledger(k, mi, pd.time, reinterpret_cast<const int *>(&sprevf)[pd.step % 3],
+                    (int)(pd.step & 1), red);
   }
 }
 
 // 4 waves per SIMD: both precisions fit 128 VGPRs without spills (the
 // FP32 build keeps its FP64 fallback to the tensile test only, and strains
 // are FP64 in both), so a grid of several rounds of blocks hides latency.
-template <typename R>
+template <typename R, bool kDeferred>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4)))
 void plate_step_kernel(KArgs k)
 {
-  plate_step_body<R>(k);
+  plate_step_body<R, kDeferred>(k);
 }
 
 // ---------------------------------------------------------------------------
+// Ledger reductions.  A member's step ledger is reduced by one 256-thread
+// block: each thread sums its blocks' 16 channels, then a fixed DPP tree per
+// wave and the 4 wave totals in order, so the order depends only on the
+// member's block count (independent of the batch around it, and the same
+// whether the reduction runs in the finalize launch of a single step, in the
+// next step's first tile or in the flush at the end of a multi-step call).
+// Separate launches make every block's partials visible without device-scope
+// fences inside a step (such fences write back the L2 and cost more than a
+// launch).
 // ---------------------------------------------------------------------------
-// Per-member ledger reduction and clock/mode update: one 1 024-thread block
-// per member.  Every thread loads its blocks' 16 channels in one round, then
-// a fixed DPP tree per wave and the 16 wave totals in order: the order
-// depends only on the member's block count (independent of the batch around
-// it).  The member record and ledger are fetched alongside the partials.  A
-// separate launch makes every step block's partials visible without
-// device-scope fences inside the step kernel.
-constexpr int kFinThreads = 1024;
+constexpr int kFinThreads = kThreads;
 
-__global__ __launch_bounds__(kFinThreads) void plate_finalize_kernel(KArgs k)
+__device__ void reduce_partials(const KArgs &k, int b0, int nb, int pslot, double (*wsum)[kCh],
+                                double *stot)
 {
-  __shared__ int sflag;
-  __shared__ double wsum[kFinThreads / 64][kCh];
-  __shared__ double stot[kCh];
-  const tdm_plate_arrays &a = k.a;
-  const int mi = blockIdx.x;
-  tdm_plate_member *mp = a.members + mi;
-  const int b0 = mp->block_first, nb = mp->nblocks;
-  tdm_plate_member m;
-  double acc[TDM_NACC];
-  double *accp = a.accounting + (size_t)mi * TDM_NACC;
-  if (threadIdx.x == 0) {
-    sflag = 0;
-    m = *mp;  // registers: no aliasing with the stores below
-#pragma unroll
-    for (int c = 0; c < TDM_NACC; ++c) acc[c] = accp[c];
-  }
-  __syncthreads();
   double sum[kCh];
 #pragma unroll
   for (int c = 0; c < kCh; ++c) sum[c] = 0.0;
-  int fl = 0;
   for (int b = threadIdx.x; b < nb; b += kFinThreads) {
     double v[kCh];
 #pragma unroll
-    for (int c = 0; c < kCh; ++c) v[c] = k.w.partials[(size_t)c * a.nblocks + b0 + b];
-    fl |= k.w.bflags[b0 + b];
+    for (int c = 0; c < kCh; ++c)
+      v[c] = k.w.partials[((size_t)pslot * kCh + c) * k.a.nblocks + b0 + b];
 #pragma unroll
     for (int c = 0; c < kCh; ++c) sum[c] += v[c];
   }
-  if (fl) atomicOr(&sflag, fl);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
   for (int c = 0; c < kCh; ++c) {
@@ -822,19 +869,104 @@ __global__ __launch_bounds__(kFinThreads) void plate_finalize_kernel(KArgs k)
     stot[threadIdx.x] = t;
   }
   __syncthreads();
+}
+
+// Step ledger of member mi (dem.py:362-378): impulses, work, damping, the
+// last-step record and the status bits; `time` is the step's start time.
+// Called by every thread of a block; thread 0 writes.
+__device__ void member_ledger(const KArgs &k, int mi, double time, int flags, int pslot,
+                              double (*wsum)[kCh])
+{
+  __shared__ double stot[kCh];
+  const tdm_plate_arrays &a = k.a;
+  tdm_plate_member *mp = a.members + mi;
+  const int b0 = mp->block_first, nb = mp->nblocks;
+  __syncthreads();  // wsum may be the caller's reduction scratch
+  reduce_partials(k, b0, nb, pslot, wsum, stot);
   if (threadIdx.x != 0) return;
+  const double dt = mp->dt, drag = mp->drag;
+  const int has_loads = mp->has_loads;
+  const double tvx = mp->tool_velocity[0], tvy = mp->tool_velocity[1];
+  double *accp = a.accounting + (size_t)mi * TDM_NACC;
+  double acc[TDM_NACC];
+#pragma unroll
+  for (int c = 0; c < TDM_NACC; ++c) acc[c] = accp[c];
   double tot[kCh];
 #pragma unroll
   for (int q = 0; q < kCh; ++q) tot[q] = stot[q];
-  const int status = sflag & 0xFFFF, rq = sflag >> 16;
-  const double dt = m.dt, time = m.time;
-  const int mode = m.mode;
-  int new_mode = mode, new_tab = m.tab;
-  if (k.op != OP_ENERGY) {
-    acc[TDM_ACC_RELEASE] += tot[C_RELEASED];
-    acc[TDM_ACC_ACQUIRED] += tot[C_ACQUIRED];
-  }
+  acc[TDM_ACC_RELEASE] += tot[C_RELEASED];
+  acc[TDM_ACC_ACQUIRED] += tot[C_ACQUIRED];
   const double rx_ = -tot[C_TOOL_FX], ry_ = -tot[C_TOOL_FY];
+  const double supx = -tot[C_SUP_X], supy = -tot[C_SUP_Y];
+  const double extx = has_loads ? tot[C_LOAD_X] : 0.0, exty = has_loads ? tot[C_LOAD_Y] : 0.0;
+  const double dimx = (-drag) * tot[C_VSUM_X], dimy = (-drag) * tot[C_VSUM_Y];
+  acc[TDM_ACC_EXT_IMP] = acc[TDM_ACC_EXT_IMP] + dt * extx;
+  acc[TDM_ACC_EXT_IMP + 1] = acc[TDM_ACC_EXT_IMP + 1] + dt * exty;
+  acc[TDM_ACC_SUP_IMP] = acc[TDM_ACC_SUP_IMP] + dt * supx;
+  acc[TDM_ACC_SUP_IMP + 1] = acc[TDM_ACC_SUP_IMP + 1] + dt * supy;
+  acc[TDM_ACC_DRAG_IMP] = acc[TDM_ACC_DRAG_IMP] + dt * dimx;
+  acc[TDM_ACC_DRAG_IMP + 1] = acc[TDM_ACC_DRAG_IMP + 1] + dt * dimy;
+  acc[TDM_ACC_TOOL_IMP] = acc[TDM_ACC_TOOL_IMP] + dt * (-rx_);
+  acc[TDM_ACC_TOOL_IMP + 1] = acc[TDM_ACC_TOOL_IMP + 1] + dt * (-ry_);
+  const double damp = (tot[C_PAIR_DISS] + (-tot[C_TOOL_DISS])) + drag * tot[C_VSQ];
+  const double tpow = -(rx_ * tvx + ry_ * tvy);
+  const double work = has_loads ? tot[C_WORK] : 0.0;
+  acc[TDM_ACC_EXT_WORK] += work;
+  acc[TDM_ACC_TOOL_WORK] += dt * tpow;
+  acc[TDM_ACC_DAMPING] += dt * damp;
+  double last[TDM_NLAST] = {time, time + dt, work, supx, supy, rx_, ry_, extx, exty,
+                            has_loads ? tot[C_MOMENT] : 0.0};
+  double *lastp = a.last + (size_t)mi * TDM_NLAST;
+#pragma unroll
+  for (int c = 0; c < TDM_NLAST; ++c) lastp[c] = last[c];
+#pragma unroll
+  for (int c = 0; c < TDM_NACC; ++c) accp[c] = acc[c];
+  if (flags & 0xFFFF) atomicOr(&mp->status, flags & 0xFFFF);
+}
+
+// clock and search phase after a step that started in (time, step, mode, tab)
+// and raised `flags`
+__device__ inline void advance_record(tdm_plate_member *mp, double time, long long step, int mode,
+                                      int tab, int flags)
+{
+  mp->time = time + mp->dt;
+  mp->step_count = step + 1;
+  if (mode == MODE_BIN) { mp->tab = tab ^ 1; mp->mode = MODE_SEARCH; }
+  else { mp->tab = tab; mp->mode = (flags >> 16) ? MODE_BIN : MODE_IDLE; }
+}
+
+// Finalize of a single launch (one step, a force evaluation or an energy
+// evaluation): one block per member.
+__global__ __launch_bounds__(kFinThreads) void plate_finalize_kernel(KArgs k)
+{
+  __shared__ int sflag;
+  __shared__ double wsum[kFinThreads / 64][kCh];
+  __shared__ double stot[kCh];
+  const tdm_plate_arrays &a = k.a;
+  const int mi = blockIdx.x;
+  tdm_plate_member *mp = a.members + mi;
+  const int b0 = mp->block_first, nb = mp->nblocks;
+  if (threadIdx.x == 0) sflag = 0;
+  __syncthreads();
+  int fl = 0;
+  for (int b = threadIdx.x; b < nb; b += kFinThreads) fl |= k.w.bflags[b0 + b];
+  if (fl) atomicOr(&sflag, fl);
+  __syncthreads();
+  const int flags = sflag;
+  if (k.op == OP_STEP) {
+    const double time = mp->time;
+    const long long step = mp->step_count;
+    const int mode = mp->mode, tab = mp->tab;
+    member_ledger(k, mi, time, flags, 0, wsum);
+    if (threadIdx.x == 0) advance_record(mp, time, step, mode, tab, flags);
+    return;
+  }
+  reduce_partials(k, b0, nb, 0, wsum, stot);
+  if (threadIdx.x != 0) return;
+  const tdm_plate_member m = *mp;
+  double tot[kCh];
+#pragma unroll
+  for (int q = 0; q < kCh; ++q) tot[q] = stot[q];
   if (k.op == OP_ENERGY) {
     double *en = a.energy + (size_t)mi * 5;
     double e0 = 0.5 * m.mass * tot[E_VSQ];
@@ -843,44 +975,47 @@ __global__ __launch_bounds__(kFinThreads) void plate_finalize_kernel(KArgs k)
     double e3 = 0.5 * m.contact_stiffness * tot[E_TOOL];
     en[0] = e0; en[1] = e1; en[2] = e2; en[3] = e3;
     en[4] = ((e0 + e1) + e2) + e3;
-    if (mode == MODE_SEARCH) new_mode = MODE_IDLE;
-  } else if (k.op == OP_STEP) {
-    const double supx = -tot[C_SUP_X], supy = -tot[C_SUP_Y];
-    const double extx = m.has_loads ? tot[C_LOAD_X] : 0.0, exty = m.has_loads ? tot[C_LOAD_Y] : 0.0;
-    const double dimx = (-m.drag) * tot[C_VSUM_X], dimy = (-m.drag) * tot[C_VSUM_Y];
-    acc[TDM_ACC_EXT_IMP] = acc[TDM_ACC_EXT_IMP] + dt * extx;
-    acc[TDM_ACC_EXT_IMP + 1] = acc[TDM_ACC_EXT_IMP + 1] + dt * exty;
-    acc[TDM_ACC_SUP_IMP] = acc[TDM_ACC_SUP_IMP] + dt * supx;
-    acc[TDM_ACC_SUP_IMP + 1] = acc[TDM_ACC_SUP_IMP + 1] + dt * supy;
-    acc[TDM_ACC_DRAG_IMP] = acc[TDM_ACC_DRAG_IMP] + dt * dimx;
-    acc[TDM_ACC_DRAG_IMP + 1] = acc[TDM_ACC_DRAG_IMP + 1] + dt * dimy;
-    acc[TDM_ACC_TOOL_IMP] = acc[TDM_ACC_TOOL_IMP] + dt * (-rx_);
-    acc[TDM_ACC_TOOL_IMP + 1] = acc[TDM_ACC_TOOL_IMP + 1] + dt * (-ry_);
-    const double damp = (tot[C_PAIR_DISS] + (-tot[C_TOOL_DISS])) + m.drag * tot[C_VSQ];
-    const double tpow = -(rx_ * m.tool_velocity[0] + ry_ * m.tool_velocity[1]);
-    const double work = m.has_loads ? tot[C_WORK] : 0.0;
-    acc[TDM_ACC_EXT_WORK] += work;
-    acc[TDM_ACC_TOOL_WORK] += dt * tpow;
-    acc[TDM_ACC_DAMPING] += dt * damp;
-    double last[TDM_NLAST] = {time, time + dt, work, supx, supy, rx_, ry_, extx, exty,
-                              m.has_loads ? tot[C_MOMENT] : 0.0};
-    double *lastp = a.last + (size_t)mi * TDM_NLAST;
-#pragma unroll
-    for (int c = 0; c < TDM_NLAST; ++c) lastp[c] = last[c];
-    mp->time = time + dt;
-    mp->step_count = m.step_count + 1;
-    if (mode == MODE_BIN) { new_tab = m.tab ^ 1; new_mode = MODE_SEARCH; }
-    else new_mode = rq ? MODE_BIN : MODE_IDLE;
   } else {
-    a.reaction[2 * mi] = rx_;
-    a.reaction[2 * mi + 1] = ry_;
-    if (mode == MODE_SEARCH) new_mode = MODE_IDLE;
+    double *accp = a.accounting + (size_t)mi * TDM_NACC;
+    accp[TDM_ACC_RELEASE] += tot[C_RELEASED];
+    accp[TDM_ACC_ACQUIRED] += tot[C_ACQUIRED];
+    a.reaction[2 * mi] = -tot[C_TOOL_FX];
+    a.reaction[2 * mi + 1] = -tot[C_TOOL_FY];
   }
+  // the launch searched for candidates if the member was in the search phase
+  if (m.mode == MODE_SEARCH) mp->mode = MODE_IDLE;
+  if (flags & 0xFFFF) mp->status = m.status | (flags & 0xFFFF);
+}
+
+// Start of a multi-step call: the member records seed the launch-to-launch
+// state (slot read by the first launch) and the flag words are cleared.
+__global__ void plate_init_kernel(const tdm_plate_member *members, int nm, MemberDyn *dyn,
+                                  int *mflags, int slot)
+{
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nm) return;
+  MemberDyn d;
+  d.time = members[t].time;
+  d.step = members[t].step_count;
+  d.mode = members[t].mode;
+  d.tab = members[t].tab;
+  d.fresh = 1;
+  d.pad = 0;
+  dyn[2 * t + slot] = d;
 #pragma unroll
-  for (int c = 0; c < TDM_NACC; ++c) accp[c] = acc[c];
-  mp->mode = new_mode;
-  mp->tab = new_tab;
-  mp->status = m.status | status;
+  for (int c = 0; c < 4; ++c) mflags[4 * t + c] = 0;
+}
+
+// End of a multi-step call: ledger of the last step and the member record
+// brought to the state after it.  k.parity is the last launch's parity.
+__global__ __launch_bounds__(kFinThreads) void plate_flush_kernel(KArgs k)
+{
+  __shared__ double wsum[kFinThreads / 64][kCh];
+  const int mi = blockIdx.x;
+  const MemberDyn st = k.w.dyn[2 * mi + k.parity];
+  const int f = k.w.mflags[4 * mi + (int)(st.step % 3)];
+  member_ledger(k, mi, st.time, f, (int)(st.step & 1), wsum);
+  if (threadIdx.x == 0) advance_record(k.a.members + mi, st.time, st.step, st.mode, st.tab, f);
 }
 
 // ---------------------------------------------------------------------------
@@ -1094,13 +1229,15 @@ int tdm_plate_rebin(const tdm_plate_arrays *a, void *stream)
   return TDM_OK;
 }
 
-static int launch_step(const tdm_plate_arrays *a, const Workspace &w, int parity, int fp32,
-                       int op, hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr)
+static KArgs make_args(const tdm_plate_arrays *a, const Workspace &w, int parity, int op,
+                       int deferred)
 {
   KArgs k;
   k.a = *a;
   k.w = w;
   k.op = op;
+  k.parity = parity;
+  k.deferred = deferred;
   const double2 *x = reinterpret_cast<const double2 *>(a->x);
   const double2 *v = reinterpret_cast<const double2 *>(a->v);
   const double2 *xa = reinterpret_cast<const double2 *>(a->x_alt);
@@ -1109,7 +1246,19 @@ static int launch_step(const tdm_plate_arrays *a, const Workspace &w, int parity
   k.v_in = parity ? va : v;
   k.x_out = const_cast<double2 *>(parity ? x : xa);
   k.v_out = const_cast<double2 *>(parity ? v : va);
-  void (*kern)(KArgs) = fp32 ? plate_step_kernel<float> : plate_step_kernel<double>;
+  return k;
+}
+
+// One launch of the step kernel; a single launch (deferred = 0) is followed
+// by its finalize.
+static int launch_step(const tdm_plate_arrays *a, const Workspace &w, int parity, int fp32,
+                       int op, hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr,
+                       int deferred = 0)
+{
+  const KArgs k = make_args(a, w, parity, op, deferred);
+  void (*kern)(KArgs) =
+      deferred ? (fp32 ? plate_step_kernel<float, true> : plate_step_kernel<double, true>)
+               : (fp32 ? plate_step_kernel<float, false> : plate_step_kernel<double, false>);
   if (ev0) {
     // events carried by the dispatch itself: they time exactly this kernel
     hipExtLaunchKernelGGL(kern, dim3(a->nblocks), dim3(kThreads), 0, s, ev0, ev1, 0, k);
@@ -1117,7 +1266,48 @@ static int launch_step(const tdm_plate_arrays *a, const Workspace &w, int parity
     kern<<<a->nblocks, kThreads, 0, s>>>(k);
   }
   HIP_TRY(hipGetLastError());
-  plate_finalize_kernel<<<a->nmembers, kFinThreads, 0, s>>>(k);
+  if (!deferred) {
+    plate_finalize_kernel<<<a->nmembers, kFinThreads, 0, s>>>(k);
+    HIP_TRY(hipGetLastError());
+  }
+  return TDM_OK;
+}
+
+// Batches of at most this many members carry the ledger from launch to
+// launch; larger batches keep one finalize launch per step (there, one
+// reduction block per member in its own launch is cheaper than a reduction
+// inside the first tile of every member).
+constexpr int kDeferMaxMembers = 4;
+
+// nsteps steps from `parity`: one launch per step.  For two or more steps
+// of a small batch the per-step finalize launches are replaced by the
+// deferred ledger (each step's first tile reduces the previous step) between
+// an init and a flush launch.  ev (optional) holds 2 events per step around
+// each step launch.
+static int launch_steps(const tdm_plate_arrays *a, const Workspace &w, int nsteps, int parity,
+                        int fp32, hipStream_t s, hipEvent_t *ev = nullptr)
+{
+  if (nsteps <= 0) return TDM_OK;
+  if (nsteps == 1 || a->nmembers > kDeferMaxMembers) {
+    for (int i = 0; i < nsteps; ++i) {
+      int rc = launch_step(a, w, parity, fp32, OP_STEP, s, ev ? ev[2 * i] : nullptr,
+                           ev ? ev[2 * i + 1] : nullptr);
+      if (rc) return rc;
+      parity ^= 1;
+    }
+    return TDM_OK;
+  }
+  plate_init_kernel<<<(a->nmembers + 255) / 256, 256, 0, s>>>(a->members, a->nmembers, w.dyn,
+                                                              w.mflags, parity ^ 1);
+  HIP_TRY(hipGetLastError());
+  for (int i = 0; i < nsteps; ++i) {
+    int rc = launch_step(a, w, parity, fp32, OP_STEP, s, ev ? ev[2 * i] : nullptr,
+                         ev ? ev[2 * i + 1] : nullptr, 1);
+    if (rc) return rc;
+    parity ^= 1;
+  }
+  const KArgs k = make_args(a, w, parity ^ 1, OP_STEP, 1);
+  plate_flush_kernel<<<a->nmembers, kFinThreads, 0, s>>>(k);
   HIP_TRY(hipGetLastError());
   return TDM_OK;
 }
@@ -1128,13 +1318,7 @@ int tdm_plate_step(const tdm_plate_arrays *a, int nsteps, int parity, int fp32, 
   int rc = check_arrays(a, &w);
   if (rc) return rc;
   if (nsteps < 0 || (parity != 0 && parity != 1)) return fail(TDM_EINVAL, "bad nsteps/parity");
-  hipStream_t s = static_cast<hipStream_t>(stream);
-  for (int i = 0; i < nsteps; ++i) {
-    rc = launch_step(a, w, parity, fp32, OP_STEP, s);
-    if (rc) return rc;
-    parity ^= 1;
-  }
-  return TDM_OK;
+  return launch_steps(a, w, nsteps, parity, fp32, static_cast<hipStream_t>(stream));
 }
 
 int tdm_plate_step_timed(const tdm_plate_arrays *a, int nsteps, int fp32, void *stream,
@@ -1150,8 +1334,7 @@ int tdm_plate_step_timed(const tdm_plate_arrays *a, int nsteps, int fp32, void *
   for (; made < 2 * nsteps; ++made)
     if (hipEventCreate(&ev[made]) != hipSuccess) break;
   if (made < 2 * nsteps) rc = fail(TDM_EHIP, "hipEventCreate failed");
-  for (int i = 0; i < nsteps && rc == TDM_OK; ++i)
-    rc = launch_step(a, w, i & 1, fp32, OP_STEP, s, ev[2 * i], ev[2 * i + 1]);
+  if (rc == TDM_OK) rc = launch_steps(a, w, nsteps, 0, fp32, s, ev);
   double total = 0.0;
   if (rc == TDM_OK) {
     hipError_t e = hipStreamSynchronize(s);
@@ -1195,7 +1378,7 @@ int tdm_plate_graph_create(const tdm_plate_arrays *a, int pairs, int fp32, void 
   if (!out || pairs <= 0) return fail(TDM_EINVAL, "bad graph request");
   hipStream_t s = static_cast<hipStream_t>(stream);
   HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-  for (int i = 0; i < 2 * pairs && rc == TDM_OK; ++i) rc = launch_step(a, w, i & 1, fp32, OP_STEP, s);
+  rc = launch_steps(a, w, 2 * pairs, 0, fp32, s);
   hipGraph_t g;
   hipError_t e = hipStreamEndCapture(s, &g);
   if (rc) return rc;
