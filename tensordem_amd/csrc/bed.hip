@@ -33,7 +33,6 @@ using namespace tdm;
 constexpr int kMaxSlots = 16;
 constexpr int kChunk = 2;        // partners gathered per round
 constexpr int kPre = 4;          // candidate slots fetched before the list length is known
-constexpr int kFinThreads = 1024;
 constexpr int kNch = 12;         // ledger channels written per block
 
 enum { MODE_IDLE = 0, MODE_SEARCH = 1, MODE_BIN = 2 };
@@ -46,9 +45,19 @@ enum {
 };
 constexpr uint32_t kFloorBit = 1u << 16, kBackBit = 1u << 17;
 
+// Clock, search phase and list state of a member at the start of a step,
+// carried from launch to launch inside one multi-step call (double-buffered
+// by the launch parity); the member record is brought up to date when the
+// call ends.
+struct BedDyn {
+  double time;
+  long long step;
+  int mode, tab, lp, rebuilds, fresh, pad;
+};
+
 struct BedWs {
   int4 *blocks;      // (member, first disk, count | list parity << 16, rank)
-  double *partials;  // [kNch][nblocks], channel-major
+  double *partials;  // [2][kNch][nblocks], by step parity, channel-major
   int *bflags;       // [nblocks]
   int *cell_count;   // [2][ncells]
   int *cell_tab;     // [2][ncells][cap]
@@ -57,12 +66,14 @@ struct BedWs {
   uint32_t *mask;    // [2][n]
   uint8_t *ncand;    // [2][n]
   double2 *xb;       // [n] position when the list was built
+  BedDyn *dyn;       // [nmembers][2]
+  int *mflags;       // [nmembers][4]: status | request << 16 of steps s % 3
 };
 
 // Carves the workspace at `base`; with base == nullptr only the byte offsets
 // are wanted and the returned pointers hold them (used by the layout query).
-inline void carve(void *base, int n, int nblocks, int ncells, int slots, int cap, BedWs *w,
-                  size_t *total)
+inline void carve(void *base, int n, int nblocks, int nmembers, int ncells, int slots, int cap,
+                  BedWs *w, size_t *total)
 {
   size_t off = 0;
   const uintptr_t b = reinterpret_cast<uintptr_t>(base);
@@ -72,7 +83,7 @@ inline void carve(void *base, int n, int nblocks, int ncells, int slots, int cap
     return p;
   };
   w->blocks = reinterpret_cast<int4 *>(take(sizeof(int4) * size_t(nblocks)));
-  w->partials = reinterpret_cast<double *>(take(sizeof(double) * kNch * size_t(nblocks)));
+  w->partials = reinterpret_cast<double *>(take(sizeof(double) * 2 * kNch * size_t(nblocks)));
   w->bflags = reinterpret_cast<int *>(take(sizeof(int) * size_t(nblocks)));
   w->cell_count = reinterpret_cast<int *>(take(sizeof(int) * 2 * size_t(ncells)));
   w->cell_tab = reinterpret_cast<int *>(take(sizeof(int) * 2 * size_t(ncells) * size_t(cap)));
@@ -81,6 +92,8 @@ inline void carve(void *base, int n, int nblocks, int ncells, int slots, int cap
   w->mask = reinterpret_cast<uint32_t *>(take(sizeof(uint32_t) * 2 * size_t(n)));
   w->ncand = reinterpret_cast<uint8_t *>(take(2 * size_t(n)));
   w->xb = reinterpret_cast<double2 *>(take(sizeof(double2) * size_t(n)));
+  w->dyn = reinterpret_cast<BedDyn *>(take(sizeof(BedDyn) * 2 * size_t(nmembers)));
+  w->mflags = reinterpret_cast<int *>(take(sizeof(int) * 4 * size_t(nmembers)));
   *total = off;
 }
 
@@ -90,6 +103,7 @@ struct BKArgs {
   const double2 *x_in, *v_in;
   double2 *x_out, *v_out;
   int op;
+  int parity;  // 0: x, v -> x_alt, v_alt; 1: the reverse
 };
 
 // The member constants one step reads, moved to scalar registers (every lane
@@ -293,10 +307,16 @@ __device__ inline int cell_of(double ox, double oy, double h, int ncx, int ncy, 
 // all disks within r_p + r_q + skin in the 3x3 cells around p, ascending,
 // written to list buffer lp^1, with the spring history of every pair that
 // was in contact carried over by a sorted merge of the old and new lists.
-// Returns the new length and mask; *status gets CAND_OVERFLOW if S is short.
+// Both lists stay in registers (static slot indices after unrolling); the
+// cell entries are visited kScan at a time, so a wave waits on one table
+// load and one partner gather per batch rather than per entry.  On return
+// cq holds the new list (slots past the length hold p).  Returns the new
+// length and mask; *status gets CAND_OVERFLOW if S is short.
+constexpr int kScan = 4;
+
 __device__ inline int rebuild_list(const BKArgs &k, const BedConst &m, int p, double2 xp,
-                                   double rp, int lp, int nc, uint32_t mask, uint32_t *nmask_out,
-                                   int *status)
+                                   double rp, int lp, int nc, uint32_t mask,
+                                   int (&cq)[kMaxSlots], uint32_t *nmask_out, int *status)
 {
   const tdm_bed_arrays &a = k.a;
   const int n = a.n, S = a.cand_slots;
@@ -305,60 +325,111 @@ __device__ inline int rebuild_list(const BKArgs &k, const BedConst &m, int p, do
   const double2 *xis = k.w.xi + lp * sn + p;
   int *ncl = k.w.cand + (lp ^ 1) * sn + p;
   double2 *nxi = k.w.xi + (lp ^ 1) * sn + p;
-  int nn = 0;
   int cx, cy;
   cell_of(m.ox, m.oy, m.cell_size, m.ncx, m.ncy, m.cell_first, xp, &cx, &cy);
   const int *cntab = k.w.cell_count + (size_t)m.tab * a.ncells;
   const int *tab = k.w.cell_tab + (size_t)m.tab * a.ncells * a.cell_cap;
-  int ccount[9];
+  // entries of the 3x3 block, flattened: cells o = 0..8 hold pre[o]..pre[o+1]-1
+  int pre[10];
+  pre[0] = 0;
 #pragma unroll
   for (int o = 0; o < 9; ++o) {
     const int gx = cx + o % 3 - 1, gy = cy + o / 3 - 1;
     const bool in = gx >= 0 && gx < m.ncx && gy >= 0 && gy < m.ncy;
-    ccount[o] = in ? min(cntab[m.cell_first + gy * m.ncx + gx], a.cell_cap) : 0;
+    pre[o + 1] = in ? min(cntab[m.cell_first + gy * m.ncx + gx], a.cell_cap) : 0;
   }
 #pragma unroll
-  for (int o = 0; o < 9; ++o) {
-    const int cn = ccount[o];
-    if (cn == 0) continue;
-    const int cell = m.cell_first + (cy + o / 3 - 1) * m.ncx + (cx + o % 3 - 1);
-    for (int e = 0; e < cn; ++e) {
-      const int q = tab[(size_t)cell * a.cell_cap + e];
+  for (int o = 0; o < 9; ++o) pre[o + 1] += pre[o];
+  const int total = pre[9];
+  int nl[kMaxSlots];
+#pragma unroll
+  for (int e = 0; e < kMaxSlots; ++e) nl[e] = INT_MAX;
+  int nn = 0;
+  for (int t0 = 0; t0 < total; t0 += kScan) {
+    int qs[kScan];
+#pragma unroll
+    for (int u = 0; u < kScan; ++u) {
+      const int t = t0 + u;
+      qs[u] = p;
+      if (t < total) {
+        int o = 0;
+#pragma unroll
+        for (int c = 1; c < 9; ++c) o += t >= pre[c];
+        int oo = 0, pr = 0;  // o and pre[o] without dynamic register indexing
+#pragma unroll
+        for (int c = 0; c < 9; ++c)
+          if (c == o) { oo = c; pr = pre[c]; }
+        const int cell = m.cell_first + (cy + oo / 3 - 1) * m.ncx + (cx + oo % 3 - 1);
+        qs[u] = tab[(size_t)cell * a.cell_cap + (t - pr)];
+      }
+    }
+    double2 xq[kScan];
+    double rq[kScan];
+#pragma unroll
+    for (int u = 0; u < kScan; ++u) {
+      xq[u] = xp;
+      rq[u] = 0.0;
+      if (qs[u] != p) { xq[u] = k.x_in[qs[u]]; rq[u] = a.radius[qs[u]]; }
+    }
+#pragma unroll
+    for (int u = 0; u < kScan; ++u) {
+      const int q = qs[u];
       if (q == p) continue;
-      const double2 xq = k.x_in[q];
-      const double rq = a.radius[q];
-      const double dx = xq.x - xp.x, dy = xq.y - xp.y;
+      const double dx = xq[u].x - xp.x, dy = xq[u].y - xp.y;
       // candidates: within r_p + r_q + skin (the slightly inclusive squared
       // form only ever adds pairs, which the contact test then rejects)
-      const double cut = (rp + rq) + m.skin;
+      const double cut = (rp + rq[u]) + m.skin;
       if (!(dx * dx + dy * dy < (cut * cut) * (1.0 + 1e-15))) continue;
       if (nn == S) { *status |= TDM_ST_CAND_OVERFLOW; continue; }
-      int pos = nn++;
-      while (pos > 0 && ncl[(size_t)(pos - 1) * n] > q) {
-        ncl[(size_t)pos * n] = ncl[(size_t)(pos - 1) * n];
-        --pos;
+      // sorted insert: one compare-exchange pass over the register list
+      int carry = q;
+#pragma unroll
+      for (int e = 0; e < kMaxSlots; ++e) {
+        const int lo = min(nl[e], carry), hi = max(nl[e], carry);
+        nl[e] = lo;
+        carry = hi;
       }
-      ncl[(size_t)pos * n] = q;
+      ++nn;
     }
   }
+  // carry each old slot that was in contact to its place in the new list
   uint32_t nmask = mask & (kFloorBit | kBackBit);
-  int eo = 0;
-  for (int e = 0; e < nn; ++e) {
-    const int q = ncl[(size_t)e * n];
-    double2 h = make_double2(0.0, 0.0);
-    while (eo < nc && cand[(size_t)eo * n] < q) ++eo;
-    if (eo < nc && cand[(size_t)eo * n] == q && ((mask >> eo) & 1u)) {
-      h = xis[(size_t)eo * n];
-      nmask |= 1u << e;
+#pragma unroll
+  for (int o = 0; o < kMaxSlots; ++o) {
+    if (o >= nc || !((mask >> o) & 1u)) continue;
+    const int q = o < kPre ? cq[o] : cand[(size_t)o * n];
+    int pos = 0;
+    bool hit = false;
+#pragma unroll
+    for (int e = 0; e < kMaxSlots; ++e) {
+      pos += nl[e] < q;
+      hit |= nl[e] == q;
     }
-    nxi[(size_t)e * n] = h;
+    if (hit) {
+      nxi[(size_t)pos * n] = xis[(size_t)o * n];
+      nmask |= 1u << pos;
+    }
+  }
+#pragma unroll
+  for (int e = 0; e < kMaxSlots; ++e) {
+    if (e < nn) ncl[(size_t)e * n] = nl[e];
+    cq[e] = e < nn ? nl[e] : p;
   }
   k.w.ncand[(size_t)(lp ^ 1) * n + p] = (uint8_t)nn;
   *nmask_out = nmask;
   return nn;
 }
 
-template <typename R>
+__device__ void bed_member_ledger(const BKArgs &k, int mi, long long step, int flags,
+                                  double (*wsum)[kCh]);
+__device__ inline BedDyn bed_advance(const BedDyn &d, int flags, double dt);
+
+// kDeferred: a launch inside a multi-step call.  The member's clock and
+// search phase come from the previous launch's state and flags instead of
+// the member record, each block flips its own block-table list parity after
+// a rebuild, and the member's first block reduces the previous step's
+// ledger after its own work (one launch per step, no finalize launch).
+template <typename R, bool kDeferred>
 __global__ __launch_bounds__(kThreads) void bed_step_kernel(BKArgs k)
 {
   __shared__ double red[4][kCh];
@@ -379,7 +450,20 @@ __global__ __launch_bounds__(kThreads) void bed_step_kernel(BKArgs k)
   int cq[kMaxSlots];
 #pragma unroll
   for (int e = 0; e < kPre; ++e) cq[e] = e < S ? k.w.cand[lp0 * sn + (size_t)e * n + p] : p;
-  const BedConst m = load_const(a.members + mi);
+  BedConst m = load_const(a.members + mi);
+  BedDyn pd, cur;
+  int pflags = 0;
+  if (kDeferred) {
+    // the state at the start of the previous step and the flags that step
+    // raised (written by the previous launch); uniform, scalar loads
+    pd = k.w.dyn[2 * mi + (k.parity ^ 1)];
+    const int4 pf4 = reinterpret_cast<const int4 *>(k.w.mflags)[mi];
+    const int r = (int)(pd.step % 3);
+    pflags = r == 0 ? pf4.x : (r == 1 ? pf4.y : pf4.z);
+    cur = pd.fresh ? pd : bed_advance(pd, pflags, m.dt);
+    m.mode = cur.mode;
+    m.tab = cur.tab;
+  }
   if (threadIdx.x < 2) sflags[threadIdx.x] = 0;
   double ch[8];
 #pragma unroll
@@ -391,14 +475,11 @@ __global__ __launch_bounds__(kThreads) void bed_step_kernel(BKArgs k)
     const double mass = m.rho_pi_t * (rp * rp);
     if (m.mode == MODE_SEARCH) {
       uint32_t nmask;
-      nc = rebuild_list(k, m, p, xp, rp, lp0, nc, mask, &nmask, &status);
+      nc = rebuild_list(k, m, p, xp, rp, lp0, nc, mask, cq, &nmask, &status);
       mask = nmask;
       lp = lp0 ^ 1;
       xb = xp;
       k.w.xb[p] = xp;
-      const int *ncl = k.w.cand + lp * sn + p;
-#pragma unroll
-      for (int e = 0; e < kMaxSlots; ++e) cq[e] = e < nc ? ncl[(size_t)e * n] : p;
     } else {
 #pragma unroll
       for (int e = kPre; e < kMaxSlots; ++e)
@@ -514,13 +595,35 @@ __global__ __launch_bounds__(kThreads) void bed_step_kernel(BKArgs k)
   if (status) atomicOr(&sflags[0], status);
   if (req) atomicOr(&sflags[1], 1);
   block_sum(ch, red);
+  const int pslot = kDeferred ? (int)(cur.step & 1) : 0;
   if (threadIdx.x == 0) {
     // partials by the member's logical block index: the reduction order does
     // not depend on where the member sits in the launch
     const size_t slot = (size_t)(m.block_first + blk.w);
 #pragma unroll
-    for (int c = 0; c < 8; ++c) k.w.partials[(size_t)c * a.nblocks + slot] = ch[c];
-    k.w.bflags[slot] = sflags[0] | (sflags[1] << 16);
+    for (int c = 0; c < 8; ++c)
+      k.w.partials[((size_t)pslot * kNch + c) * a.nblocks + slot] = ch[c];
+    const int f = sflags[0] | (sflags[1] << 16);
+    if (!kDeferred) {
+      k.w.bflags[slot] = f;
+    } else {
+      if (f) atomicOr(k.w.mflags + 4 * mi + (int)(cur.step % 3), f);
+      // this block alone reads its table entry: the next launch uses the
+      // lists this search wrote
+      if (m.mode == MODE_SEARCH) k.w.blocks[blockIdx.x].z = blk.z ^ kLpBit;
+    }
+  }
+  if (kDeferred && blk.w == 0) {
+    // the member's first block carries the member forward: this step's
+    // state for the next launch, and the ledger of the previous step (whose
+    // partials and flags are complete now)
+    if (threadIdx.x == 0) {
+      BedDyn nd = cur;
+      nd.fresh = 0;
+      k.w.dyn[2 * mi + k.parity] = nd;
+      k.w.mflags[4 * mi + (int)((cur.step + 1) % 3)] = 0;
+    }
+    if (!pd.fresh) bed_member_ledger(k, mi, pd.step, pflags, red);
   }
 }
 
@@ -595,111 +698,182 @@ __global__ __launch_bounds__(kThreads) void bed_energy_kernel(BKArgs k)
   }
 }
 
-// One workgroup per member: every thread loads its share of the member's
-// block partials (up to 4 blocks x 8 channels) in one round, then a fixed
-// DPP tree per wave and the 16 wave totals in order; the order depends only
-// on the member's block count.
-__global__ __launch_bounds__(kFinThreads) void bed_finalize_kernel(BKArgs k)
+// ---------------------------------------------------------------------------
+// Ledger reductions.  A member's partials are reduced by one 256-thread
+// block: thread t sums blocks t, t+256, t+512, ... in that order (loads
+// issued four blocks at a time), then a fixed DPP tree per wave and the four
+// wave totals in order.  The order depends only on the member's block count,
+// so it is the same in the per-step finalize, in the next step's first block
+// (multi-step calls) and in the flush at the end of a call, and independent
+// of the batch around the member.
+// ---------------------------------------------------------------------------
+template <int NC>
+__device__ void bed_reduce(const BKArgs &k, int b0, int nb, int pslot, int c_lo,
+                           double (*wsum)[kCh], double *stot)
 {
-  constexpr int kU = 4, kC = 8;
-  __shared__ int sflag;
-  __shared__ double wsum[kFinThreads / 64][kC];
-  __shared__ double stot[kC];
-  const tdm_bed_arrays &a = k.a;
-  const int mi = blockIdx.x;
-  tdm_bed_member *mp = a.members + mi;
-  const int b0 = mp->block_first, nb = mp->nblocks;
-  const int mode0 = mp->mode, lp0 = mp->list_parity;
-  const int c_lo = k.op == OP_ENERGY ? C_KIN : 0, c_n = k.op == OP_ENERGY ? 4 : 8;
-  // thread 0's tail inputs (member record, ledger) are fetched with the
-  // partials, not after them
-  tdm_bed_member m;
-  double led[8];
-  if (threadIdx.x == 0) {
-    sflag = 0;
-    m = *mp;
-    const double *accp = k.op == OP_ENERGY ? a.energy + (size_t)mi * 4
-                                           : a.accounting + (size_t)mi * TDM_BED_NACC;
+  double acc[NC];
 #pragma unroll
-    for (int c = 0; c < 8; ++c) led[c] = (k.op == OP_ENERGY && c >= 4) ? 0.0 : accp[c];
-  }
-  __syncthreads();
-  if (k.op == OP_STEP && mode0 == MODE_SEARCH) {
-    // the step just rebuilt the lists into buffer lp0^1: mirror the new
-    // parity into the member's block-table entries
-    for (int b = threadIdx.x; b < nb; b += kFinThreads) {
-      const int z = k.w.blocks[b0 + b].z;
-      k.w.blocks[b0 + b].z = (lp0 ^ 1) ? (z | kLpBit) : (z & ~kLpBit);
-    }
-  }
-  double acc[kC];
+  for (int c = 0; c < NC; ++c) acc[c] = 0.0;
+  const double *part = k.w.partials + ((size_t)pslot * kNch + c_lo) * k.a.nblocks + b0;
+  for (int base = 0; base < nb; base += 4 * kThreads) {
+    double v[4][NC];
 #pragma unroll
-  for (int c = 0; c < kC; ++c) acc[c] = 0.0;
-  int fl = 0;
-  for (int base = 0; base < nb; base += kU * kFinThreads) {
-    double v[kU][kC];
+    for (int u = 0; u < 4; ++u) {
+      const int b = base + u * kThreads + (int)threadIdx.x;
 #pragma unroll
-    for (int u = 0; u < kU; ++u) {
-      const int b = base + u * kFinThreads + (int)threadIdx.x;
-#pragma unroll
-      for (int c = 0; c < kC; ++c)
-        v[u][c] = (b < nb && c < c_n) ? k.w.partials[(size_t)(c_lo + c) * a.nblocks + b0 + b] : 0.0;
-      if (b < nb) fl |= k.w.bflags[b0 + b];
+      for (int c = 0; c < NC; ++c) v[u][c] = b < nb ? part[(size_t)c * k.a.nblocks + b] : 0.0;
     }
 #pragma unroll
-    for (int u = 0; u < kU; ++u)
+    for (int u = 0; u < 4; ++u)
 #pragma unroll
-      for (int c = 0; c < kC; ++c) acc[c] += v[u][c];
+      for (int c = 0; c < NC; ++c) acc[c] += v[u][c];
   }
-  if (fl) atomicOr(&sflag, fl);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
-  for (int c = 0; c < kC; ++c) {
+  for (int c = 0; c < NC; ++c) {
     double t = 0.0;
     if (__ballot(acc[c] != 0.0)) t = wave_sum(acc[c]);
     if (lane == 0) wsum[wave][c] = t;
   }
   __syncthreads();
-  if (threadIdx.x < kC) {
-    double t = 0.0;
-#pragma unroll
-    for (int w = 0; w < kFinThreads / 64; ++w) t += wsum[w][threadIdx.x];
-    stot[threadIdx.x] = t;
-  }
+  if (threadIdx.x < NC)
+    stot[threadIdx.x] = ((wsum[0][threadIdx.x] + wsum[1][threadIdx.x]) + wsum[2][threadIdx.x]) +
+                        wsum[3][threadIdx.x];
   __syncthreads();
-  if (threadIdx.x != 0) return;
-  const int status = sflag & 0xFFFF, rq = sflag >> 16;
-  int new_mode = m.mode, new_tab = m.tab, new_lp = m.list_parity;
-  if (k.op == OP_ENERGY) {
-    double *en = a.energy + (size_t)mi * 4;
-    en[0] = stot[C_KIN - C_KIN];
-    en[1] = stot[C_POT - C_KIN];
-    en[2] = stot[C_ELN - C_KIN];
-    en[3] = stot[C_ELT - C_KIN];
-  } else {
-    if (m.mode == MODE_SEARCH) {
-      new_lp ^= 1;
-      mp->rebuilds = m.rebuilds + 1;
-    }
-    double *acc = a.accounting + (size_t)mi * TDM_BED_NACC;
-    const double dt = m.dt;
-    acc[TDM_BED_DASHPOT] = led[TDM_BED_DASHPOT] + dt * stot[C_DASH];
-    acc[TDM_BED_FRICTION] = led[TDM_BED_FRICTION] + dt * stot[C_FRIC];
-    acc[TDM_BED_WALL_DASHPOT] = led[TDM_BED_WALL_DASHPOT] + dt * stot[C_WDASH];
-    acc[TDM_BED_WALL_FRICTION] = led[TDM_BED_WALL_FRICTION] + dt * stot[C_WFRIC];
-    acc[TDM_BED_WALL_IMPULSE] = led[TDM_BED_WALL_IMPULSE] + dt * stot[C_WFX];
-    acc[TDM_BED_WALL_IMPULSE + 1] = led[TDM_BED_WALL_IMPULSE + 1] + dt * stot[C_WFY];
-    acc[TDM_BED_CONTACTS] = stot[C_NPAIR];
-    acc[TDM_BED_WALL_CONTACTS] = stot[C_NWALL];
-    mp->time = m.time + dt;
-    mp->step_count = m.step_count + 1;
-    if (m.mode == MODE_BIN) { new_tab = m.tab ^ 1; new_mode = MODE_SEARCH; }
-    else new_mode = rq ? MODE_BIN : MODE_IDLE;
+}
+
+// Clock, search phase and list parity after a step that started in `d` and
+// raised `flags` (status | rebuild request << 16).
+__device__ inline BedDyn bed_advance(const BedDyn &d, int flags, double dt)
+{
+  BedDyn r = d;
+  r.time = d.time + dt;
+  r.step = d.step + 1;
+  r.fresh = 0;
+  if (d.mode == MODE_SEARCH) {
+    r.lp = d.lp ^ 1;
+    r.rebuilds = d.rebuilds + 1;
   }
-  mp->mode = new_mode;
-  mp->tab = new_tab;
-  mp->list_parity = new_lp;
-  mp->status = m.status | status;
+  if (d.mode == MODE_BIN) { r.tab = d.tab ^ 1; r.mode = MODE_SEARCH; }
+  else r.mode = (flags >> 16) ? MODE_BIN : MODE_IDLE;
+  return r;
+}
+
+// Step ledger of member mi: the partials of the step that started at clock
+// d.step (parity slot d.step & 1) are reduced and added to the accounting;
+// status bits are merged into the member record.  Called by every thread of
+// a 256-thread block; thread 0 writes.
+__device__ void bed_member_ledger(const BKArgs &k, int mi, long long step, int flags,
+                                  double (*wsum)[kCh])
+{
+  __shared__ double stot[8];
+  tdm_bed_member *mp = k.a.members + mi;
+  const int b0 = mp->block_first, nb = mp->nblocks;
+  __syncthreads();  // wsum may be the caller's reduction scratch
+  bed_reduce<8>(k, b0, nb, (int)(step & 1), 0, wsum, stot);
+  if (threadIdx.x != 0) return;
+  double *acc = k.a.accounting + (size_t)mi * TDM_BED_NACC;
+  const double dt = mp->dt;
+  acc[TDM_BED_DASHPOT] = acc[TDM_BED_DASHPOT] + dt * stot[C_DASH];
+  acc[TDM_BED_FRICTION] = acc[TDM_BED_FRICTION] + dt * stot[C_FRIC];
+  acc[TDM_BED_WALL_DASHPOT] = acc[TDM_BED_WALL_DASHPOT] + dt * stot[C_WDASH];
+  acc[TDM_BED_WALL_FRICTION] = acc[TDM_BED_WALL_FRICTION] + dt * stot[C_WFRIC];
+  acc[TDM_BED_WALL_IMPULSE] = acc[TDM_BED_WALL_IMPULSE] + dt * stot[C_WFX];
+  acc[TDM_BED_WALL_IMPULSE + 1] = acc[TDM_BED_WALL_IMPULSE + 1] + dt * stot[C_WFY];
+  acc[TDM_BED_CONTACTS] = stot[C_NPAIR];
+  acc[TDM_BED_WALL_CONTACTS] = stot[C_NWALL];
+  if (flags & 0xFFFF) atomicOr(&mp->status, flags & 0xFFFF);
+}
+
+__device__ inline void bed_write_record(tdm_bed_member *mp, const BedDyn &d)
+{
+  mp->time = d.time;
+  mp->step_count = d.step;
+  mp->mode = d.mode;
+  mp->tab = d.tab;
+  mp->list_parity = d.lp;
+  mp->rebuilds = d.rebuilds;
+}
+
+__device__ inline BedDyn bed_record_state(const tdm_bed_member *mp)
+{
+  BedDyn d;
+  d.time = mp->time;
+  d.step = mp->step_count;
+  d.mode = mp->mode;
+  d.tab = mp->tab;
+  d.lp = mp->list_parity;
+  d.rebuilds = mp->rebuilds;
+  d.fresh = 1;
+  d.pad = 0;
+  return d;
+}
+
+// Finalize of a single launch (one step or an energy evaluation): one block
+// per member.
+__global__ __launch_bounds__(kThreads) void bed_finalize_kernel(BKArgs k)
+{
+  __shared__ int sflag;
+  __shared__ double wsum[4][kCh];
+  __shared__ double stot[4];
+  const tdm_bed_arrays &a = k.a;
+  const int mi = blockIdx.x;
+  tdm_bed_member *mp = a.members + mi;
+  const int b0 = mp->block_first, nb = mp->nblocks;
+  if (threadIdx.x == 0) sflag = 0;
+  __syncthreads();
+  int fl = 0;
+  for (int b = threadIdx.x; b < nb; b += kThreads) fl |= k.w.bflags[b0 + b];
+  if (fl) atomicOr(&sflag, fl);
+  __syncthreads();
+  const int flags = sflag;
+  if (k.op == OP_ENERGY) {
+    bed_reduce<4>(k, b0, nb, 0, C_KIN, wsum, stot);
+    if (threadIdx.x != 0) return;
+    double *en = a.energy + (size_t)mi * 4;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) en[c] = stot[c];
+    if (flags & 0xFFFF) mp->status |= flags & 0xFFFF;
+    return;
+  }
+  const BedDyn d = bed_record_state(mp);
+  if (d.mode == MODE_SEARCH) {
+    // the step just rebuilt the lists into buffer lp^1: mirror the new
+    // parity into the member's block-table entries
+    for (int b = threadIdx.x; b < nb; b += kThreads) {
+      const int z = k.w.blocks[b0 + b].z;
+      k.w.blocks[b0 + b].z = (d.lp ^ 1) ? (z | kLpBit) : (z & ~kLpBit);
+    }
+  }
+  bed_member_ledger(k, mi, 0, flags, wsum);
+  if (threadIdx.x == 0) bed_write_record(mp, bed_advance(d, flags, mp->dt));
+}
+
+// Start of a multi-step call: the member records seed the launch-to-launch
+// state (slot read by the first launch) and the flag words are cleared.
+__global__ void bed_init_kernel(const tdm_bed_member *members, int nm, BedDyn *dyn, int *mflags,
+                                int slot)
+{
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nm) return;
+  dyn[2 * t + slot] = bed_record_state(members + t);
+#pragma unroll
+  for (int c = 0; c < 4; ++c) mflags[4 * t + c] = 0;
+}
+
+// End of a multi-step call: ledger of the last step and the member record
+// brought to the state after it.  k.parity is the last launch's parity.
+__global__ __launch_bounds__(kThreads) void bed_flush_kernel(BKArgs k)
+{
+  __shared__ double wsum[4][kCh];
+  const int mi = blockIdx.x;
+  const BedDyn st = k.w.dyn[2 * mi + k.parity];
+  const int f = k.w.mflags[4 * mi + (int)(st.step % 3)];
+  bed_member_ledger(k, mi, st.step, f, wsum);
+  if (threadIdx.x == 0) {
+    tdm_bed_member *mp = k.a.members + mi;
+    bed_write_record(mp, bed_advance(st, f, mp->dt));
+  }
 }
 
 __global__ void bed_bin_kernel(tdm_bed_arrays a, BedWs w)
@@ -736,20 +910,21 @@ int check_bed(const tdm_bed_arrays *a, BedWs *w)
   if (a->cand_slots <= 0 || a->cand_slots > kMaxSlots || a->cell_cap <= 0)
     return fail(TDM_EINVAL, "cand_slots must be in [1,16] and cell_cap > 0");
   size_t need = 0;
-  carve(a->workspace, a->n, a->nblocks, a->ncells, a->cand_slots, a->cell_cap, w, &need);
+  carve(a->workspace, a->n, a->nblocks, a->nmembers, a->ncells, a->cand_slots, a->cell_cap, w,
+        &need);
   if (need > a->workspace_bytes) return fail(TDM_ESPACE, "bed workspace too small");
   if (reinterpret_cast<uintptr_t>(a->workspace) & 255)
     return fail(TDM_EINVAL, "workspace must be 256-byte aligned");
   return TDM_OK;
 }
 
-int launch_bed(const tdm_bed_arrays *a, const BedWs &w, int parity, int fp32, int op,
-               hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr)
+BKArgs bed_args(const tdm_bed_arrays *a, const BedWs &w, int parity, int op)
 {
   BKArgs k;
   k.a = *a;
   k.w = w;
   k.op = op;
+  k.parity = parity;
   const double2 *x = reinterpret_cast<const double2 *>(a->x);
   const double2 *v = reinterpret_cast<const double2 *>(a->v);
   const double2 *xa = reinterpret_cast<const double2 *>(a->x_alt);
@@ -758,22 +933,72 @@ int launch_bed(const tdm_bed_arrays *a, const BedWs &w, int parity, int fp32, in
   k.v_in = parity ? va : v;
   k.x_out = const_cast<double2 *>(parity ? x : xa);
   k.v_out = const_cast<double2 *>(parity ? v : va);
+  return k;
+}
+
+// One launch of the step (or energy) kernel; a single launch (deferred = 0)
+// is followed by its finalize.
+int launch_bed(const tdm_bed_arrays *a, const BedWs &w, int parity, int fp32, int op,
+               hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr,
+               int deferred = 0)
+{
+  const BKArgs k = bed_args(a, w, parity, op);
   if (op == OP_ENERGY) {
     bed_energy_kernel<<<a->nblocks, kThreads, 0, s>>>(k);
-  } else if (ev0) {
-    if (fp32)
-      hipExtLaunchKernelGGL(bed_step_kernel<float>, dim3(a->nblocks), dim3(kThreads), 0, s, ev0,
-                            ev1, 0, k);
-    else
-      hipExtLaunchKernelGGL(bed_step_kernel<double>, dim3(a->nblocks), dim3(kThreads), 0, s, ev0,
-                            ev1, 0, k);
-  } else if (fp32) {
-    bed_step_kernel<float><<<a->nblocks, kThreads, 0, s>>>(k);
   } else {
-    bed_step_kernel<double><<<a->nblocks, kThreads, 0, s>>>(k);
+    void (*kern)(BKArgs) =
+        deferred ? (fp32 ? bed_step_kernel<float, true> : bed_step_kernel<double, true>)
+                 : (fp32 ? bed_step_kernel<float, false> : bed_step_kernel<double, false>);
+    if (ev0) {
+      // events carried by the dispatch itself: they time exactly this kernel
+      hipExtLaunchKernelGGL(kern, dim3(a->nblocks), dim3(kThreads), 0, s, ev0, ev1, 0, k);
+    } else {
+      kern<<<a->nblocks, kThreads, 0, s>>>(k);
+    }
   }
   HIP_TRY(hipGetLastError());
-  bed_finalize_kernel<<<a->nmembers, kFinThreads, 0, s>>>(k);
+  if (!deferred) {
+    bed_finalize_kernel<<<a->nmembers, kThreads, 0, s>>>(k);
+    HIP_TRY(hipGetLastError());
+  }
+  return TDM_OK;
+}
+
+// Batches of at most this many members carry the ledger from launch to
+// launch inside a multi-step call; larger batches keep one finalize launch
+// per step (one reduction block per member in its own launch is cheaper
+// there than a reduction inside the first block of every member).
+constexpr int kDeferMaxMembers = 4;
+
+// nsteps steps from parity 0: one launch per step.  For two or more steps of
+// a small batch the per-step finalize launches are replaced by the deferred
+// ledger between an init and a flush launch.  ev (optional) holds 2 events
+// per step around each step launch.
+int launch_bed_steps(const tdm_bed_arrays *a, const BedWs &w, int nsteps, int fp32,
+                     hipStream_t s, hipEvent_t *ev = nullptr)
+{
+  if (nsteps <= 0) return TDM_OK;
+  int parity = 0;
+  if (nsteps == 1 || a->nmembers > kDeferMaxMembers) {
+    for (int i = 0; i < nsteps; ++i) {
+      int rc = launch_bed(a, w, parity, fp32, OP_STEP, s, ev ? ev[2 * i] : nullptr,
+                          ev ? ev[2 * i + 1] : nullptr);
+      if (rc) return rc;
+      parity ^= 1;
+    }
+    return TDM_OK;
+  }
+  bed_init_kernel<<<(a->nmembers + 255) / 256, 256, 0, s>>>(a->members, a->nmembers, w.dyn,
+                                                            w.mflags, parity ^ 1);
+  HIP_TRY(hipGetLastError());
+  for (int i = 0; i < nsteps; ++i) {
+    int rc = launch_bed(a, w, parity, fp32, OP_STEP, s, ev ? ev[2 * i] : nullptr,
+                        ev ? ev[2 * i + 1] : nullptr, 1);
+    if (rc) return rc;
+    parity ^= 1;
+  }
+  const BKArgs k = bed_args(a, w, parity ^ 1, OP_STEP);
+  bed_flush_kernel<<<a->nmembers, kThreads, 0, s>>>(k);
   HIP_TRY(hipGetLastError());
   return TDM_OK;
 }
@@ -793,8 +1018,7 @@ size_t tdm_bed_workspace_bytes(int n, int nblocks, int nmembers, int ncells, int
 {
   BedWs w;
   size_t total = 0;
-  (void)nmembers;
-  carve(nullptr, n, nblocks, ncells, cand_slots, cell_cap, &w, &total);
+  carve(nullptr, n, nblocks, nmembers, ncells, cand_slots, cell_cap, &w, &total);
   return total;
 }
 
@@ -806,8 +1030,7 @@ int tdm_bed_workspace_layout(int n, int nblocks, int nmembers, int ncells, int c
     return fail(TDM_EINVAL, "null output");
   BedWs w;
   size_t total = 0;
-  (void)nmembers;
-  carve(nullptr, n, nblocks, ncells, cand_slots, cell_cap, &w, &total);
+  carve(nullptr, n, nblocks, nmembers, ncells, cand_slots, cell_cap, &w, &total);
   *cand_off = reinterpret_cast<size_t>(w.cand);
   *xi_off = reinterpret_cast<size_t>(w.xi);
   *mask_off = reinterpret_cast<size_t>(w.mask);
@@ -883,8 +1106,7 @@ int tdm_bed_step(const tdm_bed_arrays *a, int nsteps, int fp32, void *stream)
   if (rc) return rc;
   if (nsteps < 0) return fail(TDM_EINVAL, "bad nsteps");
   hipStream_t s = static_cast<hipStream_t>(stream);
-  for (int i = 0; i < nsteps && rc == TDM_OK; ++i) rc = launch_bed(a, w, i & 1, fp32, OP_STEP, s);
-  return rc;
+  return launch_bed_steps(a, w, nsteps, fp32, s);
 }
 
 int tdm_bed_step_timed(const tdm_bed_arrays *a, int nsteps, int fp32, void *stream,
@@ -900,8 +1122,7 @@ int tdm_bed_step_timed(const tdm_bed_arrays *a, int nsteps, int fp32, void *stre
   for (; made < 2 * nsteps; ++made)
     if (hipEventCreate(&ev[made]) != hipSuccess) break;
   if (made < 2 * nsteps) rc = fail(TDM_EHIP, "hipEventCreate failed");
-  for (int i = 0; i < nsteps && rc == TDM_OK; ++i)
-    rc = launch_bed(a, w, i & 1, fp32, OP_STEP, s, ev[2 * i], ev[2 * i + 1]);
+  if (rc == TDM_OK) rc = launch_bed_steps(a, w, nsteps, fp32, s, ev);
   double total = 0.0;
   if (rc == TDM_OK) {
     hipError_t e = hipStreamSynchronize(s);
@@ -936,7 +1157,7 @@ int tdm_bed_graph_create(const tdm_bed_arrays *a, int pairs, int fp32, void *str
   if (!out || pairs <= 0) return fail(TDM_EINVAL, "bad graph request");
   hipStream_t s = static_cast<hipStream_t>(stream);
   HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-  for (int i = 0; i < 2 * pairs && rc == TDM_OK; ++i) rc = launch_bed(a, w, i & 1, fp32, OP_STEP, s);
+  rc = launch_bed_steps(a, w, 2 * pairs, fp32, s);
   hipGraph_t g;
   hipError_t e = hipStreamEndCapture(s, &g);
   if (rc) return rc;
