@@ -32,7 +32,7 @@ using namespace tdm;
 
 constexpr int kMaxSlots = 16;
 constexpr int kChunk = 2;        // partners gathered per round
-constexpr int kPre = 4;          // candidate slots fetched before the list length is known
+constexpr int kPre = 6;          // candidate slots fetched before the list length is known
 constexpr int kNch = 12;         // ledger channels written per block
 
 enum { MODE_IDLE = 0, MODE_SEARCH = 1, MODE_BIN = 2 };

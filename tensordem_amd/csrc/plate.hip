@@ -27,6 +27,7 @@ constexpr int kHalo = 2;
 constexpr int kMaxLds = 1300;   // (TX+4)*(TY+4) for TX in 16..256, TY = 256/TX
 constexpr int kMaxRing = 800;   // (TX+2)*(TY+2)
 constexpr int kMaxCand = 16;    // upper bound for cand_slots
+constexpr int kScan = 4;        // cell entries gathered per round of a candidate search
 
 // ledger partial channels
 enum {
@@ -358,43 +359,70 @@ __device__ __forceinline__ void plate_step_body(const KArgs &k)
       cy = min(max(cy, 0), m.ncy - 1);
       const int *cnt = k.w.cell_count + (size_t)tab * a.ncells;
       const int *ctab = k.w.cell_tab + (size_t)tab * a.ncells * a.cell_cap;
-      // the nine cell counts are fetched together, then each cell's entries
-      int ccount[9];
+      // the nine cell counts are fetched together; the 3x3 block's entries
+      // are then visited kScan at a time (one table load and one position
+      // gather per batch) and inserted into a sorted list held in registers
+      int pre[10];
+      pre[0] = 0;
 #pragma unroll
       for (int o = 0; o < 9; ++o) {
         const int gx = cx + o % 3 - 1, gy = cy + o / 3 - 1;
         const bool in = gx >= 0 && gx < m.ncx && gy >= 0 && gy < m.ncy;
-        ccount[o] = in ? min(cnt[m.cell_first + gy * m.ncx + gx], a.cell_cap) : 0;
+        pre[o + 1] = in ? min(cnt[m.cell_first + gy * m.ncx + gx], a.cell_cap) : 0;
       }
 #pragma unroll
-      for (int o = 0; o < 9; ++o) {
-        {
-          const int gx = cx + o % 3 - 1, gy = cy + o / 3 - 1;
-          const int cn = ccount[o];
-          if (cn == 0) continue;
-          const int cell = m.cell_first + gy * m.ncx + gx;
-          for (int e = 0; e < cn; ++e) {
-            int q = ctab[(size_t)cell * a.cell_cap + e];
-            if (q == p) continue;
-            int dq = q - p;
-            bool bonded = false;
+      for (int o = 0; o < 9; ++o) pre[o + 1] += pre[o];
+      const int total = pre[9];
+      int nl[kMaxCand];
 #pragma unroll
-            for (int c = 0; c < 8; ++c)
-              if (((present >> c) & 1u) && dq == dyc(c) * nx + dxc(c)) bonded = true;
-            if (bonded) continue;
-            double2 xq = k.x_in[q];
-            double dx = xq.x - xp.x, dy = xq.y - xp.y;
-            if (!(sqrt(dx * dx + dy * dy) < cutoff)) continue;
-            if (nc == slots) { status |= TDM_ST_CAND_OVERFLOW; continue; }
-            int pos = nc++;
-            while (pos > 0 && cand[(size_t)(pos - 1) * n] > q) {
-              cand[(size_t)pos * n] = cand[(size_t)(pos - 1) * n];
-              --pos;
-            }
-            cand[(size_t)pos * n] = q;
+      for (int e = 0; e < kMaxCand; ++e) nl[e] = INT_MAX;
+      for (int t0 = 0; t0 < total; t0 += kScan) {
+        int qs[kScan];
+#pragma unroll
+        for (int u = 0; u < kScan; ++u) {
+          const int t = t0 + u;
+          qs[u] = p;
+          if (t < total) {
+            int o = 0;
+#pragma unroll
+            for (int c = 1; c < 9; ++c) o += t >= pre[c];
+            int oo = 0, pr = 0;  // o and pre[o] without dynamic register indexing
+#pragma unroll
+            for (int c = 0; c < 9; ++c)
+              if (c == o) { oo = c; pr = pre[c]; }
+            const int cell = m.cell_first + (cy + oo / 3 - 1) * m.ncx + (cx + oo % 3 - 1);
+            qs[u] = ctab[(size_t)cell * a.cell_cap + (t - pr)];
           }
         }
+        double2 xq[kScan];
+#pragma unroll
+        for (int u = 0; u < kScan; ++u) xq[u] = qs[u] != p ? k.x_in[qs[u]] : xp;
+#pragma unroll
+        for (int u = 0; u < kScan; ++u) {
+          const int q = qs[u];
+          if (q == p) continue;
+          const int dq = q - p;
+          bool bonded = false;
+#pragma unroll
+          for (int c = 0; c < 8; ++c)
+            if (((present >> c) & 1u) && dq == dyc(c) * nx + dxc(c)) bonded = true;
+          if (bonded) continue;
+          const double dx = xq[u].x - xp.x, dy = xq[u].y - xp.y;
+          if (!(sqrt(dx * dx + dy * dy) < cutoff)) continue;
+          if (nc == slots) { status |= TDM_ST_CAND_OVERFLOW; continue; }
+          int carry = q;
+#pragma unroll
+          for (int e = 0; e < kMaxCand; ++e) {
+            const int lo = min(nl[e], carry), hi = max(nl[e], carry);
+            nl[e] = lo;
+            carry = hi;
+          }
+          ++nc;
+        }
       }
+#pragma unroll
+      for (int e = 0; e < kMaxCand; ++e)
+        if (e < nc) cand[(size_t)e * n] = nl[e];
       k.w.ncand[p] = (uint8_t)nc;
       xb = make_float2((float)(xp.x - sp.x), (float)(xp.y - sp.y));
       k.w.xb[p] = xb;
