@@ -786,12 +786,15 @@ def cpu_baseline(cfg, dt, budget_s=12.0):
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50000,
-                    help="timed steps of the 1M-disk column collapse (the headline workload)")
-    ap.add_argument("--warmup", type=int, default=1000,
-                    help="untimed steps on a throw-away column engine before timing")
-    ap.add_argument("--plate-steps", type=int, default=20000)
-    ap.add_argument("--plate-warmup", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps of the headline workload: the 1M-disk column collapse "
+                         "(default 50000), or the plate with --workload plate (default 20000)")
+    ap.add_argument("--warmup", type=int, default=None,
+                    help="untimed warm-up steps (column: 1000 on a throw-away engine; "
+                         "plate with --workload plate: 200)")
+    ap.add_argument("--plate-steps", type=int, default=None,
+                    help="timed plate steps in the full run (default 20000)")
+    ap.add_argument("--plate-warmup", type=int, default=None)
     ap.add_argument("--chunk", type=int, default=100, help="plate steps per captured graph")
     ap.add_argument("--plain-steps", type=int, default=30)
     ap.add_argument("--event-launches", type=int, default=0,
@@ -817,6 +820,15 @@ def main(argv=None):
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a 512 MiB device copy (for counter calibration)")
     args = ap.parse_args(argv)
+    plate_only = args.workload == "plate"
+    if args.plate_steps is None:
+        args.plate_steps = args.steps if plate_only and args.steps is not None else 20000
+    if args.plate_warmup is None:
+        args.plate_warmup = args.warmup if plate_only and args.warmup is not None else 200
+    if args.steps is None:
+        args.steps = 50000
+    if args.warmup is None:
+        args.warmup = 1000
     if args.copy_calibration:
         copy_calibration()
         return 0
