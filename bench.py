@@ -621,7 +621,7 @@ def bench_ensemble(args):
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic_from_profile("plate_step_c4_counters.md"),
-                     "kernel": "plate_step_kernel<float, false>", "kernel_us": kernel_s * 1e6,
+                     "kernel": "plate_step_kernel<float, false, false>", "kernel_us": kernel_s * 1e6,
                      "bytes_per_disk": bytes_per_disk, "bytes_per_launch": bytes_per_disk * n},
     }
     if not args.no_plain:
@@ -949,7 +949,7 @@ def bench_plate(args):
                    "broken_bonds_end": r32["broken"], "status_bits": r32["status"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profile(),
-                     "kernel": "plate_step_kernel<float, true>",
+                     "kernel": "plate_step_kernel<float, true, true>",
                      "kernel_us": r32["kernel_s"] * 1e6,
                      "bytes_per_launch": bytes_per_launch},
         "fp64_kernel": {"value": r64["rate"], "ms_per_step": r64["ms_per_step"],

@@ -27,7 +27,9 @@ constexpr int kHalo = 2;
 constexpr int kMaxLds = 1300;   // (TX+4)*(TY+4) for TX in 16..256, TY = 256/TX
 constexpr int kMaxRing = 800;   // (TX+2)*(TY+2)
 constexpr int kMaxCand = 16;    // upper bound for cand_slots
-constexpr int kScan = 4;        // cell entries gathered per round of a candidate search
+constexpr int kScan = 4;        // entries gathered per round (candidate search, candidate forces)
+constexpr int kCandLds = 8;     // candidate slots whose pair forces are staged in LDS
+constexpr int kCandBatch = 2;   // candidates gathered per round of the force pre-pass
 
 // ledger partial channels
 enum {
@@ -224,7 +226,11 @@ __device__ inline bool contact_pair(double2 x_lo, double2 x_hi, double2 v_lo, do
 __device__ void member_ledger(const KArgs &k, int mi, double time, int flags, int pslot,
                               double (*wsum)[kCh]);
 
-template <typename R, bool kDeferred>
+// kStage: the first kCandLds candidates of a disk are gathered ahead of the
+// bonds and their pair forces staged in LDS (24 KB FP32 / 40 KB FP64 more
+// per block); used when the grid is resident in one round, where the LDS
+// does not limit occupancy.
+template <typename R, bool kDeferred, bool kStage>
 __device__ __forceinline__ void plate_step_body(const KArgs &k)
 {
   __shared__ double2 sx[kMaxLds];
@@ -234,6 +240,9 @@ __device__ __forceinline__ void plate_step_body(const KArgs &k)
   __shared__ int sflags[2];
   __shared__ MemberDyn sdyn;
   __shared__ int4 sprevf;
+  // per disk: the first kCandLds candidate indices and their signed pair forces
+  __shared__ int cQ[kCandLds][kThreads];
+  __shared__ R cFx[kCandLds][kThreads], cFy[kCandLds][kThreads];
 
   const tdm_plate_arrays &a = k.a;
   // block table entry: (member, tile x0, tile y0, rank), (nx, ny, first, tile_x)
@@ -330,6 +339,9 @@ __device__ __forceinline__ void plate_step_body(const KArgs &k)
   double ch[kCh];
 #pragma unroll
   for (int c = 0; c < kCh; ++c) ch[c] = 0.0;
+  // dissipation of unbonded candidate contacts, summed apart from that of
+  // broken bonds (list order), so the ledger does not depend on kStage
+  double dsc = 0.0;
   int status = 0, req = 0;
   if (valid) {
     const double2 xp = tile.x(ix, iy);
@@ -431,8 +443,7 @@ __device__ __forceinline__ void plate_step_body(const KArgs &k)
       double lim = 0.5 * m.skin * 0.999;
       if (ddx * ddx + ddy * ddy >= lim * lim) status |= TDM_ST_STALE_LIST;
     }
-    int split = 0;
-    while (split < nc && cand[(size_t)split * n] < p) ++split;
+    int split = 0;  // list slots below split hold partners below p
 
     // unbonded candidate q, lo -> hi oriented; accumulates into f
     auto unbonded = [&](int q, double &fx, double &fy, bool energy) {
@@ -456,12 +467,13 @@ __device__ __forceinline__ void plate_step_body(const KArgs &k)
       if (contact_pair<R>(self_lo ? xp : xq, self_lo ? xq : xp, self_lo ? vp : vq,
                           self_lo ? vq : vp, make_double2(ix0 / id, iy0 / id), m.radius, kc,
                           cdamp, Fx, Fy, diss)) {
-        if (self_lo) { fx += Fx; fy += Fy; ch[C_PAIR_DISS] += diss; }
+        if (self_lo) { fx += Fx; fy += Fy; dsc += diss; }
         else { fx += -Fx; fy += -Fy; }
       }
     };
 
     if (k.op == OP_ENERGY) {
+      while (split < nc && cand[(size_t)split * n] < p) ++split;
       // dem.py:603-624: each bond and contact counted once, at its lower end
 #pragma unroll
       for (int c = 4; c < 8; ++c) {
@@ -500,6 +512,75 @@ __device__ __forceinline__ void plate_step_body(const KArgs &k)
       if (ov > 0) ch[E_TOOL] += ov * ov;
       ch[E_VSQ] += vp.x * vp.x + vp.y * vp.y;
     } else {
+      // ---- unbonded candidates first: the first kCandLds list slots are
+      // gathered kCandBatch at a time (indices, then partner state) and their
+      // signed pair forces are left in this thread's LDS slots for the
+      // ordered sum below, so a rubble disk waits on a few gather rounds
+      // instead of two dependent loads per candidate; slots past kCandLds
+      // (rare) are evaluated inline during the sum
+      uint32_t chas = 0;
+      const int ncl = min(nc, kCandLds);
+      if constexpr (kStage) {
+        for (int e0 = 0; e0 < ncl; e0 += kCandBatch) {
+          int qs[kCandBatch];
+  #pragma unroll
+          for (int u = 0; u < kCandBatch; ++u) qs[u] = e0 + u < ncl ? cand[(size_t)(e0 + u) * n] : p;
+          double2 xq[kCandBatch], vq[kCandBatch];
+  #pragma unroll
+          for (int u = 0; u < kCandBatch; ++u) {
+            xq[u] = xp;
+            vq[u] = vp;
+            if (e0 + u < ncl) {
+              xq[u] = k.x_in[qs[u]];
+              vq[u] = k.v_in[qs[u]];
+            }
+          }
+  #pragma unroll
+          for (int u = 0; u < kCandBatch; ++u) {
+            const int e = e0 + u;
+            if (e >= ncl) continue;
+            const int q = qs[u];
+            cQ[e][threadIdx.x] = q;
+            const bool self_lo = p < q;
+            const int ql = q - first, jx = ql % nx, jy = ql / nx;
+            const double2 sq = lattice_site(jx, jy, half_span, two_r);
+            const double2 s_lo = self_lo ? sp : sq, s_hi = self_lo ? sq : sp;
+            const double ix0 = s_hi.x - s_lo.x, iy0 = s_hi.y - s_lo.y;
+            const double id = sqrt(ix0 * ix0 + iy0 * iy0);
+            double Fx, Fy, diss;
+            if (contact_pair<R>(self_lo ? xp : xq[u], self_lo ? xq[u] : xp, self_lo ? vp : vq[u],
+                                self_lo ? vq[u] : vp, make_double2(ix0 / id, iy0 / id), m.radius,
+                                kc, cdamp, Fx, Fy, diss)) {
+              // the signed contribution the sum adds (FP32 forces are float
+              // values, so the narrowing store is exact)
+              chas |= 1u << e;
+              cFx[e][threadIdx.x] = (R)(self_lo ? Fx : -Fx);
+              cFy[e][threadIdx.x] = (R)(self_lo ? Fy : -Fy);
+              if (self_lo) dsc += diss;
+            }
+          }
+        }
+      }
+      auto cand_at = [&](int e) -> int {
+        if constexpr (kStage) {
+          if (e < kCandLds) return cQ[e][threadIdx.x];
+        }
+        return cand[(size_t)e * n];
+      };
+      auto cand_add = [&](int e, int qc, double &fx_, double &fy_) {
+        if constexpr (kStage) {
+          if (e < kCandLds) {
+            if ((chas >> e) & 1u) {
+              fx_ += (double)cFx[e][threadIdx.x];
+              fy_ += (double)cFy[e][threadIdx.x];
+            }
+            return;
+          }
+        }
+        unbonded(qc, fx_, fy_, false);
+      };
+      while (split < nc && cand_at(split) < p) ++split;
+
       // ---- all present bonds, independently (4-way instruction-level
       // parallelism per group); fracture commit and force per bond --------
       // per-bond forces in the force precision (FP32 values widen exactly
@@ -658,14 +739,14 @@ __device__ __forceinline__ void plate_step_body(const KArgs &k)
         if (!((present >> c) & 1u)) continue;
         const int q = p + dyc(c) * nx + dxc(c);
         while (ci < nc) {
-          const int qc = cand[(size_t)ci * n];
+          const int qc = cand_at(ci);
           if (qc > q) break;
-          unbonded(qc, fx, fy, false);
+          cand_add(ci, qc, fx, fy);
           ++ci;
         }
         if ((has >> c) & 1u) { fx += (double)Fx[c]; fy += (double)Fy[c]; }
       }
-      for (; ci < nc; ++ci) unbonded(cand[(size_t)ci * n], fx, fy, false);
+      for (; ci < nc; ++ci) cand_add(ci, cand_at(ci), fx, fy);
       ci = 0;
 #pragma unroll
       for (int c = 0; c < 4; ++c) bond(c);
@@ -674,14 +755,14 @@ __device__ __forceinline__ void plate_step_body(const KArgs &k)
         if (!((present >> c) & 1u)) continue;
         const int q = p + dyc(c) * nx + dxc(c);
         while (ci < split) {
-          const int qc = cand[(size_t)ci * n];
+          const int qc = cand_at(ci);
           if (qc > q) break;
-          unbonded(qc, fx, fy, false);
+          cand_add(ci, qc, fx, fy);
           ++ci;
         }
         if ((has >> c) & 1u) { fx += -(double)Fx[c]; fy += -(double)Fy[c]; }
       }
-      for (; ci < split; ++ci) unbonded(cand[(size_t)ci * n], fx, fy, false);
+      for (; ci < split; ++ci) cand_add(ci, cand_at(ci), fx, fy);
       a.links[p] = (uint8_t)links;
 
       // ---- tool or hull (dem.py:294-324, hull_contact.py:7-75) ------------
@@ -817,6 +898,7 @@ __device__ __forceinline__ void plate_step_body(const KArgs &k)
   }
   if (status) atomicOr(&sflags[0], status);
   if (req) atomicOr(&sflags[1], 1);
+  ch[C_PAIR_DISS] += dsc;
   block_sum(ch, red);
   const int pslot = kDeferred ? (int)(step & 1) : 0;
   if (threadIdx.x == 0) {
@@ -848,11 +930,11 @@ __device__ __forceinline__ void plate_step_body(const KArgs &k)
 // 4 waves per SIMD: both precisions fit 128 VGPRs without spills (the
 // FP32 build keeps its FP64 fallback to the tensile test only, and strains
 // are FP64 in both), so a grid of several rounds of blocks hides latency.
-template <typename R, bool kDeferred>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4)))
+template <typename R, bool kDeferred, bool kStage>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kStage ? 2 : 4)))
 void plate_step_kernel(KArgs k)
 {
-  plate_step_body<R, kDeferred>(k);
+  plate_step_body<R, kDeferred, kStage>(k);
 }
 
 // ---------------------------------------------------------------------------
@@ -1124,6 +1206,30 @@ int check_arrays(const tdm_plate_arrays *a, Workspace *w)
   return TDM_OK;
 }
 
+// Compute units of the current device (cached per device; filled at setup,
+// before any graph capture).
+int cu_count()
+{
+  static std::atomic<int> cached[64];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+  int c = cached[dev].load(std::memory_order_relaxed);
+  if (c <= 0) {
+    if (hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        c <= 0)
+      c = 256;
+    cached[dev].store(c, std::memory_order_relaxed);
+  }
+  return c;
+}
+
+template <typename R>
+void (*pick_step(int deferred, bool stage))(KArgs)
+{
+  if (deferred) return stage ? plate_step_kernel<R, true, true> : plate_step_kernel<R, true, false>;
+  return stage ? plate_step_kernel<R, false, true> : plate_step_kernel<R, false, false>;
+}
+
 }  // namespace
 
 
@@ -1169,6 +1275,7 @@ int tdm_plate_setup(const tdm_plate_arrays *a, void *stream)
   int rc = check_arrays(a, &w);
   if (rc) return rc;
   hipStream_t s = static_cast<hipStream_t>(stream);
+  (void)cu_count();  // cached before any launch can be captured into a graph
   // block table: host copy of the member table gives tile counts
   const int nm = a->nmembers;
   tdm_plate_member *hm = new tdm_plate_member[nm];
@@ -1278,15 +1385,16 @@ static KArgs make_args(const tdm_plate_arrays *a, const Workspace &w, int parity
 }
 
 // One launch of the step kernel; a single launch (deferred = 0) is followed
-// by its finalize.
+// by its finalize.  Candidate forces are staged in LDS when the whole grid
+// is resident at two blocks per compute unit (the extra LDS then costs no
+// occupancy).
 static int launch_step(const tdm_plate_arrays *a, const Workspace &w, int parity, int fp32,
                        int op, hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr,
                        int deferred = 0)
 {
   const KArgs k = make_args(a, w, parity, op, deferred);
-  void (*kern)(KArgs) =
-      deferred ? (fp32 ? plate_step_kernel<float, true> : plate_step_kernel<double, true>)
-               : (fp32 ? plate_step_kernel<float, false> : plate_step_kernel<double, false>);
+  const bool stage = a->nblocks <= 2 * cu_count();
+  void (*kern)(KArgs) = fp32 ? pick_step<float>(deferred, stage) : pick_step<double>(deferred, stage);
   if (ev0) {
     // events carried by the dispatch itself: they time exactly this kernel
     hipExtLaunchKernelGGL(kern, dim3(a->nblocks), dim3(kThreads), 0, s, ev0, ev1, 0, k);
