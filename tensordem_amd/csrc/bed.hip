@@ -316,7 +316,7 @@ constexpr int kScan = 4;
 
 __device__ inline int rebuild_list(const BKArgs &k, const BedConst &m, int p, double2 xp,
                                    double rp, int lp, int nc, uint32_t mask,
-                                   int (&cq)[kMaxSlots], uint32_t *nmask_out, int *status)
+                                   int (&cq)[kPre], uint32_t *nmask_out, int *status)
 {
   const tdm_bed_arrays &a = k.a;
   const int n = a.n, S = a.cand_slots;
@@ -413,13 +413,14 @@ __device__ inline int rebuild_list(const BKArgs &k, const BedConst &m, int p, do
 #pragma unroll
   for (int e = 0; e < kMaxSlots; ++e) {
     if (e < nn) ncl[(size_t)e * n] = nl[e];
-    cq[e] = e < nn ? nl[e] : p;
+    if (e < kPre) cq[e] = e < nn ? nl[e] : p;
   }
   k.w.ncand[(size_t)(lp ^ 1) * n + p] = (uint8_t)nn;
   *nmask_out = nmask;
   return nn;
 }
 
+template <int U>
 __device__ void bed_member_ledger(const BKArgs &k, int mi, long long step, int flags,
                                   double (*wsum)[kCh]);
 __device__ inline BedDyn bed_advance(const BedDyn &d, int flags, double dt);
@@ -447,7 +448,7 @@ __global__ __launch_bounds__(kThreads) void bed_step_kernel(BKArgs k)
   double2 xb = k.w.xb[p];
   uint32_t mask = k.w.mask[(size_t)lp0 * n + p];
   int nc = k.w.ncand[(size_t)lp0 * n + p];
-  int cq[kMaxSlots];
+  int cq[kPre];  // the first list slots; later slots are read when reached
 #pragma unroll
   for (int e = 0; e < kPre; ++e) cq[e] = e < S ? k.w.cand[lp0 * sn + (size_t)e * n + p] : p;
   BedConst m = load_const(a.members + mi);
@@ -481,9 +482,6 @@ __global__ __launch_bounds__(kThreads) void bed_step_kernel(BKArgs k)
       xb = xp;
       k.w.xb[p] = xp;
     } else {
-#pragma unroll
-      for (int e = kPre; e < kMaxSlots; ++e)
-        cq[e] = e < nc ? k.w.cand[lp0 * sn + (size_t)e * n + p] : p;
       const double ddx = xp.x - xb.x, ddy = xp.y - xb.y;
       const double lim = 0.5 * m.skin * 0.999;
       if (ddx * ddx + ddy * ddy >= lim * lim) status |= TDM_ST_STALE_LIST;
@@ -493,18 +491,21 @@ __global__ __launch_bounds__(kThreads) void bed_step_kernel(BKArgs k)
     // ---- forces: gravity, pairs by ascending partner, floor, back wall
     double fx = 0.0, fy = mass * m.gravity_y;
     uint32_t nmask = 0;
+    int npair = 0, nwall = 0;
 #pragma unroll
     for (int c0 = 0; c0 < kMaxSlots; c0 += kChunk) {
       if (!__any(c0 < nc)) break;
       double2 xq[kChunk], vq[kChunk], hq[kChunk];
       double rq[kChunk];
+      uint32_t hi_bits = 0;  // bit j: partner j has the higher index (p is lo)
 #pragma unroll
       for (int j = 0; j < kChunk; ++j) {
         const int e = c0 + j;
         xq[j] = vq[j] = hq[j] = make_double2(0.0, 0.0);
         rq[j] = 1.0;
         if (e < nc) {
-          const int q = cq[e];
+          const int q = e < kPre ? cq[e] : k.w.cand[lp * sn + (size_t)e * n + p];
+          hi_bits |= (uint32_t)(q > p) << j;
           xq[j] = k.x_in[q];
           vq[j] = k.v_in[q];
           rq[j] = a.radius[q];
@@ -515,7 +516,7 @@ __global__ __launch_bounds__(kThreads) void bed_step_kernel(BKArgs k)
       for (int j = 0; j < kChunk; ++j) {
         const int e = c0 + j;
         if (e >= nc) continue;
-        const bool self_lo = cq[e] > p;  // the pair law is evaluated lo -> hi
+        const bool self_lo = (hi_bits >> j) & 1u;  // the pair law is evaluated lo -> hi
         const double2 xl = self_lo ? xp : xq[j], xh = self_lo ? xq[j] : xp;
         const double2 vl = self_lo ? vp : vq[j], vh = self_lo ? vq[j] : vp;
         const double rl = self_lo ? rp : rq[j], rh = self_lo ? rq[j] : rp;
@@ -526,7 +527,7 @@ __global__ __launch_bounds__(kThreads) void bed_step_kernel(BKArgs k)
         nmask |= 1u << e;
         if (self_lo) {
           fx += -Fx; fy += -Fy;
-          ch[C_DASH] += dash; ch[C_FRIC] += fric; ch[C_NPAIR] += 1.0;
+          ch[C_DASH] += dash; ch[C_FRIC] += fric; ++npair;
         } else {
           fx += Fx; fy += Fy;
         }
@@ -541,7 +542,7 @@ __global__ __launch_bounds__(kThreads) void bed_step_kernel(BKArgs k)
         nmask |= kFloorBit;
         fx += Fx; fy += Fy;
         ch[C_WDASH] += dash; ch[C_WFRIC] += fric; ch[C_WFX] += Fx; ch[C_WFY] += Fy;
-        ch[C_NWALL] += 1.0;
+        ++nwall;
       }
     }
     if (m.use_back_wall) {
@@ -552,10 +553,12 @@ __global__ __launch_bounds__(kThreads) void bed_step_kernel(BKArgs k)
         nmask |= kBackBit;
         fx += Fx; fy += Fy;
         ch[C_WDASH] += dash; ch[C_WFRIC] += fric; ch[C_WFX] += Fx; ch[C_WFY] += Fy;
-        ch[C_NWALL] += 1.0;
+        ++nwall;
       }
     }
     k.w.mask[(size_t)lp * n + p] = nmask;
+    ch[C_NPAIR] = (double)npair;
+    ch[C_NWALL] = (double)nwall;
 
     // ---- semi-implicit Euler (dem.py:363-365) -------------------------
     double vx, vy;
@@ -623,7 +626,7 @@ __global__ __launch_bounds__(kThreads) void bed_step_kernel(BKArgs k)
       k.w.dyn[2 * mi + k.parity] = nd;
       k.w.mflags[4 * mi + (int)((cur.step + 1) % 3)] = 0;
     }
-    if (!pd.fresh) bed_member_ledger(k, mi, pd.step, pflags, red);
+    if (!pd.fresh) bed_member_ledger<1>(k, mi, pd.step, pflags, red);
   }
 }
 
@@ -707,7 +710,9 @@ __global__ __launch_bounds__(kThreads) void bed_energy_kernel(BKArgs k)
 // (multi-step calls) and in the flush at the end of a call, and independent
 // of the batch around the member.
 // ---------------------------------------------------------------------------
-template <int NC>
+// U: blocks whose loads are issued together (the order of the sum does not
+// depend on it; the step kernel uses 1 to keep its register budget)
+template <int NC, int U = 4>
 __device__ void bed_reduce(const BKArgs &k, int b0, int nb, int pslot, int c_lo,
                            double (*wsum)[kCh], double *stot)
 {
@@ -715,16 +720,16 @@ __device__ void bed_reduce(const BKArgs &k, int b0, int nb, int pslot, int c_lo,
 #pragma unroll
   for (int c = 0; c < NC; ++c) acc[c] = 0.0;
   const double *part = k.w.partials + ((size_t)pslot * kNch + c_lo) * k.a.nblocks + b0;
-  for (int base = 0; base < nb; base += 4 * kThreads) {
-    double v[4][NC];
+  for (int base = 0; base < nb; base += U * kThreads) {
+    double v[U][NC];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
+    for (int u = 0; u < U; ++u) {
       const int b = base + u * kThreads + (int)threadIdx.x;
 #pragma unroll
       for (int c = 0; c < NC; ++c) v[u][c] = b < nb ? part[(size_t)c * k.a.nblocks + b] : 0.0;
     }
 #pragma unroll
-    for (int u = 0; u < 4; ++u)
+    for (int u = 0; u < U; ++u)
 #pragma unroll
       for (int c = 0; c < NC; ++c) acc[c] += v[u][c];
   }
@@ -763,6 +768,7 @@ __device__ inline BedDyn bed_advance(const BedDyn &d, int flags, double dt)
 // d.step (parity slot d.step & 1) are reduced and added to the accounting;
 // status bits are merged into the member record.  Called by every thread of
 // a 256-thread block; thread 0 writes.
+template <int U = 4>
 __device__ void bed_member_ledger(const BKArgs &k, int mi, long long step, int flags,
                                   double (*wsum)[kCh])
 {
@@ -770,7 +776,7 @@ __device__ void bed_member_ledger(const BKArgs &k, int mi, long long step, int f
   tdm_bed_member *mp = k.a.members + mi;
   const int b0 = mp->block_first, nb = mp->nblocks;
   __syncthreads();  // wsum may be the caller's reduction scratch
-  bed_reduce<8>(k, b0, nb, (int)(step & 1), 0, wsum, stot);
+  bed_reduce<8, U>(k, b0, nb, (int)(step & 1), 0, wsum, stot);
   if (threadIdx.x != 0) return;
   double *acc = k.a.accounting + (size_t)mi * TDM_BED_NACC;
   const double dt = mp->dt;
