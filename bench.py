@@ -374,6 +374,53 @@ def traffic_from_profile(name="plate_step_c2_counters.md"):
     return None
 
 
+def host_workers():
+    """Host cores the parallel CPU baseline uses: the GPU's share of the
+    host (16 cores), or every core where there are fewer."""
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+class CpuPool:
+    """Single-threaded FP64 oracle processes (oracle/cpu_worker.py), one per
+    host core used, started before this process touches the GPU and set up
+    while it works; `run` releases them together over a common window and
+    returns their aggregate particle-steps/s."""
+
+    def __init__(self, kind, dt, workers):
+        import subprocess
+        env = dict(os.environ, OMP_NUM_THREADS="1", OPENBLAS_NUM_THREADS="1",
+                   MKL_NUM_THREADS="1")
+        self.kind, self.workers = kind, workers
+        self.procs = [subprocess.Popen(
+            [sys.executable, "-m", "oracle.cpu_worker", kind, "--dt", repr(dt), "--wait"],
+            cwd=str(ROOT), env=env, stdin=subprocess.PIPE, stdout=subprocess.PIPE,
+            stderr=subprocess.PIPE, text=True) for _ in range(workers)]
+
+    def run(self, seconds):
+        for p in self.procs:
+            if p.stdout.readline().strip() != "ready":
+                raise RuntimeError(f"cpu worker failed: {p.stderr.read()[-800:]}")
+        start_at = time.time() + 0.5
+        for p in self.procs:
+            p.stdin.write(f"{start_at!r} {seconds!r}\n")
+            p.stdin.flush()
+        outs = []
+        for p in self.procs:
+            out, err = p.communicate()
+            if p.returncode != 0:
+                raise RuntimeError(f"cpu worker failed: {err[-800:]}")
+            outs.append(json.loads(out.strip().splitlines()[-1]))
+        rate = sum(o["particle_steps"] / o["seconds"] for o in outs)
+        steps = sum(o["steps"] for o in outs)
+        return rate, steps, max(o["seconds"] for o in outs)
+
+    def close(self):
+        for p in self.procs:
+            if p.poll() is None:
+                p.kill()
+                p.wait()
+
+
 def copy_calibration(mib=512):
     """A plain device copy of known size, run under rocprofv3 --pmc to scale
     FETCH_SIZE / WRITE_SIZE (each byte is read once and written once)."""
@@ -540,8 +587,23 @@ def bench_bed(args):
         res["speedup_vs_plain_fp64"] = r32["rate"] * ps / n
         del p
     if not args.no_cpu:
-        res["cpu_baseline"] = cpu_baseline_bed(cfg, x, r, dt, args.cpu_seconds)
+        one = cpu_baseline_bed(cfg, x, r, dt, args.cpu_seconds)
+        res["cpu_baseline"] = with_pool(one, getattr(args, "cpu_pools", {}).get("bed"),
+                                        args.cpu_seconds, f"the {len(r)}-disk column from release")
     return res
+
+
+def with_pool(one, pool, seconds, what):
+    """The host baseline: the aggregate of the parallel pool when there is
+    one (the single-core figure kept beside it), else the single core."""
+    if pool is None:
+        return one
+    rate, steps, secs = pool.run(seconds)
+    return dict(value=rate, unit="particle-steps/s", cores=pool.workers, kind="port",
+                sample=f"{pool.workers} concurrent single-threaded FP64 C oracle processes, each "
+                       f"on its own copy of {what} (cell grid rebuilt every step), {steps} steps "
+                       f"in total over a common {secs:.1f} s window",
+                single_core=one)
 
 
 def cpu_baseline_bed(cfg, x, r, dt, budget_s):
@@ -836,6 +898,22 @@ def main(argv=None):
         print("bench.py measures one GPU: every plate and request of this workload runs on a "
               "single MI355X and no multi-GPU path is built.", file=sys.stderr)
         return 2
+    # host-baseline workers are started before this process touches the GPU
+    args.cpu_pools = {}
+    if not args.no_cpu and args.workload in ("all", "bed"):
+        bcfg, _, r0 = c3_inputs()
+        args.cpu_pools["bed"] = CpuPool("bed", bcfg.stable_dt(float(r0.min())), host_workers())
+    if not args.no_cpu and args.workload in ("all", "plate"):
+        args.cpu_pools["plate"] = CpuPool("plate", c2_config("cpu").degree_bound_dt(),
+                                          host_workers())
+    try:
+        return run_workloads(args)
+    finally:
+        for pool in args.cpu_pools.values():
+            pool.close()
+
+
+def run_workloads(args):
     if args.workload in ("bed", "ensemble", "serve"):
         bed = {"bed": bench_bed, "ensemble": bench_ensemble, "serve": bench_serve}[
             args.workload](args)
@@ -959,7 +1037,9 @@ def bench_plate(args):
         line["plain_fp64_gpu"] = plain
         line["speedup_vs_plain_fp64"] = r32["rate"] / plain["value"]
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(c2_config("cpu"), dt, args.cpu_seconds)
+        line["cpu_baseline"] = with_pool(cpu_baseline(c2_config("cpu"), dt, args.cpu_seconds),
+                                         getattr(args, "cpu_pools", {}).get("plate"),
+                                         args.cpu_seconds, "the 1024x128 plate from rest")
     return line
 
 
