@@ -906,12 +906,16 @@ def main(argv=None):
         return 2
     # host-baseline workers are started before this process touches the GPU
     args.cpu_pools = {}
-    if not args.no_cpu and args.workload in ("all", "bed"):
-        bcfg, _, r0 = c3_inputs()
-        args.cpu_pools["bed"] = CpuPool("bed", bcfg.stable_dt(float(r0.min())), host_workers())
-    if not args.no_cpu and args.workload in ("all", "plate"):
-        args.cpu_pools["plate"] = CpuPool("plate", c2_config("cpu").degree_bound_dt(),
-                                          host_workers())
+    try:
+        if not args.no_cpu and args.workload in ("all", "bed"):
+            bcfg, _, r0 = c3_inputs()
+            args.cpu_pools["bed"] = CpuPool("bed", bcfg.stable_dt(float(r0.min())),
+                                            host_workers())
+        if not args.no_cpu and args.workload in ("all", "plate"):
+            args.cpu_pools["plate"] = CpuPool("plate", c2_config("cpu").degree_bound_dt(),
+                                              host_workers())
+    except OSError as exc:
+        print(f"bench.py: host-baseline workers not started ({exc})", file=sys.stderr)
     try:
         return run_workloads(args)
     finally:
