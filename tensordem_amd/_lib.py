@@ -142,8 +142,8 @@ def load() -> ctypes.CDLL:
         return _lib
     if not LIB_PATH.exists():
         raise ImportError(
-            f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; "
-            "g.build()'` (hipcc, gfx950). tensordem_amd has no CPU fallback.")
+            f"{LIB_PATH} is missing: build it with `python -m tensordem_amd.build` "
+            "(hipcc, gfx950). tensordem_amd has no CPU fallback.")
     lib = ctypes.CDLL(str(LIB_PATH))
     for name, (args, res) in _SIGNATURES.items():
         fn = getattr(lib, name)
