@@ -18,6 +18,64 @@ from . import _lib
 from .bedconfig import BedConfig
 
 THREADS = 256
+INT32_MAX = 2 ** 31 - 1
+
+
+def morton_keys(cx: torch.Tensor, cy: torch.Tensor) -> torch.Tensor:
+    """Morton (Z-order) code of integer cell coordinates below 2**16."""
+    def spread(v):
+        v = v.to(torch.int64) & 0xFFFF
+        v = (v | (v << 8)) & 0x00FF00FF
+        v = (v | (v << 4)) & 0x0F0F0F0F
+        v = (v | (v << 2)) & 0x33333333
+        v = (v | (v << 1)) & 0x55555555
+        return v
+    return spread(cx) | (spread(cy) << 1)
+
+
+def spatial_order(x: torch.Tensor, origin, h: float, ncx: int, ncy: int) -> torch.Tensor:
+    """Permutation putting disks in Morton order of their cells (side h from
+    `origin`, clamped into the ncx x ncy grid), ties by current index."""
+    cx = torch.clamp(torch.floor((x[:, 0] - origin[0]) / h), 0, ncx - 1).to(torch.int64)
+    cy = torch.clamp(torch.floor((x[:, 1] - origin[1]) / h), 0, ncy - 1).to(torch.int64)
+    n = len(x)
+    key = morton_keys(cx, cy) * n + torch.arange(n, device=x.device)
+    return torch.argsort(key)
+
+
+def remap_lists(cand, xi, mask, count, perm, inv, first):
+    """Candidate lists of the disks of one member after renumbering.
+
+    cand (S, m) int32 global indices, xi (S, m, 2), mask (m,) int32 and count
+    (m,) uint8 are the member's current lists in the old numbering; perm (m,)
+    gives the old global index of each new local position, inv (N,) the new
+    global index of every old one.  Returns the lists of the new positions:
+    partners renumbered and re-sorted ascending with their spring histories
+    and contact bits, each history negated where the pair's lo -> hi
+    orientation flips, wall bits (16, 17) kept."""
+    S, m = cand.shape
+    dev = cand.device
+    loc = perm - first
+    c = cand[:, loc].to(torch.int64)
+    h = xi[:, loc].clone()
+    mk = mask[loc].to(torch.int64)
+    k = count[loc].to(torch.int64)
+    slot = torch.arange(S, device=dev)[:, None]
+    valid = slot < k[None, :]
+    c_old = torch.where(valid, c, perm[None, :])
+    c_new = inv[c_old]
+    self_new = first + torch.arange(m, device=dev)
+    flip = valid & ((perm[None, :] < c_old) != (self_new[None, :] < c_new))
+    h = torch.where(flip[..., None], -h, h)
+    key = torch.where(valid, c_new, torch.full_like(c_new, INT32_MAX))
+    order = torch.argsort(key, dim=0, stable=True)
+    c_sorted = torch.gather(key, 0, order)
+    h_sorted = torch.gather(h, 0, order[..., None].expand(-1, -1, 2))
+    bits = (mk[None, :] >> order) & 1
+    bits = torch.where(valid, bits, torch.zeros_like(bits))
+    new_mask = (bits << slot).sum(dim=0) | (mk & (3 << 16))
+    out_cand = torch.where(valid, c_sorted, torch.zeros_like(c_sorted)).to(torch.int32)
+    return out_cand, h_sorted, new_mask.to(torch.int32), k.to(torch.uint8)
 
 
 @dataclass
@@ -119,6 +177,8 @@ class BedEngine:
         self.x_alt = torch.empty_like(self.x)
         self.v_alt = torch.empty_like(self.v)
         self.radius = torch.cat([pt.radius for pt in parts])
+        # original index of the disk at each storage position (renumber)
+        self.ids = torch.arange(n, dtype=torch.int64, device=dev)
         self.xi_wall = torch.zeros((n, 4), **f64)
         self.accounting = torch.zeros((nm, _lib.BED_NACC), **f64)
         self.energy = torch.zeros((nm, 4), **f64)
@@ -197,6 +257,48 @@ class BedEngine:
         g = self.graph(pairs)
         for _ in range(repeats):
             _lib.check(self.lib.tdm_graph_launch(g, self.stream))
+
+    def renumber(self) -> torch.Tensor:
+        """Store every member's disks in Morton order of their current cells.
+
+        As a collapse flows, disks that touch drift apart in memory and the
+        step's partner gathers lose cache locality; renumbering restores it.
+        All per-disk state is permuted in place (so captured graphs stay
+        valid): positions, velocities, radii, wall springs, list origins and
+        the current candidate lists with their spring histories (partners
+        renumbered and re-sorted, histories negated where a pair's lo -> hi
+        orientation flips); then the hash is rebuilt, so the next step
+        searches new lists and carries the springs over.  `ids` maps storage
+        positions to the caller's original disk indices.  Returns the
+        permutation (new position -> old position)."""
+        ms = self.read_members()
+        dev = self.device
+        perm = torch.empty(self.n, dtype=torch.int64, device=dev)
+        for m in ms:
+            sl = slice(m.first, m.first + m.n)
+            perm[sl] = m.first + spatial_order(self.x[sl], m.grid_origin, m.cell_size,
+                                               m.ncx, m.ncy)
+        inv = torch.empty_like(perm)
+        inv[perm] = torch.arange(self.n, dtype=torch.int64, device=dev)
+        S = self.cand_slots
+        cand = self._ws_view(0, torch.int32, (2, S, self.n))
+        xi = self._ws_view(1, torch.float64, (2, S, self.n, 2))
+        mask = self._ws_view(2, torch.int32, (2, self.n))
+        count = self._ws_view(3, torch.uint8, (2, self.n))
+        xb = self._ws_view(4, torch.float64, (self.n, 2))
+        for m in ms:
+            sl = slice(m.first, m.first + m.n)
+            lp = m.list_parity
+            c, h, mk, k = remap_lists(cand[lp][:, sl], xi[lp][:, sl], mask[lp][sl],
+                                      count[lp][sl], perm[sl], inv, m.first)
+            cand[lp][:, sl] = c
+            xi[lp][:, sl] = h
+            mask[lp][sl] = mk
+            count[lp][sl] = k
+        for t in (self.x, self.v, self.radius, self.xi_wall, xb, self.ids):
+            t.copy_(t[perm])
+        self.rebin()
+        return perm
 
     def read_members(self):
         raw = self.members.cpu().numpy().tobytes()
