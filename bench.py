@@ -561,6 +561,9 @@ def bench_bed(args):
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic_from_profile("bed_step_c3_counters.md" if not start
                                                      else "bed_step_c3_dense_counters.md"),
+                     "traffic_sample": "counted over the release phase (the dense phase, "
+                                       "profiles/bed_step_c3_dense_counters.md, reads 1.23x "
+                                       "more)" if not start else "counted over the dense phase",
                      "kernel": "bed_step_kernel<float, true>", "kernel_us": kernel_s * 1e6,
                      "bytes_per_disk": bpd, "bytes_per_launch": bpd * n},
     }
