@@ -119,7 +119,9 @@ typedef struct tdm_plate_arrays {
   int32_t n, nbonds, nmembers, nblocks, ncells;
   int32_t cand_slots;       /* unbonded candidate slots per disk */
   int32_t cell_cap;         /* disks per hash cell */
-  int32_t pad_;
+  int32_t max_tile_x;       /* widest member tile, or 0 if not given; a batch whose
+                               tiles are at most 128 wide may stage candidate forces
+                               in LDS (checked against the members at setup) */
 } tdm_plate_arrays;
 
 /* Library version (major*10000 + minor*100 + patch). */

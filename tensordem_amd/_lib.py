@@ -59,7 +59,7 @@ class Arrays(ctypes.Structure):
         ("workspace_bytes", ctypes.c_size_t),
         ("n", ctypes.c_int32), ("nbonds", ctypes.c_int32), ("nmembers", ctypes.c_int32),
         ("nblocks", ctypes.c_int32), ("ncells", ctypes.c_int32), ("cand_slots", ctypes.c_int32),
-        ("cell_cap", ctypes.c_int32), ("pad_", ctypes.c_int32),
+        ("cell_cap", ctypes.c_int32), ("max_tile_x", ctypes.c_int32),
     ]
 
 

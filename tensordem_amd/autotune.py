@@ -23,7 +23,8 @@ from .engine import PlateEngine, PlateSpec
 from .ensemble import with_tiles
 
 DEFAULT_CANDIDATES = (("default", 1.0), ("min_tiles", 1.0), ("min_tiles", 0.5),
-                      ("min_tiles", 1.5), (32, 1.0), (64, 1.0))
+                      ("min_tiles", 1.5), ("min_tiles_128", 1.0), ("min_tiles_128", 0.5),
+                      (32, 1.0), (64, 1.0))
 
 _memory: dict = {}
 

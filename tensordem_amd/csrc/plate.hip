@@ -28,7 +28,13 @@ constexpr int kMaxLds = 1300;   // (TX+4)*(TY+4) for TX in 16..256, TY = 256/TX
 constexpr int kMaxRing = 800;   // (TX+2)*(TY+2)
 constexpr int kMaxCand = 16;    // upper bound for cand_slots
 constexpr int kScan = 4;        // entries gathered per round (candidate search, candidate forces)
-constexpr int kCandLds = 8;     // candidate slots whose pair forces are staged in LDS
+constexpr int kCandLds = 6;     // candidate slots whose pair forces are staged in LDS
+// staged kernels serve tiles at most 128 wide, which bounds their halo and
+// ring (132 x 6, 130 x 4) and keeps an FP32 block within 40 KB of LDS: four
+// blocks per compute unit, as without staging
+constexpr int kStageMaxTile = 128;
+constexpr int kMaxLdsStage = (kStageMaxTile + 4) * (kThreads / kStageMaxTile + 4);
+constexpr int kMaxRingStage = (kStageMaxTile + 2) * (kThreads / kStageMaxTile + 2);
 constexpr int kCandBatch = 2;   // candidates gathered per round of the force pre-pass
 
 // ledger partial channels
@@ -227,14 +233,13 @@ __device__ void member_ledger(const KArgs &k, int mi, double time, int flags, in
                               double (*wsum)[kCh]);
 
 // kStage: the first kCandLds candidates of a disk are gathered ahead of the
-// bonds and their pair forces staged in LDS (24 KB FP32 / 40 KB FP64 more
-// per block); used when the grid is resident in one round, where the LDS
-// does not limit occupancy.
+// bonds and their pair forces staged in LDS (18 KB FP32 / 30 KB FP64 more
+// per block, tiles at most kStageMaxTile wide).
 template <typename R, bool kDeferred, bool kStage>
 __device__ __forceinline__ void plate_step_body(const KArgs &k)
 {
-  __shared__ double2 sx[kMaxLds];
-  __shared__ double ss[kMaxRing];
+  __shared__ double2 sx[kStage ? kMaxLdsStage : kMaxLds];
+  __shared__ double ss[kStage ? kMaxRingStage : kMaxRing];
   __shared__ double red[4][kCh];
   __shared__ tdm_plate_member sm;
   __shared__ int sflags[2];
@@ -1294,6 +1299,12 @@ int tdm_plate_setup(const tdm_plate_arrays *a, void *stream)
     total += t;
   }
   if (total != a->nblocks) { delete[] hm; return fail(TDM_EINVAL, "nblocks mismatch"); }
+  int widest = 0;
+  for (int i = 0; i < nm; ++i) widest = max(widest, (int)hm[i].tile_x);
+  if (a->max_tile_x != 0 && a->max_tile_x < widest) {
+    delete[] hm;
+    return fail(TDM_EINVAL, "max_tile_x is narrower than a member's tile");
+  }
   // Logical tile order keeps a member's tiles contiguous (row-major tiles) so
   // its ledger partials sum in a fixed order.  The physical launch order is
   // permuted so that runs of consecutive tiles share one XCD (blocks are
@@ -1385,15 +1396,17 @@ static KArgs make_args(const tdm_plate_arrays *a, const Workspace &w, int parity
 }
 
 // One launch of the step kernel; a single launch (deferred = 0) is followed
-// by its finalize.  Candidate forces are staged in LDS when the whole grid
-// is resident at two blocks per compute unit (the extra LDS then costs no
-// occupancy).
+// by its finalize.  Candidate forces are staged in LDS when every tile is at
+// most kStageMaxTile wide and the staging costs no occupancy: FP32 blocks
+// keep four per compute unit; FP64 blocks (three per unit) only when the
+// whole grid is resident at two blocks per unit.
 static int launch_step(const tdm_plate_arrays *a, const Workspace &w, int parity, int fp32,
                        int op, hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr,
                        int deferred = 0)
 {
   const KArgs k = make_args(a, w, parity, op, deferred);
-  const bool stage = a->nblocks <= 2 * cu_count();
+  const bool stage = a->max_tile_x > 0 && a->max_tile_x <= kStageMaxTile &&
+                     (fp32 || a->nblocks <= 2 * cu_count());
   void (*kern)(KArgs) = fp32 ? pick_step<float>(deferred, stage) : pick_step<double>(deferred, stage);
   if (ev0) {
     // events carried by the dispatch itself: they time exactly this kernel

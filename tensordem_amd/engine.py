@@ -179,6 +179,7 @@ class PlateEngine:
             blocks += pt.ntiles
             cells += pt.ncells
         self.n, self.nbonds, self.nmembers, self.nblocks, self.ncells = n, nb, nm, blocks, cells
+        self.max_tile_x = max(int(pt.member.tile_x) for pt in parts)
         self.cand_slots, self.cell_cap = cand_slots, cell_cap
         dev = self.device
         f64 = dict(dtype=torch.float64, device=dev)
@@ -240,6 +241,7 @@ class PlateEngine:
         a.workspace_bytes = self.workspace.numel() - (aligned - base)
         a.n, a.nbonds, a.nmembers, a.nblocks, a.ncells = self.n, self.nbonds, self.nmembers, self.nblocks, self.ncells
         a.cand_slots, a.cell_cap = self.cand_slots, self.cell_cap
+        a.max_tile_x = self.max_tile_x
 
     def _swap(self) -> None:
         self.x, self.x_alt = self.x_alt, self.x

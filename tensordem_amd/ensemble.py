@@ -54,13 +54,16 @@ def tiles(nx: int, ny: int, tile_x: int) -> int:
 def tile_for(nx: int, ny: int, policy) -> int:
     """Tile width for one member: an integer is used as given; "default" is
     the single-plate rule; "min_tiles" launches the fewest 256-disk tiles
-    (fewest idle lanes), preferring the squarer tile (less halo) on ties."""
+    (fewest idle lanes), preferring the squarer tile (less halo) on ties;
+    "min_tiles_128" does the same with tiles at most 128 wide (a batch of
+    such tiles can stage candidate forces in LDS)."""
     if isinstance(policy, int):
         return policy
     if policy == "default":
         return default_tile(nx)
-    if policy == "min_tiles":
-        return min(TILE_WIDTHS, key=lambda tx: (tiles(nx, ny, tx), abs(math.log(tx / (256 // tx)))))
+    if policy in ("min_tiles", "min_tiles_128"):
+        widths = TILE_WIDTHS if policy == "min_tiles" else tuple(w for w in TILE_WIDTHS if w <= 128)
+        return min(widths, key=lambda tx: (tiles(nx, ny, tx), abs(math.log(tx / (256 // tx)))))
     raise ValueError(f"unknown tile policy {policy!r}")
 
 
