@@ -266,8 +266,7 @@ int tdm_bed_workspace_layout(int n, int nblocks, int nmembers, int ncells, int c
                              int cell_cap, size_t *cand_off, size_t *xi_off, size_t *mask_off,
                              size_t *count_off, size_t *xb_off);
 /* Block table, cleared histories, first binning (members filled by caller).
- * Replaces FloeBed.__init__ / DiskSystem.__init__ state setup
- * (scale_ceiling/kernel.py:158-216, granular2d/engine.py:147-212). */
+ * Replaces FloeBed.__init__ state setup (scale_ceiling/kernel.py:158-216). */
 int tdm_bed_setup(const tdm_bed_arrays *a, void *stream);
 /* Re-bins x and schedules a candidate search (after the caller moved disks);
  * replaces FloeBed.build_neighbours (scale_ceiling/kernel.py:230-266). */
@@ -275,8 +274,7 @@ int tdm_bed_rebin(const tdm_bed_arrays *a, void *stream);
 /* nsteps steps of the bed law; state alternates x/v <-> x_alt/v_alt starting
  * from x/v (parity 0).  fp32 != 0: forces in FP32, contact detection, state,
  * histories and ledger in FP64.  Replaces FloeBed.step / forces
- * (scale_ceiling/kernel.py:277-452) and DiskSystem.step
- * (granular2d/engine.py:399-627); list rebuilds with history carry-over
+ * (scale_ceiling/kernel.py:277-452); list rebuilds with history carry-over
  * happen inside (scale_ceiling/kernel.py:250-275). */
 int tdm_bed_step(const tdm_bed_arrays *a, int nsteps, int fp32, void *stream);
 /* As tdm_bed_step (nsteps even) with HIP events attached to each step-kernel
@@ -290,6 +288,93 @@ int tdm_bed_energy(const tdm_bed_arrays *a, void *stream);
 /* 2*pairs bed steps captured as a graph (state ends in x/v). */
 int tdm_bed_graph_create(const tdm_bed_arrays *a, int pairs, int fp32, void *stream,
                          tdm_graph **out);
+
+/* ------------------------------------------------------------------------
+ * Rigid frictional disks with rotation (the reference's DiskSystem,
+ * src/tensordem/granular2d/engine.py:144-684): linear spring-dashpot normal
+ * force, tangential spring projected on the contact plane and capped by
+ * Coulomb friction, rolling resistance, static wall and hull segments
+ * (closest point on each segment), optional wall velocities, gravity, drag,
+ * velocity decay, periodic axes, semi-implicit Euler; FP64 throughout, each
+ * disk's sums in the reference's order, so trajectories match it bit for bit.
+ * ---------------------------------------------------------------------- */
+
+/* `out` (float64), written by every step or inspection */
+#define TDM_DISK_NORMAL_DASHPOT 0 /* ledger, accumulated while enabled [J] */
+#define TDM_DISK_FRICTION 1       /* ledger: minus the Coulomb sliding loss [J] */
+#define TDM_DISK_ROLLING 2        /* ledger: minus the rolling plastic loss [J] */
+#define TDM_DISK_ELASTIC 3        /* normal, tangential, rolling spring energy [J] */
+#define TDM_DISK_WALL_FORCE 6     /* x, y: force of the disks on the walls [N] */
+#define TDM_DISK_HULL_FORCE 8     /* x, y: force of the disks on the hull [N] */
+#define TDM_DISK_CONTACTS 10      /* pair, wall, hull contacts of the evaluation */
+#define TDM_DISK_STATUS 13        /* TDM_ST_* bits */
+#define TDM_DISK_REBUILDS 14      /* candidate-list rebuilds so far */
+#define TDM_DISK_NOUT 16
+/* per-contact record fields of tdm_disk_arrays.records */
+#define TDM_DISK_NREC 14 /* i, j, kind, n(2), lever(2), f(2), fn, ft, overlap, point(2) */
+
+typedef struct tdm_disk_params {
+  double kn, zeta2, kt;        /* kn, 2*zeta, kt_ratio*kn */
+  double mu, mu_rolling, kr_coef; /* kr_coef = kt/4 (k_r = kr_coef * r_eff^2) */
+  double dt;
+  double gravity[2], drift[2];
+  double drag_time;            /* 0: no drag */
+  double decay;                /* exp(-dt/damping_time), used when use_decay */
+  double period[2];            /* > 0: periodic axis of that length */
+  double reach;                /* candidate distance 2*max(r) + skin */
+  double skin;                 /* rebuild when a disk moved more than skin/2 */
+  double cell_h[2], grid_origin[2];
+  int32_t ncell[2];
+  int32_t use_gravity, use_decay;
+  int32_t n, n_walls, n_hull;  /* n_walls, n_hull <= 255 */
+  int32_t cand_slots, cell_cap, nblocks;
+} tdm_disk_params;
+
+typedef struct tdm_disk_arrays {
+  double *x, *v, *omega;       /* (N,2), (N,2), (N,): state at parity 0 */
+  double *x_alt, *v_alt, *omega_alt;
+  double *theta;               /* (N,) */
+  const double *radius, *mass, *inertia;
+  const double *walls, *hull;  /* (n_walls,2,2), (n_hull,2,2) segments */
+  const double *wall_velocity; /* (n_walls,2) or NULL */
+  double *out;                 /* TDM_DISK_NOUT */
+  double *records;             /* NULL, or (cand_slots+n_walls+n_hull, N, TDM_DISK_NREC) */
+  void *workspace;
+  size_t workspace_bytes;
+} tdm_disk_arrays;
+
+/* sizeof(tdm_disk_params) and sizeof(tdm_disk_arrays) as compiled. */
+int tdm_disk_abi_sizes(size_t *params_bytes, size_t *arrays_bytes);
+/* Blocks (256 disks each) for n disks. */
+int tdm_disk_blocks(int n);
+size_t tdm_disk_workspace_bytes(int n, int ncells, int cell_cap, int cand_slots, int n_walls,
+                                int n_hull);
+/* Clears lists, cell tables and contact state, schedules the first
+ * candidate search; replaces DiskSystem.__init__'s neighbour and history
+ * setup (granular2d/engine.py:197-212, 674-677). */
+int tdm_disk_setup(const tdm_disk_arrays *a, const tdm_disk_params *p, void *stream);
+/* Schedules a candidate search (after the caller moved disks). */
+int tdm_disk_rebin(const tdm_disk_arrays *a, const tdm_disk_params *p, void *stream);
+/* nsteps steps; state alternates x/v/omega <-> x_alt/v_alt/omega_alt from
+ * parity 0.  flags: TDM_DISK_LEDGER accumulates the energy ledger,
+ * TDM_DISK_RECORD writes per-contact records on the last step.  Replaces
+ * DiskSystem.step and _evaluate(advance_history=True)
+ * (granular2d/engine.py:399-596). */
+#define TDM_DISK_LEDGER 1
+#define TDM_DISK_RECORD 2
+int tdm_disk_step(const tdm_disk_arrays *a, const tdm_disk_params *p, int nsteps, int flags,
+                  void *stream);
+/* Evaluation at the parity-0 state without advancing histories or state
+ * (records when a->records is set); replaces DiskSystem.contact_forces
+ * (granular2d/engine.py:631-634). */
+int tdm_disk_inspect(const tdm_disk_arrays *a, const tdm_disk_params *p, void *stream);
+/* As tdm_disk_step (nsteps even, no flags) with HIP events on each step's
+ * two kernels; mean durations in microseconds.  Measurement helper. */
+int tdm_disk_step_timed(const tdm_disk_arrays *a, const tdm_disk_params *p, int nsteps,
+                        void *stream, double *mean_step_us, double *mean_prep_us);
+/* 2*pairs steps captured as a graph (state ends in x/v/omega). */
+int tdm_disk_graph_create(const tdm_disk_arrays *a, const tdm_disk_params *p, int pairs,
+                          int flags, void *stream, tdm_graph **out);
 
 #ifdef __cplusplus
 }

@@ -130,6 +130,20 @@ _SIGNATURES = {
     "tdm_bed_graph_create": ([ctypes.POINTER(BedArrays), ctypes.c_int, ctypes.c_int,
                               ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)], ctypes.c_int),
     "tdm_graph_destroy": ([ctypes.c_void_p], ctypes.c_int),
+    "tdm_disk_abi_sizes": ([ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)],
+                           ctypes.c_int),
+    "tdm_disk_blocks": ([ctypes.c_int], ctypes.c_int),
+    "tdm_disk_workspace_bytes": ([ctypes.c_int] * 6, ctypes.c_size_t),
+    "tdm_disk_setup": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
+    "tdm_disk_rebin": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
+    "tdm_disk_step": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                       ctypes.c_void_p], ctypes.c_int),
+    "tdm_disk_inspect": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
+    "tdm_disk_step_timed": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                             ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)],
+                            ctypes.c_int),
+    "tdm_disk_graph_create": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                               ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)], ctypes.c_int),
 }
 
 _lib = None

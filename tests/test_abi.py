@@ -106,3 +106,18 @@ def test_bed_workspace_too_small_is_reported():
     a = _bed_arrays()          # 1-byte workspace for 1000 disks
     assert lib.tdm_bed_step(ctypes.byref(a), 1, 0, None) == _lib.TDM_ESPACE
     assert lib.tdm_bed_step(None, 1, 0, None) == _lib.TDM_EINVAL
+
+
+def test_disk_host_only_entry_points():
+    from tensordem_amd.granular2d import DiskArrays, DiskParams
+    lib = _lib.load()
+    pb, ab = ctypes.c_size_t(), ctypes.c_size_t()
+    assert lib.tdm_disk_abi_sizes(ctypes.byref(pb), ctypes.byref(ab)) == 0
+    assert pb.value == ctypes.sizeof(DiskParams)
+    assert ab.value == ctypes.sizeof(DiskArrays)
+    assert lib.tdm_disk_blocks(257) == 2
+    small = lib.tdm_disk_workspace_bytes(1000, 400, 16, 24, 4, 0)
+    assert small > 1000 * 24 * 2 * (4 + 16 + 8)
+    assert lib.tdm_disk_workspace_bytes(1000, 400, 16, 24, 4, 20) > small
+    with pytest.raises(RuntimeError):
+        _lib.check(lib.tdm_disk_step(None, None, 1, 0, None))
