@@ -22,7 +22,9 @@ Reported beside the kernel rate:
     radius 0.5 m moving down at 0.5 m/s, strains 0.01 / 0.02, degree-bounded
     dt (the reference's all-pairs bound would give 5.5e-8 s), 20 000 steps;
   * c4_ensemble: 512 independent plates in one launch per step;
-  * c5_serve: the continuous-batching request loop over a Poisson window.
+  * c5_serve: the continuous-batching request loop over a Poisson window;
+  * disk_system: rigid frictional disks with rotation (the granular2d engine),
+    1 048 576 disks in a walled box under gravity, FP64.
 """
 from __future__ import annotations
 
@@ -632,6 +634,118 @@ def cpu_baseline_bed(cfg, x, r, dt, budget_s):
 
 
 # ---------------------------------------------------------------------------
+# rigid frictional disks with rotation (granular2d engine), FP64
+# ---------------------------------------------------------------------------
+DISK_SIDE = 1024
+DISK_R = 0.01
+
+
+def disk_inputs(side=DISK_SIDE):
+    """side x side disks of radius U[0.9, 1.1] x 0.01 m on a jittered square
+    grid of pitch 2.1 r (the largest neighbours overlap slightly) in a walled
+    box, seeded velocities of 0.05 m/s rms; gravity, mu = 0.5, mu_r = 0.05,
+    the engine's default stiffness, damping, ice density and skin."""
+    from tensordem_amd.granular2d import DiskConfig, box_walls
+    n = side * side
+    rng = np.random.default_rng(0)
+    r = DISK_R * rng.uniform(0.9, 1.1, size=n)
+    s = 2.1 * DISK_R
+    idx = np.arange(n)
+    slack = np.maximum(0.5 * s - r, 0.0)[:, None]
+    pos = (np.stack([idx % side, idx // side], 1) + 0.5) * s \
+        + rng.uniform(-1, 1, size=(n, 2)) * slack
+    vel = rng.normal(scale=0.05, size=(n, 2))
+    cfg = DiskConfig(mu=0.5, mu_rolling=0.05, gravity=(0.0, -9.81))
+    return cfg, pos, r, vel, box_walls(side * s, side * s)
+
+
+def disk_bytes_per_disk(cand_mean, pairs_per_disk, boundary_per_disk):
+    """Compulsory bytes per disk per launch of disk_step_kernel: own x, v,
+    omega read and written (80), radius, mass, inertia (24), angle read and
+    written (16), candidate count (4), contact mask read and written (8),
+    position at the last search (16); per candidate slot its index (4); per
+    pair in contact each disk's copy of its spring read and written
+    (2 x 2 x 24); per boundary contact its spring state read and written
+    (2 x 32)."""
+    return 148.0 + 4.0 * cand_mean + 96.0 * pairs_per_disk + 64.0 * boundary_per_disk
+
+
+def bench_disk(args):
+    """The granular2d engine on 1M disks: graph-replayed FP64 steps, kernel
+    events with list statistics, the C oracle on a CPU sample."""
+    import torch
+    from tensordem_amd.granular2d import DiskSystem
+
+    cfg, pos, r, vel, walls = disk_inputs()
+    n = len(r)
+    steps, chunk = graph_steps(args.disk_steps, 100)
+    sysd = DiskSystem(pos, r, cfg, walls=walls)
+    sysd.vel = vel
+    sysd.step(2)
+    wsteps, wchunk = graph_steps(max(2, args.disk_warmup), 100)
+    sysd.run_graph(chunk // 2, repeats=wsteps // chunk)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    sysd.run_graph(chunk // 2, repeats=steps // chunk)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    gpu_s = e0.elapsed_time(e1) * 1e-3
+    st = sysd.list_stats()
+    status = int(sysd.out[13].item())
+    rebuilds = sysd.rebuilds
+    step_us, prep_us = sysd.step_timed(200)
+    st2 = sysd.list_stats()
+    mean = {k: 0.5 * (st[k] + st2[k]) for k in st}
+    bpd = disk_bytes_per_disk(mean["cand_mean"], mean["pairs_per_disk"],
+                              mean["walls_per_disk"] + mean["hull_per_disk"])
+    achieved = bpd * n / (step_us * 1e-6) / 1e9
+    res = {
+        "workload": f"rigid frictional disks with rotation (granular2d engine), {n} disks, "
+                    f"radii U[0.9,1.1]x{DISK_R} m on a jittered grid in a walled box, "
+                    f"gravity, mu=0.5, mu_rolling=0.05, kn=3e4, zeta=0.2, dt={sysd.dt:.6g} s, "
+                    f"{steps} steps after {wsteps} warm-up steps",
+        "value": n * steps / gpu_s, "unit": "particle-steps/s", "steps": steps,
+        "warmup": wsteps, "ms_per_step": gpu_s / steps * 1e3, "dtype": "f64",
+        "status_bits": status, "list_rebuilds": rebuilds, "lists": st,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic_from_profile("disk_step_counters.md"),
+                     "kernel": "disk_step_kernel<true>", "kernel_us": step_us,
+                     "prep_kernel_us": prep_us, "bytes_per_disk": bpd,
+                     "bytes_per_launch": bpd * n},
+    }
+    if status:
+        print(f"bench.py: disk run ended with status bits {status}; its rate is not a valid "
+              "measurement", file=sys.stderr)
+    sysd.close()
+    del sysd
+    if not args.no_cpu:
+        res["cpu_baseline"] = cpu_baseline_disk(args.cpu_seconds)
+    return res
+
+
+def cpu_baseline_disk(budget_s, side=256):
+    """C oracle (FP64, one core) on a 256 x 256 box of the same construction
+    (65 536 disks; per-disk work does not depend on the box size)."""
+    from oracle.disk import OracleDisk
+    cfg, pos, r, vel, walls = disk_inputs(side)
+    sim = OracleDisk(pos, r, cfg, walls=walls)
+    sim.vel[:] = vel
+    sim.step()
+    t0 = time.perf_counter()
+    steps = 0
+    while time.perf_counter() - t0 < budget_s:
+        sim.step()
+        steps += 1
+    el = time.perf_counter() - t0
+    return dict(value=len(r) * steps / el, unit="particle-steps/s", cores=1, kind="port",
+                sample=f"{steps} steps of a {len(r)}-disk box of the same construction (FP64 C "
+                       f"oracle, cell grid rebuilt every step), {el:.1f} s")
+
+
+# ---------------------------------------------------------------------------
 # C4: 512 independent plates with Weibull strengths, one launch per step
 # ---------------------------------------------------------------------------
 def bench_ensemble(args):
@@ -873,10 +987,13 @@ def main(argv=None):
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--workload", choices=("all", "plate", "bed", "ensemble", "serve"),
+    ap.add_argument("--disk-steps", type=int, default=2000)
+    ap.add_argument("--disk-warmup", type=int, default=200)
+    ap.add_argument("--workload", choices=("all", "plate", "bed", "ensemble", "serve", "disk"),
                     default="all",
                     help="plate: the 1024x128 plate only; bed: the 1M-disk column only; "
-                         "ensemble: the 512-plate batch only; serve: the request loop only")
+                         "ensemble: the 512-plate batch only; serve: the request loop only; "
+                         "disk: the 1M-disk granular2d box only")
     ap.add_argument("--serve-window", type=float, default=60.0)
     ap.add_argument("--serve-chunk", type=int, default=250)
     ap.add_argument("--serve-slots", type=int, default=64)
@@ -927,16 +1044,17 @@ def main(argv=None):
 
 
 def run_workloads(args):
-    if args.workload in ("bed", "ensemble", "serve"):
-        bed = {"bed": bench_bed, "ensemble": bench_ensemble, "serve": bench_serve}[
-            args.workload](args)
+    if args.workload in ("bed", "ensemble", "serve", "disk"):
+        bed = {"bed": bench_bed, "ensemble": bench_ensemble, "serve": bench_serve,
+               "disk": bench_disk}[args.workload](args)
         line = {"metric": METRIC, "value": bed.pop("value"), "unit": "particle-steps/s",
                 "n_gpus": 1, "steps": bed["steps"], "warmup": bed.pop("warmup", 0),
                 "ms_per_step": bed["ms_per_step"], "higher_is_better": True, "scaling": "weak",
                 "vs_baseline": None, "dtype": bed.pop("dtype"),
                 "data": "synthetic: " + {"bed": "seeded loose column packing",
                                          "ensemble": "seeded parameter sweep of plates",
-                                         "serve": "seeded Poisson request stream"}[args.workload],
+                                         "serve": "seeded Poisson request stream",
+                                         "disk": "seeded jittered-grid disk box"}[args.workload],
                 "config": {"workload": bed.pop("workload")}}
         line.update(bed)
         print(json.dumps(line))
@@ -959,6 +1077,7 @@ def run_workloads(args):
     line["c2_plate"] = c2
     line["c4_ensemble"] = bench_ensemble(args)
     line["c5_serve"] = bench_serve(args)
+    line["disk_system"] = bench_disk(args)
     print(json.dumps(line))
     return 0
 

@@ -309,6 +309,7 @@ int tdm_bed_graph_create(const tdm_bed_arrays *a, int pairs, int fp32, void *str
 #define TDM_DISK_CONTACTS 10      /* pair, wall, hull contacts of the evaluation */
 #define TDM_DISK_STATUS 13        /* TDM_ST_* bits */
 #define TDM_DISK_REBUILDS 14      /* candidate-list rebuilds so far */
+#define TDM_DISK_CANDIDATES 15    /* candidate-list entries of the evaluation */
 #define TDM_DISK_NOUT 16
 /* per-contact record fields of tdm_disk_arrays.records */
 #define TDM_DISK_NREC 14 /* i, j, kind, n(2), lever(2), f(2), fn, ft, overlap, point(2) */
@@ -341,6 +342,7 @@ typedef struct tdm_disk_arrays {
   double *records;             /* NULL, or (cand_slots+n_walls+n_hull, N, TDM_DISK_NREC) */
   void *workspace;
   size_t workspace_bytes;
+  int64_t launches, steps;     /* kept by the library: evaluations and steps issued so far */
 } tdm_disk_arrays;
 
 /* sizeof(tdm_disk_params) and sizeof(tdm_disk_arrays) as compiled. */
@@ -352,9 +354,9 @@ size_t tdm_disk_workspace_bytes(int n, int ncells, int cell_cap, int cand_slots,
 /* Clears lists, cell tables and contact state, schedules the first
  * candidate search; replaces DiskSystem.__init__'s neighbour and history
  * setup (granular2d/engine.py:197-212, 674-677). */
-int tdm_disk_setup(const tdm_disk_arrays *a, const tdm_disk_params *p, void *stream);
+int tdm_disk_setup(tdm_disk_arrays *a, const tdm_disk_params *p, void *stream);
 /* Schedules a candidate search (after the caller moved disks). */
-int tdm_disk_rebin(const tdm_disk_arrays *a, const tdm_disk_params *p, void *stream);
+int tdm_disk_rebin(tdm_disk_arrays *a, const tdm_disk_params *p, void *stream);
 /* nsteps steps; state alternates x/v/omega <-> x_alt/v_alt/omega_alt from
  * parity 0.  flags: TDM_DISK_LEDGER accumulates the energy ledger,
  * TDM_DISK_RECORD writes per-contact records on the last step.  Replaces
@@ -362,17 +364,20 @@ int tdm_disk_rebin(const tdm_disk_arrays *a, const tdm_disk_params *p, void *str
  * (granular2d/engine.py:399-596). */
 #define TDM_DISK_LEDGER 1
 #define TDM_DISK_RECORD 2
-int tdm_disk_step(const tdm_disk_arrays *a, const tdm_disk_params *p, int nsteps, int flags,
+int tdm_disk_step(tdm_disk_arrays *a, const tdm_disk_params *p, int nsteps, int flags,
                   void *stream);
 /* Evaluation at the parity-0 state without advancing histories or state
  * (records when a->records is set); replaces DiskSystem.contact_forces
  * (granular2d/engine.py:631-634). */
-int tdm_disk_inspect(const tdm_disk_arrays *a, const tdm_disk_params *p, void *stream);
+int tdm_disk_inspect(tdm_disk_arrays *a, const tdm_disk_params *p, void *stream);
 /* As tdm_disk_step (nsteps even, no flags) with HIP events on each step's
  * two kernels; mean durations in microseconds.  Measurement helper. */
-int tdm_disk_step_timed(const tdm_disk_arrays *a, const tdm_disk_params *p, int nsteps,
+int tdm_disk_step_timed(tdm_disk_arrays *a, const tdm_disk_params *p, int nsteps,
                         void *stream, double *mean_step_us, double *mean_prep_us);
-/* 2*pairs steps captured as a graph (state ends in x/v/omega). */
+/* 2*pairs steps captured as a graph (state ends in x/v/omega).  The graph
+ * bakes in the parities of a->launches and a->steps at capture: replay it
+ * only when both have those parities again, and add 2*pairs to both per
+ * replay. */
 int tdm_disk_graph_create(const tdm_disk_arrays *a, const tdm_disk_params *p, int pairs,
                           int flags, void *stream, tdm_graph **out);
 

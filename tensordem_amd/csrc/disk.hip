@@ -4,18 +4,24 @@
 // Two kernels per step, one thread per disk:
 //
 //   disk_prep_kernel   counts each disk's wall and hull contacts at the
-//                      current positions; the last block to finish turns the
-//                      per-block counts into exclusive offsets (the rank of a
-//                      boundary contact among all boundary contacts, which is
-//                      how the reference keys their spring state,
-//                      engine.py:449-453).  When a candidate search is due, it
-//                      also drops every disk into the cell table.
+//                      current positions, per block; when a candidate search
+//                      is due it also drops every disk into the cell table.
+//                      Its first block reduces the previous step kernel's
+//                      per-block ledger partials in block order.
 //   disk_step_kernel   (search step: builds the disk's sorted candidate list
 //                      from the 3x3 cells, carrying each surviving partner's
 //                      spring over), evaluates every contact of the disk in
 //                      the reference's order, integrates, and writes the new
-//                      spring state; the last block reduces the per-block
-//                      ledger partials in a fixed order.
+//                      spring state.  A boundary contact's spring is stored at
+//                      its rank among all boundary contacts (how the reference
+//                      keys them, engine.py:449-453): each block adds up the
+//                      counts of the blocks before it.
+//
+// No kernel waits on another block: per-launch bookkeeping is indexed by the
+// launch parity, which the host passes in, so nothing needs a device-wide
+// fence (on a chip of several L2 caches an agent-scope release per block
+// costs more than the step).  A flush kernel reduces the last step's partials
+// at the end of each call.
 //
 // Every pair is evaluated by both of its disks from the same operands in the
 // same order (i = lower index), so both copies of its spring state stay
@@ -29,27 +35,26 @@
 namespace tdm {
 namespace {
 
-constexpr int kDiskCap = 24;  // candidate slots per disk held in registers
-constexpr int kDch = 13;      // ledger partial channels
+constexpr int kDiskCap = 24;  // candidate slots per disk (bits of the contact mask)
+constexpr int kDch = 14;      // ledger partial channels
+constexpr int kBatch = 4;     // candidates fetched together
+constexpr int kPrepBlocks = 512;  // grid of the prep kernel (grid-stride binning)
 enum {
   D_DASH, D_SLIDE, D_ROLL, D_OV2, D_XI2, D_KROLL2, D_WFX, D_WFY, D_HFX, D_HFY, D_NPAIR,
-  D_NWALL, D_NHULL
+  D_NWALL, D_NHULL, D_NCAND
 };
 
-// device-side step state (in the workspace)
+// device-side step state (in the workspace); [k] = launch or step parity k
 struct DiskDyn {
-  int launch;    // step-kernel launches so far (parity of the rebuild flags)
-  int adv;       // steps advanced (parity of the boundary spring state)
-  int lp;        // list buffer holding the current candidate lists
-  int cellp;     // cell table the next search bins into
+  int lpar[2];   // list buffer holding the candidate lists at that launch parity
+  int cpar[2];   // cell table a search at that launch parity bins into
+  int flag[2];   // candidate search due at the launch of that parity
+  int nst[2];    // boundary contacts stored in wst[k] (step parity)
   int rebuilds;
   int status;
-  int nst;       // boundary contacts in the stored spring state
-  int nw, nh;    // wall / hull contacts at the current positions
-  int pad0;
-  unsigned done1, done3;  // finished blocks of the current prep / step launch
-  int flag[2];   // candidate search due at the launch of that parity
-  int pad1[2];
+  int pend;      // partials of the last step kernel: 0 none, else 1 | 2 (stepped) | 4 (ledger)
+  int tot[2];    // wall and hull contacts at the current positions
+  int pad[3];
 };
 
 struct DiskWs {
@@ -57,15 +62,17 @@ struct DiskWs {
   int *cand;          // [2][S][N]
   double2 *hxi;       // [2][S][N] spring of each candidate pair
   double *hroll;      // [2][S][N] rolling angle of each candidate pair
-  int *cnt;           // [2][N] candidates per disk
+  int *cnt;           // [2][N] candidates per disk | (those below the disk's index) << 8
+  unsigned *cmask;    // [2][N] bit e: slot e was in contact at the last step (spring stored)
   double2 *xb;        // [N] position at the last search
   int *ccount;        // [2][ncells]
   int *centry;        // [2][ncells][cell_cap]
   double4 *wst;       // [2][wcap] boundary springs by rank: xi, roll
   int *blkcnt;        // [nblocks] packed wall | hull << 16 contacts per block
-  int2 *blkoff;       // [nblocks] exclusive wall, hull offsets
+  int2 *blkoff;       // [nblocks] exclusive wall, hull offsets of the blocks
   double *part;       // [nblocks][kDch]
   double *mr;         // [S + n_walls + n_hull][N] rolling torque terms
+  double *segbox;     // [n_walls + n_hull][4] bounding boxes of the segments
 };
 
 void carve(char *base, int n, int ncells, int cap, int S, int nw, int nh, DiskWs *w,
@@ -84,14 +91,16 @@ void carve(char *base, int n, int ncells, int cap, int S, int nw, int nh, DiskWs
   w->hxi = reinterpret_cast<double2 *>(take(sizeof(double2) * 2 * (size_t)S * n));
   w->hroll = reinterpret_cast<double *>(take(sizeof(double) * 2 * (size_t)S * n));
   w->cnt = reinterpret_cast<int *>(take(sizeof(int) * 2 * (size_t)n));
+  w->cmask = reinterpret_cast<unsigned *>(take(sizeof(unsigned) * 2 * (size_t)n));
   w->xb = reinterpret_cast<double2 *>(take(sizeof(double2) * (size_t)n));
   w->ccount = reinterpret_cast<int *>(take(sizeof(int) * 2 * (size_t)ncells));
   w->centry = reinterpret_cast<int *>(take(sizeof(int) * 2 * (size_t)ncells * cap));
   w->wst = reinterpret_cast<double4 *>(take(sizeof(double4) * 2 * wcap));
   w->blkcnt = reinterpret_cast<int *>(take(sizeof(int) * (size_t)nb));
   w->blkoff = reinterpret_cast<int2 *>(take(sizeof(int2) * (size_t)nb));
-  w->part = reinterpret_cast<double *>(take(sizeof(double) * kDch * (size_t)nb));
+  w->part = reinterpret_cast<double *>(take(sizeof(double) * kDch * (size_t)nb));  // [kDch][nb]
   w->mr = reinterpret_cast<double *>(take(sizeof(double) * (size_t)(S + nw + nh) * n));
+  w->segbox = reinterpret_cast<double *>(take(sizeof(double) * 4 * (size_t)(nw + nh + 1)));
   *total = off;
 }
 
@@ -105,7 +114,9 @@ struct DiskArgs {
   const double *walls, *hull, *wvel;
   double *out, *rec;
   DiskWs w;
-  int ledger;
+  int par;    // launch parity
+  int spar;   // step parity (boundary spring buffers)
+  int mode;   // 1 evaluation, | 2 stepped, | 4 ledger
 };
 
 __device__ inline double min_image(double d, double L) { return L > 0.0 ? d - L * rint(d / L) : d; }
@@ -119,6 +130,15 @@ __device__ inline double max0_second(double x) { return (x >= 0.0 || x != x) ? x
 struct SegHit {
   double n0, n1, ov, cp0, cp1;
 };
+
+// Quick reject: a disk farther than r (with a 1e-9 relative margin, far above
+// the rounding of the exact test) from the segment's bounding box `box`
+// (min x, min y, max x, max y) cannot touch it.
+__device__ inline bool seg_near(const double *box, double px, double py, double r)
+{
+  const double m = r * (1.0 + 1e-9);
+  return !(box[0] - px > m || px - box[2] > m || box[1] - py > m || py - box[3] > m);
+}
 
 __device__ inline bool seg_hit(const double *g, double px, double py, double r, SegHit &h)
 {
@@ -141,14 +161,35 @@ __device__ inline bool seg_hit(const double *g, double px, double py, double r, 
   return true;
 }
 
+__device__ inline const double *segment(const tdm_disk_params &P, const DiskArgs &A, int s)
+{
+  return s < P.n_walls ? A.walls + 4 * s : A.hull + 4 * (s - P.n_walls);
+}
+
+// Segments the disk touches: bit s of the mask (first 64 segments) and the
+// wall | hull << 16 counts.
+__device__ inline int boundary_hits(const tdm_disk_params &P, const DiskArgs &A, double px,
+                                    double py, double r, unsigned long long &mask)
+{
+  int cw = 0, chh = 0;
+  mask = 0ull;
+  SegHit h;
+  const int nseg = P.n_walls + P.n_hull;
+  for (int s = 0; s < nseg; ++s) {
+    if (!seg_near(A.w.segbox + 4 * s, px, py, r)) continue;
+    if (!seg_hit(segment(P, A, s), px, py, r, h)) continue;
+    if (s < 64) mask |= 1ull << s;
+    if (s < P.n_walls) ++cw;
+    else ++chh;
+  }
+  return cw | (chh << 16);
+}
+
 __device__ inline int boundary_counts(const tdm_disk_params &P, const DiskArgs &A, double px,
                                       double py, double r)
 {
-  int cw = 0, chh = 0;
-  SegHit h;
-  for (int s = 0; s < P.n_walls; ++s) cw += seg_hit(A.walls + 4 * s, px, py, r, h);
-  for (int s = 0; s < P.n_hull; ++s) chh += seg_hit(A.hull + 4 * s, px, py, r, h);
-  return cw | (chh << 16);
+  unsigned long long mask;
+  return boundary_hits(P, A, px, py, r, mask);
 }
 
 __device__ inline int cell_coord(const tdm_disk_params &P, int a, double x)
@@ -164,6 +205,18 @@ __device__ inline int cell_coord(const tdm_disk_params &P, int a, double x)
   double f = floor((x - P.grid_origin[a]) / P.cell_h[a]);
   if (!(f >= 0.0)) return 0;
   return f >= (double)(nc - 1) ? nc - 1 : (int)f;
+}
+
+// Disks [256 lb, 256 lb + 256) of logical block lb.  Hardware blocks go to
+// the 8 XCDs round robin; this mapping gives each XCD a contiguous run of
+// logical blocks, so the partners of a disk (neighbouring indices and, in a
+// spatially ordered assembly, the next rows) are mostly read through the
+// same L2.
+__device__ inline int logical_block()
+{
+  const int nb = gridDim.x;
+  if (nb & 7) return blockIdx.x;
+  return (blockIdx.x & 7) * (nb >> 3) + (blockIdx.x >> 3);
 }
 
 // Neighbouring cells along one axis: wrapped and de-duplicated on a periodic
@@ -184,18 +237,6 @@ __device__ inline int axis_cells(const tdm_disk_params &P, int a, int c, int (&o
     if (!dup) out[k++] = q;
   }
   return k;
-}
-
-__device__ inline double atomic_load_f64(const double *p)
-{
-  unsigned long long u = __hip_atomic_load(reinterpret_cast<const unsigned long long *>(p),
-                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  return __longlong_as_double((long long)u);
-}
-
-__device__ inline int atomic_load_i32(const int *p)
-{
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // One contact of the law (engine.py:455-534).  (n0, n1) points from j to
@@ -266,51 +307,60 @@ __device__ inline void clear_record(double *rec, size_t at)
   rec[at * TDM_DISK_NREC + 2] = -1.0;
 }
 
-__global__ __launch_bounds__(kThreads) void disk_prep_kernel(tdm_disk_params P, DiskArgs A)
+// Reduces the pending per-block partials of the last step kernel into
+// `out` in block order (one block; deterministic).
+__device__ void reduce_pending(const tdm_disk_params &P, const DiskArgs &A, double (*red)[kCh])
 {
-  __shared__ int bsum;
-  __shared__ int last;
   DiskDyn *dyn = A.w.dyn;
-  const int p = blockIdx.x * kThreads + threadIdx.x;
-  const int launch = dyn->launch;
-  const bool rebuild = dyn->flag[launch & 1] != 0;
-  const int cellp = dyn->cellp;
-  if (threadIdx.x == 0) bsum = 0;
-  __syncthreads();
-  if (p < P.n) {
-    const double2 xp = A.x[p];
-    const int packed = boundary_counts(P, A, xp.x, xp.y, A.radius[p]);
-    if (packed) atomicAdd(&bsum, packed);
-    if (rebuild) {
-      const int ncells = P.ncell[0] * P.ncell[1];
-      const int c = cell_coord(P, 0, xp.x) + P.ncell[0] * cell_coord(P, 1, xp.y);
-      const size_t cc = (size_t)cellp * ncells + c;
-      const int slot = atomicAdd(&A.w.ccount[cc], 1);
-      if (slot < P.cell_cap) A.w.centry[cc * P.cell_cap + slot] = p;
-      else atomicOr(&dyn->status, TDM_ST_CELL_OVERFLOW);
-    }
-  }
-  __syncthreads();
+  const int pend = dyn->pend;
+  if (!pend) return;
+  double acc[kDch];
+#pragma unroll
+  for (int k = 0; k < kDch; ++k) acc[k] = 0.0;
+  // channel-major partials: consecutive threads read consecutive blocks
+#pragma unroll 4
+  for (int b = threadIdx.x; b < P.nblocks; b += kThreads)
+#pragma unroll
+    for (int k = 0; k < kDch; ++k) acc[k] += A.w.part[(size_t)k * P.nblocks + b];
+  block_sum<kDch>(acc, red);
   if (threadIdx.x == 0) {
-    A.w.blkcnt[blockIdx.x] = bsum;
-    __threadfence();
-    last = atomicAdd(&dyn->done1, 1u) == gridDim.x - 1;
+    double *out = A.out;
+    if (pend & 4) {
+      out[TDM_DISK_NORMAL_DASHPOT] = out[TDM_DISK_NORMAL_DASHPOT] + acc[D_DASH] * P.dt;
+      out[TDM_DISK_FRICTION] = out[TDM_DISK_FRICTION] - acc[D_SLIDE];
+      out[TDM_DISK_ROLLING] = out[TDM_DISK_ROLLING] - acc[D_ROLL];
+    }
+    out[TDM_DISK_ELASTIC] = 0.5 * P.kn * acc[D_OV2];
+    out[TDM_DISK_ELASTIC + 1] = 0.5 * P.kt * acc[D_XI2];
+    out[TDM_DISK_ELASTIC + 2] = P.mu_rolling > 0.0 ? 0.5 * acc[D_KROLL2] : 0.0;
+    out[TDM_DISK_WALL_FORCE] = -acc[D_WFX];
+    out[TDM_DISK_WALL_FORCE + 1] = -acc[D_WFY];
+    out[TDM_DISK_HULL_FORCE] = -acc[D_HFX];
+    out[TDM_DISK_HULL_FORCE + 1] = -acc[D_HFY];
+    out[TDM_DISK_CONTACTS] = acc[D_NPAIR];
+    out[TDM_DISK_CONTACTS + 1] = acc[D_NWALL];
+    out[TDM_DISK_CONTACTS + 2] = acc[D_NHULL];
+    out[TDM_DISK_STATUS] = (double)dyn->status;
+    out[TDM_DISK_REBUILDS] = (double)dyn->rebuilds;
+    out[TDM_DISK_CANDIDATES] = acc[D_NCAND];
+    dyn->pend = 0;
   }
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
-  // the last block: exclusive offsets of the wall and hull counts per block
-  const int nb = gridDim.x;
+}
+
+// Exclusive offsets of the per-block wall and hull contact counts (one block).
+__device__ void scan_blocks(const tdm_disk_params &P, const DiskArgs &A)
+{
+  __shared__ int sw_s[kThreads], sh_s[kThreads];
+  const int nb = P.nblocks;
   const int per = (nb + kThreads - 1) / kThreads;
-  const int b0 = threadIdx.x * per, b1 = min(nb, b0 + per);
+  const int b0 = min(nb, (int)threadIdx.x * per), b1 = min(nb, b0 + per);
   int sw = 0, sh = 0;
+#pragma unroll 8
   for (int b = b0; b < b1; ++b) {
-    const int v = atomic_load_i32(&A.w.blkcnt[b]);
+    const int v = A.w.blkcnt[b];
     sw += v & 0xFFFF;
     sh += v >> 16;
   }
-  // inclusive scan of (sw, sh) over the threads, 2 x 256 ints in LDS
-  __shared__ int sw_s[kThreads], sh_s[kThreads];
   sw_s[threadIdx.x] = sw;
   sh_s[threadIdx.x] = sh;
   __syncthreads();
@@ -324,38 +374,92 @@ __global__ __launch_bounds__(kThreads) void disk_prep_kernel(tdm_disk_params P, 
   }
   int ow = sw_s[threadIdx.x] - sw, oh = sh_s[threadIdx.x] - sh;
   for (int b = b0; b < b1; ++b) {
-    const int v = atomic_load_i32(&A.w.blkcnt[b]);
+    const int v = A.w.blkcnt[b];
     A.w.blkoff[b] = make_int2(ow, oh);
     ow += v & 0xFFFF;
     oh += v >> 16;
   }
   if (threadIdx.x == kThreads - 1) {
-    dyn->nw = sw_s[kThreads - 1];
-    dyn->nh = sh_s[kThreads - 1];
-    dyn->done1 = 0u;
+    A.w.dyn->tot[0] = sw_s[kThreads - 1];
+    A.w.dyn->tot[1] = sh_s[kThreads - 1];
   }
 }
 
-template <bool kAdvance>
+// Per-block wall and hull contact counts of the disks at A.x (start of a
+// call; later steps get them from the step kernel, after integrating).
+__global__ __launch_bounds__(kThreads) void disk_count_kernel(tdm_disk_params P, DiskArgs A)
+{
+  __shared__ int bsum;
+  const int lb = logical_block();
+  const int p = lb * kThreads + threadIdx.x;
+  if (threadIdx.x == 0) bsum = 0;
+  __syncthreads();
+  if (p < P.n) {
+    const double2 xp = A.x[p];
+    const int packed = boundary_counts(P, A, xp.x, xp.y, A.radius[p]);
+    if (packed) atomicAdd(&bsum, packed);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) A.w.blkcnt[lb] = bsum;
+}
+
+__global__ __launch_bounds__(kThreads) void disk_prep_kernel(tdm_disk_params P, DiskArgs A)
+{
+  __shared__ double red[4][kCh];
+  DiskDyn *dyn = A.w.dyn;
+  const int p = blockIdx.x * kThreads + threadIdx.x;
+  const bool rebuild = dyn->flag[A.par] != 0;
+  const int cellp = dyn->cpar[A.par];
+  if (blockIdx.x == 0) {
+    reduce_pending(P, A, red);
+    if (P.n_walls + P.n_hull) {
+      __syncthreads();
+      scan_blocks(P, A);
+    }
+    // the step kernel of this launch raises the flag for the next one
+    if (threadIdx.x == 0) dyn->flag[A.par ^ 1] = 0;
+  }
+  if (!rebuild) return;
+  // a small grid: most launches only do block 0's work
+  for (int q = p; q < P.n; q += gridDim.x * kThreads) {
+    const double2 xq = A.x[q];
+    const int ncells = P.ncell[0] * P.ncell[1];
+    const int c = cell_coord(P, 0, xq.x) + P.ncell[0] * cell_coord(P, 1, xq.y);
+    const size_t cc = (size_t)cellp * ncells + c;
+    const int slot = atomicAdd(&A.w.ccount[cc], 1);
+    if (slot < P.cell_cap) A.w.centry[cc * P.cell_cap + slot] = q;
+    else atomicOr(&dyn->status, TDM_ST_CELL_OVERFLOW);
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void disk_flush_kernel(tdm_disk_params P, DiskArgs A)
+{
+  __shared__ double red[4][kCh];
+  reduce_pending(P, A, red);
+}
+
+// kSums: accumulate the ledger, energy, reaction and count partials (steps
+// whose sums nobody reads skip them)
+template <bool kAdvance, bool kSums>
 __global__ __launch_bounds__(kThreads) void disk_step_kernel(tdm_disk_params P, DiskArgs A)
 {
   __shared__ double red[4][kCh];
   __shared__ int scan[kThreads];
-  __shared__ int last;
+  __shared__ int bsum;
   DiskDyn *dyn = A.w.dyn;
   const DiskWs &W = A.w;
   const int N = P.n, S = P.cand_slots;
-  const int p = blockIdx.x * kThreads + threadIdx.x;
+  const int lb = logical_block();
+  const int p = lb * kThreads + threadIdx.x;
   const bool live = p < N;
-  const int launch = dyn->launch;
-  const int par = launch & 1;
+  const int par = A.par;
   const bool rebuild = dyn->flag[par] != 0;
-  const int lp = dyn->lp;
+  const int lp = dyn->lpar[par];
   const int lq = rebuild ? lp ^ 1 : lp;
-  const int cellp = dyn->cellp;
-  const int adv = dyn->adv;
-  const int nst = dyn->nst, nw_tot = dyn->nw;
+  const int cellp = dyn->cpar[par];
+  const int nst = dyn->nst[A.spar];
   const int ncells = P.ncell[0] * P.ncell[1];
+  const bool bounded = P.n_walls + P.n_hull > 0;
   double ch[kDch];
 #pragma unroll
   for (int k = 0; k < kDch; ++k) ch[k] = 0.0;
@@ -365,35 +469,50 @@ __global__ __launch_bounds__(kThreads) void disk_step_kernel(tdm_disk_params P, 
     for (int c = blockIdx.x * kThreads + threadIdx.x; c < ncells; c += gridDim.x * kThreads)
       cc[c] = 0;
   }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    // bookkeeping read only by the next launch (the other parity)
+    dyn->lpar[par ^ 1] = lq;
+    dyn->cpar[par ^ 1] = rebuild ? cellp ^ 1 : cellp;
+    if (rebuild) dyn->rebuilds += 1;
+    if (kSums) dyn->pend = A.mode;
+  }
 
   double2 xp = make_double2(0.0, 0.0), vp = xp;
   double omp = 0.0, rp = 0.0, mp = 1.0;
   int packed = 0;
+  unsigned long long smask = 0ull;
   if (live) {
     xp = A.x[p];
     vp = A.v[p];
     omp = A.om[p];
     rp = A.radius[p];
     mp = A.mass[p];
-    packed = boundary_counts(P, A, xp.x, xp.y, rp);
+    if (bounded) packed = boundary_hits(P, A, xp.x, xp.y, rp, smask);
   }
-  // exclusive prefix of the packed boundary counts inside the block
-  scan[threadIdx.x] = packed;
-  __syncthreads();
-  for (int d = 1; d < kThreads; d <<= 1) {
-    const int a = threadIdx.x >= d ? scan[threadIdx.x - d] : 0;
+  // boundary-contact ranks: the block's offsets, then the prefix inside it
+  // (blocks without boundary contacts skip the scan)
+  int rank_w0 = 0, rank_h0 = 0;
+  if (bounded && __syncthreads_or(packed != 0)) {
+    scan[threadIdx.x] = packed;
     __syncthreads();
-    scan[threadIdx.x] += a;
-    __syncthreads();
+    for (int d = 1; d < kThreads; d <<= 1) {
+      const int a = threadIdx.x >= d ? scan[threadIdx.x - d] : 0;
+      __syncthreads();
+      scan[threadIdx.x] += a;
+      __syncthreads();
+    }
+    const int excl = scan[threadIdx.x] - packed;
+    const int2 boff = W.blkoff[lb];
+    rank_w0 = boff.x + (excl & 0xFFFF);
+    rank_h0 = dyn->tot[0] + boff.y + (excl >> 16);
   }
-  const int excl = scan[threadIdx.x] - packed;
-  const int2 boff = W.blkoff[blockIdx.x];
-  const int rank_w0 = boff.x + (excl & 0xFFFF);
-  const int rank_h0 = nw_tot + boff.y + (excl >> 16);
+  if (kAdvance && blockIdx.x == 0 && threadIdx.x == 0)
+    dyn->nst[A.spar ^ 1] = bounded ? dyn->tot[0] + dyn->tot[1] : 0;
 
   // ---- candidate search (engine.py:298-357 gives the pair set) -------------
   if (live && rebuild) {
-    int nl[kDiskCap];
+    // the list is kept sorted in place in the disk's own slots (cached)
+    int *cl = W.cand + (size_t)lq * S * N + p;
     int k = 0;
     const double reach2 = P.reach * P.reach;
     const int cx = cell_coord(P, 0, xp.x), cy = cell_coord(P, 1, xp.y);
@@ -415,69 +534,108 @@ __global__ __launch_bounds__(kThreads) void disk_step_kernel(tdm_disk_params P, 
             continue;
           }
           int e = k++;
-          for (; e > 0 && nl[e - 1] > q; --e) nl[e] = nl[e - 1];
-          nl[e] = q;
+          for (; e > 0; --e) {
+            const int prev = cl[(size_t)(e - 1) * N];
+            if (prev < q) break;
+            cl[(size_t)e * N] = prev;
+          }
+          cl[(size_t)e * N] = q;
         }
       }
-    // carry the springs of partners already listed (both lists ascending)
-    const int ocnt = W.cnt[(size_t)lp * N + p];
+    // carry the springs of partners in contact at the last step (both
+    // lists ascending); other slots need no spring
+    const int ocnt = W.cnt[(size_t)lp * N + p] & 0xFF;
+    const unsigned omask = W.cmask[(size_t)lp * N + p];
+    unsigned nmask = 0u;
+    int below = 0;
     int o = 0;
     int oq = ocnt > 0 ? W.cand[((size_t)lp * S) * N + p] : INT_MAX;
     for (int e = 0; e < k; ++e) {
-      const int q = nl[e];
+      const int q = cl[(size_t)e * N];
+      below += q < p;
       while (oq < q) {
         ++o;
         oq = o < ocnt ? W.cand[((size_t)lp * S + o) * N + p] : INT_MAX;
       }
-      double2 h = make_double2(0.0, 0.0);
-      double hr = 0.0;
-      if (oq == q) {
-        h = W.hxi[((size_t)lp * S + o) * N + p];
-        hr = W.hroll[((size_t)lp * S + o) * N + p];
-      }
       const size_t at = ((size_t)lq * S + e) * N + p;
-      W.cand[at] = q;
-      W.hxi[at] = h;
-      W.hroll[at] = hr;
+      if (oq == q && (omask >> o & 1u)) {
+        W.hxi[at] = W.hxi[((size_t)lp * S + o) * N + p];
+        W.hroll[at] = W.hroll[((size_t)lp * S + o) * N + p];
+        nmask |= 1u << e;
+      }
     }
-    W.cnt[(size_t)lq * N + p] = k;
+    W.cnt[(size_t)lq * N + p] = k | (below << 8);
+    W.cmask[(size_t)lq * N + p] = nmask;
     W.xb[p] = xp;
   }
 
   double F0 = 0.0, F1 = 0.0, T = 0.0;
+  int packed_new = 0;
   unsigned touch = 0u;  // candidate slots in contact
   int nbc = 0;          // boundary contacts of this disk
   const bool rolling = P.mu_rolling > 0.0;
   if (live) {
-    const int cnt = W.cnt[(size_t)lq * N + p];
-    int e0 = 0;
-    while (e0 < cnt && W.cand[((size_t)lq * S + e0) * N + p] < p) ++e0;
-    // pairs where p is i (partners above p), then where p is j (below)
-    for (int pass = 0; pass < 2; ++pass) {
-      const int ea = pass ? 0 : e0, eb = pass ? e0 : cnt;
-      const bool owner = pass == 0;
-      for (int e = ea; e < eb; ++e) {
+    const int cw = W.cnt[(size_t)lq * N + p];
+    const int cnt = cw & 0xFF, e0 = cw >> 8;
+    const unsigned mk = W.cmask[(size_t)lq * N + p];
+    if (kSums) ch[D_NCAND] = (double)cnt;
+    const int *cl = W.cand + (size_t)lq * S * N + p;
+    // Phase 1, every candidate: does it overlap?  The test uses the same
+    // operands as the evaluation (|x_q - x_p| is the canonical |x_j - x_i|
+    // and r_p + r_q the canonical r_i + r_j), with a squared-distance
+    // pre-test whose 1e-12 relative margin lies far above the rounding of the
+    // exact test.  kBatch candidates are fetched together.  Bit j of `hit` is
+    // candidate j in evaluation order: partners above p (slots e0..cnt-1,
+    // where p is i), then below (slots 0..e0-1, where p is j).
+    const int nup = cnt - e0;
+    unsigned hit = 0u;
+    for (int j0 = 0; j0 < cnt; j0 += kBatch) {
+      int qb[kBatch];
+      double2 xb_[kBatch];
+      double rb[kBatch];
+#pragma unroll
+      for (int u = 0; u < kBatch; ++u) {
+        const int j = j0 + u;
+        const int e = j < nup ? e0 + j : j - nup;
+        qb[u] = j < cnt ? cl[(size_t)e * N] : p;
+      }
+#pragma unroll
+      for (int u = 0; u < kBatch; ++u) {
+        xb_[u] = A.x[qb[u]];
+        rb[u] = A.radius[qb[u]];
+      }
+#pragma unroll
+      for (int u = 0; u < kBatch; ++u) {
+        const int j = j0 + u;
+        const double l0 = min_image(xb_[u].x - xp.x, P.period[0]);
+        const double l1 = min_image(xb_[u].y - xp.y, P.period[1]);
+        const double l2 = l0 * l0 + l1 * l1;
+        const double rs = rp + rb[u];
+        if (j < cnt && !(l2 > rs * rs * (1.0 + 1e-12)) && rs - sqrt(l2) > 0.0)
+          hit |= 1u << j;
+      }
+    }
+    // Phase 2, contacts only, in evaluation order (a wave runs the law as
+    // often as its busiest disk has contacts, not once per candidate)
+    while (hit) {
+      const int j = __builtin_ctz(hit);
+      hit &= hit - 1u;
+      {
+        const int e = j < nup ? e0 + j : j - nup;
+        const bool owner = j < nup;
+        touch |= 1u << e;
         const size_t at = ((size_t)lq * S + e) * N + p;
-        const int q = W.cand[at];
+        const int q = cl[(size_t)e * N];
         const double2 xq = A.x[q];
+        const double rq = A.radius[q];
         const double2 xi_ = owner ? xp : xq, xj_ = owner ? xq : xp;
         const double l0 = min_image(xj_.x - xi_.x, P.period[0]);
         const double l1 = min_image(xj_.y - xi_.y, P.period[1]);
+        const double ri = owner ? rp : rq, rj = owner ? rq : rp;
         const double dist = sqrt(l0 * l0 + l1 * l1);
+        const double ov = (ri + rj) - dist;
         const double dm = fmax(dist, 1e-30);
         const double nij0 = l0 / dm, nij1 = l1 / dm;
-        const double rq = A.radius[q];
-        const double ri = owner ? rp : rq, rj = owner ? rq : rp;
-        const double ov = (ri + rj) - dist;
-        if (!(ov > 0.0)) {
-          if (kAdvance) {
-            W.hxi[at] = make_double2(0.0, 0.0);
-            W.hroll[at] = 0.0;
-          }
-          if (A.rec && owner) clear_record(A.rec, (size_t)e * N + p);
-          continue;
-        }
-        touch |= 1u << e;
         const double mq = A.mass[q];
         const double mi = owner ? mp : mq, mj = owner ? mq : mp;
         const double2 vq = A.v[q];
@@ -493,8 +651,12 @@ __global__ __launch_bounds__(kThreads) void disk_step_kernel(tdm_disk_params P, 
         const double wj0 = omj * lj0, wj1 = omj * lj1;
         const double vi0 = vi.x + -wi1, vi1 = vi.y + wi0;
         const double vj0 = vj.x + -wj1, vj1 = vj.y + wj0;
-        double2 h = W.hxi[at];
-        double hr = W.hroll[at];
+        double2 h = make_double2(0.0, 0.0);
+        double hr = 0.0;
+        if (mk >> e & 1u) {
+          h = W.hxi[at];
+          hr = W.hroll[at];
+        }
         const LawOut o = contact_law(P, n0, n1, ov, meff, reff, vi0 - vj0, vi1 - vj1,
                                      omi - omj, h.x, h.y, hr);
         if (owner) {
@@ -502,14 +664,16 @@ __global__ __launch_bounds__(kThreads) void disk_step_kernel(tdm_disk_params P, 
           F1 += o.t1;
           T += li0 * o.t1 - li1 * o.t0;
           if (rolling) W.mr[(size_t)e * N + p] = -o.m_r;
-          const double vn = (vi0 - vj0) * n0 + (vi1 - vj1) * n1;
-          ch[D_DASH] += (o.fn - P.kn * ov) * vn;
-          ch[D_SLIDE] += o.slide;
-          ch[D_ROLL] += o.rloss;
-          ch[D_OV2] += ov * ov;
-          ch[D_XI2] += h.x * h.x + h.y * h.y;
-          ch[D_KROLL2] += o.kr * (hr * hr);
-          ch[D_NPAIR] += 1.0;
+          if (kSums) {
+            const double vn = (vi0 - vj0) * n0 + (vi1 - vj1) * n1;
+            ch[D_DASH] += (o.fn - P.kn * ov) * vn;
+            ch[D_SLIDE] += o.slide;
+            ch[D_ROLL] += o.rloss;
+            ch[D_OV2] += ov * ov;
+            ch[D_XI2] += h.x * h.x + h.y * h.y;
+            ch[D_KROLL2] += o.kr * (hr * hr);
+            ch[D_NPAIR] += 1.0;
+          }
           if (A.rec)
             put_record(A.rec, (size_t)e * N + p, (double)p, (double)q, 0.0, n0, n1, l0, l1, o,
                        ov, xi_.x + ri * nij0, xi_.y + ri * nij1);
@@ -525,24 +689,29 @@ __global__ __launch_bounds__(kThreads) void disk_step_kernel(tdm_disk_params P, 
         }
       }
     }
+    if (kAdvance) W.cmask[(size_t)lq * N + p] = touch;
     if (A.rec)
       for (int e = 0; e < S; ++e)
-        if (e < e0 || e >= cnt) clear_record(A.rec, (size_t)e * N + p);
+        if (e < e0 || e >= cnt || !(touch >> e & 1u)) clear_record(A.rec, (size_t)e * N + p);
 
     // walls, then hull (engine.py:420-453, 506-521)
-    const double4 *wold = W.wst + (size_t)(adv & 1) * ((size_t)N * (P.n_walls + P.n_hull) + 1);
-    double4 *wnew = W.wst + (size_t)((adv + 1) & 1) * ((size_t)N * (P.n_walls + P.n_hull) + 1);
+    const double4 *wold = W.wst + (size_t)A.spar * ((size_t)N * (P.n_walls + P.n_hull) + 1);
+    double4 *wnew = W.wst + (size_t)(A.spar ^ 1) * ((size_t)N * (P.n_walls + P.n_hull) + 1);
     int kw = 0, kh = 0;
     const int nseg = P.n_walls + P.n_hull;
     for (int s = 0; s < nseg; ++s) {
       const bool wall = s < P.n_walls;
-      const double *g = wall ? A.walls + 4 * s : A.hull + 4 * (s - P.n_walls);
+      const double *g = segment(P, A, s);
       SegHit hh;
       const size_t rat = (size_t)(S + s) * N + p;
-      if (!seg_hit(g, xp.x, xp.y, rp, hh)) {
+      const bool touching = s < 64 ? (smask >> s & 1ull) != 0ull
+                                   : seg_near(W.segbox + 4 * s, xp.x, xp.y, rp) &&
+                                         seg_hit(g, xp.x, xp.y, rp, hh);
+      if (!touching) {
         if (A.rec) clear_record(A.rec, rat);
         continue;
       }
+      if (s < 64) seg_hit(g, xp.x, xp.y, rp, hh);
       const int rank = wall ? rank_w0 + kw++ : rank_h0 + kh++;
       double4 st = make_double4(0.0, 0.0, 0.0, 0.0);
       if (rank < nst) st = wold[rank];
@@ -562,21 +731,23 @@ __global__ __launch_bounds__(kThreads) void disk_step_kernel(tdm_disk_params P, 
       T += li0 * o.t1 - li1 * o.t0;
       if (rolling) W.mr[(size_t)(S + nbc) * N + p] = -o.m_r;
       ++nbc;
-      const double vn = (vi0 - vj0) * n0 + (vi1 - vj1) * n1;
-      ch[D_DASH] += (o.fn - P.kn * hh.ov) * vn;
-      ch[D_SLIDE] += o.slide;
-      ch[D_ROLL] += o.rloss;
-      ch[D_OV2] += hh.ov * hh.ov;
-      ch[D_XI2] += st.x * st.x + st.y * st.y;
-      ch[D_KROLL2] += o.kr * (st.z * st.z);
-      if (wall) {
-        ch[D_WFX] += o.t0;
-        ch[D_WFY] += o.t1;
-        ch[D_NWALL] += 1.0;
-      } else {
-        ch[D_HFX] += o.t0;
-        ch[D_HFY] += o.t1;
-        ch[D_NHULL] += 1.0;
+      if (kSums) {
+        const double vn = (vi0 - vj0) * n0 + (vi1 - vj1) * n1;
+        ch[D_DASH] += (o.fn - P.kn * hh.ov) * vn;
+        ch[D_SLIDE] += o.slide;
+        ch[D_ROLL] += o.rloss;
+        ch[D_OV2] += hh.ov * hh.ov;
+        ch[D_XI2] += st.x * st.x + st.y * st.y;
+        ch[D_KROLL2] += o.kr * (st.z * st.z);
+        if (wall) {
+          ch[D_WFX] += o.t0;
+          ch[D_WFY] += o.t1;
+          ch[D_NWALL] += 1.0;
+        } else {
+          ch[D_HFX] += o.t0;
+          ch[D_HFY] += o.t1;
+          ch[D_NHULL] += 1.0;
+        }
       }
       if (kAdvance) wnew[rank] = st;
       if (A.rec)
@@ -633,60 +804,36 @@ __global__ __launch_bounds__(kThreads) void disk_step_kernel(tdm_disk_params P, 
       const double d1 = min_image(xs2[1] - xb.y, P.period[1]);
       const double half = 0.5 * P.skin;
       if (!(d0 * d0 + d1 * d1 <= half * half)) atomicOr(&dyn->flag[par ^ 1], 1);
+      if (bounded) packed_new = boundary_counts(P, A, xs2[0], xs2[1], rp);
     }
+  }
+  // boundary contacts at the new positions, for the next launch's ranks
+  if (kAdvance && bounded) {
+    if (threadIdx.x == 0) bsum = 0;
+    __syncthreads();
+    if (packed_new) atomicAdd(&bsum, packed_new);
+    __syncthreads();
+    if (threadIdx.x == 0) W.blkcnt[lb] = bsum;
   }
 
-  // ---- ledger partials; the last block reduces them in block order -------
-  block_sum<kDch>(ch, red);
-  if (threadIdx.x == 0) {
-    for (int k = 0; k < kDch; ++k) W.part[(size_t)blockIdx.x * kDch + k] = ch[k];
-    __threadfence();
-    last = atomicAdd(&dyn->done3, 1u) == gridDim.x - 1;
+  // ---- ledger partials, reduced by the next launch (or the flush) --------
+  if (kSums) {
+    block_sum<kDch>(ch, red);
+    if (threadIdx.x == 0)
+      for (int k = 0; k < kDch; ++k) W.part[(size_t)k * gridDim.x + lb] = ch[k];
   }
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
-  double acc[kDch];
-#pragma unroll
-  for (int k = 0; k < kDch; ++k) acc[k] = 0.0;
-  for (int b = threadIdx.x; b < (int)gridDim.x; b += kThreads)
-#pragma unroll
-    for (int k = 0; k < kDch; ++k) acc[k] += atomic_load_f64(&W.part[(size_t)b * kDch + k]);
-  __syncthreads();
-  block_sum<kDch>(acc, red);
-  if (threadIdx.x == 0) {
-    double *out = A.out;
-    if (kAdvance && A.ledger) {
-      out[TDM_DISK_NORMAL_DASHPOT] = out[TDM_DISK_NORMAL_DASHPOT] + acc[D_DASH] * P.dt;
-      out[TDM_DISK_FRICTION] = out[TDM_DISK_FRICTION] - acc[D_SLIDE];
-      out[TDM_DISK_ROLLING] = out[TDM_DISK_ROLLING] - acc[D_ROLL];
-    }
-    out[TDM_DISK_ELASTIC] = 0.5 * P.kn * acc[D_OV2];
-    out[TDM_DISK_ELASTIC + 1] = 0.5 * P.kt * acc[D_XI2];
-    out[TDM_DISK_ELASTIC + 2] = rolling ? 0.5 * acc[D_KROLL2] : 0.0;
-    out[TDM_DISK_WALL_FORCE] = -acc[D_WFX];
-    out[TDM_DISK_WALL_FORCE + 1] = -acc[D_WFY];
-    out[TDM_DISK_HULL_FORCE] = -acc[D_HFX];
-    out[TDM_DISK_HULL_FORCE + 1] = -acc[D_HFY];
-    out[TDM_DISK_CONTACTS] = acc[D_NPAIR];
-    out[TDM_DISK_CONTACTS + 1] = acc[D_NWALL];
-    out[TDM_DISK_CONTACTS + 2] = acc[D_NHULL];
-    if (rebuild) {
-      dyn->lp = lp ^ 1;
-      dyn->cellp = cellp ^ 1;
-      dyn->rebuilds += 1;
-    }
-    dyn->flag[par] = 0;
-    dyn->launch = launch + 1;
-    if (kAdvance) {
-      dyn->adv = adv + 1;
-      dyn->nst = dyn->nw + dyn->nh;
-    }
-    // other blocks OR into status during this launch: read it from L2
-    out[TDM_DISK_STATUS] = (double)atomic_load_i32(&dyn->status);
-    out[TDM_DISK_REBUILDS] = (double)dyn->rebuilds;
-    dyn->done3 = 0u;
-  }
+}
+
+__global__ void disk_segbox_kernel(tdm_disk_params P, DiskArgs A)
+{
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= P.n_walls + P.n_hull) return;
+  const double *g = segment(P, A, s);
+  double *b = A.w.segbox + 4 * s;
+  b[0] = fmin(g[0], g[2]);
+  b[1] = fmin(g[1], g[3]);
+  b[2] = fmax(g[0], g[2]);
+  b[3] = fmax(g[1], g[3]);
 }
 
 __global__ void disk_init_kernel(DiskDyn *dyn)
@@ -698,9 +845,9 @@ __global__ void disk_init_kernel(DiskDyn *dyn)
   }
 }
 
-__global__ void disk_mark_kernel(DiskDyn *dyn)
+__global__ void disk_mark_kernel(DiskDyn *dyn, int par)
 {
-  if (threadIdx.x == 0 && blockIdx.x == 0) dyn->flag[dyn->launch & 1] = 1;
+  if (threadIdx.x == 0 && blockIdx.x == 0) dyn->flag[par] = 1;
 }
 
 int check_disk(const tdm_disk_arrays *a, const tdm_disk_params *p, DiskWs *w)
@@ -735,11 +882,12 @@ int check_disk(const tdm_disk_arrays *a, const tdm_disk_params *p, DiskWs *w)
   return TDM_OK;
 }
 
-DiskArgs disk_args(const tdm_disk_arrays *a, const DiskWs &w, int parity, int ledger,
-                   bool records)
+// Kernel arguments of launch i of a call: state buffers alternate from x/v/
+// omega (i even), parities continue the caller's launch and step counts.
+DiskArgs disk_args(const tdm_disk_arrays *a, const DiskWs &w, int i, int mode, bool records)
 {
   DiskArgs A;
-  const bool odd = parity & 1;
+  const bool odd = i & 1;
   A.x = reinterpret_cast<const double2 *>(odd ? a->x_alt : a->x);
   A.v = reinterpret_cast<const double2 *>(odd ? a->v_alt : a->v);
   A.om = odd ? a->omega_alt : a->omega;
@@ -756,7 +904,9 @@ DiskArgs disk_args(const tdm_disk_arrays *a, const DiskWs &w, int parity, int le
   A.out = a->out;
   A.rec = records ? a->records : nullptr;
   A.w = w;
-  A.ledger = ledger;
+  A.par = (int)((a->launches + i) & 1);
+  A.spar = (int)((a->steps + (mode & 2 ? i : 0)) & 1);
+  A.mode = mode;
   return A;
 }
 
@@ -764,16 +914,23 @@ int launch_disk_steps(const tdm_disk_arrays *a, const tdm_disk_params *p, const 
                       int nsteps, int flags, hipStream_t s, hipEvent_t *ev = nullptr)
 {
   const int nb = p->nblocks;
+  const int mode = 1 | 2 | (flags & TDM_DISK_LEDGER ? 4 : 0);
+  if (nsteps > 0 && p->n_walls + p->n_hull > 0)
+    disk_count_kernel<<<nb, kThreads, 0, s>>>(*p, disk_args(a, w, 0, mode, false));
   for (int i = 0; i < nsteps; ++i) {
     const bool rec = (flags & TDM_DISK_RECORD) && a->records && i == nsteps - 1;
-    const DiskArgs A = disk_args(a, w, i, flags & TDM_DISK_LEDGER, rec);
+    const DiskArgs A = disk_args(a, w, i, mode, rec);
     if (ev) HIP_TRY(hipEventRecord(ev[4 * i], s));
-    disk_prep_kernel<<<nb, kThreads, 0, s>>>(*p, A);
+    disk_prep_kernel<<<min(nb, kPrepBlocks), kThreads, 0, s>>>(*p, A);
     if (ev) HIP_TRY(hipEventRecord(ev[4 * i + 1], s));
     if (ev) HIP_TRY(hipEventRecord(ev[4 * i + 2], s));
-    disk_step_kernel<true><<<nb, kThreads, 0, s>>>(*p, A);
+    if ((flags & TDM_DISK_LEDGER) || i == nsteps - 1)
+      disk_step_kernel<true, true><<<nb, kThreads, 0, s>>>(*p, A);
+    else
+      disk_step_kernel<true, false><<<nb, kThreads, 0, s>>>(*p, A);
     if (ev) HIP_TRY(hipEventRecord(ev[4 * i + 3], s));
   }
+  if (nsteps > 0) disk_flush_kernel<<<1, kThreads, 0, s>>>(*p, disk_args(a, w, 0, mode, false));
   HIP_TRY(hipGetLastError());
   return TDM_OK;
 }
@@ -804,7 +961,7 @@ size_t tdm_disk_workspace_bytes(int n, int ncells, int cell_cap, int cand_slots,
   return total;
 }
 
-int tdm_disk_setup(const tdm_disk_arrays *a, const tdm_disk_params *p, void *stream)
+int tdm_disk_setup(tdm_disk_arrays *a, const tdm_disk_params *p, void *stream)
 {
   DiskWs w;
   int rc = check_disk(a, p, &w);
@@ -812,46 +969,61 @@ int tdm_disk_setup(const tdm_disk_arrays *a, const tdm_disk_params *p, void *str
   hipStream_t s = static_cast<hipStream_t>(stream);
   const size_t ncells = (size_t)p->ncell[0] * p->ncell[1];
   fill_kernel<int><<<256, 256, 0, s>>>(w.cnt, 2 * (size_t)p->n, 0);
+  fill_kernel<unsigned><<<256, 256, 0, s>>>(w.cmask, 2 * (size_t)p->n, 0u);
   fill_kernel<int><<<256, 256, 0, s>>>(w.ccount, 2 * ncells, 0);
   disk_init_kernel<<<1, 64, 0, s>>>(w.dyn);
+  if (p->n_walls + p->n_hull > 0)
+    disk_segbox_kernel<<<(p->n_walls + p->n_hull + 63) / 64, 64, 0, s>>>(
+        *p, disk_args(a, w, 0, 0, false));
   HIP_TRY(hipGetLastError());
+  a->launches = 0;
+  a->steps = 0;
   return TDM_OK;
 }
 
-int tdm_disk_rebin(const tdm_disk_arrays *a, const tdm_disk_params *p, void *stream)
+int tdm_disk_rebin(tdm_disk_arrays *a, const tdm_disk_params *p, void *stream)
 {
   DiskWs w;
   int rc = check_disk(a, p, &w);
   if (rc) return rc;
-  disk_mark_kernel<<<1, 64, 0, static_cast<hipStream_t>(stream)>>>(w.dyn);
+  disk_mark_kernel<<<1, 64, 0, static_cast<hipStream_t>(stream)>>>(w.dyn,
+                                                                    (int)(a->launches & 1));
   HIP_TRY(hipGetLastError());
   return TDM_OK;
 }
 
-int tdm_disk_step(const tdm_disk_arrays *a, const tdm_disk_params *p, int nsteps, int flags,
+int tdm_disk_step(tdm_disk_arrays *a, const tdm_disk_params *p, int nsteps, int flags,
                   void *stream)
 {
   DiskWs w;
   int rc = check_disk(a, p, &w);
   if (rc) return rc;
   if (nsteps < 0) return fail(TDM_EINVAL, "nsteps must be nonnegative");
-  return launch_disk_steps(a, p, w, nsteps, flags, static_cast<hipStream_t>(stream));
+  rc = launch_disk_steps(a, p, w, nsteps, flags, static_cast<hipStream_t>(stream));
+  if (rc == TDM_OK) {
+    a->launches += nsteps;
+    a->steps += nsteps;
+  }
+  return rc;
 }
 
-int tdm_disk_inspect(const tdm_disk_arrays *a, const tdm_disk_params *p, void *stream)
+int tdm_disk_inspect(tdm_disk_arrays *a, const tdm_disk_params *p, void *stream)
 {
   DiskWs w;
   int rc = check_disk(a, p, &w);
   if (rc) return rc;
   hipStream_t s = static_cast<hipStream_t>(stream);
-  const DiskArgs A = disk_args(a, w, 0, 0, a->records != nullptr);
-  disk_prep_kernel<<<p->nblocks, kThreads, 0, s>>>(*p, A);
-  disk_step_kernel<false><<<p->nblocks, kThreads, 0, s>>>(*p, A);
+  const DiskArgs A = disk_args(a, w, 0, 1, a->records != nullptr);
+  if (p->n_walls + p->n_hull > 0) disk_count_kernel<<<p->nblocks, kThreads, 0, s>>>(*p, A);
+  disk_prep_kernel<<<min(p->nblocks, kPrepBlocks), kThreads, 0, s>>>(*p, A);
+  disk_step_kernel<false, true><<<p->nblocks, kThreads, 0, s>>>(*p, A);
+  disk_flush_kernel<<<1, kThreads, 0, s>>>(*p, A);
   HIP_TRY(hipGetLastError());
+  a->launches += 1;
   return TDM_OK;
 }
 
-int tdm_disk_step_timed(const tdm_disk_arrays *a, const tdm_disk_params *p, int nsteps,
+int tdm_disk_step_timed(tdm_disk_arrays *a, const tdm_disk_params *p, int nsteps,
                         void *stream, double *mean_step_us, double *mean_prep_us)
 {
   DiskWs w;
@@ -867,6 +1039,10 @@ int tdm_disk_step_timed(const tdm_disk_arrays *a, const tdm_disk_params *p, int 
     if (hipEventCreate(&ev[made]) != hipSuccess) break;
   if (made < nev) rc = fail(TDM_EHIP, "hipEventCreate failed");
   if (rc == TDM_OK) rc = launch_disk_steps(a, p, w, nsteps, 0, s, ev);
+  if (rc == TDM_OK) {
+    a->launches += nsteps;
+    a->steps += nsteps;
+  }
   double tstep = 0.0, tprep = 0.0;
   if (rc == TDM_OK) {
     hipError_t e = hipStreamSynchronize(s);

@@ -158,12 +158,13 @@ class DiskArrays(ctypes.Structure):
     _fields_ = [(name, ctypes.c_void_p) for name in (
         "x", "v", "omega", "x_alt", "v_alt", "omega_alt", "theta", "radius", "mass", "inertia",
         "walls", "hull", "wall_velocity", "out", "records", "workspace")] + [
-        ("workspace_bytes", ctypes.c_size_t)]
+        ("workspace_bytes", ctypes.c_size_t), ("launches", ctypes.c_int64),
+        ("steps", ctypes.c_int64)]
 
 
 NOUT, NREC = 16, 14
 OUT_DASH, OUT_FRIC, OUT_ROLL, OUT_ELASTIC, OUT_WALL, OUT_HULL, OUT_COUNTS = 0, 1, 2, 3, 6, 8, 10
-OUT_STATUS, OUT_REBUILDS = 13, 14
+OUT_STATUS, OUT_REBUILDS, OUT_CANDIDATES = 13, 14, 15
 FLAG_LEDGER, FLAG_RECORD = 1, 2
 LEDGER_KEYS = ("normal_dashpot", "friction", "rolling")
 
@@ -507,6 +508,14 @@ class DiskSystem:
     def rebuilds(self) -> int:
         return int(self.out[OUT_REBUILDS].item())
 
+    def list_stats(self) -> dict:
+        """Candidates, pair, wall and hull contacts per disk at the last
+        evaluation (pair contacts counted once)."""
+        o = self.out.tolist()
+        n = self.n
+        return dict(cand_mean=o[OUT_CANDIDATES] / n, pairs_per_disk=o[OUT_COUNTS] / n,
+                    walls_per_disk=o[OUT_COUNTS + 1] / n, hull_per_disk=o[OUT_COUNTS + 2] / n)
+
     # ---- stepping ----------------------------------------------------------------
     def step(self, n: int = 1, *, record: bool = False) -> None:
         """Advance n steps with semi-implicit Euler (engine.py:579-596)."""
@@ -541,7 +550,9 @@ class DiskSystem:
 
     def graph(self, pairs: int):
         """2*pairs steps captured as a graph (state ends in the same buffers)."""
-        key = (pairs, self.energy_ledger is not None, self._x.data_ptr())
+        a = self._arrays
+        key = (pairs, self.energy_ledger is not None, self._x.data_ptr(), a.launches & 1,
+               a.steps & 1)
         if key not in self._graphs:
             self._sync_edits()
             g = ctypes.c_void_p()
@@ -561,6 +572,9 @@ class DiskSystem:
         self._sync_edits()
         for _ in range(repeats):
             _lib.check(self.lib.tdm_graph_launch(g, self._stream))
+            # the library's launch and step counts, which the graph baked in
+            self._arrays.launches += 2 * pairs
+            self._arrays.steps += 2 * pairs
         for _ in range(2 * pairs * repeats):
             self.time += self.dt
         self.steps += 2 * pairs * repeats
