@@ -712,7 +712,7 @@ def bench_disk(args):
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic_from_profile("disk_step_counters.md"),
-                     "kernel": "disk_step_kernel<true>", "kernel_us": step_us,
+                     "kernel": "disk_step_kernel<true, false>", "kernel_us": step_us,
                      "prep_kernel_us": prep_us, "bytes_per_disk": bpd,
                      "bytes_per_launch": bpd * n},
     }

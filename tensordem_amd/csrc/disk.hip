@@ -572,7 +572,18 @@ __global__ __launch_bounds__(kThreads) void disk_step_kernel(tdm_disk_params P, 
   double F0 = 0.0, F1 = 0.0, T = 0.0;
   int packed_new = 0;
   unsigned touch = 0u;  // candidate slots in contact
-  int nbc = 0;          // boundary contacts of this disk
+  // rolling torques, added after every lever torque in contact order: the
+  // first four in registers, later ones in the disk's scratch column
+  int nroll = 0;
+  double rr0 = 0.0, rr1 = 0.0, rr2 = 0.0, rr3 = 0.0;
+  auto push_roll = [&](double v) {
+    if (nroll == 0) rr0 = v;
+    else if (nroll == 1) rr1 = v;
+    else if (nroll == 2) rr2 = v;
+    else if (nroll == 3) rr3 = v;
+    else W.mr[(size_t)(nroll - 4) * N + p] = v;
+    ++nroll;
+  };
   const bool rolling = P.mu_rolling > 0.0;
   if (live) {
     const int cw = W.cnt[(size_t)lq * N + p];
@@ -663,7 +674,7 @@ __global__ __launch_bounds__(kThreads) void disk_step_kernel(tdm_disk_params P, 
           F0 += o.t0;
           F1 += o.t1;
           T += li0 * o.t1 - li1 * o.t0;
-          if (rolling) W.mr[(size_t)e * N + p] = -o.m_r;
+          if (rolling) push_roll(-o.m_r);
           if (kSums) {
             const double vn = (vi0 - vj0) * n0 + (vi1 - vj1) * n1;
             ch[D_DASH] += (o.fn - P.kn * ov) * vn;
@@ -681,7 +692,7 @@ __global__ __launch_bounds__(kThreads) void disk_step_kernel(tdm_disk_params P, 
           F0 += -o.t0;
           F1 += -o.t1;
           T += lj0 * -o.t1 - lj1 * -o.t0;
-          if (rolling) W.mr[(size_t)e * N + p] = o.m_r;
+          if (rolling) push_roll(o.m_r);
         }
         if (kAdvance) {
           W.hxi[at] = h;
@@ -729,8 +740,7 @@ __global__ __launch_bounds__(kThreads) void disk_step_kernel(tdm_disk_params P, 
       F0 += o.t0;
       F1 += o.t1;
       T += li0 * o.t1 - li1 * o.t0;
-      if (rolling) W.mr[(size_t)(S + nbc) * N + p] = -o.m_r;
-      ++nbc;
+      if (rolling) push_roll(-o.m_r);
       if (kSums) {
         const double vn = (vi0 - vj0) * n0 + (vi1 - vj1) * n1;
         ch[D_DASH] += (o.fn - P.kn * hh.ov) * vn;
@@ -757,11 +767,11 @@ __global__ __launch_bounds__(kThreads) void disk_step_kernel(tdm_disk_params P, 
 
     // rolling torques after every lever torque, in the same contact order
     if (rolling) {
-      for (int e = e0; e < cnt; ++e)
-        if (touch >> e & 1u) T += W.mr[(size_t)e * N + p];
-      for (int e = 0; e < e0; ++e)
-        if (touch >> e & 1u) T += W.mr[(size_t)e * N + p];
-      for (int k = 0; k < nbc; ++k) T += W.mr[(size_t)(S + k) * N + p];
+      if (nroll > 0) T += rr0;
+      if (nroll > 1) T += rr1;
+      if (nroll > 2) T += rr2;
+      if (nroll > 3) T += rr3;
+      for (int k = 4; k < nroll; ++k) T += W.mr[(size_t)(k - 4) * N + p];
     }
     if (P.use_gravity) {
       F0 = F0 + mp * P.gravity[0];

@@ -197,3 +197,29 @@ def test_graph_replays_match_eager_steps():
         for name in ("pos", "vel", "omega", "theta"):
             assert torch.equal(getattr(eager, name), getattr(graph, name)), (rep, name)
     assert eager.digest() == graph.digest()
+
+
+def test_crowded_disks_with_many_contacts_match_oracle():
+    # a hexagonal cluster squeezed so the inner disks touch six neighbours
+    # (more rolling torques per disk than the kernel keeps in registers)
+    # plus a floor and a side wall
+    rows, cols, r = 7, 7, 0.01
+    pitch = 1.9 * r
+    pos = np.array([[(c + 0.5 * (k % 2)) * pitch + 0.03, k * pitch * math.sqrt(3) / 2 + 0.0095]
+                    for k in range(rows) for c in range(cols)])
+    rad = np.full(len(pos), r)
+    cfg = DiskConfig(mu=0.6, mu_rolling=0.2, gravity=(0.0, -9.81), dt=1e-6)
+    walls = np.array([[[-1.0, 0.0], [1.0, 0.0]], [[0.03 - 0.0095, -1.0], [0.03 - 0.0095, 1.0]]])
+    dev, ref = _pair(cfg, pos, rad, walls=walls)
+    w = np.random.default_rng(12).normal(scale=2.0, size=len(pos))
+    dev.omega = w
+    ref.omega[:] = w
+    dev.enable_energy_ledger()
+    ref.enable_energy_ledger()
+    _advance_both(dev, ref, 3, 100)
+    _ledgers_agree(dev, ref)
+    got, c = dev.contact_forces(), ref.contact_forces()
+    for k, v in c.items():
+        assert np.array_equal(got[k], v), k
+    per_disk = np.bincount(np.r_[c["i"], c["j"][c["j"] >= 0]], minlength=len(pos))
+    assert per_disk.max() >= 6
