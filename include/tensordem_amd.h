@@ -353,7 +353,7 @@ size_t tdm_disk_workspace_bytes(int n, int ncells, int cell_cap, int cand_slots,
                                 int n_hull);
 /* Clears lists, cell tables and contact state, schedules the first
  * candidate search; replaces DiskSystem.__init__'s neighbour and history
- * setup (granular2d/engine.py:197-212, 674-677). */
+ * setup (granular2d/engine.py:194-209, 683-685). */
 int tdm_disk_setup(tdm_disk_arrays *a, const tdm_disk_params *p, void *stream);
 /* Schedules a candidate search (after the caller moved disks). */
 int tdm_disk_rebin(tdm_disk_arrays *a, const tdm_disk_params *p, void *stream);
@@ -361,14 +361,14 @@ int tdm_disk_rebin(tdm_disk_arrays *a, const tdm_disk_params *p, void *stream);
  * parity 0.  flags: TDM_DISK_LEDGER accumulates the energy ledger,
  * TDM_DISK_RECORD writes per-contact records on the last step.  Replaces
  * DiskSystem.step and _evaluate(advance_history=True)
- * (granular2d/engine.py:399-596). */
+ * (granular2d/engine.py:399-627). */
 #define TDM_DISK_LEDGER 1
 #define TDM_DISK_RECORD 2
 int tdm_disk_step(tdm_disk_arrays *a, const tdm_disk_params *p, int nsteps, int flags,
                   void *stream);
 /* Evaluation at the parity-0 state without advancing histories or state
  * (records when a->records is set); replaces DiskSystem.contact_forces
- * (granular2d/engine.py:631-634). */
+ * (granular2d/engine.py:675-678). */
 int tdm_disk_inspect(tdm_disk_arrays *a, const tdm_disk_params *p, void *stream);
 /* As tdm_disk_step (nsteps even, no flags) with HIP events on each step's
  * two kernels; mean durations in microseconds.  Measurement helper. */

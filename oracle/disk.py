@@ -3,8 +3,8 @@ infrastructure only).
 
 `OracleDisk` drives `ord_eval` of oracle/disk2d.c with numpy state and
 mirrors `DiskSystem` of granular2d/engine.py:144-684: masses and inertias
-(engine.py:167-168), the stability bound (engine.py:222-226), the contact
-state carried by key between steps, the energy ledger (engine.py:598-629)
+(engine.py:170-171), the stability bound (engine.py:231-235), the contact
+state carried by key between steps, the energy ledger (engine.py:578-588, 630-673)
 and the state digest (engine.py:289-295).  Only tests, the smoke check and
 the benchmark's CPU baseline use this module.
 """

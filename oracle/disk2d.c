@@ -3,29 +3,29 @@
  * baseline use it; the product never links it).
  *
  * Restates `DiskSystem._evaluate` and `DiskSystem.step` of
- * granular2d/engine.py:399-596 for one step:
+ * granular2d/engine.py:399-596 and 599-618 for one step:
  *
  *   contacts   pairs i < j with overlap r_i + r_j - |l| > 0, l = x_j - x_i
- *              under the minimum image on periodic axes (engine.py:298-357,
+ *              under the minimum image on periodic axes (engine.py:298-350,
  *              411-418), in ascending key i*n + j; then wall segments and
  *              hull segments, disk-major, with the closest point
  *              t = clip(ap.ab / max(|ab|^2, 1e-30), 0, 1), overlap r - |d|,
- *              kept when overlap > 0 and |d| > 1e-12 (engine.py:359-380).
+ *              kept when overlap > 0 and |d| > 1e-12 (engine.py:352-375).
  *   state      spring xi (2) and rolling angle per contact, loaded by key
- *              (engine.py:389-397, 449-453): pairs by i*n + j, boundary
+ *              (engine.py:388-397, 452-456): pairs by i*n + j, boundary
  *              contacts by n*n + k, k their rank among this step's wall then
  *              hull contacts (a boundary contact inherits the state of the
  *              previous step's boundary contact of the same rank).
  *   law        f_n = max(0, kn*ov - 2*zeta*sqrt(kn*m_eff)*v_n); xi += v_t dt,
  *              projected on the tangent; f_t = -k_t xi capped at mu f_n;
  *              rolling M_r = k_r theta_rel capped at mu_r r_eff f_n
- *              (engine.py:455-534).
+ *              (engine.py:458-537).
  *   sums       force: pairs on i, pairs on j (negated), walls, hull, then
  *              gravity and drag; torque: lever crosses in the same order, then
- *              the rolling torques (engine.py:496-534) -- the order of
+ *              the rolling torques (engine.py:499-544) -- the order of
  *              numpy's np.add.at, so each disk's sums are bit-identical.
  *   step       semi-implicit Euler with the optional velocity decay and the
- *              periodic wrap x mod L (engine.py:579-596).
+ *              periodic wrap x mod L (engine.py:599-618).
  *
  * Ledger and energy sums (ODS_*) add contacts in key order; numpy sums them
  * pairwise, so they agree with the reference to rounding, not bit for bit.
@@ -45,7 +45,7 @@ enum {
   ODS_DASH, ODS_SLIDE, ODS_ROLL, ODS_EL_N, ODS_EL_T, ODS_EL_R, ODS_WALL_FX, ODS_WALL_FY,
   ODS_HULL_FX, ODS_HULL_FY, ODS_NPAIR, ODS_NWALL, ODS_NHULL, ODS_NSCAL
 };
-/* per-contact record fields (contact_forces, engine.py:536-565) */
+/* per-contact record fields (contact_forces, engine.py:546-575) */
 enum {
   ODR_I, ODR_J, ODR_KIND, ODR_NX, ODR_NY, ODR_LX, ODR_LY, ODR_FX, ODR_FY, ODR_FN, ODR_FT,
   ODR_OV, ODR_PX, ODR_PY, ODR_NREC

@@ -14,7 +14,7 @@
 //                      the reference's order, integrates, and writes the new
 //                      spring state.  A boundary contact's spring is stored at
 //                      its rank among all boundary contacts (how the reference
-//                      keys them, engine.py:449-453): each block adds up the
+//                      keys them, engine.py:452-456): each block adds up the
 //                      counts of the blocks before it.
 //
 // No kernel waits on another block: per-launch bookkeeping is indexed by the
@@ -125,7 +125,7 @@ __device__ inline double min_image(double d, double L) { return L > 0.0 ? d - L 
 __device__ inline double max0_first(double x) { return (0.0 >= x) ? 0.0 : x; }
 __device__ inline double max0_second(double x) { return (x >= 0.0 || x != x) ? x : 0.0; }
 
-// Closest point of a segment (engine.py:359-380): contact when the overlap
+// Closest point of a segment (engine.py:352-375): contact when the overlap
 // r - |d| is positive and |d| > 1e-12; n points from the segment to the disk.
 struct SegHit {
   double n0, n1, ov, cp0, cp1;
@@ -239,7 +239,7 @@ __device__ inline int axis_cells(const tdm_disk_params &P, int a, int c, int (&o
   return k;
 }
 
-// One contact of the law (engine.py:455-534).  (n0, n1) points from j to
+// One contact of the law (engine.py:458-537).  (n0, n1) points from j to
 // i, (vr0, vr1) = v_i - v_j at the contact, omrel = omega_i - omega_j.
 // Updates the spring (xi, roll) and returns the force on i in (t0, t1) and
 // the rolling torque m_r (on i: -m_r).
@@ -509,7 +509,7 @@ __global__ __launch_bounds__(kThreads) void disk_step_kernel(tdm_disk_params P, 
   if (kAdvance && blockIdx.x == 0 && threadIdx.x == 0)
     dyn->nst[A.spar ^ 1] = bounded ? dyn->tot[0] + dyn->tot[1] : 0;
 
-  // ---- candidate search (engine.py:298-357 gives the pair set) -------------
+  // ---- candidate search (engine.py:298-350 gives the pair set) -------------
   if (live && rebuild) {
     // the list is kept sorted in place in the disk's own slots (cached)
     int *cl = W.cand + (size_t)lq * S * N + p;
@@ -705,7 +705,7 @@ __global__ __launch_bounds__(kThreads) void disk_step_kernel(tdm_disk_params P, 
       for (int e = 0; e < S; ++e)
         if (e < e0 || e >= cnt || !(touch >> e & 1u)) clear_record(A.rec, (size_t)e * N + p);
 
-    // walls, then hull (engine.py:420-453, 506-521)
+    // walls, then hull (engine.py:420-421, 509-520)
     const double4 *wold = W.wst + (size_t)A.spar * ((size_t)N * (P.n_walls + P.n_hull) + 1);
     double4 *wnew = W.wst + (size_t)(A.spar ^ 1) * ((size_t)N * (P.n_walls + P.n_hull) + 1);
     int kw = 0, kh = 0;

@@ -100,7 +100,7 @@ def box_walls(width: float, height: float, *, left: bool = True, right: bool = T
 
 def ship_segments(length: float, beam: float, origin=(0.0, 0.0), *, bow: float = 0.0,
                   stern: float = 0.0, n_bow: int = 6) -> np.ndarray:
-    """Closed hull polyline in the x-y plane, bow towards +x (engine.py:117-141):
+    """Closed hull polyline in the x-y plane, bow towards +x (engine.py:119-141):
     flat sides of half-beam, a transom whose corners are set in by `stern`, and
     a bow of length `bow` tapering linearly to the stem in n_bow pieces."""
     for name, value in (("length", length), ("beam", beam)):
@@ -170,7 +170,7 @@ LEDGER_KEYS = ("normal_dashpot", "friction", "rolling")
 
 
 class _Ledger(dict):
-    """The energy ledger dict (engine.py:598-629); the three work entries are
+    """The energy ledger dict (engine.py:630-653); the three work entries are
     read from the device whenever they are looked up."""
 
     def __init__(self, system: "DiskSystem", kinetic_initial: float):
@@ -518,7 +518,7 @@ class DiskSystem:
 
     # ---- stepping ----------------------------------------------------------------
     def step(self, n: int = 1, *, record: bool = False) -> None:
-        """Advance n steps with semi-implicit Euler (engine.py:579-596)."""
+        """Advance n steps with semi-implicit Euler (engine.py:599-627)."""
         if n < 0:
             raise ValueError("n must be nonnegative")
         if n == 0:
@@ -615,7 +615,7 @@ class DiskSystem:
 
     def contact_forces(self) -> dict:
         """Evaluate the contacts at the current state without integrating or
-        advancing any spring (engine.py:631-634); per-contact records."""
+        advancing any spring (engine.py:675-678); per-contact records."""
         self._ensure_records()
         self._sync_edits()
         _lib.check(self.lib.tdm_disk_inspect(ctypes.byref(self._arrays),
@@ -633,13 +633,13 @@ class DiskSystem:
         return self._contacts
 
     def enable_energy_ledger(self) -> dict:
-        """Start (or reset) the non-conservative work ledger (engine.py:598-621)."""
+        """Start (or reset) the non-conservative work ledger (engine.py:630-653)."""
         self.out[:3] = 0.0
         self.energy_ledger = _Ledger(self, self.kinetic_energy())
         return self.energy_ledger
 
     def energies(self) -> dict:
-        """Kinetic, gravitational and stored spring energies (engine.py:623-634)."""
+        """Kinetic, gravitational and stored spring energies (engine.py:655-666)."""
         g = np.asarray(self.config.gravity, dtype=np.float64)
         if np.any(g != 0.0):
             gt = torch.as_tensor(g, device=self.device)

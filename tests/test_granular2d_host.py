@@ -83,7 +83,7 @@ def test_box_walls_and_polyline():
 
 
 def test_recommended_dt_formula():
-    # dt = 0.1 / sqrt(2 kn / (m_min / 2)), m = rho pi r^2 t (engine.py:167, 222-226)
+    # dt = 0.1 / sqrt(2 kn / (m_min / 2)), m = rho pi r^2 t (engine.py:170, 231-235)
     cfg = DiskConfig()
     m = cfg.density * math.pi * 0.01**2 * cfg.thickness
     want = 0.1 / math.sqrt(2.0 * cfg.kn / (0.5 * m))

@@ -18,7 +18,7 @@ reference's own tests and the mathematics fix:
   feels the force of the same pair moved into the interior.
 
 The boundary spring state is carried by rank among the boundary contacts
-(engine.py:449-453); no reference fixture covers a changing boundary-contact
+(engine.py:452-456); no reference fixture covers a changing boundary-contact
 set, so that rule is pinned only through the trajectories above."""
 import math
 
