@@ -497,19 +497,26 @@ __global__ __launch_bounds__(kThreads) void bed_step_kernel(BKArgs k)
       if (!__any(c0 < nc)) break;
       double2 xq[kChunk], vq[kChunk], hq[kChunk];
       double rq[kChunk];
+      int qq[kChunk];
       uint32_t hi_bits = 0;  // bit j: partner j has the higher index (p is lo)
 #pragma unroll
       for (int j = 0; j < kChunk; ++j) {
         const int e = c0 + j;
         xq[j] = vq[j] = hq[j] = make_double2(0.0, 0.0);
         rq[j] = 1.0;
+        qq[j] = p;
         if (e < nc) {
           const int q = e < kPre ? cq[e] : k.w.cand[lp * sn + (size_t)e * n + p];
+          qq[j] = q;
           hi_bits |= (uint32_t)(q > p) << j;
           xq[j] = k.x_in[q];
-          vq[j] = k.v_in[q];
           rq[j] = a.radius[q];
-          if ((mask >> e) & 1u) hq[j] = xis[(size_t)e * n];
+          // a partner's velocity is needed only in contact: fetched with the
+          // round for the pairs in contact last step, on demand for new ones
+          if ((mask >> e) & 1u) {
+            vq[j] = k.v_in[q];
+            hq[j] = xis[(size_t)e * n];
+          }
         }
       }
 #pragma unroll
@@ -518,8 +525,16 @@ __global__ __launch_bounds__(kThreads) void bed_step_kernel(BKArgs k)
         if (e >= nc) continue;
         const bool self_lo = (hi_bits >> j) & 1u;  // the pair law is evaluated lo -> hi
         const double2 xl = self_lo ? xp : xq[j], xh = self_lo ? xq[j] : xp;
-        const double2 vl = self_lo ? vp : vq[j], vh = self_lo ? vq[j] : vp;
         const double rl = self_lo ? rp : rq[j], rh = self_lo ? rq[j] : rp;
+        {
+          // hm_pair's overlap test, before the velocity is needed
+          const double dx = xh.x - xl.x, dy = xh.y - xl.y;
+          const double s2 = dx * dx + dy * dy;
+          const double rs = rl + rh;
+          if (s2 > (rs * rs) * (1.0 + 1e-15) || !(rs - sqrt(s2) > 0)) continue;
+        }
+        if (!((mask >> e) & 1u)) vq[j] = k.v_in[qq[j]];
+        const double2 vl = self_lo ? vp : vq[j], vh = self_lo ? vq[j] : vp;
         double2 h = hq[j];
         double Fx, Fy, dash, fric;
         if (!hm_pair<R>(m, xl, xh, vl, vh, rl, rh, h, Fx, Fy, dash, fric)) continue;
