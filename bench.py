@@ -645,18 +645,18 @@ def disk_inputs(side=DISK_SIDE):
     grid of pitch 2.1 r (the largest neighbours overlap slightly) in a walled
     box, seeded velocities of 0.05 m/s rms; gravity, mu = 0.5, mu_r = 0.05,
     the engine's default stiffness, damping, ice density and skin."""
-    from tensordem_amd.granular2d import DiskConfig, box_walls
-    n = side * side
-    rng = np.random.default_rng(0)
-    r = DISK_R * rng.uniform(0.9, 1.1, size=n)
-    s = 2.1 * DISK_R
-    idx = np.arange(n)
-    slack = np.maximum(0.5 * s - r, 0.0)[:, None]
-    pos = (np.stack([idx % side, idx // side], 1) + 0.5) * s \
-        + rng.uniform(-1, 1, size=(n, 2)) * slack
-    vel = rng.normal(scale=0.05, size=(n, 2))
+    from tensordem_amd.granular2d import DiskConfig, jittered_box
+    pos, r, vel, walls = jittered_box(side, DISK_R, seed=0)
     cfg = DiskConfig(mu=0.5, mu_rolling=0.05, gravity=(0.0, -9.81))
-    return cfg, pos, r, vel, box_walls(side * s, side * s)
+    return cfg, pos, r, vel, walls
+
+
+def disk_dt():
+    """The 1M-disk box's time step (the engine's stability bound, from its
+    lightest disk); also valid for the smaller boxes of the host baseline."""
+    cfg, _, r, _, _ = disk_inputs()
+    m_min = cfg.density * math.pi * float(r.min()) ** 2 * cfg.thickness
+    return 0.1 / math.sqrt(2.0 * cfg.kn / (0.5 * m_min))
 
 
 def disk_bytes_per_disk(cand_mean, pairs_per_disk, boundary_per_disk):
@@ -722,7 +722,10 @@ def bench_disk(args):
     sysd.close()
     del sysd
     if not args.no_cpu:
-        res["cpu_baseline"] = cpu_baseline_disk(args.cpu_seconds)
+        one = cpu_baseline_disk(args.cpu_seconds)
+        res["cpu_baseline"] = with_pool(one, getattr(args, "cpu_pools", {}).get("disk"),
+                                        args.cpu_seconds, "a 65 536-disk box of the same "
+                                        "construction")
     return res
 
 
@@ -730,8 +733,10 @@ def cpu_baseline_disk(budget_s, side=256):
     """C oracle (FP64, one core) on a 256 x 256 box of the same construction
     (65 536 disks; per-disk work does not depend on the box size)."""
     from oracle.disk import OracleDisk
+    from tensordem_amd.granular2d import DiskConfig
     cfg, pos, r, vel, walls = disk_inputs(side)
-    sim = OracleDisk(pos, r, cfg, walls=walls)
+    sim = OracleDisk(pos, r, DiskConfig(mu=cfg.mu, mu_rolling=cfg.mu_rolling,
+                                        gravity=cfg.gravity, dt=disk_dt()), walls=walls)
     sim.vel[:] = vel
     sim.step()
     t0 = time.perf_counter()
@@ -1034,6 +1039,8 @@ def main(argv=None):
         if not args.no_cpu and args.workload in ("all", "plate"):
             args.cpu_pools["plate"] = CpuPool("plate", c2_config("cpu").degree_bound_dt(),
                                               host_workers())
+        if not args.no_cpu and args.workload in ("all", "disk"):
+            args.cpu_pools["disk"] = CpuPool("disk", disk_dt(), host_workers())
     except OSError as exc:
         print(f"bench.py: host-baseline workers not started ({exc})", file=sys.stderr)
     try:

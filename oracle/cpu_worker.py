@@ -8,10 +8,13 @@ runs this module.
 
     python -m oracle.cpu_worker bed --dt DT --seconds T
     python -m oracle.cpu_worker plate --dt DT --wait
+    python -m oracle.cpu_worker disk --dt DT --wait
 
 `bed`: the 1 048 576-disk column (radii U[0.8, 1.2] x 0.01 m, seed 0) from
 release; `plate`: the 1024 x 128 plate indentation run from rest (indenter
-r = 0.5 m at 0.5 m/s, strains 0.01 / 0.02).  With --wait, the worker prints
+r = 0.5 m at 0.5 m/s, strains 0.01 / 0.02); `disk`: a 256 x 256 box of rigid
+disks with rotation (radii U[0.9, 1.1] x 0.01 m on a jittered grid, walls,
+gravity, mu = 0.5, mu_rolling = 0.05, seed 0).  With --wait, the worker prints
 "ready" after its set-up and then reads one line "START_AT SECONDS" from
 stdin: it sleeps until the wall-clock instant START_AT and runs for SECONDS,
 so that concurrent workers time the same window.
@@ -44,14 +47,24 @@ def _plate(dt):
     return OraclePlate(cfg, dt=dt), cfg.nx * cfg.ny
 
 
+def _disk(dt):
+    from oracle.disk import OracleDisk
+    from tensordem_amd.granular2d import DiskConfig, jittered_box
+    pos, r, vel, walls = jittered_box(256, 0.01, seed=0)
+    sim = OracleDisk(pos, r, DiskConfig(mu=0.5, mu_rolling=0.05, gravity=(0.0, -9.81), dt=dt),
+                     walls=walls)
+    sim.vel[:] = vel
+    return sim, len(r)
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
-    ap.add_argument("kind", choices=("bed", "plate"))
+    ap.add_argument("kind", choices=("bed", "plate", "disk"))
     ap.add_argument("--dt", type=float, required=True)
     ap.add_argument("--seconds", type=float, default=10.0)
     ap.add_argument("--wait", action="store_true")
     args = ap.parse_args(argv)
-    sim, n = (_bed if args.kind == "bed" else _plate)(args.dt)
+    sim, n = {"bed": _bed, "plate": _plate, "disk": _disk}[args.kind](args.dt)
     sim.step()  # warm-up: first-touch of the oracle's buffers
     if args.wait:
         print("ready", flush=True)

@@ -125,6 +125,24 @@ def ship_segments(length: float, beam: float, origin=(0.0, 0.0), *, bow: float =
     return polyline_segments(np.asarray(starboard + stem + port, dtype=np.float64))
 
 
+def jittered_box(side: int, radius: float = 0.01, seed: int = 0, speed: float = 0.05):
+    """side x side disks of radius U[0.9, 1.1] x radius on a square grid of
+    pitch 2.1 radius, each jittered inside its cell (the largest neighbours
+    overlap slightly), with seeded Gaussian velocities of `speed` rms per
+    axis, and the four walls of the side*pitch square box.  Returns
+    (positions, radii, velocities, walls)."""
+    n = side * side
+    rng = np.random.default_rng(seed)
+    r = radius * rng.uniform(0.9, 1.1, size=n)
+    pitch = 2.1 * radius
+    idx = np.arange(n)
+    slack = np.maximum(0.5 * pitch - r, 0.0)[:, None]
+    pos = (np.stack([idx % side, idx // side], 1) + 0.5) * pitch \
+        + rng.uniform(-1, 1, size=(n, 2)) * slack
+    vel = rng.normal(scale=speed, size=(n, 2))
+    return pos, r, vel, box_walls(side * pitch, side * pitch)
+
+
 def _as_segments(value, name: str) -> np.ndarray:
     if value is None:
         return np.zeros((0, 2, 2))
