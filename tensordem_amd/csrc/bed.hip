@@ -32,6 +32,12 @@ using namespace tdm;
 
 constexpr int kMaxSlots = 16;
 constexpr int kChunk = 2;        // partners gathered per round
+// A disk that moved this fraction of the skin since its list was built
+// requests a rebuild; the list stays complete up to half the skin, and the
+// bin and search steps follow within two steps (motion per step is far below
+// the remaining 0.1 skin; a list used beyond half the skin raises the stale
+// status).
+constexpr double kRebuildFrac = 0.4;
 constexpr int kPre = 6;          // candidate slots fetched before the list length is known
 constexpr int kNch = 12;         // ledger channels written per block
 
@@ -590,7 +596,7 @@ __global__ __launch_bounds__(kThreads) void bed_step_kernel(BKArgs k)
     k.v_out[p] = make_double2(vx, vy);
     if (!isfinite(xn) || !isfinite(yn) || !isfinite(vx) || !isfinite(vy)) status |= TDM_ST_NONFINITE;
     const double ddx = xn - xb.x, ddy = yn - xb.y;
-    const double q4 = 0.25 * m.skin;
+    const double q4 = kRebuildFrac * m.skin;
     if (ddx * ddx + ddy * ddy >= q4 * q4) req = 1;
     if (m.mode == MODE_BIN) {
       int cx, cy;
