@@ -293,6 +293,7 @@ class DiskSystem:
         self.out = torch.zeros(NOUT, **f64)
         self.records = None
         self._records_on = False
+        self._records_valid = False  # records describe the last evaluation
         self._contacts = None
         self.energy_ledger: dict | None = None
         self.hull_force_history: list = []
@@ -564,6 +565,7 @@ class DiskSystem:
         if n % 2:
             self._swap()
         self._contacts = None
+        self._records_valid = self._records_on
         self._check_status()
 
     def graph(self, pairs: int):
@@ -597,6 +599,7 @@ class DiskSystem:
             self.time += self.dt
         self.steps += 2 * pairs * repeats
         self._contacts = None
+        self._records_valid = False  # captured steps write no records
 
     def step_timed(self, nsteps: int) -> tuple[float, float]:
         """nsteps (even) steps with each kernel timed by HIP events; returns the
@@ -610,6 +613,7 @@ class DiskSystem:
             self.time += self.dt
         self.steps += nsteps
         self._contacts = None
+        self._records_valid = False
         return a.value, b.value
 
     # ---- contacts, energies --------------------------------------------------------
@@ -639,13 +643,17 @@ class DiskSystem:
         _lib.check(self.lib.tdm_disk_inspect(ctypes.byref(self._arrays),
                                              ctypes.byref(self._params), self._stream))
         self._check_status()
+        self._records_valid = True
         self._contacts = self._compact_records()
         return self._contacts
 
     @property
     def contacts(self) -> dict:
+        """Records of the last evaluation (the last step of the last `step`
+        call); where none were written (before records were first requested,
+        after graph replays), an evaluation of the current state."""
         if self._contacts is None:
-            if not self._records_on:
+            if not self._records_valid:
                 return self.contact_forces()
             self._contacts = self._compact_records()
         return self._contacts

@@ -197,6 +197,15 @@ def test_graph_replays_match_eager_steps():
         for name in ("pos", "vel", "omega", "theta"):
             assert torch.equal(getattr(eager, name), getattr(graph, name)), (rep, name)
     assert eager.digest() == graph.digest()
+    # replays write no records: `contacts` then evaluates the current state
+    before = graph.contact_forces()
+    graph.run_graph(1)
+    now = graph.contacts
+    again = graph.contact_forces()
+    assert len(now["i"]) > 0
+    for k, v in again.items():
+        assert np.array_equal(now[k], v), k
+    assert not np.array_equal(before["f"], now["f"])
 
 
 def test_crowded_disks_with_many_contacts_match_oracle():
