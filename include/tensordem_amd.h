@@ -329,6 +329,8 @@ typedef struct tdm_disk_params {
   int32_t use_gravity, use_decay;
   int32_t n, n_walls, n_hull;  /* n_walls, n_hull <= 255 */
   int32_t cand_slots, cell_cap, nblocks;
+  double seg_box[8][4];        /* written by tdm_disk_setup: bounding boxes (min x,
+                                  min y, max x, max y) of the first 8 segments */
 } tdm_disk_params;
 
 typedef struct tdm_disk_arrays {
@@ -354,7 +356,7 @@ size_t tdm_disk_workspace_bytes(int n, int ncells, int cell_cap, int cand_slots,
 /* Clears lists, cell tables and contact state, schedules the first
  * candidate search; replaces DiskSystem.__init__'s neighbour and history
  * setup (granular2d/engine.py:194-209, 683-685). */
-int tdm_disk_setup(tdm_disk_arrays *a, const tdm_disk_params *p, void *stream);
+int tdm_disk_setup(tdm_disk_arrays *a, tdm_disk_params *p, void *stream);
 /* Schedules a candidate search (after the caller moved disks). */
 int tdm_disk_rebin(tdm_disk_arrays *a, const tdm_disk_params *p, void *stream);
 /* nsteps steps; state alternates x/v/omega <-> x_alt/v_alt/omega_alt from

@@ -38,7 +38,8 @@ namespace {
 constexpr int kDiskCap = 24;  // candidate slots per disk (bits of the contact mask)
 constexpr int kDch = 14;      // ledger partial channels
 constexpr int kBatch = 4;     // candidates fetched together
-constexpr int kPrepBlocks = 512;  // grid of the prep kernel (grid-stride binning)
+constexpr int kPrepBlocks = 512;
+constexpr int kParamSegs = 8;     // segment boxes carried in tdm_disk_params.seg_box  // grid of the prep kernel (grid-stride binning)
 enum {
   D_DASH, D_SLIDE, D_ROLL, D_OV2, D_XI2, D_KROLL2, D_WFX, D_WFY, D_HFX, D_HFY, D_NPAIR,
   D_NWALL, D_NHULL, D_NCAND
@@ -68,7 +69,7 @@ struct DiskWs {
   int *ccount;        // [2][ncells]
   int *centry;        // [2][ncells][cell_cap]
   double4 *wst;       // [2][wcap] boundary springs by rank: xi, roll
-  int *blkcnt;        // [nblocks] packed wall | hull << 16 contacts per block
+  int *blkcnt;        // [2][nblocks] packed wall | hull << 16 contacts per block, by step parity
   int2 *blkoff;       // [nblocks] exclusive wall, hull offsets of the blocks
   double *part;       // [nblocks][kDch]
   double *mr;         // [S + n_walls + n_hull][N] rolling torque terms
@@ -96,7 +97,7 @@ void carve(char *base, int n, int ncells, int cap, int S, int nw, int nh, DiskWs
   w->ccount = reinterpret_cast<int *>(take(sizeof(int) * 2 * (size_t)ncells));
   w->centry = reinterpret_cast<int *>(take(sizeof(int) * 2 * (size_t)ncells * cap));
   w->wst = reinterpret_cast<double4 *>(take(sizeof(double4) * 2 * wcap));
-  w->blkcnt = reinterpret_cast<int *>(take(sizeof(int) * (size_t)nb));
+  w->blkcnt = reinterpret_cast<int *>(take(sizeof(int) * 2 * (size_t)nb));
   w->blkoff = reinterpret_cast<int2 *>(take(sizeof(int2) * (size_t)nb));
   w->part = reinterpret_cast<double *>(take(sizeof(double) * kDch * (size_t)nb));  // [kDch][nb]
   w->mr = reinterpret_cast<double *>(take(sizeof(double) * (size_t)(S + nw + nh) * n));
@@ -168,6 +169,10 @@ __device__ inline const double *segment(const tdm_disk_params &P, const DiskArgs
 
 // Segments the disk touches: bit s of the mask (first 64 segments) and the
 // wall | hull << 16 counts.
+// Segments the disk touches: bit s of the mask (first 64 segments) and the
+// wall | hull << 16 counts.  The first kParamSegs boxes come with the launch
+// parameters (the loop over them is unrolled, so they stay in scalar
+// registers), the rest from the workspace.
 __device__ inline int boundary_hits(const tdm_disk_params &P, const DiskArgs &A, double px,
                                     double py, double r, unsigned long long &mask)
 {
@@ -175,7 +180,16 @@ __device__ inline int boundary_hits(const tdm_disk_params &P, const DiskArgs &A,
   mask = 0ull;
   SegHit h;
   const int nseg = P.n_walls + P.n_hull;
-  for (int s = 0; s < nseg; ++s) {
+#pragma unroll
+  for (int s = 0; s < kParamSegs; ++s) {
+    if (s >= nseg) break;
+    if (!seg_near(P.seg_box[s], px, py, r)) continue;
+    if (!seg_hit(segment(P, A, s), px, py, r, h)) continue;
+    mask |= 1ull << s;
+    if (s < P.n_walls) ++cw;
+    else ++chh;
+  }
+  for (int s = kParamSegs; s < nseg; ++s) {
     if (!seg_near(A.w.segbox + 4 * s, px, py, r)) continue;
     if (!seg_hit(segment(P, A, s), px, py, r, h)) continue;
     if (s < 64) mask |= 1ull << s;
@@ -350,38 +364,54 @@ __device__ void reduce_pending(const tdm_disk_params &P, const DiskArgs &A, doub
 // Exclusive offsets of the per-block wall and hull contact counts (one block).
 __device__ void scan_blocks(const tdm_disk_params &P, const DiskArgs &A)
 {
-  __shared__ int sw_s[kThreads], sh_s[kThreads];
+  __shared__ int sw_s[kThreads / 64], sh_s[kThreads / 64];
   const int nb = P.nblocks;
   const int per = (nb + kThreads - 1) / kThreads;
   const int b0 = min(nb, (int)threadIdx.x * per), b1 = min(nb, b0 + per);
+  const int *cnt = A.w.blkcnt + (size_t)A.spar * nb;
   int sw = 0, sh = 0;
 #pragma unroll 8
   for (int b = b0; b < b1; ++b) {
-    const int v = A.w.blkcnt[b];
+    const int v = cnt[b];
     sw += v & 0xFFFF;
     sh += v >> 16;
   }
-  sw_s[threadIdx.x] = sw;
-  sh_s[threadIdx.x] = sh;
-  __syncthreads();
-  for (int d = 1; d < kThreads; d <<= 1) {
-    const int aw = threadIdx.x >= d ? sw_s[threadIdx.x - d] : 0;
-    const int ah = threadIdx.x >= d ? sh_s[threadIdx.x - d] : 0;
-    __syncthreads();
-    sw_s[threadIdx.x] += aw;
-    sh_s[threadIdx.x] += ah;
-    __syncthreads();
+  // inclusive scan: inside each wave by shuffles, then the wave totals
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  int iw = sw, ih = sh;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int tw = __shfl_up(iw, d), th = __shfl_up(ih, d);
+    if (lane >= d) {
+      iw += tw;
+      ih += th;
+    }
   }
-  int ow = sw_s[threadIdx.x] - sw, oh = sh_s[threadIdx.x] - sh;
+  if (lane == 63) {
+    sw_s[wave] = iw;
+    sh_s[wave] = ih;
+  }
+  __syncthreads();
+  int bw = 0, bh = 0, tw = 0, th = 0;
+#pragma unroll
+  for (int k = 0; k < kThreads / 64; ++k) {
+    if (k < wave) {
+      bw += sw_s[k];
+      bh += sh_s[k];
+    }
+    tw += sw_s[k];
+    th += sh_s[k];
+  }
+  int ow = bw + iw - sw, oh = bh + ih - sh;
   for (int b = b0; b < b1; ++b) {
-    const int v = A.w.blkcnt[b];
+    const int v = cnt[b];
     A.w.blkoff[b] = make_int2(ow, oh);
     ow += v & 0xFFFF;
     oh += v >> 16;
   }
-  if (threadIdx.x == kThreads - 1) {
-    A.w.dyn->tot[0] = sw_s[kThreads - 1];
-    A.w.dyn->tot[1] = sh_s[kThreads - 1];
+  if (threadIdx.x == 0) {
+    A.w.dyn->tot[0] = tw;
+    A.w.dyn->tot[1] = th;
   }
 }
 
@@ -400,7 +430,7 @@ __global__ __launch_bounds__(kThreads) void disk_count_kernel(tdm_disk_params P,
     if (packed) atomicAdd(&bsum, packed);
   }
   __syncthreads();
-  if (threadIdx.x == 0) A.w.blkcnt[lb] = bsum;
+  if (threadIdx.x == 0) A.w.blkcnt[(size_t)A.spar * P.nblocks + lb] = bsum;
 }
 
 __global__ __launch_bounds__(kThreads) void disk_prep_kernel(tdm_disk_params P, DiskArgs A)
@@ -410,6 +440,11 @@ __global__ __launch_bounds__(kThreads) void disk_prep_kernel(tdm_disk_params P, 
   const int p = blockIdx.x * kThreads + threadIdx.x;
   const bool rebuild = dyn->flag[A.par] != 0;
   const int cellp = dyn->cpar[A.par];
+  if (P.n_walls + P.n_hull) {
+    // the step kernel of this launch adds the next positions' counts here
+    int *next = A.w.blkcnt + (size_t)(A.spar ^ 1) * P.nblocks;
+    for (int b = p; b < P.nblocks; b += gridDim.x * kThreads) next[b] = 0;
+  }
   if (blockIdx.x == 0) {
     reduce_pending(P, A, red);
     if (P.n_walls + P.n_hull) {
@@ -445,7 +480,6 @@ __global__ __launch_bounds__(kThreads) void disk_step_kernel(tdm_disk_params P, 
 {
   __shared__ double red[4][kCh];
   __shared__ int scan[kThreads];
-  __shared__ int bsum;
   DiskDyn *dyn = A.w.dyn;
   const DiskWs &W = A.w;
   const int N = P.n, S = P.cand_slots;
@@ -490,9 +524,14 @@ __global__ __launch_bounds__(kThreads) void disk_step_kernel(tdm_disk_params P, 
     if (bounded) packed = boundary_hits(P, A, xp.x, xp.y, rp, smask);
   }
   // boundary-contact ranks: the block's offsets, then the prefix inside it
-  // (blocks without boundary contacts skip the scan)
+  // (blocks without boundary contacts, known from the offsets, skip it)
   int rank_w0 = 0, rank_h0 = 0;
-  if (bounded && __syncthreads_or(packed != 0)) {
+  int2 boff = make_int2(0, 0), bnext = boff;
+  if (bounded) {
+    boff = W.blkoff[lb];
+    bnext = lb + 1 < (int)gridDim.x ? W.blkoff[lb + 1] : make_int2(dyn->tot[0], dyn->tot[1]);
+  }
+  if (bounded && (bnext.x - boff.x) + (bnext.y - boff.y) > 0) {
     scan[threadIdx.x] = packed;
     __syncthreads();
     for (int d = 1; d < kThreads; d <<= 1) {
@@ -502,7 +541,6 @@ __global__ __launch_bounds__(kThreads) void disk_step_kernel(tdm_disk_params P, 
       __syncthreads();
     }
     const int excl = scan[threadIdx.x] - packed;
-    const int2 boff = W.blkoff[lb];
     rank_w0 = boff.x + (excl & 0xFFFF);
     rank_h0 = dyn->tot[0] + boff.y + (excl >> 16);
   }
@@ -817,13 +855,14 @@ __global__ __launch_bounds__(kThreads) void disk_step_kernel(tdm_disk_params P, 
       if (bounded) packed_new = boundary_counts(P, A, xs2[0], xs2[1], rp);
     }
   }
-  // boundary contacts at the new positions, for the next launch's ranks
-  if (kAdvance && bounded) {
-    if (threadIdx.x == 0) bsum = 0;
-    __syncthreads();
-    if (packed_new) atomicAdd(&bsum, packed_new);
-    __syncthreads();
-    if (threadIdx.x == 0) W.blkcnt[lb] = bsum;
+  // boundary contacts at the new positions, for the next launch's ranks:
+  // a wave with any adds its total to the block's count (cleared by the prep
+  // kernel of this launch)
+  if (kAdvance && bounded && __ballot(packed_new != 0)) {
+    int v = packed_new;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    if ((threadIdx.x & 63) == 0) atomicAdd(&W.blkcnt[(size_t)(A.spar ^ 1) * P.nblocks + lb], v);
   }
 
   // ---- ledger partials, reduced by the next launch (or the flush) --------
@@ -971,7 +1010,7 @@ size_t tdm_disk_workspace_bytes(int n, int ncells, int cell_cap, int cand_slots,
   return total;
 }
 
-int tdm_disk_setup(tdm_disk_arrays *a, const tdm_disk_params *p, void *stream)
+int tdm_disk_setup(tdm_disk_arrays *a, tdm_disk_params *p, void *stream)
 {
   DiskWs w;
   int rc = check_disk(a, p, &w);
@@ -982,10 +1021,18 @@ int tdm_disk_setup(tdm_disk_arrays *a, const tdm_disk_params *p, void *stream)
   fill_kernel<unsigned><<<256, 256, 0, s>>>(w.cmask, 2 * (size_t)p->n, 0u);
   fill_kernel<int><<<256, 256, 0, s>>>(w.ccount, 2 * ncells, 0);
   disk_init_kernel<<<1, 64, 0, s>>>(w.dyn);
-  if (p->n_walls + p->n_hull > 0)
-    disk_segbox_kernel<<<(p->n_walls + p->n_hull + 63) / 64, 64, 0, s>>>(
-        *p, disk_args(a, w, 0, 0, false));
+  const int nseg = p->n_walls + p->n_hull;
+  if (nseg > 0)
+    disk_segbox_kernel<<<(nseg + 63) / 64, 64, 0, s>>>(*p, disk_args(a, w, 0, 0, false));
   HIP_TRY(hipGetLastError());
+  // the first boxes travel with every launch's parameters
+  for (int k = 0; k < kParamSegs; ++k)
+    for (int c = 0; c < 4; ++c) p->seg_box[k][c] = 0.0;
+  if (nseg > 0) {
+    HIP_TRY(hipMemcpyAsync(p->seg_box, w.segbox, sizeof(double) * 4 * min(nseg, kParamSegs),
+                           hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+  }
   a->launches = 0;
   a->steps = 0;
   return TDM_OK;

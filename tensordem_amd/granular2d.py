@@ -167,7 +167,7 @@ class DiskParams(ctypes.Structure):
         ("use_gravity", ctypes.c_int32), ("use_decay", ctypes.c_int32), ("n", ctypes.c_int32),
         ("n_walls", ctypes.c_int32), ("n_hull", ctypes.c_int32),
         ("cand_slots", ctypes.c_int32), ("cell_cap", ctypes.c_int32),
-        ("nblocks", ctypes.c_int32),
+        ("nblocks", ctypes.c_int32), ("seg_box", (ctypes.c_double * 4) * 8),
     ]
 
 
@@ -329,32 +329,35 @@ class DiskSystem:
     # ---- set-up helpers --------------------------------------------------------
     def _grid(self, pos: np.ndarray) -> None:
         """Cell grid of edge >= reach: the period on a periodic axis, otherwise
-        the box around the disks and boundaries with room to move (disks that
-        leave it are clamped into the edge cells, which stays correct)."""
+        the box around the disks with room to move (segments are not binned;
+        disks that leave the box are clamped into its edge cells, which stays
+        correct).  At most a few cells per disk in total."""
         p = self._params
-        pts = [pos]
-        for segs in (self.walls, self.hull):
-            if len(segs):
-                pts.append(segs.reshape(-1, 2))
-        pts = np.concatenate(pts)
-        lo, hi = pts.min(0), pts.max(0)
+        lo, hi = pos.min(0), pos.max(0)
         span = float(max(hi - lo)) + p.reach
         n = len(pos)
         h0 = p.reach * (1.0 + 1e-9)
+        ext, org = [0.0, 0.0], [0.0, 0.0]
         for a, period in enumerate((self.periodic_x, self.periodic_y)):
             p.period[a] = period or 0.0
             if period is not None:
-                nc = max(1, min(int(math.floor(period / h0)), 1 << 15))
-                p.ncell[a], p.cell_h[a], p.grid_origin[a] = nc, period / nc, 0.0
+                ext[a], org[a] = period, 0.0
             else:
-                o = float(lo[a]) - 0.25 * span - 2.0 * h0
-                ext = float(hi[a]) + 0.25 * span + 2.0 * h0 - o
-                h = h0
-                # keep the table near a few cells per disk
-                limit = max(1 << 12, 8 * n)
-                while math.ceil(ext / h) > limit or math.ceil(ext / h) > (1 << 15):
-                    h *= 2.0
-                p.ncell[a], p.cell_h[a], p.grid_origin[a] = int(math.ceil(ext / h)) + 1, h, o
+                org[a] = float(lo[a]) - 0.25 * span - 2.0 * h0
+                ext[a] = float(hi[a]) + 0.25 * span + 2.0 * h0 - org[a]
+        cap = max(1 << 12, 4 * n)
+        scale = 1.0
+        while True:
+            nc = [max(1, int(math.floor(ext[a] / (h0 * scale)))) if p.period[a] > 0 else
+                  int(math.ceil(ext[a] / (h0 * scale))) + 1 for a in range(2)]
+            if nc[0] * nc[1] <= cap:
+                break
+            scale *= 1.25
+        for a in range(2):
+            if p.period[a] > 0:
+                p.ncell[a], p.cell_h[a], p.grid_origin[a] = nc[a], ext[a] / nc[a], 0.0
+            else:
+                p.ncell[a], p.cell_h[a], p.grid_origin[a] = nc[a], h0 * scale, org[a]
 
     @property
     def _stream(self) -> int:
