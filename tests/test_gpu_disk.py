@@ -232,3 +232,36 @@ def test_crowded_disks_with_many_contacts_match_oracle():
         assert np.array_equal(got[k], v), k
     per_disk = np.bincount(np.r_[c["i"], c["j"][c["j"] >= 0]], minlength=len(pos))
     assert per_disk.max() >= 6
+
+
+def test_doubly_periodic_wrap_and_cell_area_match_reference_case():
+    # tests/test_granular2d_energy_minimization.py:231-241: two large disks in
+    # a 2 x 2 cell drifting at (10, -7) stay inside [0, 2) on both axes
+    pos = np.array([[0.5, 0.5], [1.5, 1.5]])
+    cfg = DiskConfig(kn=1.0, mu=0.0, density=1.0, thickness=1.0)
+    dev, ref = _pair(cfg, pos, np.array([0.2, 0.2]), periodic_x=2.0, periodic_y=2.0)
+    assert dev.domain_area() == pytest.approx(4.0)
+    assert dev.n_periodic_axes == 2
+    dev.set_uniform_velocity((10.0, -7.0))
+    ref.vel[:] = (10.0, -7.0)
+    _advance_both(dev, ref, 3, 1)
+    p = dev.pos.cpu().numpy()
+    assert np.all((p >= 0.0) & (p < 2.0))
+
+
+def test_periodic_shear_channel_with_moving_walls_matches_oracle():
+    # periodic along x, floor and lid walls sliding in opposite directions:
+    # the minimum image, the boundary springs and the wall velocities at once
+    pos, r, w, h = _packing(600, 13, aspect=4.0)
+    cfg = DiskConfig(mu=0.5, mu_rolling=0.05, zeta=0.3, skin=0.002)
+    walls = box_walls(w, h, left=False, right=False)
+    dev, ref = _pair(cfg, pos, r, walls=walls, periodic_x=w,
+                     wall_velocity=[[-0.3, 0.0], [0.3, 0.0]])
+    dev.enable_energy_ledger()
+    ref.enable_energy_ledger()
+    _advance_both(dev, ref, 4, 250)
+    _ledgers_agree(dev, ref)
+    assert ref.last[11] > 0                                   # wall contacts
+    assert np.allclose(dev.wall_force, ref.wall_force, rtol=1e-9, atol=1e-12)
+    p = dev.pos.cpu().numpy()
+    assert np.all((p[:, 0] >= 0.0) & (p[:, 0] < w))
