@@ -370,7 +370,7 @@ __device__ void scan_blocks(const tdm_disk_params &P, const DiskArgs &A)
   const int b0 = min(nb, (int)threadIdx.x * per), b1 = min(nb, b0 + per);
   const int *cnt = A.w.blkcnt + (size_t)A.spar * nb;
   int sw = 0, sh = 0;
-#pragma unroll 8
+#pragma unroll 16
   for (int b = b0; b < b1; ++b) {
     const int v = cnt[b];
     sw += v & 0xFFFF;
@@ -403,6 +403,7 @@ __device__ void scan_blocks(const tdm_disk_params &P, const DiskArgs &A)
     th += sh_s[k];
   }
   int ow = bw + iw - sw, oh = bh + ih - sh;
+#pragma unroll 16
   for (int b = b0; b < b1; ++b) {
     const int v = cnt[b];
     A.w.blkoff[b] = make_int2(ow, oh);
