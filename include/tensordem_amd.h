@@ -359,6 +359,13 @@ size_t tdm_disk_workspace_bytes(int n, int ncells, int cell_cap, int cand_slots,
 int tdm_disk_setup(tdm_disk_arrays *a, tdm_disk_params *p, void *stream);
 /* Schedules a candidate search (after the caller moved disks). */
 int tdm_disk_rebin(tdm_disk_arrays *a, const tdm_disk_params *p, void *stream);
+/* Re-derives the segment bounding boxes (device table and p->seg_box) after
+ * the caller moved wall or hull segments in place; contact state and lists
+ * are kept.  The reference reads DiskSystem.walls / hull afresh at every
+ * evaluation (granular2d/engine.py:352-375, 411-412), so callers move plates
+ * and gates between steps (granular_identifiability/targets.py:241-246,
+ * segregation/bed2d.py:170-174).  Waits for the boxes on `stream`. */
+int tdm_disk_segments(tdm_disk_arrays *a, tdm_disk_params *p, void *stream);
 /* nsteps steps; state alternates x/v/omega <-> x_alt/v_alt/omega_alt from
  * parity 0.  flags: TDM_DISK_LEDGER accumulates the energy ledger,
  * TDM_DISK_RECORD writes per-contact records on the last step.  Replaces

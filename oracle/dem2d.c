@@ -629,6 +629,25 @@ void orc_mech_energy(const double *prm, int n, const double *x, const double *v,
   free(rowptr); free(col); free(cp);
 }
 
+/* polyline_contact_forces (hull_contact.py:7-75) on its own: per-particle
+ * force (n,2) from the nearest point of the polyline `vert` (m vertices, in
+ * global position) moving with `hv`, and the hull reaction (2) = -sum of
+ * the forces.  The plate step calls the same per-particle law. */
+void orc_hull_forces(int n, const double *x, const double *v, const double *vert, int m,
+                     const double *hv, double r_p, double r_h, double k, double c,
+                     double *force, double *reaction)
+{
+  double sx = 0.0, sy = 0.0;
+  for (int p = 0; p < n; ++p) {
+    hull_particle(x + 2 * p, v + 2 * p, vert, m, hv, r_p, r_h, k, c, force + 2 * p,
+                  force + 2 * p + 1);
+    sx += force[2 * p];
+    sy += force[2 * p + 1];
+  }
+  reaction[0] = -sx;
+  reaction[1] = -sy;
+}
+
 /* Brute-force list of all pairs (i<j) closer than `cutoff`: the pair-set
  * oracle for the cell hash (O(n^2), small n only). */
 int orc_pairs_within(int n, const double *x, double cutoff, int32_t *out_ij, int cap)

@@ -59,6 +59,9 @@ def library() -> ctypes.CDLL:
     lib.orc_mech_energy.restype = None
     lib.orc_pairs_within.argtypes = [c_int, d, ctypes.c_double, i32, c_int]
     lib.orc_pairs_within.restype = c_int
+    c_dbl = ctypes.c_double
+    lib.orc_hull_forces.argtypes = [c_int, d, d, d, c_int, d, c_dbl, c_dbl, c_dbl, c_dbl, d, d]
+    lib.orc_hull_forces.restype = None
     _LIB = lib
     return lib
 
@@ -269,3 +272,20 @@ def pairs_within(x: np.ndarray, cutoff: float) -> np.ndarray:
         if cnt <= cap:
             return out[:cnt].copy()
         cap = cnt
+
+
+def hull_forces(positions, velocities, vertices, hull_velocity, *, particle_radius,
+                hull_radius, stiffness, damping=0.0):
+    """The oracle's polyline hull law on its own (hull_contact.py:7-75):
+    per-particle forces (N,2) and the hull reaction (2,)."""
+    x = np.ascontiguousarray(positions, dtype=np.float64).reshape(-1, 2)
+    v = np.ascontiguousarray(velocities, dtype=np.float64).reshape(-1, 2)
+    vert = np.ascontiguousarray(vertices, dtype=np.float64).reshape(-1, 2)
+    hv = np.ascontiguousarray(hull_velocity, dtype=np.float64).reshape(2)
+    force = np.zeros_like(x)
+    reaction = np.zeros(2)
+    d = ctypes.c_double
+    library().orc_hull_forces(len(x), _p(x, d), _p(v, d), _p(vert, d), len(vert), _p(hv, d),
+                              float(particle_radius), float(hull_radius), float(stiffness),
+                              float(damping), _p(force, d), _p(reaction, d))
+    return force, reaction

@@ -136,6 +136,7 @@ _SIGNATURES = {
     "tdm_disk_workspace_bytes": ([ctypes.c_int] * 6, ctypes.c_size_t),
     "tdm_disk_setup": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
     "tdm_disk_rebin": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
+    "tdm_disk_segments": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
     "tdm_disk_step": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
                        ctypes.c_void_p], ctypes.c_int),
     "tdm_disk_inspect": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),

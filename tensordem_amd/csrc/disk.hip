@@ -985,6 +985,26 @@ int launch_disk_steps(const tdm_disk_arrays *a, const tdm_disk_params *p, const 
   return TDM_OK;
 }
 
+// Bounding boxes of the wall and hull segments into the workspace, and the
+// first kParamSegs of them into the launch parameters (they travel with every
+// launch); waits for the copy.
+int segment_boxes(const tdm_disk_arrays *a, tdm_disk_params *p, const DiskWs &w,
+                  hipStream_t s)
+{
+  const int nseg = p->n_walls + p->n_hull;
+  if (nseg > 0)
+    disk_segbox_kernel<<<(nseg + 63) / 64, 64, 0, s>>>(*p, disk_args(a, w, 0, 0, false));
+  HIP_TRY(hipGetLastError());
+  for (int k = 0; k < kParamSegs; ++k)
+    for (int c = 0; c < 4; ++c) p->seg_box[k][c] = 0.0;
+  if (nseg > 0) {
+    HIP_TRY(hipMemcpyAsync(p->seg_box, w.segbox, sizeof(double) * 4 * min(nseg, kParamSegs),
+                           hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+  }
+  return TDM_OK;
+}
+
 }  // namespace
 }  // namespace tdm
 
@@ -999,6 +1019,10 @@ int tdm_disk_abi_sizes(size_t *params_bytes, size_t *arrays_bytes)
   *arrays_bytes = sizeof(tdm_disk_arrays);
   return TDM_OK;
 }
+
+}  // extern "C"
+
+extern "C" {
 
 int tdm_disk_blocks(int n) { return n <= 0 ? 0 : (n + kThreads - 1) / kThreads; }
 
@@ -1022,21 +1046,20 @@ int tdm_disk_setup(tdm_disk_arrays *a, tdm_disk_params *p, void *stream)
   fill_kernel<unsigned><<<256, 256, 0, s>>>(w.cmask, 2 * (size_t)p->n, 0u);
   fill_kernel<int><<<256, 256, 0, s>>>(w.ccount, 2 * ncells, 0);
   disk_init_kernel<<<1, 64, 0, s>>>(w.dyn);
-  const int nseg = p->n_walls + p->n_hull;
-  if (nseg > 0)
-    disk_segbox_kernel<<<(nseg + 63) / 64, 64, 0, s>>>(*p, disk_args(a, w, 0, 0, false));
   HIP_TRY(hipGetLastError());
-  // the first boxes travel with every launch's parameters
-  for (int k = 0; k < kParamSegs; ++k)
-    for (int c = 0; c < 4; ++c) p->seg_box[k][c] = 0.0;
-  if (nseg > 0) {
-    HIP_TRY(hipMemcpyAsync(p->seg_box, w.segbox, sizeof(double) * 4 * min(nseg, kParamSegs),
-                           hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipStreamSynchronize(s));
-  }
+  rc = segment_boxes(a, p, w, s);
+  if (rc) return rc;
   a->launches = 0;
   a->steps = 0;
   return TDM_OK;
+}
+
+int tdm_disk_segments(tdm_disk_arrays *a, tdm_disk_params *p, void *stream)
+{
+  DiskWs w;
+  int rc = check_disk(a, p, &w);
+  if (rc) return rc;
+  return segment_boxes(a, p, w, static_cast<hipStream_t>(stream));
 }
 
 int tdm_disk_rebin(tdm_disk_arrays *a, const tdm_disk_params *p, void *stream)

@@ -288,8 +288,12 @@ class DiskSystem:
         self.inertia = torch.tensor(inertia, **f64)
         self._walls_d = torch.tensor(self.walls.reshape(-1, 4), **f64) if len(self.walls) else None
         self._hull_d = torch.tensor(self.hull.reshape(-1, 4), **f64) if len(self.hull) else None
-        self.wall_velocity = wv
+        self._wall_velocity = wv
         self._wv_d = torch.tensor(wv, **f64) if wv is not None else None
+        # what the device holds, to notice in-place edits of walls, hull and
+        # wall velocities between calls (the reference reads them afresh at
+        # every evaluation, engine.py:352-375, 411-412)
+        self._uploaded = self._segment_state()
         self.out = torch.zeros(NOUT, **f64)
         self.records = None
         self._records_on = False
@@ -386,8 +390,70 @@ class DiskSystem:
         self._refresh()
         self._xver = self._x._version
 
+    def _segment_state(self):
+        wv = self._wall_velocity
+        return (self.walls.copy(), self.hull.copy(), None if wv is None else wv.copy())
+
+    @property
+    def wall_velocity(self):
+        """(n_walls, 2) velocities of the wall segments, or None; assign it or
+        edit it in place between steps."""
+        return self._wall_velocity
+
+    @wall_velocity.setter
+    def wall_velocity(self, value) -> None:
+        if value is None:
+            self._wall_velocity = None
+            return
+        wv = np.array(value, dtype=np.float64).reshape(-1, 2)
+        if wv.shape != (len(self.walls), 2):
+            raise ValueError("wall_velocity must have shape (n_walls, 2)")
+        if not np.isfinite(wv).all():
+            raise ValueError("wall_velocity must be finite")
+        self._wall_velocity = wv
+
+    def _sync_segments(self) -> None:
+        """Upload walls, hull and wall velocities the caller changed since the
+        last call, refresh the segment boxes and drop captured graphs (their
+        launch parameters hold the old boxes)."""
+        walls, hull, wv = self._uploaded
+        cur_wv = self._wall_velocity
+        same_wv = (wv is None and cur_wv is None) or (
+            wv is not None and cur_wv is not None and np.array_equal(wv, cur_wv))
+        if np.array_equal(walls, self.walls) and np.array_equal(hull, self.hull) and same_wv:
+            return
+        new_walls = _as_segments(self.walls, "walls")
+        new_hull = _as_segments(self.hull, "hull")
+        if len(new_walls) != len(walls) or len(new_hull) != len(hull):
+            raise ValueError("the number of wall and hull segments is fixed at construction; "
+                             "move segments in place instead")
+        if not np.isfinite(new_walls).all() or not np.isfinite(new_hull).all():
+            raise ValueError("walls and hull must be finite")
+        if cur_wv is not None:
+            cur_wv = np.asarray(cur_wv, dtype=np.float64).reshape(-1, 2)
+            if cur_wv.shape != (len(new_walls), 2) or not np.isfinite(cur_wv).all():
+                raise ValueError("wall_velocity must have shape (n_walls, 2) and be finite")
+        f64 = dict(dtype=torch.float64, device=self.device)
+        if self._walls_d is not None:
+            self._walls_d.copy_(torch.as_tensor(new_walls.reshape(-1, 4)))
+        if self._hull_d is not None:
+            self._hull_d.copy_(torch.as_tensor(new_hull.reshape(-1, 4)))
+        if cur_wv is None:
+            self._wv_d = None
+        elif self._wv_d is None:
+            self._wv_d = torch.tensor(cur_wv, **f64)
+        else:
+            self._wv_d.copy_(torch.as_tensor(cur_wv))
+        self._refresh()
+        _lib.check(self.lib.tdm_disk_segments(ctypes.byref(self._arrays),
+                                              ctypes.byref(self._params), self._stream))
+        self.close()
+        self._uploaded = self._segment_state()
+
     def _sync_edits(self) -> None:
-        """A caller that moved disks gets a fresh candidate search."""
+        """A caller that moved disks gets a fresh candidate search; moved or
+        re-timed segments are uploaded."""
+        self._sync_segments()
         if self._x._version != self._xver:
             _lib.check(self.lib.tdm_disk_rebin(ctypes.byref(self._arrays),
                                                ctypes.byref(self._params), self._stream))
@@ -591,8 +657,8 @@ class DiskSystem:
         return self._graphs[key]
 
     def run_graph(self, pairs: int, repeats: int = 1) -> None:
-        g = self.graph(pairs)
         self._sync_edits()
+        g = self.graph(pairs)
         for _ in range(repeats):
             _lib.check(self.lib.tdm_graph_launch(g, self._stream))
             # the library's launch and step counts, which the graph baked in
@@ -603,6 +669,7 @@ class DiskSystem:
         self.steps += 2 * pairs * repeats
         self._contacts = None
         self._records_valid = False  # captured steps write no records
+        self._check_status()  # the replays' flush wrote the status word
 
     def step_timed(self, nsteps: int) -> tuple[float, float]:
         """nsteps (even) steps with each kernel timed by HIP events; returns the
@@ -653,11 +720,16 @@ class DiskSystem:
     @property
     def contacts(self) -> dict:
         """Records of the last evaluation (the last step of the last `step`
-        call); where none were written (before records were first requested,
-        after graph replays), an evaluation of the current state."""
+        call).  Where none were written (before records were first requested,
+        after graph replays) they describe the current state, evaluated without
+        touching the energies, wall and hull forces or counts of the last step;
+        from then on every step writes records, as the reference's does."""
         if self._contacts is None:
             if not self._records_valid:
-                return self.contact_forces()
+                saved = self.out.clone()
+                rec = self.contact_forces()
+                self.out.copy_(saved)
+                return rec
             self._contacts = self._compact_records()
         return self._contacts
 

@@ -18,6 +18,12 @@ device; `step()` and `forces()` are single kernel launches
   threshold are re-checked in FP64).
 * `dt_policy="degree"` bounds dt by 8 partners per disk instead of all N-1
   (the reference bound shrinks like 1/N, dem.py:61-72).
+* `DEMConfig(device="cpu")`, the reference's default, runs on the ROCm
+  device as well (`self.device` is the ROCm device; `config` is kept as
+  given, so restarts carry the caller's setting).
+* `snapshot()` writes the reference's all-pairs `tensordem.restart/1` up to
+  `restart.MAX_REFERENCE_PAIRS` pairs and a bond-indexed layout under
+  `tensordem_amd.restart_bonds/1` above it; `from_snapshot` reads both.
 """
 from __future__ import annotations
 
@@ -29,7 +35,7 @@ from dataclasses import asdict
 import numpy as np
 import torch
 
-from . import _lib
+from . import _lib, restart
 from .config import DEMConfig
 from .engine import PlateEngine, PlateSpec
 
@@ -62,11 +68,12 @@ class IceDEM:
         self.device = torch.device(config.device)
         if self.device.type not in ("cpu", "cuda"):
             raise ValueError("Only CPU and CUDA devices are supported")
-        if self.device.type != "cuda":
-            raise ValueError("tensordem_amd.IceDEM runs on the ROCm device: use "
-                             "DEMConfig(device='cuda')")
         if not torch.cuda.is_available():
-            raise ValueError("CUDA requested but unavailable")
+            raise ValueError("tensordem_amd.IceDEM runs on the ROCm device, which is unavailable")
+        if self.device.type == "cpu":
+            # the reference's default DEMConfig(device="cpu") runs here too:
+            # the solver only exists on the ROCm device
+            self.device = torch.device("cuda")
         if precision not in ("fp64", "fp32"):
             raise ValueError("precision must be 'fp64' or 'fp32'")
         if dt_policy not in ("all_pairs", "degree"):
@@ -418,18 +425,44 @@ class IceDEM:
         return result
 
     # ---- restart (dem.py:486-601) ------------------------------------------------
-    def snapshot(self) -> dict:
-        """Complete JSON restart; derived lattice topology is rebuilt and checked."""
+    def snapshot(self, layout: str | None = None) -> dict:
+        """Complete JSON restart; derived lattice topology is rebuilt and checked.
+
+        layout "reference" writes the reference's `tensordem.restart/1`
+        (alive and failure records over all N(N-1)/2 pairs, dem.py:486-601),
+        which the reference loads; "bonds" writes them per bond under
+        `tensordem_amd.restart_bonds/1`.  The default is "reference" while
+        the pair arrays stay within restart.MAX_REFERENCE_PAIRS, else "bonds".
+        """
+        n = self._eng.n
+        if layout is None:
+            layout = "reference" if restart.pair_count(n) <= restart.MAX_REFERENCE_PAIRS \
+                else "bonds"
+        if layout not in ("reference", "bonds"):
+            raise ValueError("layout must be 'reference' or 'bonds'")
+        per_bond = dict(alive=self.alive.cpu().numpy(),
+                        failure_mode=self.failure_mode.cpu().numpy(),
+                        failure_time_s=self.failure_time_s.cpu().numpy(),
+                        failure_extension_m=self.failure_extension_m.cpu().numpy(),
+                        failure_energy_J=self.failure_energy_J.cpu().numpy())
+        if layout == "reference":
+            e = self._eng
+            fields = restart.bonds_to_pairs(per_bond, n, e.bond_i.cpu().numpy(),
+                                            e.bond_j.cpu().numpy())
+            schema = restart.REFERENCE_SCHEMA
+        else:
+            fields = {k: v.tolist() for k, v in per_bond.items()}
+            schema = restart.BOND_SCHEMA
         cfg = asdict(self.config)
-        return dict(schema="tensordem.restart/1", config=cfg,
+        return dict(schema=schema, config=cfg,
                     dt_s=self.dt, time_s=self.time, step_count=self.step_count,
                     positions_m=self.positions.tolist(), velocities_m_s=self.velocities.tolist(),
                     initial_positions_m=self.initial_positions.tolist(), fixed=self.fixed.tolist(),
-                    alive=self.alive.tolist(),
-                    failure_mode=self.failure_mode.tolist(),
-                    failure_time_s=self.failure_time_s.tolist(),
-                    failure_extension_m=self.failure_extension_m.tolist(),
-                    failure_energy_J=self.failure_energy_J.tolist(),
+                    alive=fields["alive"],
+                    failure_mode=fields["failure_mode"],
+                    failure_time_s=fields["failure_time_s"],
+                    failure_extension_m=fields["failure_extension_m"],
+                    failure_energy_J=fields["failure_energy_J"],
                     tool_start_m=self.tool_start.tolist(),
                     tool_velocity_m_s=self.tool_velocity.tolist(),
                     hull_profile_m=None if self.hull_profile is None else self.hull_profile.tolist(),
@@ -439,8 +472,12 @@ class IceDEM:
 
     @classmethod
     def from_snapshot(cls, state: dict, *, device: str | None = None, **kwargs) -> "IceDEM":
-        """Fail closed on missing fields, incompatible topology or nonfinite data."""
-        if not isinstance(state, dict) or state.get("schema") != "tensordem.restart/1":
+        """Fail closed on missing fields, incompatible topology or nonfinite data.
+
+        Reads both the reference's all-pairs `tensordem.restart/1` and the
+        bond-indexed `tensordem_amd.restart_bonds/1` (see `snapshot`)."""
+        if not isinstance(state, dict) or state.get("schema") not in (restart.REFERENCE_SCHEMA,
+                                                                      restart.BOND_SCHEMA):
             raise ValueError("unsupported DEM restart schema")
         required = {"schema", "config", "dt_s", "time_s", "step_count", "positions_m",
                     "velocities_m_s", "initial_positions_m", "fixed", "alive", "failure_mode",
@@ -465,6 +502,11 @@ class IceDEM:
         if abs(state["time_s"] - count * result.dt) > 1e-10 * max(1.0, state["time_s"]):
             raise ValueError("restart time inconsistent with step count")
         dev = result.device
+        if state["schema"] == restart.REFERENCE_SCHEMA:
+            e = result._eng
+            state = dict(state)
+            state.update(restart.pairs_to_bonds(state, e.n, e.bond_i.cpu().numpy(),
+                                                e.bond_j.cpu().numpy()))
         for key, attribute in (("positions_m", "positions"), ("velocities_m_s", "velocities"),
                                ("initial_positions_m", "initial_positions"),
                                ("tool_start_m", "tool_start"), ("tool_velocity_m_s", "tool_velocity")):

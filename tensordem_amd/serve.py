@@ -225,7 +225,7 @@ class _Batch:
     steps each has done.  All device work of a round is queued on the batch
     stream; a new engine is built on a side stream meanwhile."""
 
-    def __init__(self, kind: str, fp32: bool, device, keep_state: bool):
+    def __init__(self, kind: str, fp32: bool, device, keep_state):
         self.kind, self.fp32, self.device = kind, fp32, torch.device(device)
         self.keep_state = keep_state
         self.engine = None
@@ -287,12 +287,14 @@ class _Batch:
                 x = eng.x[m.first:m.first + n]
                 vals = [x.sum().reshape(1), eng.accounting[k]]
                 extra = {}
+                keep = self.keep_state is True or (
+                    not isinstance(self.keep_state, bool) and r.rid in self.keep_state)
                 if self.kind == "plate":
                     alive = eng.alive[m.bond_first:m.bond_first + m.nbonds]
                     vals.insert(1, (~alive).sum().to(torch.float64).reshape(1))
-                    if self.keep_state:
+                    if keep:
                         extra["alive"] = alive
-                if self.keep_state:
+                if keep:
                     extra["x"], extra["v"] = x, eng.v[m.first:m.first + n]
                 dev_parts = {"scalars": torch.cat(vals),
                              "member": eng.members.view(-1, sz)[k].clone(), **extra}
@@ -384,11 +386,13 @@ class Server:
 
     hbm_budget: bytes the admitted requests' estimated footprints may use;
     max_active: members per batch; chunk: steps between scheduling points
-    (every request's step count must be a multiple of it).  The host runs at
-    most one round ahead of the device."""
+    (every request's step count must be a multiple of it); keep_state: True
+    copies every finished request's positions, velocities (and alive mask)
+    into its result, a set of request ids only those requests'.  The host
+    runs at most one round ahead of the device."""
 
     def __init__(self, *, hbm_budget: float = 288e9, max_active: int = 64, chunk: int = 250,
-                 fp32: bool = True, device="cuda", keep_state: bool = False):
+                 fp32: bool = True, device="cuda", keep_state=False):
         if chunk <= 0 or chunk % 2:
             raise ValueError("chunk must be a positive even number of steps")
         self.hbm_budget, self.max_active, self.chunk = float(hbm_budget), int(max_active), chunk

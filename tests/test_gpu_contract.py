@@ -157,3 +157,35 @@ def test_energy_drift_of_conservative_lattice():
         m.step()
         worst = max(worst, abs(float(m.mechanical_energy()["mechanical_J"]) - e0))
     assert worst / e0 <= ACCEPT
+
+
+def test_restart_layouts_with_broken_bonds():
+    """The 7x3 indentation run to step 200 (bonds broken): the reference
+    layout (all N(N-1)/2 pairs) and the bond layout restore the same state,
+    and both continue bit for bit; a reference-default DEMConfig (device
+    'cpu') runs on the ROCm device."""
+    cfg = DEMConfig(nx=7, ny=3, tool_speed=0.2)
+    assert cfg.device == "cpu"
+    m = IceDEM(cfg)
+    assert m.device.type == "cuda"
+    for _ in range(200):
+        m.step()
+    assert m.broken_bonds > 0
+    n = 21
+    ref = m.snapshot()
+    assert ref["schema"] == "tensordem.restart/1"
+    assert len(ref["alive"]) == len(ref["failure_time_s"]) == n * (n - 1) // 2
+    assert sum(not a for a in ref["alive"]) == n * (n - 1) // 2 - m.alive.numel() + m.broken_bonds
+    bonds = m.snapshot(layout="bonds")
+    assert bonds["schema"] == "tensordem_amd.restart_bonds/1"
+    assert len(bonds["alive"]) == m.alive.numel()
+    clones = [IceDEM.from_snapshot(ref), IceDEM.from_snapshot(bonds)]
+    for c in clones:
+        assert torch.equal(c.alive, m.alive) and torch.equal(c.failure_mode, m.failure_mode)
+        assert torch.equal(c.failure_time_s, m.failure_time_s)
+    for _ in range(100):
+        m.step()
+        for c in clones:
+            c.step()
+    for c in clones:
+        assert torch.equal(c.positions, m.positions) and torch.equal(c.alive, m.alive)

@@ -265,3 +265,93 @@ def test_periodic_shear_channel_with_moving_walls_matches_oracle():
     assert np.allclose(dev.wall_force, ref.wall_force, rtol=1e-9, atol=1e-12)
     p = dev.pos.cpu().numpy()
     assert np.all((p[:, 0] >= 0.0) & (p[:, 0] < w))
+
+
+def test_walls_moved_in_place_and_wall_velocity_set_later_match_oracle():
+    # granular_identifiability/targets.py:241-246: a plate is translated by
+    # editing DiskSystem.walls in place between steps, after wall_velocity
+    # was assigned to the constructed system
+    pos, r, w, h = _packing(300, 21)
+    cfg = DiskConfig(mu=0.5, mu_rolling=0.05, zeta=0.2, gravity=(0.0, -9.81), skin=0.002)
+    walls = box_walls(w, h + 0.004)
+    dev = DiskSystem(pos, r, cfg, walls=walls.copy())
+    ref = OracleDisk(pos, r, cfg, walls=walls.copy())
+    speed = 0.2
+    wv = np.zeros((len(walls), 2))
+    wv[1] = (0.0, -speed)
+    dev.wall_velocity = wv
+    ref.wall_velocity = wv.copy()
+    for k in range(6):
+        for _ in range(40):
+            dev.walls[1, :, 1] -= speed * dev.dt
+            ref.walls[1, :, 1] -= speed * ref.dt
+            dev.step(1)
+            ref.step(1)
+        _same_state(dev, ref, f"lid lowered for {(k + 1) * 40} steps")
+    assert ref.last[11] > 0                                    # the lid touches disks
+    assert np.allclose(dev.wall_force, ref.wall_force, rtol=1e-9, atol=1e-12)
+    # graph replays pick the moved lid up too (a cached graph holds old boxes)
+    g0 = dev.wall_force.copy()
+    dev.run_graph(10)
+    for _ in range(20):
+        ref.step(1)
+    dev.walls[1, :, 1] -= 20 * speed * dev.dt
+    ref.walls[1, :, 1] -= 20 * speed * ref.dt
+    dev.run_graph(10)
+    for _ in range(20):
+        ref.step(1)
+    _same_state(dev, ref, "graph replays around a lid move")
+    # the velocity of a wall, edited in place
+    dev.wall_velocity[1] = (0.0, 0.0)
+    ref.wall_velocity[1] = (0.0, 0.0)
+    dev.step(30)
+    ref.step(30)
+    _same_state(dev, ref, "lid stopped")
+    assert not np.array_equal(g0, dev.wall_force)
+    with pytest.raises(ValueError, match="fixed at construction"):
+        dev.walls = dev.walls[:2].copy()
+        dev.step(1)
+
+
+def test_boundary_spring_rank_passes_on_like_the_oracle():
+    # two disks on a floor; the first is lifted off and the second inherits
+    # the spring of rank 0 (engine.py:432-456)
+    r, dt = 0.01, 1e-5
+    cfg = DiskConfig(kn=3.0e4, zeta=0.0, mu=10.0, mu_rolling=0.0, dt=dt)
+    floor = np.array([[[-1.0, 0.0], [1.0, 0.0]]])
+    pos = np.array([[-0.2, r - 1e-4], [0.2, r - 2e-4]])
+    dev, ref = _pair(cfg, pos, np.array([r, r]), walls=floor)
+    v = np.array([[0.3, 0.0], [-0.1, 0.0]])
+    dev.vel = v
+    ref.vel[:] = v
+    dev.step(1)
+    ref.step(1)
+    _same_state(dev, ref, "both on the floor")
+    lifted = np.array([-0.2, 0.5])
+    p = dev.pos.cpu().numpy()
+    p[0] = lifted
+    dev.pos = p
+    ref.pos[0] = lifted
+    dev.step(3)
+    ref.step(3)
+    _same_state(dev, ref, "after the lift-off")
+    got, want = dev.contacts, ref.contacts
+    for k, val in want.items():
+        assert np.array_equal(got[k], val), k
+    assert got["ft"][0] > 0.5 * cfg.kt_ratio * cfg.kn * 0.3 * dt
+
+
+def test_graph_replays_raise_on_candidate_overflow_and_contacts_keep_energies():
+    pos, r, w, h = _packing(400, 5)
+    cfg = DiskConfig(mu=0.4, gravity=(0.0, -9.81), skin=0.002)
+    sysd = DiskSystem(pos, r, cfg, walls=box_walls(w, h), cand_slots=1)
+    with pytest.raises(RuntimeError, match="cand_slots"):
+        sysd.run_graph(2)
+    ok = DiskSystem(pos, r, cfg, walls=box_walls(w, h))
+    ok.step(10)
+    ok.run_graph(5)
+    e0, wf0 = ok.energies(), ok.wall_force.copy()
+    rec = ok.contacts                      # evaluated on the current state
+    assert len(rec["i"]) > 0
+    assert ok.energies() == e0
+    assert np.array_equal(ok.wall_force, wf0)

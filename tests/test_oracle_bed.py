@@ -9,7 +9,7 @@ import math
 import numpy as np
 import pytest
 
-from oracle.bed import OracleBed, BS_NPAIRS
+from oracle.bed import OracleBed, BS_DASH, BS_FRIC, BS_NPAIRS
 from oracle.plate import pairs_within
 from tensordem_amd.bedconfig import BedConfig, column_packing
 
@@ -169,3 +169,84 @@ def test_column_packing_is_overlap_free_and_tall():
     h = x[:, 1].max() - x[:, 1].min()
     w = x[:, 0].max() - x[:, 0].min()
     assert h / w == pytest.approx(2.0, rel=0.1)
+
+
+# --- pair-pair branch: R* = r_i r_j / (r_i + r_j), m* = m_i m_j / (m_i + m_j),
+# forces lo -> hi (scale_ceiling/kernel.py:293-322 restated in 2-D) ----------
+
+def _pair(delta, vi, vj, r=R, cfg=None, dt=1e-7):
+    cfg = cfg or free(friction=0.5)
+    x = np.array([[0.0, 0.0], [2 * r - delta, 0.0]])
+    b = OracleBed(cfg, x, np.array([r, r]), dt=dt)
+    b.velocities[0] = vi
+    b.velocities[1] = vj
+    return cfg, b
+
+
+def test_pair_stick_stiffness_uses_half_radius():
+    """Disk 0 slides along +y against disk 1 for one step: xi = v_t dt and the
+    tangential force on 1 is +s_t xi with s_t = 8 G* sqrt(R* d), R* = r/2."""
+    delta, vt, dt = 2e-5, 1e-3, 1e-7
+    cfg, b = _pair(delta, [0.0, vt], [0.0, 0.0], dt=dt)
+    b.step(integrate=False)
+    s_t = 8 * cfg.g_star * math.sqrt(0.5 * R * delta)
+    assert b.hist_keys.tolist() == [1]          # key i * n + j of the pair (0, 1)
+    assert b.hist_xi[0] == pytest.approx([0.0, vt * dt], rel=1e-15)
+    assert b.force[1, 1] == pytest.approx(s_t * vt * dt, rel=1e-12)
+    assert b.force[0, 1] == -b.force[1, 1]
+    # planted mistakes: the wall's R* = r, or no tangential force on the partner
+    assert b.force[1, 1] != pytest.approx(8 * cfg.g_star * math.sqrt(R * delta) * vt * dt,
+                                          rel=1e-3)
+    assert b.force[1, 1] > 0
+
+
+def test_pair_normal_damping_uses_reduced_mass():
+    """Approaching at v_n: f_n = (2/3) s_n d + C sqrt(s_n m*) v_n with m* = m/2,
+    and the dashpot power (f_n - f_el) v_n = C sqrt(s_n m*) v_n^2 >= 0."""
+    delta, vn = 3e-5, 0.02
+    cfg, b = _pair(delta, [vn, 0.0], [0.0, 0.0], cfg=free())
+    b.step(integrate=False)
+    m = cfg.rho_pi_t * R * R
+    s_n = 2 * cfg.e_star * math.sqrt(0.5 * R * delta)
+    f_el = (2.0 / 3.0) * s_n * delta
+    visc = cfg.damping_coefficient * math.sqrt(s_n * 0.5 * m) * vn
+    assert b.force[1, 0] == pytest.approx(f_el + visc, rel=1e-12)
+    assert b.force[0, 0] == -b.force[1, 0]
+    assert b.last[BS_DASH] == pytest.approx(visc * vn, rel=1e-12) and b.last[BS_DASH] > 0
+    # planted: full mass instead of the reduced mass
+    wrong = f_el + cfg.damping_coefficient * math.sqrt(s_n * m) * vn
+    assert b.force[1, 0] != pytest.approx(wrong, rel=1e-4)
+    # separating: the viscous term lowers f_n, never below zero
+    cfg, b = _pair(delta, [-50.0, 0.0], [0.0, 0.0], cfg=free())
+    b.step(integrate=False)
+    assert b.force[1, 0] == 0.0 and b.force[0, 0] == 0.0
+
+
+def test_pair_coulomb_cap_and_friction_power():
+    """Held at constant overlap and sliding at v_t, the spring grows by v_t dt
+    per step until s_t |xi| = mu f_n; from then on the friction power is
+    mu f_n v_t (energy absorbed, >= 0) and the force on each disk opposes
+    the relative slip."""
+    delta, vt, dt, mu = 2e-5, 0.3, 1e-6, 0.5
+    cfg, b = _pair(delta, [0.0, vt], [0.0, 0.0], cfg=free(friction=mu), dt=dt)
+    s_n = 2 * cfg.e_star * math.sqrt(0.5 * R * delta)
+    f_n = (2.0 / 3.0) * s_n * delta
+    s_t = 8 * cfg.g_star * math.sqrt(0.5 * R * delta)
+    steps_to_cap = mu * f_n / (s_t * vt * dt)
+    assert 2 < steps_to_cap < 200
+    for k in range(int(steps_to_cap) + 3):
+        b.step(integrate=False)
+    assert s_t * abs(b.hist_xi[0, 1]) == pytest.approx(mu * f_n, rel=1e-12)
+    assert b.force[0, 1] == pytest.approx(-mu * f_n, rel=1e-12)   # on the sliding disk
+    assert b.force[1, 1] == pytest.approx(mu * f_n, rel=1e-12)
+    assert b.last[BS_FRIC] == pytest.approx(mu * f_n * vt, rel=1e-12)
+    # unequal radii: R* and m* are the harmonic forms
+    cfg = free(friction=mu)
+    r0, r1 = 0.8 * R, 1.2 * R
+    b = OracleBed(cfg, np.array([[0.0, 0.0], [r0 + r1 - delta, 0.0]]), np.array([r0, r1]),
+                  dt=dt)
+    b.velocities[0] = [0.0, 1e-4]
+    b.step(integrate=False)
+    reff = r0 * r1 / (r0 + r1)
+    assert b.force[1, 1] == pytest.approx(8 * cfg.g_star * math.sqrt(reff * delta) * 1e-4 * dt,
+                                          rel=1e-12)

@@ -17,9 +17,12 @@ reference's own tests and the mathematics fix:
   velocity decay is exp(-dt/T) per step; a periodic pair across the seam
   feels the force of the same pair moved into the interior.
 
-The boundary spring state is carried by rank among the boundary contacts
-(engine.py:452-456); no reference fixture covers a changing boundary-contact
-set, so that rule is pinned only through the trajectories above."""
+* the boundary-spring rank rule (engine.py:432-456: a boundary contact's
+  spring is keyed n*n + rank, ranks over the wall contacts by disk then
+  segment and then the hull contacts): two disks on a floor, the first lifts
+  off, and the second inherits its spring -- the stored value and the
+  record's |f_t| against the hand-derived u dt + v_c dt; and a hull contact
+  of a lower-index disk ranking after a wall contact."""
 import math
 
 import numpy as np
@@ -217,3 +220,64 @@ def test_digest_is_reproducible_and_seed_sensitive():
         return s.digest()
     assert digest(11) == digest(11)
     assert digest(11) != digest(12)
+
+
+# --- boundary springs are keyed by rank among the step's boundary contacts ---
+# (engine.py:432-456: keys n*n + rank, ranks over wall contacts ordered by
+# disk then segment, then hull contacts; the spring of rank k is whatever
+# rank k held at the previous step)
+
+def _two_on_floor(u, w, dt=1e-5):
+    r = 0.01
+    cfg = DiskConfig(kn=3.0e4, zeta=0.0, mu=10.0, mu_rolling=0.0, dt=dt)
+    floor = np.array([[[-1.0, 0.0], [1.0, 0.0]]])
+    s = OracleDisk([[-0.2, r - 1e-4], [0.2, r - 2e-4]], [r, r], cfg, walls=floor)
+    s.vel[0] = [u, 0.0]
+    s.vel[1] = [w, 0.0]
+    return s, cfg, r
+
+
+def test_boundary_spring_passes_to_the_next_contact_of_its_rank():
+    u, w, dt = 0.3, -0.1, 1e-5
+    s, cfg, r = _two_on_floor(u, w, dt)
+    n = 2
+    s.step()
+    # both disks on the floor: rank 0 is disk 0, rank 1 is disk 1
+    assert s.st_key.tolist() == [n * n, n * n + 1]
+    assert s.st_xi[0] == pytest.approx([u * dt, 0.0], rel=1e-15)
+    assert s.st_xi[1] == pytest.approx([w * dt, 0.0], rel=1e-15)
+    # disk 0 leaves the floor; disk 1 becomes rank 0 and inherits disk 0's
+    # spring: xi = u dt + v_c dt, v_c the slip speed of disk 1's contact point
+    # (centre speed plus r omega: the lever to the floor is (0, -r))
+    s.pos[0] = [-0.2, 0.5]
+    k_t = cfg.kt_ratio * cfg.kn
+    omega1 = dt * (r * (-k_t * w * dt)) / s.inertia[1]   # torque of the first step
+    assert s.omega[1] == pytest.approx(omega1, rel=1e-12)
+    v_c = float(s.vel[1, 0]) + r * float(s.omega[1])
+    s.step()
+    assert s.st_key.tolist() == [n * n]
+    expect = u * dt + v_c * dt
+    assert s.st_xi[0, 0] == pytest.approx(expect, rel=1e-13)
+    assert abs(s.st_xi[0, 1]) < 1e-12 * abs(expect)   # normal (0, 1) up to round-off
+    rec = s.contacts
+    assert rec["i"].tolist() == [1] and rec["kind"].tolist() == [1]
+    assert rec["ft"][0] == pytest.approx(k_t * abs(expect), rel=1e-13)
+    # planted: a spring keyed by disk would have kept only disk 1's history
+    assert abs(s.st_xi[0, 0] - (w * dt + v_c * dt)) > 1e-3 * abs(expect)
+
+
+def test_wall_contacts_rank_before_hull_contacts():
+    """Disk 0 touches a hull segment, disk 1 the floor: the wall contact of
+    disk 1 takes rank 0 although disk 0 has the lower index."""
+    r, dt, u, w = 0.01, 1e-5, 0.2, 0.05
+    cfg = DiskConfig(kn=3.0e4, zeta=0.0, mu=10.0, mu_rolling=0.0, dt=dt)
+    floor = np.array([[[-1.0, 0.0], [1.0, 0.0]]])
+    hull = np.array([[[-1.0, 1.0], [1.0, 1.0]]])
+    s = OracleDisk([[-0.2, 1.0 - r + 1e-4], [0.2, r - 1e-4]], [r, r], cfg, walls=floor,
+                   hull=hull)
+    s.vel[0] = [u, 0.0]
+    s.vel[1] = [w, 0.0]
+    s.step()
+    assert s.st_key.tolist() == [4, 5]
+    assert s.st_xi[0] == pytest.approx([w * dt, 0.0], rel=1e-15)   # floor, disk 1
+    assert s.st_xi[1] == pytest.approx([u * dt, 0.0], rel=1e-15)   # hull, disk 0
