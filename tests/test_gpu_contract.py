@@ -160,7 +160,7 @@ def test_energy_drift_of_conservative_lattice():
 
 
 def test_restart_layouts_with_broken_bonds():
-    """The 7x3 indentation run to step 200 (bonds broken): the reference
+    """The 7x3 indentation run to step 345 (21 bonds broken): the reference
     layout (all N(N-1)/2 pairs) and the bond layout restore the same state,
     and both continue bit for bit; a reference-default DEMConfig (device
     'cpu') runs on the ROCm device."""
@@ -168,9 +168,9 @@ def test_restart_layouts_with_broken_bonds():
     assert cfg.device == "cpu"
     m = IceDEM(cfg)
     assert m.device.type == "cuda"
-    for _ in range(200):
+    for _ in range(345):
         m.step()
-    assert m.broken_bonds > 0
+    assert m.broken_bonds == 21
     n = 21
     ref = m.snapshot()
     assert ref["schema"] == "tensordem.restart/1"
