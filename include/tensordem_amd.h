@@ -193,6 +193,30 @@ int tdm_plate_forces(const tdm_plate_arrays *a, int parity, int fp32, void *stre
  */
 int tdm_plate_energy(const tdm_plate_arrays *a, int parity, void *stream);
 
+/*
+ * IceDEM.fragments (dem.py:462-484): connected components of one plate's
+ * surviving bonds, isolated disks included.  `links` (nx*ny,) are the plate's
+ * link bits (tdm_plate_arrays.links from the plate's first disk: bit c set
+ * when the bond towards direction code c is alive, codes 4..7 pointing to
+ * +1, +nx-1, +nx, +nx+1), `x` / `v` its (nx*ny,2) positions and velocities.
+ * Outputs (device, int32 unless noted):
+ *   labels (n,): component of each disk; components are numbered in the
+ *     order of their lowest disk, the order the reference lists them in;
+ *   order (n,): disk indices grouped by component, ascending inside each
+ *     (the reference's sorted particle_ids, concatenated);
+ *   comp_first (n+1,): component c is order[comp_first[c] .. comp_first[c+1]);
+ *   ncomp (1,): number of components;
+ *   comp_sums (n,4) float64: per component the sums of x, y, vx, vy over its
+ *     disks (centroid and mean velocity are these over the disk count).
+ * Deterministic: sums in a fixed order, no float atomics.  `workspace` holds
+ * tdm_fragments_workspace_bytes(nx*ny) bytes, 256-byte aligned; n <= 2^24.
+ */
+size_t tdm_fragments_workspace_bytes(int n);
+int tdm_plate_fragments(const uint8_t *links, const double *x, const double *v, int nx, int ny,
+                        int32_t *labels, int32_t *order, int32_t *comp_first, int32_t *ncomp,
+                        double *comp_sums, void *workspace, size_t workspace_bytes,
+                        void *stream);
+
 /* Opaque captured sequence of steps (a hipGraph of 2*pairs steps). */
 typedef struct tdm_graph tdm_graph;
 

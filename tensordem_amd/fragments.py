@@ -1,32 +1,72 @@
 """Connected components of surviving bonds (IceDEM.fragments, dem.py:462-484).
 
-Minimum-label propagation over the alive bonds on the device: every disk
-starts with its own index as label and repeatedly takes the smallest label
-among itself and its bonded partners until nothing changes; each component
-then carries the index of its lowest disk, the same representative the
-reference's breadth-first search starts from.
+`tdm_plate_fragments` (csrc/fragments.hip) labels the components of one
+plate on the device: union-find over the lattice link bits with the lowest
+disk of each component as its root, components numbered in the order of
+their lowest disk, disk indices grouped by a stable radix sort, and
+per-component sums of positions and velocities in a fixed order.  The host
+only turns those arrays into the reference's list of dicts.
 """
 from __future__ import annotations
 
+import ctypes
+
 import torch
 
+from . import _lib
 
-def label_components(engine) -> torch.Tensor:
-    n = engine.n
+
+class Components:
+    """Device outputs of one fragment labelling (see tdm_plate_fragments)."""
+
+    def __init__(self, labels, order, comp_first, ncomp, sums):
+        self.labels, self.order, self.comp_first = labels, order, comp_first
+        self.ncomp, self.sums = ncomp, sums
+
+
+def label_plate(engine, k: int = 0, workspace: torch.Tensor | None = None) -> Components:
+    """Component labelling of member k of a PlateEngine at its current state."""
+    m = engine._members_host[k]
+    nx, ny, first = int(m.nx), int(m.ny), int(m.first)
+    n = nx * ny
     dev = engine.x.device
-    labels = torch.arange(n, device=dev, dtype=torch.int64)
-    i = engine.bond_i[engine.alive]
-    j = engine.bond_j[engine.alive]
-    if i.numel() == 0:
-        return labels
-    while True:
-        li, lj = labels[i], labels[j]
-        low = torch.minimum(li, lj)
-        new = labels.clone()
-        new.scatter_reduce_(0, i, low, reduce="amin")
-        new.scatter_reduce_(0, j, low, reduce="amin")
-        # pointer jumping: follow labels to their own labels
-        new = new[new]
-        if torch.equal(new, labels):
-            return labels
-        labels = new
+    i32 = dict(dtype=torch.int32, device=dev)
+    labels = torch.empty(n, **i32)
+    order = torch.empty(n, **i32)
+    comp_first = torch.empty(n + 1, **i32)
+    ncomp = torch.empty(1, **i32)
+    sums = torch.empty((n, 4), dtype=torch.float64, device=dev)
+    lib = engine.lib
+    need = int(lib.tdm_fragments_workspace_bytes(n))
+    if workspace is None or workspace.numel() < need + 256:
+        workspace = torch.empty(need + 256, dtype=torch.uint8, device=dev)
+    base = workspace.data_ptr()
+    ws = (base + 255) & ~255
+    x = engine.x[first:first + n]
+    v = engine.v[first:first + n]
+    links = engine.links[first:first + n]
+    _lib.check(lib.tdm_plate_fragments(
+        links.data_ptr(), x.data_ptr(), v.data_ptr(), nx, ny, labels.data_ptr(),
+        order.data_ptr(), comp_first.data_ptr(), ncomp.data_ptr(), sums.data_ptr(),
+        ctypes.c_void_p(ws), ctypes.c_size_t(workspace.numel() - (ws - base)),
+        engine.stream))
+    return Components(labels, order, comp_first, ncomp, sums)
+
+
+def fragment_records(c: Components, mass: float) -> list[dict]:
+    """The reference's records: particle_ids (ascending), mass_kg,
+    centroid_m, mean_velocity_m_s, fragments in the order of their lowest
+    disk."""
+    nc = int(c.ncomp.item())
+    first = c.comp_first[:nc + 1].cpu().tolist()
+    order = c.order.cpu().tolist()
+    sums = c.sums[:nc].cpu().tolist()
+    out = []
+    for k in range(nc):
+        ids = order[first[k]:first[k + 1]]
+        cnt = len(ids)
+        s = sums[k]
+        out.append(dict(particle_ids=ids, mass_kg=cnt * mass,
+                        centroid_m=[s[0] / cnt, s[1] / cnt],
+                        mean_velocity_m_s=[s[2] / cnt, s[3] / cnt]))
+    return out

@@ -405,24 +405,11 @@ class IceDEM:
         return events
 
     def fragments(self) -> list[dict]:
-        """Connected components of surviving bonds, including isolated disks."""
-        from .fragments import label_components
-        labels = label_components(self._eng)
-        lab = labels.cpu()
-        pos = self.positions.cpu()
-        vel = self.velocities.cpu()
-        order = torch.argsort(lab, stable=True)
-        uniq, counts = torch.unique_consecutive(lab[order], return_counts=True)
-        result = []
-        start = 0
-        for cnt in counts.tolist():
-            ids = sorted(order[start:start + cnt].tolist())
-            start += cnt
-            result.append(dict(particle_ids=ids, mass_kg=len(ids) * self.config.mass,
-                               centroid_m=pos[ids].mean(0).tolist(),
-                               mean_velocity_m_s=vel[ids].mean(0).tolist()))
-        result.sort(key=lambda c: c["particle_ids"][0])
-        return result
+        """Connected components of surviving bonds, including isolated disks
+        (labelled on the device, tdm_plate_fragments)."""
+        from .fragments import fragment_records, label_plate
+        self._sync_inputs()
+        return fragment_records(label_plate(self._eng, 0), self.config.mass)
 
     # ---- restart (dem.py:486-601) ------------------------------------------------
     def snapshot(self, layout: str | None = None) -> dict:

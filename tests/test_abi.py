@@ -121,3 +121,15 @@ def test_disk_host_only_entry_points():
     assert lib.tdm_disk_workspace_bytes(1000, 400, 16, 24, 4, 20) > small
     with pytest.raises(RuntimeError):
         _lib.check(lib.tdm_disk_step(None, None, 1, 0, None))
+
+
+def test_fragment_entry_point_checks_arguments_on_the_host():
+    lib = _lib.load()
+    n = 1024 * 128
+    need = lib.tdm_fragments_workspace_bytes(n)
+    assert need >= 7 * 4 * n + 4 * 256 * (n // 256)
+    assert lib.tdm_fragments_workspace_bytes(0) == 0
+    null = [None] * 3
+    assert lib.tdm_plate_fragments(*null, 4, 4, *([None] * 5), None, 0, None) == _lib.TDM_EINVAL
+    assert lib.tdm_plate_fragments(*null, 1 << 13, 1 << 12, *([None] * 5), None, 0,
+                                   None) == _lib.TDM_EINVAL
