@@ -217,6 +217,22 @@ int tdm_plate_fragments(const uint8_t *links, const double *x, const double *v, 
                         double *comp_sums, void *workspace, size_t workspace_bytes,
                         void *stream);
 
+/*
+ * tdm_plate_fragments without a workspace argument and without allocating:
+ * its scratch is a file-scope device array of 8 MiB (7 int arrays and the
+ * radix histogram for up to 262 144 disks), which each GPU the library runs
+ * on holds until the process ends.  No counterpart in the reference; it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * outputs, bit for bit, as tdm_plate_fragments; nx * ny above 262 144 is
+ * rejected with TDM_EINVAL.  Every call on one GPU shares that GPU's array:
+ * calls that overlap on one GPU (concurrent streams or host threads) race.
+ * `stream` must belong to the current device.
+ */
+int tdm_plate_fragments_noalloc(const uint8_t *links, const double *x, const double *v, int nx,
+                                int ny, int32_t *labels, int32_t *order, int32_t *comp_first,
+                                int32_t *ncomp, double *comp_sums, void *stream);
+
 /* Opaque captured sequence of steps (a hipGraph of 2*pairs steps). */
 typedef struct tdm_graph tdm_graph;
 

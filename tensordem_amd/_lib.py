@@ -119,6 +119,8 @@ _SIGNATURES = {
     "tdm_fragments_workspace_bytes": ([ctypes.c_int], ctypes.c_size_t),
     "tdm_plate_fragments": ([ctypes.c_void_p] * 3 + [ctypes.c_int] * 2 + [ctypes.c_void_p] * 6
                             + [ctypes.c_size_t, ctypes.c_void_p], ctypes.c_int),
+    "tdm_plate_fragments_noalloc": ([ctypes.c_void_p] * 3 + [ctypes.c_int] * 2
+                                    + [ctypes.c_void_p] * 6, ctypes.c_int),
     "tdm_bed_blocks": ([ctypes.c_int], ctypes.c_int),
     "tdm_bed_workspace_bytes": ([ctypes.c_int] * 6, ctypes.c_size_t),
     "tdm_bed_workspace_layout": ([ctypes.c_int] * 6 + [ctypes.POINTER(ctypes.c_size_t)] * 5,

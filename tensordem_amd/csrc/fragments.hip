@@ -16,6 +16,8 @@
 //     mean velocities are deterministic; no float atomics anywhere.
 #include "common.h"
 
+#include <mutex>
+
 namespace tdm {
 namespace {
 
@@ -339,6 +341,45 @@ int launch_fragments(const uint8_t *links, const double *x, const double *v, int
   return TDM_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Allocation-free variant: the scratch lives in a file-scope device array of
+// a fixed size (each GPU the library runs on holds its own copy until the
+// process ends).  In host code `scratch` names a host-side stand-in, not the
+// device address: the address comes from hipGetSymbolAddress, once per
+// device, with that device current, and only that pointer is passed on.
+// ---------------------------------------------------------------------------
+constexpr int kNoAllocMaxDisks = 1 << 18;  // 262 144 disks (a 512 x 512 plate)
+constexpr size_t kIntArrayBytes = ((sizeof(int) * (size_t)kNoAllocMaxDisks + 255) / 256) * 256;
+constexpr size_t kHistBytes =
+    ((sizeof(int) * 256 * (size_t)((kNoAllocMaxDisks + kThreads - 1) / kThreads) + 255) / 256) *
+    256;
+constexpr size_t kNoAllocBytes = 7 * kIntArrayBytes + kHistBytes;  // 8 MiB
+constexpr int kMaxDevices = 64;
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+std::mutex g_scratch_mutex;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of this device's scratch array (looked up on the first call
+// on each device and kept).
+int scratch_address(char **out)
+{
+  int dev = -1;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(TDM_EINVAL, "fragments: device id out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(TDM_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = p;
+  }
+  *out = static_cast<char *>(g_scratch_ptr[dev]);
+  return TDM_OK;
+}
+
 }  // namespace
 }  // namespace tdm
 
@@ -366,6 +407,23 @@ int tdm_plate_fragments(const uint8_t *links, const double *x, const double *v, 
     return fail(TDM_ESPACE, "fragments: workspace too small");
   return launch_fragments(links, x, v, nx, ny, labels, order, comp_first, ncomp, comp_sums,
                           static_cast<char *>(workspace), static_cast<hipStream_t>(stream));
+}
+
+int tdm_plate_fragments_noalloc(const uint8_t *links, const double *x, const double *v, int nx,
+                                int ny, int32_t *labels, int32_t *order, int32_t *comp_first,
+                                int32_t *ncomp, double *comp_sums, void *stream)
+{
+  int rc = check_fragments(links, x, v, nx, ny, labels, order, comp_first, ncomp, comp_sums);
+  if (rc) return rc;
+  if ((long long)nx * ny > kNoAllocMaxDisks)
+    return fail(TDM_EINVAL, "fragments_noalloc: at most 262144 disks (nx * ny) per call");
+  if (tdm_fragments_workspace_bytes(nx * ny) > kNoAllocBytes)
+    return fail(TDM_ESPACE, "fragments_noalloc: scratch smaller than the workspace layout");
+  char *ws = nullptr;
+  rc = scratch_address(&ws);
+  if (rc) return rc;
+  return launch_fragments(links, x, v, nx, ny, labels, order, comp_first, ncomp, comp_sums, ws,
+                          static_cast<hipStream_t>(stream));
 }
 
 }  // extern "C"

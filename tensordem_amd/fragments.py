@@ -24,8 +24,15 @@ class Components:
         self.ncomp, self.sums = ncomp, sums
 
 
-def label_plate(engine, k: int = 0, workspace: torch.Tensor | None = None) -> Components:
-    """Component labelling of member k of a PlateEngine at its current state."""
+NOALLOC_MAX_DISKS = 1 << 18
+
+
+def label_plate(engine, k: int = 0, workspace: torch.Tensor | None = None, *,
+                allocation_free: bool = False) -> Components:
+    """Component labelling of member k of a PlateEngine at its current state.
+
+    allocation_free: use tdm_plate_fragments_noalloc (the library's static
+    scratch, at most NOALLOC_MAX_DISKS disks) instead of a workspace."""
     m = engine._members_host[k]
     nx, ny, first = int(m.nx), int(m.ny), int(m.first)
     n = nx * ny
@@ -37,17 +44,22 @@ def label_plate(engine, k: int = 0, workspace: torch.Tensor | None = None) -> Co
     ncomp = torch.empty(1, **i32)
     sums = torch.empty((n, 4), dtype=torch.float64, device=dev)
     lib = engine.lib
+    x = engine.x[first:first + n]
+    v = engine.v[first:first + n]
+    links = engine.links[first:first + n]
+    outs = (labels.data_ptr(), order.data_ptr(), comp_first.data_ptr(), ncomp.data_ptr(),
+            sums.data_ptr())
+    if allocation_free:
+        _lib.check(lib.tdm_plate_fragments_noalloc(links.data_ptr(), x.data_ptr(), v.data_ptr(),
+                                                   nx, ny, *outs, engine.stream))
+        return Components(labels, order, comp_first, ncomp, sums)
     need = int(lib.tdm_fragments_workspace_bytes(n))
     if workspace is None or workspace.numel() < need + 256:
         workspace = torch.empty(need + 256, dtype=torch.uint8, device=dev)
     base = workspace.data_ptr()
     ws = (base + 255) & ~255
-    x = engine.x[first:first + n]
-    v = engine.v[first:first + n]
-    links = engine.links[first:first + n]
     _lib.check(lib.tdm_plate_fragments(
-        links.data_ptr(), x.data_ptr(), v.data_ptr(), nx, ny, labels.data_ptr(),
-        order.data_ptr(), comp_first.data_ptr(), ncomp.data_ptr(), sums.data_ptr(),
+        links.data_ptr(), x.data_ptr(), v.data_ptr(), nx, ny, *outs,
         ctypes.c_void_p(ws), ctypes.c_size_t(workspace.numel() - (ws - base)),
         engine.stream))
     return Components(labels, order, comp_first, ncomp, sums)
