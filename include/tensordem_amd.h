@@ -261,7 +261,8 @@ int tdm_graph_destroy(tdm_graph *g);
 #define TDM_BED_WALL_IMPULSE 4 /* x, y: impulse of the walls on the disks [N s] */
 #define TDM_BED_CONTACTS 6     /* pair contacts during the last step */
 #define TDM_BED_WALL_CONTACTS 7
-#define TDM_BED_NACC 8
+#define TDM_BED_WALL_WORK 8    /* work of the moving right wall on the disks [J] */
+#define TDM_BED_NACC 9
 
 typedef struct tdm_bed_member {
   double e_star, g_star;       /* pair moduli E/(2(1-nu^2)), G/(2(2-nu)) */
@@ -271,6 +272,10 @@ typedef struct tdm_bed_member {
   double gravity_y;            /* [m/s^2], e.g. -9.81 */
   double dt;
   double floor_y, back_wall_x;
+  /* right wall (gate or indenter, scale_ceiling/kernel.py:156-197, 338-401):
+   * at x = right_wall_x + right_wall_v * time during the step that starts at
+   * `time`, inward normal -x, moving with (right_wall_v, 0) */
+  double right_wall_x, right_wall_v;
   double time, skin, cell_size, grid_origin[2];
   int64_t step_count;
   int32_t first, n;            /* disks [first, first+n) */
@@ -279,12 +284,13 @@ typedef struct tdm_bed_member {
   int32_t block_first, nblocks;
   int32_t mode, tab, list_parity, status;
   int32_t rebuilds;             /* candidate-list rebuilds so far */
+  int32_t use_right_wall, pad_;
 } tdm_bed_member;
 
 typedef struct tdm_bed_arrays {
   double *x, *v, *x_alt, *v_alt; /* (N,2) state and second buffers */
   const double *radius;          /* (N,) */
-  double *xi_wall;               /* (N,4) floor and back-wall spring history */
+  double *xi_wall;               /* (N,6) floor, back-wall and right-wall spring history */
   double *accounting;            /* (nmembers, TDM_BED_NACC) */
   double *energy;                /* (nmembers,4): kinetic, gravity, Hertz normal, tangential */
   tdm_bed_member *members;
@@ -293,6 +299,8 @@ typedef struct tdm_bed_arrays {
   int32_t n, nmembers, nblocks, ncells, cand_slots, cell_cap;
 } tdm_bed_arrays;
 
+/* sizeof(tdm_bed_member) and sizeof(tdm_bed_arrays) as compiled. */
+int tdm_bed_abi_sizes(size_t *member_bytes, size_t *arrays_bytes);
 /* Blocks (256 disks each) a member of n disks occupies. */
 int tdm_bed_blocks(int n);
 size_t tdm_bed_workspace_bytes(int n, int nblocks, int nmembers, int ncells, int cand_slots,

@@ -13,8 +13,9 @@ import numpy as np
 from .plate import _p, library
 
 B_ESTAR, B_GSTAR, B_CDAMP, B_MU, B_MU_WALL, B_RHO_PI_T, B_GRAV_Y, B_DT, B_Y_FLOOR, B_X_LEFT, \
-    B_USE_FLOOR, B_USE_LEFT, B_NPARAM = range(13)
-BS_DASH, BS_FRIC, BS_WALL_DASH, BS_WALL_FRIC, BS_WALL_FX, BS_WALL_FY, BS_NPAIRS, BS_NWALL = range(8)
+    B_USE_FLOOR, B_USE_LEFT, B_X_RIGHT, B_V_RIGHT, B_USE_RIGHT, B_TIME, B_NPARAM = range(17)
+BS_DASH, BS_FRIC, BS_WALL_DASH, BS_WALL_FRIC, BS_WALL_FX, BS_WALL_FY, BS_NPAIRS, BS_NWALL, \
+    BS_WALL_POWER, BS_NSCAL = range(10)
 
 
 def _bind():
@@ -24,7 +25,7 @@ def _bind():
         i64 = ctypes.POINTER(ctypes.c_int64)
         i32 = ctypes.POINTER(ctypes.c_int32)
         lib.orb_step.argtypes = [d, ctypes.c_int, d, d, d, ctypes.c_int, i64, d, ctypes.c_int,
-                                 i64, d, d, d, d, d, ctypes.c_int]
+                                 i64, d, d, d, d, d, d, ctypes.c_int]
         lib.orb_step.restype = ctypes.c_int
         lib.orb_contacts.argtypes = [ctypes.c_int, d, d, ctypes.c_int, i32]
         lib.orb_contacts.restype = ctypes.c_int
@@ -46,11 +47,17 @@ class OracleBed:
         self.hist_xi = np.zeros((0, 2))
         self.xi_floor = np.zeros((n, 2))
         self.xi_left = np.zeros((n, 2))
+        self.xi_right = np.zeros((n, 2))
+        # right wall: x = right_x + right_v * t (the config's, changeable)
+        self.right_wall = bool(config.right_wall)
+        self.right_x = float(config.right_wall_x)
+        self.right_v = float(config.right_wall_speed)
         self.force = np.zeros((n, 2))
         self.time = 0.0
         self.step_count = 0
-        self.acc = np.zeros(6)   # dash, fric, wall dash, wall fric (J); wall impulse x, y
-        self.last = np.zeros(8)
+        # dash, fric, wall dash, wall fric (J); wall impulse x, y (N s); wall work (J)
+        self.acc = np.zeros(7)
+        self.last = np.zeros(BS_NSCAL)
 
     def _params(self) -> np.ndarray:
         c = self.config
@@ -60,20 +67,39 @@ class OracleBed:
         p[B_GRAV_Y], p[B_DT] = -c.gravity, self.dt
         p[B_Y_FLOOR], p[B_X_LEFT] = c.floor_y, c.back_wall_x
         p[B_USE_FLOOR], p[B_USE_LEFT] = float(c.floor), float(c.back_wall)
+        p[B_X_RIGHT], p[B_V_RIGHT] = self.right_x, self.right_v
+        p[B_USE_RIGHT], p[B_TIME] = float(self.right_wall), self.time
         return p
+
+    def set_right_wall(self, *, x=None, speed=None, enabled=None) -> None:
+        """Move, re-time or switch the right wall now: it is at x from the
+        current time on, moving at `speed`."""
+        if speed is not None:
+            if x is None:
+                x = self.right_x + self.right_v * self.time
+            self.right_v = float(speed)
+        if x is not None:
+            self.right_x = float(x) - self.right_v * self.time
+        if enabled is not None:
+            self.right_wall = bool(enabled)
+
+    @property
+    def right_wall_position(self) -> float:
+        return self.right_x + self.right_v * self.time
 
     def step(self, integrate: bool = True) -> None:
         n = len(self.radii)
         cap = max(16, 4 * n)
         ok = np.zeros(cap, dtype=np.int64)
         ox = np.zeros((cap, 2))
-        scal = np.zeros(8)
+        scal = np.zeros(BS_NSCAL)
         d, i64 = ctypes.c_double, ctypes.c_int64
         cnt = self.lib.orb_step(_p(self._params(), d), n, _p(self.positions, d),
                                 _p(self.velocities, d), _p(self.radii, d), len(self.hist_keys),
                                 _p(self.hist_keys, i64), _p(self.hist_xi, d), cap, _p(ok, i64),
                                 _p(ox, d), _p(self.xi_floor, d), _p(self.xi_left, d),
-                                _p(self.force, d), _p(scal, d), int(integrate))
+                                _p(self.xi_right, d), _p(self.force, d), _p(scal, d),
+                                int(integrate))
         if cnt == -1:
             raise RuntimeError("oracle contact capacity exceeded")
         if cnt == -2:
@@ -89,6 +115,7 @@ class OracleBed:
             self.acc[3] += dt * scal[BS_WALL_FRIC]
             self.acc[4] += dt * scal[BS_WALL_FX]
             self.acc[5] += dt * scal[BS_WALL_FY]
+            self.acc[6] += dt * scal[BS_WALL_POWER]
             self.time += dt
             self.step_count += 1
 
@@ -119,7 +146,8 @@ class OracleBed:
             reff = self.radii[i] * self.radii[j] / (self.radii[i] + self.radii[j])
             el += 0.4 * (2.0 / 3.0) * 2 * c.e_star * np.sqrt(reff * d) * d * d
         for gap, use in ((self.positions[:, 1] - c.floor_y, c.floor),
-                         (self.positions[:, 0] - c.back_wall_x, c.back_wall)):
+                         (self.positions[:, 0] - c.back_wall_x, c.back_wall),
+                         (self.right_wall_position - self.positions[:, 0], self.right_wall)):
             if use:
                 d = np.maximum(self.radii - gap, 0.0)
                 el += float((0.4 * (2.0 / 3.0) * 2 * c.e_star * np.sqrt(self.radii * d) * d * d).sum())

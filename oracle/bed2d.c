@@ -17,6 +17,10 @@
  *     pair law (the reference's wall branch divides by |xi| only);
  *   - floor y = y_floor and back wall x = x_left (granular2d/
  *     column_collapse.py:236-266: no gate wall once released), gravity;
+ *   - a right wall (the floe bed's indenter, scale_ceiling/kernel.py:156-197,
+ *     338-401, 514-519) at x = x_right + v_right * t during the step that
+ *     starts at time t, inward normal -x, the law evaluated on the velocity
+ *     relative to the wall; it doubles as the column's gate (v_right = 0);
  *   - semi-implicit Euler v += dt F / m, x += dt v as IceDEM.step
  *     (dem.py:363-365).
  * Per disk the force is gravity, then pair forces by ascending partner
@@ -32,12 +36,13 @@
 
 enum {
   B_ESTAR = 0, B_GSTAR, B_CDAMP, B_MU, B_MU_WALL, B_RHO_PI_T, B_GRAV_Y, B_DT, B_Y_FLOOR,
-  B_X_LEFT, B_USE_FLOOR, B_USE_LEFT, B_NPARAM
+  B_X_LEFT, B_USE_FLOOR, B_USE_LEFT, B_X_RIGHT, B_V_RIGHT, B_USE_RIGHT, B_TIME, B_NPARAM
 };
-/* scalar outputs: powers absorbed (>= 0 when dissipating) and counts */
+/* scalar outputs: powers absorbed (>= 0 when dissipating), counts, and the
+ * power of the moving right wall on the disks */
 enum {
   BS_DASH = 0, BS_FRIC, BS_WALL_DASH, BS_WALL_FRIC, BS_WALL_FX, BS_WALL_FY, BS_NPAIRS,
-  BS_NWALL, BS_NSCAL
+  BS_NWALL, BS_WALL_POWER, BS_NSCAL
 };
 
 typedef struct { int64_t key; int32_t idx; } bkeyed;
@@ -159,12 +164,13 @@ static void wall(const double *prm, double r, double m, double gap, double nx, d
  */
 int orb_step(const double *prm, int n, double *x, double *v, const double *r, int nh,
              const int64_t *hk, const double *hx, int cap_out, int64_t *ok, double *ox,
-             double *xi_floor, double *xi_left, double *force, double *scal, int integrate)
+             double *xi_floor, double *xi_left, double *xi_right, double *force, double *scal,
+             int integrate)
 {
   int np = 0;
   bpair *pr = bed_pairs(n, x, r, &np);
   if (np > cap_out) { free(pr); return -1; }
-  double dash = 0, fric = 0, wdash = 0, wfric = 0, wfx = 0, wfy = 0;
+  double dash = 0, fric = 0, wdash = 0, wfric = 0, wfx = 0, wfy = 0, wpow = 0;
   int nwall = 0;
   bcontrib *cb = (bcontrib *)malloc(sizeof(bcontrib) * (size_t)(2 * np > 0 ? 2 * np : 1));
   int32_t *deg = (int32_t *)calloc((size_t)n + 1, sizeof(int32_t));
@@ -237,6 +243,16 @@ int orb_step(const double *prm, int n, double *x, double *v, const double *r, in
            prm[B_MU_WALL], &wx, &wy, &wdash, &wfric);
       if (act) { fx += wx; fy += wy; wfx += wx; wfy += wy; ++nwall; }
     }
+    if (prm[B_USE_RIGHT] != 0) {
+      const double xr = prm[B_X_RIGHT] + prm[B_V_RIGHT] * prm[B_TIME];
+      const double vrel[2] = {v[2 * p] - prm[B_V_RIGHT], v[2 * p + 1]};
+      wall(prm, r[p], m, xr - x[2 * p], -1.0, 0.0, vrel, xi_right + 2 * p, &act,
+           prm[B_MU_WALL], &wx, &wy, &wdash, &wfric);
+      if (act) {
+        fx += wx; fy += wy; wfx += wx; wfy += wy; ++nwall;
+        wpow += wx * prm[B_V_RIGHT];
+      }
+    }
     force[2 * p] = fx;
     force[2 * p + 1] = fy;
   }
@@ -255,6 +271,7 @@ int orb_step(const double *prm, int n, double *x, double *v, const double *r, in
   scal[BS_DASH] = dash; scal[BS_FRIC] = fric; scal[BS_WALL_DASH] = wdash;
   scal[BS_WALL_FRIC] = wfric; scal[BS_WALL_FX] = wfx; scal[BS_WALL_FY] = wfy;
   scal[BS_NPAIRS] = (double)np; scal[BS_NWALL] = (double)nwall;
+  scal[BS_WALL_POWER] = wpow;
   free(pr); free(cb); free(deg); free(pfx); free(pfy); free(fill);
   return bad ? -2 : np;
 }

@@ -69,7 +69,9 @@ class BedMember(ctypes.Structure):
         ("e_star", ctypes.c_double), ("g_star", ctypes.c_double), ("damping", ctypes.c_double),
         ("friction", ctypes.c_double), ("wall_friction", ctypes.c_double),
         ("rho_pi_t", ctypes.c_double), ("gravity_y", ctypes.c_double), ("dt", ctypes.c_double),
-        ("floor_y", ctypes.c_double), ("back_wall_x", ctypes.c_double), ("time", ctypes.c_double),
+        ("floor_y", ctypes.c_double), ("back_wall_x", ctypes.c_double),
+        ("right_wall_x", ctypes.c_double), ("right_wall_v", ctypes.c_double),
+        ("time", ctypes.c_double),
         ("skin", ctypes.c_double), ("cell_size", ctypes.c_double),
         ("grid_origin", ctypes.c_double * 2), ("step_count", ctypes.c_int64),
         ("first", ctypes.c_int32), ("n", ctypes.c_int32), ("use_floor", ctypes.c_int32),
@@ -77,6 +79,7 @@ class BedMember(ctypes.Structure):
         ("cell_first", ctypes.c_int32), ("block_first", ctypes.c_int32),
         ("nblocks", ctypes.c_int32), ("mode", ctypes.c_int32), ("tab", ctypes.c_int32),
         ("list_parity", ctypes.c_int32), ("status", ctypes.c_int32), ("rebuilds", ctypes.c_int32),
+        ("use_right_wall", ctypes.c_int32), ("pad_", ctypes.c_int32),
     ]
 
 
@@ -93,7 +96,7 @@ class BedArrays(ctypes.Structure):
 
 
 BED_DASHPOT, BED_FRICTION, BED_WALL_DASHPOT, BED_WALL_FRICTION, BED_WALL_IMPULSE = 0, 1, 2, 3, 4
-BED_CONTACTS, BED_WALL_CONTACTS, BED_NACC = 6, 7, 8
+BED_CONTACTS, BED_WALL_CONTACTS, BED_WALL_WORK, BED_NACC = 6, 7, 8, 9
 
 # every entry point declared in include/tensordem_amd.h, with its ctypes signature
 _SIGNATURES = {
@@ -121,6 +124,8 @@ _SIGNATURES = {
                             + [ctypes.c_size_t, ctypes.c_void_p], ctypes.c_int),
     "tdm_plate_fragments_noalloc": ([ctypes.c_void_p] * 3 + [ctypes.c_int] * 2
                                     + [ctypes.c_void_p] * 6, ctypes.c_int),
+    "tdm_bed_abi_sizes": ([ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)],
+                          ctypes.c_int),
     "tdm_bed_blocks": ([ctypes.c_int], ctypes.c_int),
     "tdm_bed_workspace_bytes": ([ctypes.c_int] * 6, ctypes.c_size_t),
     "tdm_bed_workspace_layout": ([ctypes.c_int] * 6 + [ctypes.POINTER(ctypes.c_size_t)] * 5,
@@ -174,6 +179,10 @@ def load() -> ctypes.CDLL:
     if mb.value != ctypes.sizeof(Member) or ab.value != ctypes.sizeof(Arrays):
         raise ImportError(f"ABI mismatch: library structs {mb.value}/{ab.value} B, binding "
                           f"{ctypes.sizeof(Member)}/{ctypes.sizeof(Arrays)} B")
+    lib.tdm_bed_abi_sizes(ctypes.byref(mb), ctypes.byref(ab))
+    if mb.value != ctypes.sizeof(BedMember) or ab.value != ctypes.sizeof(BedArrays):
+        raise ImportError(f"ABI mismatch: library bed structs {mb.value}/{ab.value} B, binding "
+                          f"{ctypes.sizeof(BedMember)}/{ctypes.sizeof(BedArrays)} B")
     _lib = lib
     return lib
 

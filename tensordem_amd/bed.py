@@ -52,7 +52,7 @@ def remap_lists(cand, xi, mask, count, perm, inv, first):
     global index of every old one.  Returns the lists of the new positions:
     partners renumbered and re-sorted ascending with their spring histories
     and contact bits, each history negated where the pair's lo -> hi
-    orientation flips, wall bits (16, 17) kept."""
+    orientation flips, wall bits (16, 17, 18) kept."""
     S, m = cand.shape
     dev = cand.device
     loc = perm - first
@@ -73,7 +73,7 @@ def remap_lists(cand, xi, mask, count, perm, inv, first):
     h_sorted = torch.gather(h, 0, order[..., None].expand(-1, -1, 2))
     bits = (mk[None, :] >> order) & 1
     bits = torch.where(valid, bits, torch.zeros_like(bits))
-    new_mask = (bits << slot).sum(dim=0) | (mk & (3 << 16))
+    new_mask = (bits << slot).sum(dim=0) | (mk & (7 << 16))
     out_cand = torch.where(valid, c_sorted, torch.zeros_like(c_sorted)).to(torch.int32)
     return out_cand, h_sorted, new_mask.to(torch.int32), k.to(torch.uint8)
 
@@ -118,6 +118,8 @@ def bed_parts(s: BedSpec, *, device="cuda", skin_factor: float = 0.4,
     m.dt = float(s.dt if s.dt is not None else
                  (c.dt if c.dt is not None else c.stable_dt(float(r.min()))))
     m.floor_y, m.back_wall_x = c.floor_y, c.back_wall_x
+    m.right_wall_x, m.right_wall_v = c.right_wall_x, c.right_wall_speed
+    m.use_right_wall = int(c.right_wall)
     m.time, m.step_count = 0.0, 0
     r0 = float(r.mean())
     m.skin = skin_factor * r0
@@ -179,7 +181,7 @@ class BedEngine:
         self.radius = torch.cat([pt.radius for pt in parts])
         # original index of the disk at each storage position (renumber)
         self.ids = torch.arange(n, dtype=torch.int64, device=dev)
-        self.xi_wall = torch.zeros((n, 4), **f64)
+        self.xi_wall = torch.zeros((n, 6), **f64)   # floor, back wall, right wall
         self.accounting = torch.zeros((nm, _lib.BED_NACC), **f64)
         self.energy = torch.zeros((nm, 4), **f64)
         self._members_host = members
@@ -300,6 +302,30 @@ class BedEngine:
         self.rebin()
         return perm
 
+    def set_right_wall(self, member: int = 0, *, x=None, speed=None, enabled=None) -> None:
+        """Move, re-time or switch member's right wall (the indenter, or the
+        column's gate) between calls: from the member's current time on it
+        stands at x and moves at `speed` along +x (scale_ceiling/kernel.py:
+        514-519 sets the indenter velocity and advances it by v dt per step)."""
+        ms = self.read_members()
+        m = ms[member]
+        t = m.time
+        if speed is not None:
+            if x is None:
+                x = m.right_wall_x + m.right_wall_v * t
+            m.right_wall_v = float(speed)
+        if x is not None:
+            m.right_wall_x = float(x) - m.right_wall_v * t
+        if enabled is not None:
+            m.use_right_wall = int(bool(enabled))
+        if not (math.isfinite(m.right_wall_x) and math.isfinite(m.right_wall_v)):
+            raise ValueError("right wall position and speed must be finite")
+        self.write_members(ms)
+
+    def right_wall_position(self, member: int = 0) -> float:
+        m = self.read_members()[member]
+        return m.right_wall_x + m.right_wall_v * m.time
+
     def read_members(self):
         raw = self.members.cpu().numpy().tobytes()
         return (_lib.BedMember * self.nmembers).from_buffer_copy(raw)
@@ -336,7 +362,7 @@ class BedEngine:
         radii, law, clock, ledger, every active pair spring history (pairs
         i < j with their xi, in the member's local indices) and the wall
         springs.  Restoring it continues the run bit for bit."""
-        from dataclasses import asdict
+        from dataclasses import asdict, replace
         ms = self.read_members()
         m = ms[member]
         sl = slice(m.first, m.first + m.n)
@@ -356,9 +382,11 @@ class BedEngine:
         pi, qi, hx = np.concatenate(p_loc), np.concatenate(q_loc), np.concatenate(h)
         order = np.lexsort((qi, pi))
         spec = self.specs[member]
+        cfg = replace(spec.config, right_wall=bool(m.use_right_wall),
+                      right_wall_x=float(m.right_wall_x), right_wall_speed=float(m.right_wall_v))
         return {
             "schema": self.RESTART_SCHEMA,
-            "config": asdict(spec.config), "dt": m.dt, "time": m.time,
+            "config": asdict(cfg), "dt": m.dt, "time": m.time,
             "step_count": int(m.step_count),
             "positions": self.x[sl].cpu().numpy().copy(),
             "velocities": self.v[sl].cpu().numpy().copy(),
@@ -367,7 +395,7 @@ class BedEngine:
             "pairs": np.stack((pi[order], qi[order]), 1).astype(np.int64),
             "pair_xi": hx[order].copy(),
             "wall_xi": self.xi_wall[sl].cpu().numpy().copy(),
-            "wall_active": ((mask >> 16) & 3).astype(np.uint8),
+            "wall_active": ((mask >> 16) & 7).astype(np.uint8),
         }
 
     @classmethod
@@ -405,8 +433,14 @@ class BedEngine:
         eng._ws_view(3, torch.uint8, (2, n))[0].copy_(
             torch.from_numpy(counts.astype(np.uint8)).to(dev))
         eng._ws_view(4, torch.float64, (n, 2)).copy_(eng.x)
-        eng.xi_wall.copy_(torch.from_numpy(np.asarray(state["wall_xi"], np.float64)).to(dev))
-        eng.accounting[0].copy_(torch.from_numpy(np.asarray(state["accounting"])).to(dev))
+        wxi = np.zeros((n, 6))
+        old = np.asarray(state["wall_xi"], np.float64)
+        wxi[:, :old.shape[1]] = old      # files of two-wall beds hold (n, 4)
+        eng.xi_wall.copy_(torch.from_numpy(wxi).to(dev))
+        acc = np.zeros(_lib.BED_NACC)
+        old_acc = np.asarray(state["accounting"], np.float64)
+        acc[:len(old_acc)] = old_acc     # files written before the wall-work channel
+        eng.accounting[0].copy_(torch.from_numpy(acc).to(dev))
         ms = eng.read_members()
         ms[0].time, ms[0].step_count = float(state["time"]), int(state["step_count"])
         eng.write_members(ms)
@@ -465,11 +499,12 @@ def validate_bed_restart(state: dict) -> None:
         raise ValueError("bed restart fields missing or unknown")
     x = np.asarray(state["positions"], np.float64)
     n = len(x)
-    shapes = {"positions": (n, 2), "velocities": (n, 2), "radii": (n,),
-              "accounting": (_lib.BED_NACC,), "wall_xi": (n, 4), "wall_active": (n,)}
+    shapes = {"positions": [(n, 2)], "velocities": [(n, 2)], "radii": [(n,)],
+              "accounting": [(_lib.BED_NACC,), (_lib.BED_NACC - 1,)],
+              "wall_xi": [(n, 6), (n, 4)], "wall_active": [(n,)]}
     for k, shp in shapes.items():
         a = np.asarray(state[k])
-        if a.shape != shp:
+        if a.shape not in shp:
             raise ValueError(f"invalid bed restart {k} shape")
         if not np.isfinite(a.astype(np.float64)).all():
             raise ValueError(f"nonfinite bed restart {k}")
@@ -487,8 +522,10 @@ def validate_bed_restart(state: dict) -> None:
         if len(np.unique(pairs[:, 0] * n + pairs[:, 1])) != len(pairs):
             raise ValueError("repeated bed restart pair")
     wa = np.asarray(state["wall_active"])
-    if wa.dtype.kind not in "iu" or (wa.astype(np.int64) & ~3).any():
+    if wa.dtype.kind not in "iu" or (wa.astype(np.int64) & ~7).any():
         raise ValueError("invalid bed restart wall_active")
+    if np.asarray(state["wall_xi"]).shape[1] == 4 and (wa.astype(np.int64) & 4).any():
+        raise ValueError("bed restart marks right wall contacts but stores no right wall springs")
     dt, t, sc = state["dt"], state["time"], state["step_count"]
     if not (math.isfinite(dt) and dt > 0 and math.isfinite(t) and t >= 0):
         raise ValueError("invalid bed restart time or timestep")
@@ -517,3 +554,52 @@ def load_bed_restart(path) -> dict:
         raise ValueError(f"{path}: not a tensordem_amd bed restart file")
     validate_bed_restart(state)
     return state
+
+
+def settled_column(n: int, r0: float = 0.01, *, seed: int = 0, config: BedConfig | None = None,
+                   aspect: float = 2.0, fp32: bool = True, chunk: int = 2000,
+                   max_steps: int = 200_000, rest: float = 1e-6, device="cuda",
+                   log=None) -> tuple[dict, dict]:
+    """The release state of a column collapse: a deposited random loose
+    column (`column_deposit`) settled under gravity between the back wall,
+    the floor and a gate at its width (a right wall at rest), on the device,
+    until its kinetic energy is below `rest` times m g H summed over the
+    disks (or `max_steps`).  Returns the bed restart state with the gate
+    already removed, the clock at zero and an empty ledger: the column then
+    stands as the reference's protocol releases it (granular2d/
+    column_collapse.py:236-266, the gate is the absence of a right wall),
+    with its contact springs; and a dict describing the settling."""
+    from dataclasses import asdict, replace
+    from .bedconfig import column_deposit
+    cfg = config or BedConfig()
+    x, r, width = column_deposit(n, r0, seed=seed, aspect=aspect)
+    gated = replace(cfg, right_wall=True, right_wall_x=width, right_wall_speed=0.0)
+    eng = BedEngine([BedSpec(gated, x, r)], fp32=fp32, device=device)
+    height = float((x[:, 1] + r).max())
+    m = gated.rho_pi_t * (r * r)
+    scale = float((m * gated.gravity * height).sum())
+    done = 0
+    ke = math.inf
+    while done < max_steps:
+        eng.run_graph(chunk // 2)
+        done += chunk
+        if done % (5 * chunk) == 0:
+            ke = float(eng.energies()[0, 0].item())
+            if log:
+                log(f"settle: {done} steps, kinetic {ke:.3e} J ({ke / scale:.2e} of m g H)")
+            if ke < rest * scale:
+                break
+    if eng.status()[0]:
+        raise RuntimeError(f"settling the column raised status bits {eng.status()[0]}")
+    state = eng.snapshot()
+    eng.close()
+    # remove the gate: no right wall, no right-wall springs
+    state["config"] = asdict(replace(BedConfig(**state["config"]), right_wall=False))
+    state["wall_xi"] = np.asarray(state["wall_xi"]).copy()
+    state["wall_xi"][:, 4:6] = 0.0
+    state["wall_active"] = (np.asarray(state["wall_active"]) & 3).astype(np.uint8)
+    state["time"], state["step_count"] = 0.0, 0
+    state["accounting"] = np.zeros_like(np.asarray(state["accounting"]))
+    info = dict(settle_steps=done, kinetic_J=ke, kinetic_rel=ke / scale, width_m=width,
+                contact_pairs=len(state["pairs"]), height_m=height)
+    return state, info

@@ -3,8 +3,8 @@
 The law restates, in two dimensions, the reference's floe-bed contact model
 (src/tensordem/scale_ceiling/kernel.py:38-129, 216-227, 277-401): Hertz
 normal force with a restitution-derived viscous term, Cundall-Strack
-tangential spring capped by Coulomb friction, floor and back-wall contacts,
-gravity.  The column geometry follows the 2-D column-collapse protocol
+tangential spring capped by Coulomb friction, floor, back-wall and moving
+right-wall (indenter or gate) contacts, gravity.  The column geometry follows the 2-D column-collapse protocol
 (src/tensordem/granular2d/column_collapse.py:196-266): a column of aspect
 ratio H/L resting on a rough floor against a back wall, released by the
 absence of the gate wall.
@@ -32,6 +32,12 @@ class BedConfig:
     floor: bool = True
     back_wall: bool = True
     dt: float | None = None
+    # right wall: the floe bed's indenter (scale_ceiling/kernel.py:156-197,
+    # 338-401, 514-519), or the column's gate when it stands still; at
+    # x = right_wall_x + right_wall_speed * t, pushing towards -x
+    right_wall: bool = False
+    right_wall_x: float = 0.0       # position at t = 0 [m]
+    right_wall_speed: float = 0.0   # velocity along +x [m/s]
 
     def __post_init__(self) -> None:
         for name in ("young", "density", "thickness"):
@@ -48,6 +54,9 @@ class BedConfig:
                 raise ValueError(f"{name} must be finite and nonnegative")
         if self.dt is not None and (not math.isfinite(self.dt) or self.dt <= 0):
             raise ValueError("dt must be finite and positive")
+        for name in ("right_wall_x", "right_wall_speed"):
+            if not math.isfinite(getattr(self, name)):
+                raise ValueError(f"{name} must be finite")
 
     # -- contact constants (scale_ceiling/kernel.py:119-129, 216-227) --------
     @property
@@ -113,3 +122,55 @@ def column_packing(n: int, r0: float, *, polydispersity: float = 0.2, aspect: fl
     y = y + rng.uniform(-slack, slack, size=n) * 0.5
     del rows
     return np.stack((x, y), axis=1), r
+
+
+def column_deposit(n: int, r0: float, *, polydispersity: float = 0.2, aspect: float = 2.0,
+                   seed: int = 0, gap: float = 0.15, packing: float = 0.755):
+    """Random loose column of n disks built by sequential deposition: radii
+    uniform in [(1-p) r0, (1+p) r0]; row by row from the back wall (x = 0),
+    each disk placed right of its row neighbour with a random gap in
+    [0, gap r0) and lowered until it rests on a disk of the rows below or on
+    the floor (y = 0).  Every disk touches at least one support, no two
+    overlap, the area fraction comes out near 0.755 and the width
+    L = sqrt(n pi <r^2> / (aspect packing)) makes H / L close to `aspect`.
+    Returns (x (n,2), r (n,), L), in row-major (spatially coherent) order;
+    the column is not yet in equilibrium (settle it behind a gate at x = L,
+    `bed.settled_column`)."""
+    rng = np.random.default_rng(seed)
+    r = r0 * rng.uniform(1.0 - polydispersity, 1.0 + polydispersity, size=n)
+    width = math.sqrt(n * math.pi * float(np.mean(r * r)) / (aspect * packing))
+    rmax = r0 * (1.0 + polydispersity)
+    # a disk's supports lie within r + rmax of its centre; row neighbours are
+    # at least 2 r_min apart, so five consecutive disks of a row cover that reach
+    reach_w = 2 + int(math.ceil((r0 + rmax) / (r0 * (1.0 - polydispersity))))
+    x = np.empty(n)
+    y = np.empty(n)
+    rows: list[tuple[np.ndarray, np.ndarray, np.ndarray]] = []
+    per_row = int(width / (2.0 * r0 * (1.0 - polydispersity))) + 2
+    k = 0
+    while k < n:
+        m = min(n - k, per_row)
+        rr = r[k:k + m]
+        g = gap * r0 * rng.uniform(0.0, 1.0, size=m)
+        xs = rr[0] + g[0] + np.concatenate(([0.0], np.cumsum(rr[:-1] + rr[1:] + g[1:])))
+        cnt = max(1, int(np.count_nonzero(xs + rr <= width)))
+        rx, rr = xs[:cnt], rr[:cnt]
+        ry = rr.copy()
+        # the rows below that can reach this one: disks of a row are at most
+        # 2 rmax above the lowest support of the row before it
+        for px, py, pr in rows[-4:]:
+            lo = np.searchsorted(px, rx - (rr + rmax))
+            idx = lo[:, None] + np.arange(reach_w)[None, :]
+            ok = idx < len(px)
+            idc = np.minimum(idx, len(px) - 1)
+            dx = px[idc] - rx[:, None]
+            s = pr[idc] + rr[:, None]
+            h2 = s * s - dx * dx
+            ok &= h2 > 0.0
+            need = np.where(ok, py[idc] + np.sqrt(np.maximum(h2, 0.0)), -np.inf)
+            ry = np.maximum(ry, need.max(axis=1))
+        rows.append((rx, ry, rr))
+        x[k:k + cnt] = rx
+        y[k:k + cnt] = ry
+        k += cnt
+    return np.stack((x, y), axis=1), r, width

@@ -45,11 +45,17 @@ enum { MODE_IDLE = 0, MODE_SEARCH = 1, MODE_BIN = 2 };
 enum { OP_STEP = 1, OP_ENERGY = 2 };
 constexpr int kLpBit = 1 << 16;  // list parity, mirrored into each block-table entry
 // ledger partial channels
+// C_COUNTS carries pair contacts + 2^32 x wall contacts: both are integers
+// far below 2^21, so the sums over blocks stay exact in FP64 (a ninth step
+// channel would cost the step kernel a wave per SIMD)
 enum {
-  C_DASH = 0, C_FRIC, C_WDASH, C_WFRIC, C_WFX, C_WFY, C_NPAIR, C_NWALL,
+  C_DASH = 0, C_FRIC, C_WDASH, C_WFRIC, C_WFX, C_WFY, C_COUNTS, C_WWORK,
   C_KIN = 8, C_POT, C_ELN, C_ELT
 };
-constexpr uint32_t kFloorBit = 1u << 16, kBackBit = 1u << 17;
+constexpr int kStepCh = 8;       // step channels C_DASH .. C_WWORK
+constexpr double kWallCount = 4294967296.0;  // 2^32
+constexpr uint32_t kFloorBit = 1u << 16, kBackBit = 1u << 17, kRightBit = 1u << 18;
+constexpr uint32_t kWallBits = kFloorBit | kBackBit | kRightBit;
 
 // Clock, search phase and list state of a member at the start of a step,
 // carried from launch to launch inside one multi-step call (double-buffered
@@ -116,8 +122,8 @@ struct BKArgs {
 // of a workgroup belongs to the same member).
 struct BedConst {
   double e_star, g_star, damping, friction, wall_friction, rho_pi_t, gravity_y, dt;
-  double floor_y, back_wall_x, skin, cell_size, ox, oy;
-  int use_floor, use_back_wall, ncx, ncy, cell_first, block_first, nblocks, mode, tab;
+  double floor_y, back_wall_x, right_x, right_v, time, skin, cell_size, ox, oy;
+  int use_floor, use_back_wall, use_right, ncx, ncy, cell_first, block_first, nblocks, mode, tab;
 };
 
 __device__ inline double uniform(double v)
@@ -140,12 +146,16 @@ __device__ inline BedConst load_const(const tdm_bed_member *mp)
   c.dt = uniform(mp->dt);
   c.floor_y = uniform(mp->floor_y);
   c.back_wall_x = uniform(mp->back_wall_x);
+  c.right_x = uniform(mp->right_wall_x);
+  c.right_v = uniform(mp->right_wall_v);
+  c.time = uniform(mp->time);
   c.skin = uniform(mp->skin);
   c.cell_size = uniform(mp->cell_size);
   c.ox = uniform(mp->grid_origin[0]);
   c.oy = uniform(mp->grid_origin[1]);
   c.use_floor = uniform(mp->use_floor);
   c.use_back_wall = uniform(mp->use_back_wall);
+  c.use_right = uniform(mp->use_right_wall);
   c.ncx = uniform(mp->ncx);
   c.ncy = uniform(mp->ncy);
   c.cell_first = uniform(mp->cell_first);
@@ -399,7 +409,7 @@ __device__ inline int rebuild_list(const BKArgs &k, const BedConst &m, int p, do
     }
   }
   // carry each old slot that was in contact to its place in the new list
-  uint32_t nmask = mask & (kFloorBit | kBackBit);
+  uint32_t nmask = mask & kWallBits;
 #pragma unroll
   for (int o = 0; o < kMaxSlots; ++o) {
     if (o >= nc || !((mask >> o) & 1u)) continue;
@@ -470,11 +480,12 @@ __global__ __launch_bounds__(kThreads) void bed_step_kernel(BKArgs k)
     cur = pd.fresh ? pd : bed_advance(pd, pflags, m.dt);
     m.mode = cur.mode;
     m.tab = cur.tab;
+    m.time = cur.time;
   }
   if (threadIdx.x < 2) sflags[threadIdx.x] = 0;
-  double ch[8];
+  double ch[kStepCh];
 #pragma unroll
-  for (int c = 0; c < 8; ++c) ch[c] = 0.0;
+  for (int c = 0; c < kStepCh; ++c) ch[c] = 0.0;
   int status = 0, req = 0;
   int lp = lp0;
 
@@ -554,32 +565,33 @@ __global__ __launch_bounds__(kThreads) void bed_step_kernel(BKArgs k)
         }
       }
     }
-    double2 *xw = reinterpret_cast<double2 *>(a.xi_wall) + 2 * p;
-    if (m.use_floor) {
-      double2 h = (mask & kFloorBit) ? xw[0] : make_double2(0.0, 0.0);
+    // floor, back wall, right wall: one loop so the law is inlined once
+    double2 *xw = reinterpret_cast<double2 *>(a.xi_wall) + 3 * p;
+#pragma unroll 1
+    for (int w = 0; w < 3; ++w) {
+      if (!(w == 0 ? m.use_floor : (w == 1 ? m.use_back_wall : m.use_right))) continue;
+      // the law sees the disk's velocity relative to the wall (kernel.py:381-389);
+      // the floor and the back wall stand still
+      const double wv = w == 2 ? m.right_v : 0.0;
+      const double gap = w == 0 ? xp.y - m.floor_y
+                                : (w == 1 ? xp.x - m.back_wall_x
+                                          : (m.right_x + m.right_v * m.time) - xp.x);
+      const double nx = w == 0 ? 0.0 : (w == 1 ? 1.0 : -1.0), ny = w == 0 ? 1.0 : 0.0;
+      const uint32_t bit = kFloorBit << w;
+      double2 h = (mask & bit) ? xw[w] : make_double2(0.0, 0.0);
+      const double2 vr = make_double2(vp.x - wv, vp.y);
       double Fx, Fy, dash, fric;
-      if (hm_wall<R>(m, rp, mass, xp.y - m.floor_y, 0.0, 1.0, vp, h, Fx, Fy, dash, fric)) {
-        xw[0] = h;
-        nmask |= kFloorBit;
+      if (hm_wall<R>(m, rp, mass, gap, nx, ny, vr, h, Fx, Fy, dash, fric)) {
+        xw[w] = h;
+        nmask |= bit;
         fx += Fx; fy += Fy;
         ch[C_WDASH] += dash; ch[C_WFRIC] += fric; ch[C_WFX] += Fx; ch[C_WFY] += Fy;
-        ++nwall;
-      }
-    }
-    if (m.use_back_wall) {
-      double2 h = (mask & kBackBit) ? xw[1] : make_double2(0.0, 0.0);
-      double Fx, Fy, dash, fric;
-      if (hm_wall<R>(m, rp, mass, xp.x - m.back_wall_x, 1.0, 0.0, vp, h, Fx, Fy, dash, fric)) {
-        xw[1] = h;
-        nmask |= kBackBit;
-        fx += Fx; fy += Fy;
-        ch[C_WDASH] += dash; ch[C_WFRIC] += fric; ch[C_WFX] += Fx; ch[C_WFY] += Fy;
+        ch[C_WWORK] += Fx * wv;
         ++nwall;
       }
     }
     k.w.mask[(size_t)lp * n + p] = nmask;
-    ch[C_NPAIR] = (double)npair;
-    ch[C_NWALL] = (double)nwall;
+    ch[C_COUNTS] = (double)npair + kWallCount * (double)nwall;
 
     // ---- semi-implicit Euler (dem.py:363-365) -------------------------
     double vx, vy;
@@ -625,7 +637,7 @@ __global__ __launch_bounds__(kThreads) void bed_step_kernel(BKArgs k)
     // not depend on where the member sits in the launch
     const size_t slot = (size_t)(m.block_first + blk.w);
 #pragma unroll
-    for (int c = 0; c < 8; ++c)
+    for (int c = 0; c < kStepCh; ++c)
       k.w.partials[((size_t)pslot * kNch + c) * a.nblocks + slot] = ch[c];
     const int f = sflags[0] | (sflags[1] << 16);
     if (!kDeferred) {
@@ -698,15 +710,16 @@ __global__ __launch_bounds__(kThreads) void bed_energy_kernel(BKArgs k)
         ch[3] += 0.5 * ((8 * m.g_star) * sqrt(reff * d)) * (h.x * h.x + h.y * h.y);
       }
     }
-    const double gaps[2] = {xp.y - m.floor_y, xp.x - m.back_wall_x};
-    const int use[2] = {m.use_floor, m.use_back_wall};
-    for (int w = 0; w < 2; ++w) {
+    const double gaps[3] = {xp.y - m.floor_y, xp.x - m.back_wall_x,
+                            (m.right_x + m.right_v * m.time) - xp.x};
+    const int use[3] = {m.use_floor, m.use_back_wall, m.use_right};
+    for (int w = 0; w < 3; ++w) {
       const double d = rp - gaps[w];
       if (!use[w] || !(d > 0)) continue;
       const double sn_ = (2 * m.e_star) * sqrt(rp * d);
       ch[2] += 0.4 * (((2.0 / 3.0) * sn_) * d) * d;
       if ((mask >> (16 + w)) & 1u) {
-        const double2 h = reinterpret_cast<const double2 *>(a.xi_wall)[2 * p + w];
+        const double2 h = reinterpret_cast<const double2 *>(a.xi_wall)[3 * p + w];
         ch[3] += 0.5 * ((8 * m.g_star) * sqrt(rp * d)) * (h.x * h.x + h.y * h.y);
       }
     }
@@ -793,11 +806,11 @@ template <int U = 4>
 __device__ void bed_member_ledger(const BKArgs &k, int mi, long long step, int flags,
                                   double (*wsum)[kCh])
 {
-  __shared__ double stot[8];
+  __shared__ double stot[kStepCh];
   tdm_bed_member *mp = k.a.members + mi;
   const int b0 = mp->block_first, nb = mp->nblocks;
   __syncthreads();  // wsum may be the caller's reduction scratch
-  bed_reduce<8, U>(k, b0, nb, (int)(step & 1), 0, wsum, stot);
+  bed_reduce<kStepCh, U>(k, b0, nb, (int)(step & 1), 0, wsum, stot);
   if (threadIdx.x != 0) return;
   double *acc = k.a.accounting + (size_t)mi * TDM_BED_NACC;
   const double dt = mp->dt;
@@ -807,8 +820,10 @@ __device__ void bed_member_ledger(const BKArgs &k, int mi, long long step, int f
   acc[TDM_BED_WALL_FRICTION] = acc[TDM_BED_WALL_FRICTION] + dt * stot[C_WFRIC];
   acc[TDM_BED_WALL_IMPULSE] = acc[TDM_BED_WALL_IMPULSE] + dt * stot[C_WFX];
   acc[TDM_BED_WALL_IMPULSE + 1] = acc[TDM_BED_WALL_IMPULSE + 1] + dt * stot[C_WFY];
-  acc[TDM_BED_CONTACTS] = stot[C_NPAIR];
-  acc[TDM_BED_WALL_CONTACTS] = stot[C_NWALL];
+  const double nwall = floor(stot[C_COUNTS] / kWallCount);
+  acc[TDM_BED_CONTACTS] = stot[C_COUNTS] - kWallCount * nwall;
+  acc[TDM_BED_WALL_CONTACTS] = nwall;
+  acc[TDM_BED_WALL_WORK] = acc[TDM_BED_WALL_WORK] + dt * stot[C_WWORK];
   if (flags & 0xFFFF) atomicOr(&mp->status, flags & 0xFFFF);
 }
 
@@ -1034,6 +1049,14 @@ int launch_bed_steps(const tdm_bed_arrays *a, const BedWs &w, int nsteps, int fp
 
 extern "C" {
 
+int tdm_bed_abi_sizes(size_t *member_bytes, size_t *arrays_bytes)
+{
+  if (!member_bytes || !arrays_bytes) return fail(TDM_EINVAL, "null output");
+  *member_bytes = sizeof(tdm_bed_member);
+  *arrays_bytes = sizeof(tdm_bed_arrays);
+  return TDM_OK;
+}
+
 int tdm_bed_blocks(int n)
 {
   if (n <= 0) return fail(TDM_EINVAL, "bed member needs n > 0");
@@ -1107,7 +1130,7 @@ int tdm_bed_setup(const tdm_bed_arrays *a, void *stream)
   if (e != hipSuccess) return fail(TDM_EHIP, hipGetErrorString(e));
   fill_kernel<int><<<64, 256, 0, s>>>(w.bflags, (size_t)a->nblocks, 0);
   fill_kernel<uint32_t><<<1024, 256, 0, s>>>(w.mask, 2 * (size_t)a->n, 0u);
-  fill_kernel<double><<<1024, 256, 0, s>>>(a->xi_wall, 4 * (size_t)a->n, 0.0);
+  fill_kernel<double><<<1024, 256, 0, s>>>(a->xi_wall, 6 * (size_t)a->n, 0.0);
   HIP_TRY(hipGetLastError());
   return tdm_bed_rebin(a, stream);
 }

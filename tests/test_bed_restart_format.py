@@ -42,7 +42,9 @@ def test_round_trip_is_exact(tmp_path):
     (lambda s: s.update(pairs=np.array([[0, 5], [1, 3]])), "i < j"),
     (lambda s: s.update(pairs=np.array([[1, 3], [1, 3]])), "repeated"),
     (lambda s: s.update(pair_xi=np.zeros((3, 2))), "pair_xi shape"),
-    (lambda s: s.update(wall_active=np.array([4, 0, 0, 0, 0])), "wall_active"),
+    (lambda s: s.update(wall_active=np.array([8, 0, 0, 0, 0])), "wall_active"),
+    (lambda s: s.update(wall_active=np.array([4, 0, 0, 0, 0])), "right wall"),
+    (lambda s: s.update(wall_xi=np.zeros((5, 5))), "wall_xi shape"),
     (lambda s: s.update(dt=0.0), "timestep"),
     (lambda s: s.update(step_count=-1), "step_count"),
 ])
@@ -51,3 +53,10 @@ def test_inconsistent_states_are_rejected(edit, msg):
     edit(s)
     with pytest.raises(ValueError, match=msg):
         validate_bed_restart(s)
+
+
+def test_three_wall_states_validate():
+    s = state()
+    s["wall_xi"] = np.zeros((5, 6))
+    s["wall_active"] = np.array([4, 5, 7, 0, 2], np.uint8)
+    validate_bed_restart(s)

@@ -250,3 +250,126 @@ def test_pair_coulomb_cap_and_friction_power():
     reff = r0 * r1 / (r0 + r1)
     assert b.force[1, 1] == pytest.approx(8 * cfg.g_star * math.sqrt(reff * delta) * 1e-4 * dt,
                                           rel=1e-12)
+
+
+# --- the right wall: indenter (moving) or gate (at rest), law on the velocity
+# relative to the wall (scale_ceiling/kernel.py:338-401) ---------------------
+
+def _right(delta, v, wall_v=0.0, dt=1e-7, **kw):
+    base = dict(gravity=0.0, floor=False, back_wall=False, right_wall=True, right_wall_x=1.0,
+                right_wall_speed=wall_v)
+    base.update(kw)
+    cfg = BedConfig(**base)
+    b = OracleBed(cfg, np.array([[1.0 - (R - delta), 0.5]]), np.array([R]), dt=dt)
+    b.velocities[0] = v
+    return cfg, b
+
+
+def test_right_wall_static_hertz_and_galilean_invariance():
+    delta = 4e-5
+    cfg, b = _right(delta, [0.0, 0.0])
+    b.step(integrate=False)
+    delta = R - (1.0 - b.positions[0, 0])      # the overlap as stored, to the last bit
+    expect = (4.0 / 3.0) * cfg.e_star * math.sqrt(R) * delta ** 1.5
+    assert b.force[0, 0] == pytest.approx(-expect, rel=1e-12)
+    assert b.force[0, 1] == 0.0
+    static = b.force.copy()
+    # disk and wall moving together: no viscous term, the same force bit for bit
+    cfg, b = _right(delta, [-0.7, 0.0], wall_v=-0.7)
+    b.step(integrate=False)
+    assert np.array_equal(b.force, static)
+    # a disk carried along at the wall's speed sees no work from it either
+    # (the work is F . v_wall with F on the disk, here the static force)
+    assert b.last[8] == pytest.approx(static[0, 0] * -0.7, rel=1e-15)
+
+
+def test_right_wall_approach_damping_and_slip():
+    delta, U = 4e-5, 0.3
+    cfg, b = _right(delta, [0.0, 0.0], wall_v=-U, restitution=0.5)
+    b.step(integrate=False)
+    delta = R - (1.0 - b.positions[0, 0])
+    m = cfg.rho_pi_t * R * R
+    s_n = 2 * cfg.e_star * math.sqrt(R * delta)
+    f_el = (2.0 / 3.0) * s_n * delta
+    assert b.force[0, 0] == pytest.approx(-(f_el + cfg.damping_coefficient *
+                                            math.sqrt(s_n * m) * U), rel=1e-12)
+    # sliding along the wall at w: xi = w dt, force -s_t xi on the disk
+    w, dt = 2e-3, 1e-7
+    cfg, b = _right(delta, [0.0, w], dt=dt, wall_friction=10.0)
+    b.step(integrate=False)
+    delta = R - (1.0 - b.positions[0, 0])
+    s_t = 8 * cfg.g_star * math.sqrt(R * delta)
+    assert b.xi_right[0] == pytest.approx([0.0, w * dt], rel=1e-15)
+    assert b.force[0, 1] == pytest.approx(-s_t * w * dt, rel=1e-12)
+
+
+def test_right_wall_moves_with_the_clock():
+    """A free disk 2 mm left of the contact distance and a wall closing at
+    0.4 m/s: the first step in contact is the first whose start time puts
+    the wall within r of the disk centre."""
+    U, dt, gap = 0.4, 1e-6, 2e-3
+    cfg = BedConfig(gravity=0.0, floor=False, back_wall=False, right_wall=True,
+                    right_wall_x=1.0, right_wall_speed=-U)
+    b = OracleBed(cfg, np.array([[1.0 - R - gap, 0.5]]), np.array([R]), dt=dt)
+    first = None
+    for k in range(10000):
+        b.step()
+        if b.last[7] > 0:
+            first = k
+            break
+    t_k = [0.0]
+    for _ in range(first):
+        t_k.append(t_k[-1] + dt)
+    assert (1.0 - U * t_k[-1]) - (1.0 - R - gap) < R
+    assert (1.0 - U * t_k[-2]) - (1.0 - R - gap) >= R
+    assert first == pytest.approx(gap / (U * dt), abs=1)
+
+
+@pytest.mark.parametrize("e", [1.0, 0.5])
+def test_disk_struck_by_a_moving_indenter(e):
+    """A wall at constant speed U strikes a resting free disk.  In the wall's
+    frame this is the floor impact: the disk leaves at U (1 + e_real) with
+    e_real the restitution of the same impact (RK4 reference, 1 %); the wall
+    impulse in the ledger is m v_final and its work is the kinetic energy
+    gained plus what the wall's dashpot absorbed (to the discretisation of
+    the impact)."""
+    U, dt = 0.2, 2e-7
+    cfg = BedConfig(gravity=0.0, floor=False, back_wall=False, right_wall=True,
+                    right_wall_x=1.0, right_wall_speed=-U, restitution=e, wall_friction=0.0)
+    b = OracleBed(cfg, np.array([[1.0 - R - 1e-6, 0.5]]), np.array([R]), dt=dt)
+    touched = False
+    for _ in range(200000):
+        b.step()
+        touched = touched or b.last[7] > 0
+        if touched and b.last[7] == 0:
+            break
+    m = cfg.rho_pi_t * R * R
+    v = -b.velocities[0, 0]
+    e_real = 1.0 if e == 1.0 else _impact_restitution_rk4(cfg, U, R)
+    assert v == pytest.approx(U * (1.0 + e_real), rel=0.01)
+    assert b.acc[4] == pytest.approx(-m * v, rel=1e-12)        # wall impulse on the disk
+    # work of the wall = kinetic energy gained + energy its dashpot absorbed
+    assert b.acc[6] == pytest.approx(0.5 * m * v * v + b.acc[2], rel=0.002)
+    if e < 1.0:
+        assert b.acc[2] > 0.1 * b.acc[6]
+    if e == 1.0:
+        assert b.acc[2] == 0.0 and abs(b.acc[6] - 0.5 * m * v * v) < 2e-3 * 0.5 * m * v * v
+
+
+def test_deposited_column_is_loose_overlap_free_and_supported():
+    from tensordem_amd.bedconfig import column_deposit
+    x, r, width = column_deposit(20000, R, seed=2)
+    assert (x[:, 0] - r).min() >= 0 and (x[:, 0] + r).max() <= width and (x[:, 1] - r).min() >= 0
+    pr = pairs_within(x, 2.4 * R * 1.0001)
+    d = np.sqrt(((x[pr[:, 1]] - x[pr[:, 0]]) ** 2).sum(1))
+    gap = d - (r[pr[:, 0]] + r[pr[:, 1]])
+    assert gap.min() > -1e-12 * R                       # no overlaps beyond round-off
+    touch = np.abs(gap) < 1e-12 * R
+    supported = np.zeros(len(r), bool)
+    supported[pr[touch, 1]] = True                       # resting on a lower-index disk
+    supported |= np.abs(x[:, 1] - r) < 1e-15
+    assert supported.all()
+    h = (x[:, 1] + r).max()
+    phi = math.pi * (r * r).sum() / (width * h)
+    assert 0.73 < phi < 0.78
+    assert h / width == pytest.approx(2.0, rel=0.05)
