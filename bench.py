@@ -461,119 +461,231 @@ def graph_steps(k, cap):
     return steps, chunk
 
 
+BED_FULL_STEPS = 50000
+BED_WINDOWS = 10
+_C3_RELEASE = None
+
+
 def c3_inputs():
-    from tensordem_amd.bedconfig import BedConfig, column_packing
-    x, r = column_packing(BED_N, BED_R0, seed=0)
+    """Law, deposited positions and radii of the column (the host baseline's
+    sample; the GPU runs start from the settled state of the same column)."""
+    from tensordem_amd.bedconfig import BedConfig, column_deposit
+    x, r, _ = column_deposit(BED_N, BED_R0, seed=0)
     return BedConfig(), x, r
 
 
+def c3_release():
+    """The release state of the column collapse: the deposited column
+    settled to rest behind a gate on the device, gate removed (computed
+    once per process, about 250 000 FP32 steps)."""
+    global _C3_RELEASE
+    if _C3_RELEASE is None:
+        from tensordem_amd.bed import settled_column
+        t0 = time.perf_counter()
+        state, info = settled_column(BED_N, BED_R0, seed=0, max_steps=400000)
+        info["seconds"] = time.perf_counter() - t0
+        print(f"bench.py: column settled in {info['settle_steps']} steps "
+              f"({info['seconds']:.1f} s), {info['contact_pairs']} contact pairs", file=sys.stderr)
+        _C3_RELEASE = (state, info)
+    return _C3_RELEASE
+
+
+def bed_windows(steps, full=BED_FULL_STEPS, windows=BED_WINDOWS):
+    """[(start, n)]: the timed steps split into windows spread over the
+    collapse when fewer than `full` steps are timed (start steps even), else
+    one window from release."""
+    if steps >= full:
+        return [(0, steps)]
+    # windows of at least 4 steps: each short window carries one gap between
+    # graph launches (a few microseconds)
+    w = max(1, min(windows, steps // 4))
+    per = [steps // w + (1 if k < steps % w else 0) for k in range(w)]
+    span = full - 2 * max(per)   # a short window runs k untimed lead steps first
+    starts = [0 if w == 1 else 2 * round(k * span / (w - 1) / 2) for k in range(w)]
+    return list(zip(starts, per))
+
+
+def _advance(eng, k):
+    if k > 0:
+        eng.step(k)
+
+
 def bench_bed(args):
-    """The 1M-disk column collapse: FP32 and FP64 forces over the full step
-    count from release, kernel events with list statistics on a second pass,
-    the plain FP64 path from the half-way state, the oracle on a CPU sample.
-    With --bed-from S (or --bed-start FILE) every engine starts instead from
-    the state S steps into the collapse, carried over with its spring
-    histories by the bed restart."""
+    """The 1M-disk column collapse from its settled release state: FP32 and
+    FP64 forces over the timed steps (windows spread over the 50 000-step
+    collapse when fewer are timed, advanced untimed between them), kernel
+    events and list statistics at the same windows, the plain FP64 path
+    from the state at the middle window, the oracle on a CPU sample.  With
+    --bed-from S (or --bed-start FILE) every engine starts instead from the
+    state S steps into the collapse, carried over with its spring histories
+    by the bed restart, and the steps run in one window."""
     import torch
     from tensordem_amd import _lib
-    from tensordem_amd.bed import BedEngine, BedSpec, load_bed_restart, save_bed_restart
+    from tensordem_amd.bed import BedEngine, load_bed_restart, save_bed_restart
 
-    cfg, x, r = c3_inputs()
-    n = len(r)
-    steps, chunk = graph_steps(args.steps, 1000)
-    state = None
+    cfg = None
     if args.bed_start:
-        state = load_bed_restart(args.bed_start)
-    elif args.bed_from:
-        adv = BedEngine([BedSpec(cfg, x, r)], fp32=True)
-        adv.run_graph(chunk // 2, repeats=max(1, args.bed_from // chunk))
-        state = adv.snapshot()
-        adv.close()
-        del adv
-        if args.bed_save_start:
-            save_bed_restart(state, args.bed_save_start)
+        state, info = load_bed_restart(args.bed_start), {}
+    else:
+        state, info = c3_release()
+        if args.bed_from:
+            adv = BedEngine.from_snapshot(state, fp32=True)
+            _advance(adv, args.bed_from)
+            state = adv.snapshot()
+            adv.close()
+            del adv
+            if args.bed_save_start:
+                save_bed_restart(state, args.bed_save_start)
+    n = len(state["radii"])
+    start = int(state["step_count"])
+    steps = max(2, int(args.steps))
+    wins = bed_windows(steps, windows=args.bed_windows) if start == 0 else [(0, steps)]
 
     def engine(fp32):
-        if state is not None:
-            return BedEngine.from_snapshot(state, fp32=fp32)
-        return BedEngine([BedSpec(cfg, x, r)], fp32=fp32)
+        return BedEngine.from_snapshot(state, fp32=fp32)
 
     warm = engine(True)
-    wsteps, wchunk = graph_steps(max(2, args.warmup), 1000)
-    warm.run_graph(wchunk // 2, repeats=wsteps // wchunk)
+    wsteps = max(0, int(args.warmup))
+    _advance(warm, wsteps)
     torch.cuda.synchronize()
-    dt = warm.read_members()[0].dt
-    skin = warm.read_members()[0].skin
+    m0 = warm.read_members()[0]
+    dt, skin = m0.dt, m0.skin
     warm.close()
     del warm
-    start = 0 if state is None else int(state["step_count"])
+
+    def run(eng, timed):
+        """Walk the windows: advance untimed to each start, then `timed(eng,
+        k)`, which returns (result, steps it advanced)."""
+        at, out = 0, []
+        for s0, k in wins:
+            _advance(eng, s0 - at)
+            res, adv = timed(eng, k)
+            out.append(res)
+            at = s0 + adv
+        return out
+
+    # a short window replays its graph once untimed first (the first replay
+    # of a graph uploads it), so it times steps k .. 2k after its start
+    lead = [k - (k & 1) if k < 1000 else 0 for _, k in wins]
+
+    def graph_window(eng, k):
+        """k timed steps from graph replays.  A short window first replays its
+        graph once untimed and queues the timed replay right behind it, with
+        no host wait in between: the GPU goes from the lead replay straight
+        into the timed one, so the start event does not count the host's
+        launch latency."""
+        stream = torch.cuda.current_stream()
+        even = k - (k & 1)
+        _, chunk = graph_steps(even, 1000) if even >= 2 else (0, 0)
+        short = k < 1000
+        if chunk:
+            eng.graph(chunk // 2)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        if short and chunk:
+            eng.run_graph(chunk // 2, repeats=even // chunk)   # untimed lead
+        e0.record(stream)
+        if chunk:
+            eng.run_graph(chunk // 2, repeats=even // chunk)
+        if k % 2:
+            eng.step(1)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) * 1e-3, k + (even if short else 0)
+
     out = {}
     for fp32 in (True, False):
         eng = engine(fp32)
-        eng.graph(chunk // 2)
-        stream = torch.cuda.current_stream()
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        torch.cuda.synchronize()
-        e0.record(stream)
-        eng.run_graph(chunk // 2, repeats=steps // chunk)
-        e1.record(stream)
-        torch.cuda.synchronize()
-        gpu_s = e0.elapsed_time(e1) * 1e-3
+        secs = run(eng, graph_window)
+        gpu_s = sum(secs)
         acc = eng.accounting[0].cpu().numpy()
         out["fp32" if fp32 else "fp64"] = dict(
             seconds=gpu_s, ms_per_step=gpu_s / steps * 1e3, rate=n * steps / gpu_s,
+            window_ms_per_step=[s_ / k * 1e3 for s_, (_, k) in zip(secs, wins)],
             status=int(eng.status()[0]), contacts_end=int(acc[_lib.BED_CONTACTS]),
             stats_end=eng.list_stats(), rebuilds=int(eng.read_members()[0].rebuilds))
         eng.close()
         del eng
-    # kernel-event pass over the same run, sampling the lists for the byte count
+    # kernel-event pass over the same windows (HIP events on every step-kernel
+    # dispatch; event timing needs an even step count per call), sampling the
+    # lists for the byte count; the plain path starts from the middle window
     eng = engine(True)
-    kus, byts, half = [], [], None
-    nchunks = steps // chunk
-    stride = max(1, nchunks // 10)
-    for k in range(nchunks):
-        kus.append(eng.step_timed(chunk))
-        if k % stride == stride - 1 or k == nchunks - 1:
+    half_at = wins[len(wins) // 2][0]
+    half = None
+
+    def event_window(eng, k):
+        """Kernel events over the window.  A short window is timed by one call
+        queued right behind its untimed lead steps (the GPU stays busy, so its
+        clocks do not drop in a host gap); a long one in ten calls, the lists
+        sampled after each."""
+        nonlocal half
+        lk = k - (k & 1) if k < 1000 else 0
+        _advance(eng, lk)
+        kk = k + (k & 1)
+        calls = 1 if k < 1000 else 10
+        stride = max(2, (kk // calls) + ((kk // calls) & 1))
+        us, byts, done = [], [], 0
+        while done < kk:
+            c = min(stride, kk - done)
+            us.append((eng.step_timed(c), c))
+            done += c
             st = eng.list_stats()
             acc = eng.accounting[0].cpu().numpy()
             byts.append(bed_bytes_per_disk(st["cand_mean"], st["contacts_per_disk"],
                                            acc[_lib.BED_WALL_CONTACTS] / n))
-        if k == max(0, nchunks // 2 - 1):
-            half = (eng.x.cpu().numpy(), eng.v.cpu().numpy())
-    kernel_s = sum(kus) / len(kus) * 1e-6
+            if half is None and eng.read_members()[0].step_count - start >= half_at:
+                half = (eng.x.cpu().numpy(), eng.v.cpu().numpy())
+        return (us, byts), lk + kk
+
+    ev = run(eng, event_window)
     eng.close()
     del eng
+    us_all = [u for w_us, _ in ev for u in w_us]
+    kernel_s = sum(u * c for u, c in us_all) / sum(c for _, c in us_all) * 1e-6
+    byts = [b for _, w_b in ev for b in w_b]
     bpd = sum(byts) / len(byts)
     achieved = bpd * n / kernel_s / 1e9
     r32, r64 = out["fp32"], out["fp64"]
+    windowed = len(wins) > 1
     res = {
         "workload": f"granular column collapse, {n} disks, radii U[0.8,1.2]x{BED_R0} m, "
                     "E=1 GPa, nu=0.3, mu=0.5, e=0.5 (viscous), rho=917, t=0.02 m, "
-                    f"aspect 2 loose column (seed 0), floor + back wall, gravity, "
-                    f"dt={dt:.6g} s, {steps} steps from "
-                    + ("release" if not start else f"step {start} of the collapse"),
+                    "aspect 2 random loose column (sequential deposition, seed 0) settled to "
+                    "rest behind a gate, released, floor + back wall, gravity, "
+                    f"dt={dt:.6g} s, "
+                    + (f"{steps} steps in {len(wins)} windows spread over the "
+                       f"{BED_FULL_STEPS}-step collapse" if windowed else
+                       f"{steps} steps from "
+                       + ("release" if not start else f"step {start} of the collapse")),
         "value": r32["rate"], "unit": "particle-steps/s", "steps": steps, "warmup": wsteps,
         "ms_per_step": r32["ms_per_step"], "dtype": "f32-force/f64-state",
-        "skin_m": skin, "graph_chunk_steps": chunk, "start_step": start,
+        "windows": [{"start_step": s0 + ld + start, "steps": k}
+                    for (s0, k), ld in zip(wins, lead)],
+        "window_ms_per_step": r32["window_ms_per_step"],
+        "skin_m": skin, "start_step": start,
+        "release_state": {k: info[k] for k in ("settle_steps", "contact_pairs", "kinetic_rel",
+                                                "width_m", "height_m") if k in info},
         "status_bits": r32["status"], "contact_pairs_end": r32["contacts_end"],
         "lists_end": r32["stats_end"], "list_rebuilds": r32["rebuilds"],
         "fp64_kernel": {"value": r64["rate"], "ms_per_step": r64["ms_per_step"],
                         "status_bits": r64["status"], "contact_pairs_end": r64["contacts_end"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": traffic_from_profile("bed_step_c3_counters.md" if not start
-                                                     else "bed_step_c3_dense_counters.md"),
-                     "traffic_sample": "counted over the release phase (the dense phase, "
-                                       "profiles/bed_step_c3_dense_counters.md, reads 1.23x "
-                                       "more)" if not start else "counted over the dense phase",
+                     "traffic": traffic_from_profile("bed_step_c3_counters.md"),
+                     "traffic_sample": "counted over the collapse windows of "
+                                       "profiles/bed_step_c3_counters.md",
                      "kernel": "bed_step_kernel<float, true>", "kernel_us": kernel_s * 1e6,
+                     "kernel_timing": "HIP events on every step-kernel dispatch of "
+                                      "tdm_bed_step_timed calls at the timed windows",
                      "bytes_per_disk": bpd, "bytes_per_launch": bpd * n},
     }
     for name, rr in out.items():
         if rr["status"]:
             print(f"bench.py: bed {name} run ended with status bits {rr['status']}; its rate "
                   "is not a valid measurement", file=sys.stderr)
-    if not args.no_plain:
+    if not args.no_plain and half is not None:
+        r = np.asarray(state["radii"])
+        cfg = BedConfig_from(state)
         p = PlainBedFP64(cfg, half[0], r, dt, skin, v=half[1])
         for _ in range(3):
             p.step()
@@ -588,14 +700,39 @@ def bench_bed(args):
             steps=args.bed_plain_steps, list_rebuilds=p.rebuilds - 1,
             what="eager PyTorch FP64 bed step on the same GPU (sort-based cell list, Verlet "
                  "list rebuilt at |dx|inf > skin/4, searchsorted history remap, index_add_), "
-                 f"from the state at step {start + steps // 2}")
+                 f"from the state at step {start + half_at}")
         res["speedup_vs_plain_fp64"] = r32["rate"] * ps / n
         del p
     if not args.no_cpu:
-        one = cpu_baseline_bed(cfg, x, r, dt, args.cpu_seconds)
+        ccfg, cx, cr = c3_inputs()
+        one = cpu_baseline_bed(ccfg, cx, cr, dt, args.cpu_seconds)
         res["cpu_baseline"] = with_pool(one, getattr(args, "cpu_pools", {}).get("bed"),
-                                        args.cpu_seconds, f"the {len(r)}-disk column from release")
+                                        args.cpu_seconds,
+                                        f"the {len(cr)}-disk deposited column (before settling)")
     return res
+
+
+def save_bed_phases(out: Path):
+    """Release state of the column and the states 25 000 and 48 000 steps
+    into its collapse (FP32 forces), as bed restart files for profiling runs
+    that should not replay the settling or the collapse under a profiler."""
+    from tensordem_amd.bed import BedEngine, save_bed_restart
+    out.mkdir(parents=True, exist_ok=True)
+    state, _ = c3_release()
+    save_bed_restart(state, out / "c3_release.npz")
+    eng = BedEngine.from_snapshot(state, fp32=True)
+    at = 0
+    for name, s_ in (("c3_mid.npz", 25000), ("c3_late.npz", 48000)):
+        _advance(eng, s_ - at)
+        at = s_
+        save_bed_restart(eng.snapshot(), out / name)
+    eng.close()
+    print(f"bench.py: wrote {out}/c3_release.npz, c3_mid.npz, c3_late.npz", file=sys.stderr)
+
+
+def BedConfig_from(state):
+    from tensordem_amd.bedconfig import BedConfig
+    return BedConfig(**state["config"])
 
 
 def with_pool(one, pool, seconds, what):
@@ -618,7 +755,7 @@ def with_pool(one, pool, seconds, what):
 
 
 def cpu_baseline_bed(cfg, x, r, dt, budget_s):
-    """C oracle (FP64, one core) on the same 1M-disk column from release."""
+    """C oracle (FP64, one core) on the 1M-disk deposited column."""
     from oracle.bed import OracleBed
     sim = OracleBed(cfg, x, r, dt=dt)
     sim.step()
@@ -629,7 +766,7 @@ def cpu_baseline_bed(cfg, x, r, dt, budget_s):
         steps += 1
     el = time.perf_counter() - t0
     return dict(value=len(r) * steps / el, unit="particle-steps/s", cores=1, kind="port",
-                sample=f"{steps} steps of the {len(r)}-disk column from release (FP64 C oracle, "
+                sample=f"{steps} steps of the {len(r)}-disk deposited column (FP64 C oracle, "
                        f"cell grid rebuilt every step), {el:.1f} s")
 
 
@@ -1010,6 +1147,12 @@ def main(argv=None):
                     help="start the bed timing from a bed restart file")
     ap.add_argument("--bed-save-start", default=None,
                     help="write the --bed-from start state to this restart file")
+    ap.add_argument("--bed-windows", type=int, default=BED_WINDOWS,
+                    help="timed windows spread over the collapse when fewer than 50000 "
+                         "column steps are timed (1: one window from release)")
+    ap.add_argument("--bed-save-phases", default=None,
+                    help="settle the column, write its release state and the states 25000 "
+                         "and 48000 steps into the collapse to this directory, and exit")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a 512 MiB device copy (for counter calibration)")
     args = ap.parse_args(argv)
@@ -1024,6 +1167,9 @@ def main(argv=None):
         args.warmup = 1000
     if args.copy_calibration:
         copy_calibration()
+        return 0
+    if args.bed_save_phases:
+        save_bed_phases(Path(args.bed_save_phases))
         return 0
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: every plate and request of this workload runs on a "

@@ -10,8 +10,8 @@ runs this module.
     python -m oracle.cpu_worker plate --dt DT --wait
     python -m oracle.cpu_worker disk --dt DT --wait
 
-`bed`: the 1 048 576-disk column (radii U[0.8, 1.2] x 0.01 m, seed 0) from
-release; `plate`: the 1024 x 128 plate indentation run from rest (indenter
+`bed`: the 1 048 576-disk deposited column (radii U[0.8, 1.2] x 0.01 m,
+seed 0) before settling; `plate`: the 1024 x 128 plate indentation run from rest (indenter
 r = 0.5 m at 0.5 m/s, strains 0.01 / 0.02); `disk`: a 256 x 256 box of rigid
 disks with rotation (radii U[0.9, 1.1] x 0.01 m on a jittered grid, walls,
 gravity, mu = 0.5, mu_rolling = 0.05, seed 0).  With --wait, the worker prints
@@ -34,8 +34,8 @@ if str(ROOT) not in sys.path:
 
 def _bed(dt):
     from oracle.bed import OracleBed
-    from tensordem_amd.bedconfig import BedConfig, column_packing
-    x, r = column_packing(1 << 20, 0.01, seed=0)
+    from tensordem_amd.bedconfig import BedConfig, column_deposit
+    x, r, _ = column_deposit(1 << 20, 0.01, seed=0)
     return OracleBed(BedConfig(), x, r, dt=dt), len(r)
 
 
