@@ -730,6 +730,54 @@ def save_bed_phases(out: Path):
     print(f"bench.py: wrote {out}/c3_release.npz, c3_mid.npz, c3_late.npz", file=sys.stderr)
 
 
+def bench_horizon(args):
+    """How long FP32-force runs of the 1024 x 128 plate keep the FP64
+    broken-bond set, against the re-check margin and the time step: the FP64
+    run (bit-identical to the FP64 oracle over its 20 000 steps) and FP32 runs
+    with fp32_margin 1e-3, 1e-2 and 1e-1 step in lockstep; every 100 steps
+    their alive masks are compared.  The same at dt / 2 over the same
+    simulated time.  Reports each run's first differing checkpoint, its
+    broken bonds at the end and its mean time per step."""
+    import torch
+    from tensordem_amd.engine import PlateEngine, PlateSpec
+
+    cfg = c2_config()
+    dt0 = cfg.degree_bound_dt()
+    rows = []
+    for dt, total, margins in ((dt0, args.plate_steps, (1e-3, 1e-2, 1e-1)),
+                               (0.5 * dt0, 2 * args.plate_steps, (1e-3, 1e-1))):
+        ref = PlateEngine([PlateSpec(cfg, dt=dt)], fp32=False)
+        runs = {m: PlateEngine([PlateSpec(cfg, dt=dt)], fp32=True, fp32_margin=m)
+                for m in margins}
+        first = {m: None for m in margins}
+        secs = {m: 0.0 for m in margins}
+        every = 100
+        for k in range(every, total + 1, every):
+            ref.step(every)
+            for m, eng in runs.items():
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                eng.step(every)
+                torch.cuda.synchronize()
+                secs[m] += time.perf_counter() - t0
+                if first[m] is None and not torch.equal(eng.alive, ref.alive):
+                    first[m] = k
+        nb_ref = int((~ref.alive).sum())
+        for m, eng in runs.items():
+            rows.append({"dt_s": dt, "steps": total, "fp32_margin": m,
+                         "first_difference_step": first[m],
+                         "equal_through_step": (first[m] - every) if first[m] else total,
+                         "equal_through_time_s": ((first[m] - every) if first[m] else total) * dt,
+                         "broken_end": int((~eng.alive).sum()), "broken_end_fp64": nb_ref,
+                         "ms_per_step": secs[m] / total * 1e3,
+                         "status_bits": int(eng.status()[0])})
+            eng.close()
+        ref.close()
+    return {"workload": "1024x128 plate indentation, FP32 forces vs FP64, broken-bond sets "
+                        "compared every 100 steps", "value": 0.0, "steps": args.plate_steps,
+            "ms_per_step": 0.0, "dtype": "f32-force/f64-state", "horizon": rows}
+
+
 def BedConfig_from(state):
     from tensordem_amd.bedconfig import BedConfig
     return BedConfig(**state["config"])
@@ -1131,11 +1179,13 @@ def main(argv=None):
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--disk-steps", type=int, default=2000)
     ap.add_argument("--disk-warmup", type=int, default=200)
-    ap.add_argument("--workload", choices=("all", "plate", "bed", "ensemble", "serve", "disk"),
+    ap.add_argument("--workload", choices=("all", "plate", "bed", "ensemble", "serve", "disk",
+                                           "horizon"),
                     default="all",
                     help="plate: the 1024x128 plate only; bed: the 1M-disk column only; "
                          "ensemble: the 512-plate batch only; serve: the request loop only; "
-                         "disk: the 1M-disk granular2d box only")
+                         "disk: the 1M-disk granular2d box only; horizon: FP32 broken-bond "
+                         "horizon of the 1024x128 plate against fp32_margin and dt")
     ap.add_argument("--serve-window", type=float, default=60.0)
     ap.add_argument("--serve-chunk", type=int, default=250)
     ap.add_argument("--serve-slots", type=int, default=64)
@@ -1197,6 +1247,9 @@ def main(argv=None):
 
 
 def run_workloads(args):
+    if args.workload == "horizon":
+        print(json.dumps(bench_horizon(args)))
+        return 0
     if args.workload in ("bed", "ensemble", "serve", "disk"):
         bed = {"bed": bench_bed, "ensemble": bench_ensemble, "serve": bench_serve,
                "disk": bench_disk}[args.workload](args)

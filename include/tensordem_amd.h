@@ -297,6 +297,10 @@ typedef struct tdm_bed_arrays {
   void *workspace;
   size_t workspace_bytes;
   int32_t n, nmembers, nblocks, ncells, cand_slots, cell_cap;
+  int32_t fp32_histories;        /* 1: the spring histories in the workspace are float2 (a bed
+                                    stepped with FP32 forces), 0: double2; every step call
+                                    must pass the matching fp32 flag */
+  int32_t pad_;
 } tdm_bed_arrays;
 
 /* sizeof(tdm_bed_member) and sizeof(tdm_bed_arrays) as compiled. */
@@ -309,7 +313,9 @@ size_t tdm_bed_workspace_bytes(int n, int nblocks, int nmembers, int ncells, int
  * [2][slots][N]), their spring histories (double2 [2][slots][N]), the
  * contact masks (uint32 [2][N]: bit e = slot e in contact, bits 16/17 =
  * floor/back wall) and counts (uint8 [2][N]), index 0/1 being the list
- * parity; and of each disk's position when its list was built (double2 [N]). */
+ * parity; and of each disk's position when its list was built (double2 [N]).
+ * With fp32_histories the histories are float2 [2][slots][N] at the same
+ * offset. */
 int tdm_bed_workspace_layout(int n, int nblocks, int nmembers, int ncells, int cand_slots,
                              int cell_cap, size_t *cand_off, size_t *xi_off, size_t *mask_off,
                              size_t *count_off, size_t *xb_off);

@@ -92,6 +92,7 @@ class BedArrays(ctypes.Structure):
         ("workspace", ctypes.c_void_p), ("workspace_bytes", ctypes.c_size_t),
         ("n", ctypes.c_int32), ("nmembers", ctypes.c_int32), ("nblocks", ctypes.c_int32),
         ("ncells", ctypes.c_int32), ("cand_slots", ctypes.c_int32), ("cell_cap", ctypes.c_int32),
+        ("fp32_histories", ctypes.c_int32), ("pad_", ctypes.c_int32),
     ]
 
 

@@ -215,6 +215,7 @@ class BedEngine:
         a.workspace_bytes = self.workspace.numel() - (a.workspace - self.workspace.data_ptr())
         a.n, a.nmembers, a.nblocks, a.ncells = self.n, self.nmembers, self.nblocks, self.ncells
         a.cand_slots, a.cell_cap = self.cand_slots, self.cell_cap
+        a.fp32_histories = int(self.fp32)
 
     def _swap(self) -> None:
         self.x, self.x_alt = self.x_alt, self.x
@@ -284,7 +285,7 @@ class BedEngine:
         inv[perm] = torch.arange(self.n, dtype=torch.int64, device=dev)
         S = self.cand_slots
         cand = self._ws_view(0, torch.int32, (2, S, self.n))
-        xi = self._ws_view(1, torch.float64, (2, S, self.n, 2))
+        xi = self.hist_view()
         mask = self._ws_view(2, torch.int32, (2, self.n))
         count = self._ws_view(3, torch.uint8, (2, self.n))
         xb = self._ws_view(4, torch.float64, (self.n, 2))
@@ -336,6 +337,12 @@ class BedEngine:
     def status(self) -> np.ndarray:
         return np.array([m.status for m in self.read_members()], dtype=np.int32)
 
+    def hist_view(self) -> torch.Tensor:
+        """The spring histories of both list buffers, (2, slots, n, 2), in the
+        precision the bed steps in (float32 with FP32 forces)."""
+        dt = torch.float32 if self.fp32 else torch.float64
+        return self._ws_view(1, dt, (2, self.cand_slots, self.n, 2))
+
     def _ws_view(self, which: int, dtype, shape):
         off = self._ws_base() - self.workspace.data_ptr() + self._layout[which]
         nbytes = int(np.prod(shape)) * torch.empty((), dtype=dtype).element_size()
@@ -369,7 +376,7 @@ class BedEngine:
         S = self.cand_slots
         lp = m.list_parity
         cand = self._ws_view(0, torch.int32, (2, S, self.n))[lp][:, sl].cpu().numpy()
-        xi = self._ws_view(1, torch.float64, (2, S, self.n, 2))[lp][:, sl].cpu().numpy()
+        xi = self.hist_view()[lp][:, sl].double().cpu().numpy()
         mask = self._ws_view(2, torch.int32, (2, self.n))[lp][sl].cpu().numpy().astype(np.int64)
         nc = self._ws_view(3, torch.uint8, (2, self.n))[lp][sl].cpu().numpy()
         p_loc, q_loc, h = [], [], []
@@ -427,7 +434,7 @@ class BedEngine:
         np.add.at(mask, disk, (1 << slot).astype(np.int64))
         mask |= np.asarray(state["wall_active"], np.int64) << 16
         eng._ws_view(0, torch.int32, (2, S, n))[0].copy_(torch.from_numpy(cand).to(dev))
-        eng._ws_view(1, torch.float64, (2, S, n, 2))[0].copy_(torch.from_numpy(xi).to(dev))
+        eng.hist_view()[0].copy_(torch.from_numpy(xi).to(dev))
         eng._ws_view(2, torch.int32, (2, n))[0].copy_(
             torch.from_numpy(mask.astype(np.uint32).view(np.int32)).to(dev))
         eng._ws_view(3, torch.uint8, (2, n))[0].copy_(

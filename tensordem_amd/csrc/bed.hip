@@ -82,6 +82,21 @@ struct BedWs {
   int *mflags;       // [nmembers][4]: status | request << 16 of steps s % 3
 };
 
+// Spring histories are stored in the precision of the forces: FP64 steps keep
+// double2, FP32 steps float2 (their histories are computed in FP32, so the
+// narrower store is exact) in the first half of the same region.
+template <typename R> struct HistT { using type = double2; };
+template <> struct HistT<float> { using type = float2; };
+template <typename R> using Hist = typename HistT<R>::type;
+__device__ inline double2 widen(double2 h) { return h; }
+__device__ inline double2 widen(float2 h) { return make_double2((double)h.x, (double)h.y); }
+template <typename H> __device__ inline H narrow(double2 h);
+template <> __device__ inline double2 narrow<double2>(double2 h) { return h; }
+template <> __device__ inline float2 narrow<float2>(double2 h)
+{
+  return make_float2((float)h.x, (float)h.y);
+}
+
 // Carves the workspace at `base`; with base == nullptr only the byte offsets
 // are wanted and the returned pointers hold them (used by the layout query).
 inline void carve(void *base, int n, int nblocks, int nmembers, int ncells, int slots, int cap,
@@ -330,6 +345,7 @@ __device__ inline int cell_of(double ox, double oy, double h, int ncx, int ncy, 
 // length and mask; *status gets CAND_OVERFLOW if S is short.
 constexpr int kScan = 4;
 
+template <typename R>
 __device__ inline int rebuild_list(const BKArgs &k, const BedConst &m, int p, double2 xp,
                                    double rp, int lp, int nc, uint32_t mask,
                                    int (&cq)[kPre], uint32_t *nmask_out, int *status)
@@ -338,9 +354,9 @@ __device__ inline int rebuild_list(const BKArgs &k, const BedConst &m, int p, do
   const int n = a.n, S = a.cand_slots;
   const size_t sn = (size_t)S * n;
   const int *cand = k.w.cand + lp * sn + p;
-  const double2 *xis = k.w.xi + lp * sn + p;
+  const Hist<R> *xis = reinterpret_cast<const Hist<R> *>(k.w.xi) + lp * sn + p;
   int *ncl = k.w.cand + (lp ^ 1) * sn + p;
-  double2 *nxi = k.w.xi + (lp ^ 1) * sn + p;
+  Hist<R> *nxi = reinterpret_cast<Hist<R> *>(k.w.xi) + (lp ^ 1) * sn + p;
   int cx, cy;
   cell_of(m.ox, m.oy, m.cell_size, m.ncx, m.ncy, m.cell_first, xp, &cx, &cy);
   const int *cntab = k.w.cell_count + (size_t)m.tab * a.ncells;
@@ -493,7 +509,7 @@ __global__ __launch_bounds__(kThreads) void bed_step_kernel(BKArgs k)
     const double mass = m.rho_pi_t * (rp * rp);
     if (m.mode == MODE_SEARCH) {
       uint32_t nmask;
-      nc = rebuild_list(k, m, p, xp, rp, lp0, nc, mask, cq, &nmask, &status);
+      nc = rebuild_list<R>(k, m, p, xp, rp, lp0, nc, mask, cq, &nmask, &status);
       mask = nmask;
       lp = lp0 ^ 1;
       xb = xp;
@@ -503,7 +519,7 @@ __global__ __launch_bounds__(kThreads) void bed_step_kernel(BKArgs k)
       const double lim = 0.5 * m.skin * 0.999;
       if (ddx * ddx + ddy * ddy >= lim * lim) status |= TDM_ST_STALE_LIST;
     }
-    double2 *xis = k.w.xi + lp * sn + p;
+    Hist<R> *xis = reinterpret_cast<Hist<R> *>(k.w.xi) + lp * sn + p;
 
     // ---- forces: gravity, pairs by ascending partner, floor, back wall
     double fx = 0.0, fy = mass * m.gravity_y;
@@ -532,7 +548,7 @@ __global__ __launch_bounds__(kThreads) void bed_step_kernel(BKArgs k)
           // round for the pairs in contact last step, on demand for new ones
           if ((mask >> e) & 1u) {
             vq[j] = k.v_in[q];
-            hq[j] = xis[(size_t)e * n];
+            hq[j] = widen(xis[(size_t)e * n]);
           }
         }
       }
@@ -555,7 +571,7 @@ __global__ __launch_bounds__(kThreads) void bed_step_kernel(BKArgs k)
         double2 h = hq[j];
         double Fx, Fy, dash, fric;
         if (!hm_pair<R>(m, xl, xh, vl, vh, rl, rh, h, Fx, Fy, dash, fric)) continue;
-        xis[(size_t)e * n] = h;
+        xis[(size_t)e * n] = narrow<Hist<R>>(h);
         nmask |= 1u << e;
         if (self_lo) {
           fx += -Fx; fy += -Fy;
@@ -689,6 +705,7 @@ __global__ __launch_bounds__(kThreads) void bed_energy_kernel(BKArgs k)
     const int nc = k.w.ncand[(size_t)lp * n + p];
     const int *cand = k.w.cand + lp * sn + p;
     const double2 *xis = k.w.xi + lp * sn + p;
+    const float2 *xis32 = reinterpret_cast<const float2 *>(k.w.xi) + lp * sn + p;
     const double ddx = xp.x - xb.x, ddy = xp.y - xb.y;
     const double lim = 0.5 * m.skin * 0.999;
     if (ddx * ddx + ddy * ddy >= lim * lim) status |= TDM_ST_STALE_LIST;
@@ -706,7 +723,7 @@ __global__ __launch_bounds__(kThreads) void bed_energy_kernel(BKArgs k)
       const double sn_ = (2 * m.e_star) * sqrt(reff * d);
       ch[2] += 0.4 * (((2.0 / 3.0) * sn_) * d) * d;
       if ((mask >> e) & 1u) {
-        const double2 h = xis[(size_t)e * n];
+        const double2 h = a.fp32_histories ? widen(xis32[(size_t)e * n]) : xis[(size_t)e * n];
         ch[3] += 0.5 * ((8 * m.g_star) * sqrt(reff * d)) * (h.x * h.x + h.y * h.y);
       }
     }
@@ -1154,6 +1171,9 @@ int tdm_bed_step(const tdm_bed_arrays *a, int nsteps, int fp32, void *stream)
   BedWs w;
   int rc = check_bed(a, &w);
   if (rc) return rc;
+  if ((fp32 != 0) != (a->fp32_histories != 0))
+    return fail(TDM_EINVAL, "the bed's spring histories are stored in the other precision "
+                            "(fp32_histories must match the step's fp32 flag)");
   if (nsteps < 0) return fail(TDM_EINVAL, "bad nsteps");
   hipStream_t s = static_cast<hipStream_t>(stream);
   return launch_bed_steps(a, w, nsteps, fp32, s);
@@ -1165,6 +1185,9 @@ int tdm_bed_step_timed(const tdm_bed_arrays *a, int nsteps, int fp32, void *stre
   BedWs w;
   int rc = check_bed(a, &w);
   if (rc) return rc;
+  if ((fp32 != 0) != (a->fp32_histories != 0))
+    return fail(TDM_EINVAL, "the bed's spring histories are stored in the other precision "
+                            "(fp32_histories must match the step's fp32 flag)");
   if (nsteps <= 0 || (nsteps & 1) || !mean_kernel_us) return fail(TDM_EINVAL, "bad timing request");
   hipStream_t s = static_cast<hipStream_t>(stream);
   hipEvent_t *ev = new hipEvent_t[2 * nsteps]();
@@ -1204,6 +1227,9 @@ int tdm_bed_graph_create(const tdm_bed_arrays *a, int pairs, int fp32, void *str
   BedWs w;
   int rc = check_bed(a, &w);
   if (rc) return rc;
+  if ((fp32 != 0) != (a->fp32_histories != 0))
+    return fail(TDM_EINVAL, "the bed's spring histories are stored in the other precision "
+                            "(fp32_histories must match the step's fp32 flag)");
   if (!out || pairs <= 0) return fail(TDM_EINVAL, "bad graph request");
   hipStream_t s = static_cast<hipStream_t>(stream);
   HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));

@@ -207,8 +207,8 @@ def _carry_bed(old: BedEngine, old_members, eng: BedEngine, members, keep) -> No
     n_old, n_new = old.n, eng.n
     oc = old._ws_view(0, torch.int32, (2, S, n_old))
     eng._ws_view(0, torch.int32, (2, S, n_new))[0][:, dd] = oc.permute(1, 0, 2)[:, lp, sd] + sh
-    ox = old._ws_view(1, torch.float64, (2, S, n_old, 2))
-    eng._ws_view(1, torch.float64, (2, S, n_new, 2))[0][:, dd] = ox.permute(1, 0, 2, 3)[:, lp, sd]
+    ox = old.hist_view()
+    eng.hist_view()[0][:, dd] = ox.permute(1, 0, 2, 3)[:, lp, sd]
     eng._ws_view(2, torch.int32, (2, n_new))[0][dd] = old._ws_view(2, torch.int32, (2, n_old))[lp, sd]
     eng._ws_view(3, torch.uint8, (2, n_new))[0][dd] = old._ws_view(3, torch.uint8, (2, n_old))[lp, sd]
     eng._ws_view(4, torch.float64, (n_new, 2))[dd] = old._ws_view(4, torch.float64, (n_old, 2))[sd]
