@@ -38,6 +38,7 @@ extern "C" {
 #define TDM_ST_STALE_LIST 2     /* a disk moved >= skin/2 since the contact list was built */
 #define TDM_ST_CAND_OVERFLOW 4  /* more unbonded contact candidates than slots */
 #define TDM_ST_CELL_OVERFLOW 8  /* more disks in one hash cell than its capacity */
+#define TDM_ST_HIST_OVERFLOW 16 /* more friction springs on one sphere than history slots */
 
 /* ledger layout (accounting, per member, float64) -- the dict of dem.py:139-144 */
 #define TDM_ACC_EXT_WORK 0
@@ -443,6 +444,103 @@ int tdm_disk_step_timed(tdm_disk_arrays *a, const tdm_disk_params *p, int nsteps
  * replay. */
 int tdm_disk_graph_create(const tdm_disk_arrays *a, const tdm_disk_params *p, int pairs,
                           int flags, void *stream, tdm_graph **out);
+
+/* ---------------------------------------------------------------------------
+ * Bonded spheres in 3-D: the force evaluation and step of IceDEM3D
+ * (src/tensordem/dem3d.py:564-1967), FP64, one simulation per call.
+ * Options of the reference's DEM3DConfig outside these parameters (rolling
+ * resistance, friction regularisation, Mindlin partial slip, platen
+ * friction, buoyancy, Maxwell-Wiechert bonds, creep) are not evaluated here;
+ * the host refuses them.
+ * ------------------------------------------------------------------------- */
+/* `out` (float64), written by every evaluation */
+#define TDM_D3_REACTION 0       /* x, y, z: tool reaction, -sum of the tool force [N] */
+#define TDM_D3_TOP_REACTION 3   /* top platen reaction z [N] */
+#define TDM_D3_BOTTOM_REACTION 4
+#define TDM_D3_LATERAL 5        /* x_min, x_max, y_min, y_max platen reactions [N] */
+#define TDM_D3_FRICTION_POWER 9 /* sum of F_t . (v_i - v_j) over contacts [W] */
+#define TDM_D3_CONTACTS 10      /* contacts evaluated (pairs without a live bond) */
+#define TDM_D3_BREAKS 11        /* bonds broken by this evaluation */
+#define TDM_D3_STATUS 12        /* TDM_ST_* bits, sticky until tdm_d3_clear_status */
+#define TDM_D3_NOUT 13
+/* flags of tdm_d3_forces */
+#define TDM_D3_UPDATE_FRACTURE 1
+#define TDM_D3_UPDATE_HISTORY 2
+
+typedef struct tdm_d3_params {
+  double radius, mass, dt, drag;
+  double kb, kc, kt, mu;          /* bond, contact, tangential stiffness, friction */
+  double eps_t, eps_s;            /* tensile (when thr is NULL) and shear thresholds */
+  double hz_pair;                 /* (4/3) E* sqrt(R*), pair */
+  double hz_pair_u;               /* (8/15) E* sqrt(R*), pair potential */
+  double hz_rigid;                /* (4/3) E* sqrt(R*), sphere on a rigid surface */
+  double hz_kt;                   /* 8 G* sqrt(R*), Mindlin tangential stiffness */
+  double zeta2;                   /* 2 zeta of the restitution coefficient */
+  double gamma_mobile, gamma_pinned; /* linear dashpots at m/2 and m */
+  double cdamp;                   /* contact_damping: tool, and the absolute pair dashpot */
+  double tool_r;
+  double platen_k, platen_c, top_z0, bot_z0, top_v, bot_v, lat_k, lat_c;
+  double mg;                      /* mass * gravity */
+  double cell_w;                  /* cell width, 2 r */
+  int32_t hertz, damp;            /* damp: 0 none, 1 restitution, 2 absolute */
+  int32_t friction, mindlin_kt, top, bot, lateral, gravity;
+  int32_t n, nbonds, ncells, cell_cap, kh, nblocks;  /* ncells a power of two */
+} tdm_d3_params;
+
+typedef struct tdm_d3_arrays {
+  double *x, *v;                  /* (N,3): state at entry and at exit */
+  double *x_alt, *v_alt;          /* (N,3) */
+  const double *x0;               /* (N,3) initial positions */
+  const uint8_t *fixed;           /* (N,) */
+  const int32_t *nbr;             /* (N,26) partners above (ascending), then below, -1 absent */
+  const int32_t *bond_of_slot;    /* (N,26) bond index of each partner */
+  uint8_t *alive;                 /* (B,) */
+  const double *thr;              /* (B,) per-bond tensile threshold, or NULL */
+  uint8_t *failure_mode;          /* (B,) 1 tensile, 2 shear, 3 both */
+  double *failure_time_s, *failure_extension_m, *failure_energy_J; /* (B,) */
+  const double *loads;            /* (N,3) nodal loads, or NULL */
+  double *force;                  /* (N,3) written by every evaluation but energy */
+  double *pair_energy;            /* (N,) written by tdm_d3_energy */
+  const double *tool;             /* (6,) start, velocity */
+  double *clock;                  /* (10,) time, steps, lateral positions (4), velocities (4) */
+  double *out;                    /* TDM_D3_NOUT */
+  void *workspace;
+  size_t workspace_bytes;
+  int64_t evals, hist_flips, steps; /* kept by the library */
+} tdm_d3_arrays;
+
+int tdm_d3_abi_sizes(size_t *params_bytes, size_t *arrays_bytes);
+/* Blocks (256 spheres each) for n spheres. */
+int tdm_d3_blocks(int n);
+size_t tdm_d3_workspace_bytes(int n, int ncells, int cell_cap, int kh);
+/* Inverse reference covariance of every sphere, cleared cell tables,
+ * friction springs and status; replaces IceDEM3D.__init__'s derived state
+ * (dem3d.py:579-760). */
+int tdm_d3_setup(tdm_d3_arrays *a, const tdm_d3_params *p, void *stream);
+int tdm_d3_clear_status(tdm_d3_arrays *a, const tdm_d3_params *p, void *stream);
+/* Drops every friction spring (a restart: the reference's state_dict does
+ * not carry them, dem3d.py:1828-1852). */
+int tdm_d3_clear_history(tdm_d3_arrays *a, const tdm_d3_params *p, void *stream);
+/* nsteps steps of IceDEM3D.step (dem3d.py:1644-1662), each one force
+ * evaluation with fracture and friction history committed. */
+int tdm_d3_step(tdm_d3_arrays *a, const tdm_d3_params *p, int nsteps, void *stream);
+/* IceDEM3D.forces(external_forces, update_fracture, update_history)
+ * (dem3d.py:1300-1642): force (N,3) and out, state unchanged. */
+int tdm_d3_forces(tdm_d3_arrays *a, const tdm_d3_params *p, int flags, void *stream);
+/* Contact energy of every pair without a live bond, to its lower sphere
+ * (dem3d.py:1748-1772). */
+int tdm_d3_energy(tdm_d3_arrays *a, const tdm_d3_params *p, void *stream);
+/* Objective shear strain of every sphere at x (dem3d.py:875-901). */
+int tdm_d3_shear(tdm_d3_arrays *a, const tdm_d3_params *p, double *shear, void *stream);
+/* 2*pairs steps captured as a graph.  The graph bakes in the parities of
+ * a->evals and a->hist_flips at capture; a replay leaves both unchanged. */
+int tdm_d3_graph_create(tdm_d3_arrays *a, const tdm_d3_params *p, int pairs, void *stream,
+                        tdm_graph **out);
+/* As tdm_d3_step (nsteps even) with HIP events around each evaluation's
+ * strain and step kernels; mean durations in microseconds.  Measurement
+ * helper. */
+int tdm_d3_step_timed(tdm_d3_arrays *a, const tdm_d3_params *p, int nsteps, void *stream,
+                      double *mean_step_us, double *mean_strain_us);
 
 #ifdef __cplusplus
 }
