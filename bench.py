@@ -936,6 +936,99 @@ def cpu_baseline_disk(budget_s, side=256):
 
 
 # ---------------------------------------------------------------------------
+# IceDEM3D: the reference's 10k-100k scale study, 47^3 bonded spheres
+# ---------------------------------------------------------------------------
+D3_SIDE = 47
+D3_REFERENCE_MS = 15872.0   # docs/dem3d_scale_benchmark.md:62-66, 103 823 spheres, loong64
+
+
+def d3_config(side=D3_SIDE):
+    """scripts/benchmark_dem3d_scale.py:49-70: r = 0.025 m, k_b = 2000, k_c = 5000,
+    c = 5, drag = 2, tool r = 0.1 m at 0.01 m/s, 1 cm gap, free edges,
+    dt = 0.1 x recommended_dt."""
+    import dataclasses
+    from tensordem_amd.dem3dconfig import DEM3DConfig
+    base = DEM3DConfig(nx=side, ny=side, nz=side, radius=0.025, density=917.0,
+                       bond_stiffness=2000.0, contact_stiffness=5000.0, contact_damping=5.0,
+                       drag=2.0, tool_radius=0.1, tool_speed=0.01, tool_gap=0.01,
+                       breaking_strain=0.015, shear_breaking_strain=0.03, fix_x_edges=False,
+                       fix_bottom=False)
+    return dataclasses.replace(base, dt=0.10 * base.recommended_dt)
+
+
+def d3_bytes_per_sphere(nbonds_per_sphere, ncells_per_sphere):
+    """Compulsory bytes per sphere per launch of d3_step_kernel: own x and v
+    read (48), the new x and v and the force written (72), initial position
+    (24), fixed flag (1), the 26-slot partner and bond-index rows (208), own
+    shear strain (8), the alive byte of each of its bonds once (B/N), its
+    cell-table entry read (4) and the bucket counts cleared and read
+    (2 x 4 x ncells/N)."""
+    return 48.0 + 72.0 + 24.0 + 1.0 + 208.0 + 8.0 + nbonds_per_sphere + 4.0 + 8.0 * ncells_per_sphere
+
+
+def bench_dem3d(args):
+    """IceDEM3D on 47^3 = 103 823 spheres: graph-replayed FP64 steps, kernel
+    events, the C oracle on a CPU sample."""
+    import torch
+    from tensordem_amd.dem3d import IceDEM3D
+
+    cfg = d3_config()
+    sim = IceDEM3D(cfg)
+    n, nb = len(sim.positions), len(sim.pairs)
+    steps, chunk = graph_steps(args.d3_steps, 64)
+    sim.run(max(64, args.d3_warmup), pairs=chunk // 2)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    sim.run(steps, pairs=chunk // 2)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    gpu_s = e0.elapsed_time(e1) * 1e-3
+    step_us, strain_us = sim.step_timed(200)
+    bps = d3_bytes_per_sphere(nb / n, sim._params.ncells / n)
+    achieved = bps * n / (step_us * 1e-6) / 1e9
+    ms = gpu_s / steps * 1e3
+    res = {
+        "workload": f"IceDEM3D bonded-sphere lattice {D3_SIDE}^3 = {n} spheres, {nb} bonds "
+                    "(the reference's scale_100k case: r=0.025 m, k_b=2000, k_c=5000, c=5, "
+                    f"drag=2, tool r=0.1 m at 0.01 m/s, dt=0.1 x recommended = {sim.dt:.6g} s), "
+                    f"{steps} steps",
+        "value": n * steps / gpu_s, "unit": "particle-steps/s", "steps": steps,
+        "ms_per_step": ms, "dtype": "f64", "broken_bonds": sim.broken_bonds,
+        "reference_ms_per_step": D3_REFERENCE_MS,
+        "reference_over_this": D3_REFERENCE_MS / ms,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic_from_profile("dem3d_step_scale100k_counters.md"),
+                     "kernel": "d3_step_kernel", "kernel_us": step_us,
+                     "strain_kernel_us": strain_us, "bytes_per_sphere": bps,
+                     "bytes_per_launch": bps * n},
+    }
+    del sim
+    if not args.no_cpu:
+        res["cpu_baseline"] = cpu_baseline_dem3d(args.cpu_seconds)
+    return res
+
+
+def cpu_baseline_dem3d(budget_s):
+    """C oracle (FP64, one core, sorted cell list) on the same 47^3 lattice."""
+    from oracle.dem3d import OracleDEM3D
+    sim = OracleDEM3D(d3_config())
+    sim.step()
+    t0 = time.perf_counter()
+    steps = 0
+    while time.perf_counter() - t0 < budget_s:
+        sim.step()
+        steps += 1
+    el = time.perf_counter() - t0
+    n = len(sim.positions)
+    return dict(value=n * steps / el, unit="particle-steps/s", cores=1, kind="port",
+                sample=f"{steps} steps of the same {n}-sphere lattice (FP64 C oracle, sorted "
+                       f"cell list rebuilt every step), {el:.1f} s")
+
+
+# ---------------------------------------------------------------------------
 # C4: 512 independent plates with Weibull strengths, one launch per step
 # ---------------------------------------------------------------------------
 def bench_ensemble(args):
@@ -1179,13 +1272,16 @@ def main(argv=None):
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--disk-steps", type=int, default=2000)
     ap.add_argument("--disk-warmup", type=int, default=200)
+    ap.add_argument("--d3-steps", type=int, default=2048)
+    ap.add_argument("--d3-warmup", type=int, default=128)
     ap.add_argument("--workload", choices=("all", "plate", "bed", "ensemble", "serve", "disk",
-                                           "horizon"),
+                                           "horizon", "dem3d"),
                     default="all",
                     help="plate: the 1024x128 plate only; bed: the 1M-disk column only; "
                          "ensemble: the 512-plate batch only; serve: the request loop only; "
                          "disk: the 1M-disk granular2d box only; horizon: FP32 broken-bond "
-                         "horizon of the 1024x128 plate against fp32_margin and dt")
+                         "horizon of the 1024x128 plate against fp32_margin and dt; dem3d: "
+                         "the 47^3-sphere IceDEM3D lattice only")
     ap.add_argument("--serve-window", type=float, default=60.0)
     ap.add_argument("--serve-chunk", type=int, default=250)
     ap.add_argument("--serve-slots", type=int, default=64)
@@ -1250,9 +1346,9 @@ def run_workloads(args):
     if args.workload == "horizon":
         print(json.dumps(bench_horizon(args)))
         return 0
-    if args.workload in ("bed", "ensemble", "serve", "disk"):
+    if args.workload in ("bed", "ensemble", "serve", "disk", "dem3d"):
         bed = {"bed": bench_bed, "ensemble": bench_ensemble, "serve": bench_serve,
-               "disk": bench_disk}[args.workload](args)
+               "disk": bench_disk, "dem3d": bench_dem3d}[args.workload](args)
         line = {"metric": METRIC, "value": bed.pop("value"), "unit": "particle-steps/s",
                 "n_gpus": 1, "steps": bed["steps"], "warmup": bed.pop("warmup", 0),
                 "ms_per_step": bed["ms_per_step"], "higher_is_better": True, "scaling": "weak",
@@ -1260,7 +1356,8 @@ def run_workloads(args):
                 "data": "synthetic: " + {"bed": "seeded loose column packing",
                                          "ensemble": "seeded parameter sweep of plates",
                                          "serve": "seeded Poisson request stream",
-                                         "disk": "seeded jittered-grid disk box"}[args.workload],
+                                         "disk": "seeded jittered-grid disk box",
+                                         "dem3d": "cubic bonded-sphere lattice"}[args.workload],
                 "config": {"workload": bed.pop("workload")}}
         line.update(bed)
         print(json.dumps(line))
@@ -1284,6 +1381,7 @@ def run_workloads(args):
     line["c4_ensemble"] = bench_ensemble(args)
     line["c5_serve"] = bench_serve(args)
     line["disk_system"] = bench_disk(args)
+    line["dem3d_scale"] = bench_dem3d(args)
     print(json.dumps(line))
     return 0
 

@@ -31,6 +31,9 @@
  * Hertz forces against rigid surfaces are c d sqrt(d) (the reference writes
  * c d^1.5; both are within an ulp).
  *
+ * Contact partners come from a sorted cell list of width 2r (every pair closer
+ * than 2r lies in adjacent cells); I3_BRUTE tests every pair instead.
+ *
  * Per sphere the force is summed as: drag, gravity, load, forward bonds
  * (partner above, ascending), backward bonds (partner below, ascending),
  * contacts by ascending partner (friction, then normal), tool, top and
@@ -53,7 +56,7 @@ enum {
 };
 enum {
   I3_HERTZ = 0, I3_DAMP, I3_FRICTION, I3_MINDLIN_KT, I3_TOP, I3_BOT, I3_LATERAL, I3_GRAVITY,
-  I3_UPDATE_FRACTURE, I3_UPDATE_HISTORY, I3_OP, I3_KH, I3_NIPARAM
+  I3_UPDATE_FRACTURE, I3_UPDATE_HISTORY, I3_OP, I3_KH, I3_BRUTE, I3_NIPARAM
 };
 enum { DAMP_NONE = 0, DAMP_RESTITUTION = 1, DAMP_ABSOLUTE = 2 };
 enum { OP_STEP = 0, OP_FORCES = 1, OP_ENERGY = 2 };
@@ -204,6 +207,70 @@ static const double *hist_find(const history *h, int kh, int p, int q)
   return NULL;
 }
 
+typedef struct { int64_t c[3]; int32_t idx; } cellkey;
+
+static int cmp_cellkey(const void *a, const void *b)
+{
+  const cellkey *u = (const cellkey *)a, *v = (const cellkey *)b;
+  for (int k = 0; k < 3; ++k)
+    if (u->c[k] != v->c[k]) return u->c[k] < v->c[k] ? -1 : 1;
+  return u->idx - v->idx;
+}
+
+static int cmp_int(const void *a, const void *b)
+{
+  int u = *(const int *)a, v = *(const int *)b;
+  return (u > v) - (u < v);
+}
+
+static int64_t cell_of(double x, double w)
+{
+  double c = floor(x / w);
+  if (c < -1e15) c = -1e15;
+  if (c > 1e15) c = 1e15;
+  return (int64_t)c;
+}
+
+/* first entry of the sorted keys with cell >= c */
+static int lower_cell(const cellkey *k, int m, const int64_t *c)
+{
+  int lo = 0, hi = m;
+  while (lo < hi) {
+    int mid = (lo + hi) / 2;
+    cellkey probe = {{c[0], c[1], c[2]}, -1};
+    if (cmp_cellkey(&k[mid], &probe) < 0) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+/* the candidate partners of p: every sphere of the 27 cells of width 2r
+ * around it (brute: every sphere), ascending */
+static int candidates(int n, const double *x, double w, const cellkey *keys, int m, int p,
+                      int brute, int *out)
+{
+  int c = 0;
+  if (brute) {
+    for (int q = 0; q < n; ++q)
+      if (q != p) out[c++] = q;
+    return c;
+  }
+  if (!isfinite(x[3 * p]) || !isfinite(x[3 * p + 1]) || !isfinite(x[3 * p + 2])) return 0;
+  int64_t cp[3] = {cell_of(x[3 * p], w), cell_of(x[3 * p + 1], w), cell_of(x[3 * p + 2], w)};
+  for (int dz = -1; dz <= 1; ++dz)
+    for (int dy = -1; dy <= 1; ++dy)
+      for (int dx = -1; dx <= 1; ++dx) {
+        int64_t cc[3] = {cp[0] + dx, cp[1] + dy, cp[2] + dz};
+        for (int e = lower_cell(keys, m, cc); e < m; ++e) {
+          const cellkey *k = keys + e;
+          if (k->c[0] != cc[0] || k->c[1] != cc[1] || k->c[2] != cc[2]) break;
+          if (k->idx != p) out[c++] = k->idx;
+        }
+      }
+  qsort(out, (size_t)c, sizeof(int), cmp_int);
+  return c;
+}
+
 /*
  * One evaluation.  x, v: state (advanced in place by OP_STEP); alive and the
  * failure arrays: bond state (committed when I3_UPDATE_FRACTURE); hist_in /
@@ -271,6 +338,19 @@ int orc3_eval(int n, int nb, const double *dp, const int *ip, double *x, double 
     for (int p = 0; p < n; ++p) hout.cnt[p] = 0;
 
   const double two_r = 2.0 * r;
+  const int brute = ip[I3_BRUTE];
+  cellkey *keys = (cellkey *)malloc(sizeof(cellkey) * (size_t)(n > 0 ? n : 1));
+  int *cand = (int *)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+  int nkeys = 0;
+  if (!brute) {
+    for (int p = 0; p < n; ++p) {
+      if (!isfinite(x[3 * p]) || !isfinite(x[3 * p + 1]) || !isfinite(x[3 * p + 2])) continue;
+      cellkey k = {{cell_of(x[3 * p], two_r), cell_of(x[3 * p + 1], two_r),
+                    cell_of(x[3 * p + 2], two_r)}, p};
+      keys[nkeys++] = k;
+    }
+    qsort(keys, (size_t)nkeys, sizeof(cellkey), cmp_cellkey);
+  }
   for (int p = 0; p < n; ++p) {
     double f[3];
     for (int k = 0; k < 3; ++k) f[k] = -dp[D3_DRAG] * v[3 * p + k];
@@ -288,8 +368,9 @@ int orc3_eval(int n, int nb, const double *dp, const int *ip, double *x, double 
         for (int k = 0; k < 3; ++k) f[k] -= bf[3 * b + k];
     }
     double epair = 0.0;
-    for (int q = 0; q < n; ++q) {
-      if (q == p) continue;
+    const int nc = candidates(n, x, two_r, keys, nkeys, p, brute, cand);
+    for (int e = 0; e < nc; ++e) {
+      const int q = cand[e];
       int i = p < q ? p : q, j = p < q ? q : p;
       double cd[3];
       for (int k = 0; k < 3; ++k) cd[k] = x[3 * j + k] - x[3 * i + k];
@@ -456,7 +537,7 @@ int orc3_eval(int n, int nb, const double *dp, const int *ip, double *x, double 
         if (!isfinite(xx) || !isfinite(vv)) status |= ST_NONFINITE;
       }
   }
-  free(shear); free(ext); free(bf);
+  free(shear); free(ext); free(bf); free(keys); free(cand);
   free_adjacency(&g);
   out[O3_STATUS] = status;
   return status;

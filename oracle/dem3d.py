@@ -27,7 +27,7 @@ D3_LAT_POS = 35
 D3_LAT_VEL = 39
 D3_NPARAM = 43
 (I3_HERTZ, I3_DAMP, I3_FRICTION, I3_MINDLIN_KT, I3_TOP, I3_BOT, I3_LATERAL, I3_GRAVITY,
- I3_UPDATE_FRACTURE, I3_UPDATE_HISTORY, I3_OP, I3_KH, I3_NIPARAM) = range(13)
+ I3_UPDATE_FRACTURE, I3_UPDATE_HISTORY, I3_OP, I3_KH, I3_BRUTE, I3_NIPARAM) = range(14)
 DAMP_NONE, DAMP_RESTITUTION, DAMP_ABSOLUTE = 0, 1, 2
 OP_STEP, OP_FORCES, OP_ENERGY = 0, 1, 2
 O3_REACTION, O3_TOP_Z, O3_BOT_Z, O3_LAT = 0, 3, 4, 5
@@ -138,8 +138,9 @@ def pair_constants(cfg):
 class OracleDEM3D:
     """numpy mirror of IceDEM3D driven by the C oracle."""
 
-    def __init__(self, config, *, dt=None, bond_thresholds=None):
+    def __init__(self, config, *, dt=None, bond_thresholds=None, brute_force=False):
         c = self.config = config
+        self.brute_force = bool(brute_force)
         self.lib = _bind()
         self.dt = float(dt if dt is not None else (c.dt if c.dt is not None else c.recommended_dt))
         pos, fixed, pairs = reference_lattice(c)
@@ -221,6 +222,7 @@ class OracleDEM3D:
         ip[I3_GRAVITY] = c.gravity > 0.0
         ip[I3_UPDATE_FRACTURE], ip[I3_UPDATE_HISTORY] = update_fracture, update_history
         ip[I3_OP], ip[I3_KH] = op, KH
+        ip[I3_BRUTE] = self.brute_force
         return dp, ip
 
     def _eval(self, op, external_forces=None, update_fracture=True, update_history=True):

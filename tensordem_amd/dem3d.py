@@ -478,6 +478,19 @@ class IceDEM3D:
         if st & _lib.ST_NONFINITE:
             raise FloatingPointError("non-finite DEM3D state; reduce dt or contact stiffness")
 
+    def step_timed(self, steps: int) -> tuple[float, float]:
+        """`steps` (even) steps with HIP events on each step and strain kernel
+        launch; mean durations in microseconds (measurement helper)."""
+        self._sync_params()
+        step_us, strain_us = ctypes.c_double(), ctypes.c_double()
+        _lib.check(self.lib.tdm_d3_step_timed(ctypes.byref(self._arrays),
+                                              ctypes.byref(self._params), int(steps),
+                                              self._stream_ptr, ctypes.byref(step_us),
+                                              ctypes.byref(strain_us)))
+        self._advance_clock(int(steps))
+        self._finish_steps()
+        return step_us.value, strain_us.value
+
     def shear_strain(self) -> torch.Tensor:
         """Objective shear strain of every sphere at the current state."""
         out = torch.empty(len(self._x), dtype=self.dtype, device=self.device)

@@ -452,3 +452,29 @@ def test_fixed_spheres_stay_put():
         sim.step()
     assert np.array_equal(sim.positions[sim.fixed.astype(bool)], fixed0)
     assert sim.step_count == 20
+
+
+def test_cell_list_equals_brute_force():
+    """dem3d.py:825-858's guarantee (every overlapping pair found): the
+    oracle's sorted cell list and an all-pairs scan give identical forces on a
+    random cloud with many overlaps (tests/test_dem3d.py:83-98) and over an
+    indentation that fractures the lattice."""
+    cfg = DEM3DConfig(nx=3, ny=3, nz=2, drag=0.0, fix_x_edges=False, friction_coefficient=0.3)
+    a = OracleDEM3D(cfg)
+    b = OracleDEM3D(cfg, brute_force=True)
+    x = np.random.default_rng(73).normal(size=a.positions.shape) * 0.035
+    a.positions, b.positions = x.copy(), x.copy()
+    fa, _ = a.forces()
+    fb, _ = b.forces()
+    assert a.out[10] == b.out[10] > 0
+    assert np.array_equal(fa, fb)
+    cfg = DEM3DConfig(nx=11, ny=5, nz=4, radius=0.01, tool_radius=0.03, tool_speed=1.0,
+                      tool_gap=0.0002, breaking_strain=0.004, shear_breaking_strain=0.01,
+                      contact_damping=1.0, friction_coefficient=0.3)
+    a, b = OracleDEM3D(cfg), OracleDEM3D(cfg, brute_force=True)
+    for _ in range(600):
+        a.step()
+        b.step()
+    assert a.broken_bonds > 50 and a.out[10] > 0
+    assert np.array_equal(a.positions, b.positions)
+    assert np.array_equal(a.hist_xi, b.hist_xi)
