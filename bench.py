@@ -956,14 +956,16 @@ def d3_config(side=D3_SIDE):
     return dataclasses.replace(base, dt=0.10 * base.recommended_dt)
 
 
-def d3_bytes_per_sphere(nbonds_per_sphere, ncells_per_sphere):
-    """Compulsory bytes per sphere per launch of d3_step_kernel: own x and v
-    read (48), the new x and v and the force written (72), initial position
-    (24), fixed flag (1), the 26-slot partner and bond-index rows (208), own
-    shear strain (8), the alive byte of each of its bonds once (B/N), its
-    cell-table entry read (4) and the bucket counts cleared and read
-    (2 x 4 x ncells/N)."""
-    return 48.0 + 72.0 + 24.0 + 1.0 + 208.0 + 8.0 + nbonds_per_sphere + 4.0 + 8.0 * ncells_per_sphere
+def d3_bytes_per_sphere(bonds_per_sphere, cells_per_sphere, list_per_sphere):
+    """Compulsory (unique) bytes per sphere per launch of d3_step_kernel: own x
+    and v read (48), the new x and v and the force written (72), initial
+    position (24), fixed flag (1), the 26-slot partner and bond-index rows
+    (208), each of its bonds' alive byte and bond force once (25 B/N), its
+    candidate count and list (4 + 4 per entry), its position at the last
+    list build (24), and its share of the cell counts cleared for the next
+    evaluation (4 x ncells/N)."""
+    return (48.0 + 72.0 + 24.0 + 1.0 + 208.0 + 25.0 * bonds_per_sphere + 4.0
+            + 4.0 * list_per_sphere + 24.0 + 4.0 * cells_per_sphere)
 
 
 def bench_dem3d(args):
@@ -985,8 +987,12 @@ def bench_dem3d(args):
     e1.record(stream)
     torch.cuda.synchronize()
     gpu_s = e0.elapsed_time(e1) * 1e-3
-    step_us, strain_us = sim.step_timed(200)
-    bps = d3_bytes_per_sphere(nb / n, sim._params.ncells / n)
+    step_us, strain_us, bond_us = sim.step_timed(200)
+    # the lattice barely moves in this workload: each sphere lists its face
+    # neighbours (rest length 2r < 2r + skin), no others
+    assert sim.broken_bonds == 0
+    listed = 2.0 * float((sim.rest_lengths < 2.0 * cfg.radius * 1.05).sum()) / n
+    bps = d3_bytes_per_sphere(nb / n, sim._params.ncells / n, listed)
     achieved = bps * n / (step_us * 1e-6) / 1e9
     ms = gpu_s / steps * 1e3
     res = {
@@ -1002,8 +1008,10 @@ def bench_dem3d(args):
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic_from_profile("dem3d_step_scale100k_counters.md"),
                      "kernel": "d3_step_kernel", "kernel_us": step_us,
-                     "strain_kernel_us": strain_us, "bytes_per_sphere": bps,
-                     "bytes_per_launch": bps * n},
+                     "strain_kernel_us": strain_us, "bond_kernel_us": bond_us,
+                     "bytes_per_sphere": bps,
+                     "bytes_per_launch": bps * n, "list_entries_per_sphere": listed,
+                     "list_rebuilds": sim.rebuilds},
     }
     del sim
     if not args.no_cpu:

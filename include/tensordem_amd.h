@@ -462,7 +462,8 @@ int tdm_disk_graph_create(const tdm_disk_arrays *a, const tdm_disk_params *p, in
 #define TDM_D3_CONTACTS 10      /* contacts evaluated (pairs without a live bond) */
 #define TDM_D3_BREAKS 11        /* bonds broken by this evaluation */
 #define TDM_D3_STATUS 12        /* TDM_ST_* bits, sticky until tdm_d3_clear_status */
-#define TDM_D3_NOUT 13
+#define TDM_D3_REBUILDS 13      /* candidate-list rebuilds so far */
+#define TDM_D3_NOUT 14
 /* flags of tdm_d3_forces */
 #define TDM_D3_UPDATE_FRACTURE 1
 #define TDM_D3_UPDATE_HISTORY 2
@@ -481,7 +482,8 @@ typedef struct tdm_d3_params {
   double tool_r;
   double platen_k, platen_c, top_z0, bot_z0, top_v, bot_v, lat_k, lat_c;
   double mg;                      /* mass * gravity */
-  double cell_w;                  /* cell width, 2 r */
+  double cell_w;                  /* cell width, at least 2 r + skin */
+  double skin;                    /* candidate lists hold partners within 2 r + skin */
   int32_t hertz, damp;            /* damp: 0 none, 1 restitution, 2 absolute */
   int32_t friction, mindlin_kt, top, bot, lateral, gravity;
   int32_t n, nbonds, ncells, cell_cap, kh, nblocks;  /* ncells a power of two */
@@ -494,6 +496,7 @@ typedef struct tdm_d3_arrays {
   const uint8_t *fixed;           /* (N,) */
   const int32_t *nbr;             /* (N,26) partners above (ascending), then below, -1 absent */
   const int32_t *bond_of_slot;    /* (N,26) bond index of each partner */
+  const int32_t *pairs;           /* (B,2) bonds (i < j), sorted by (i, j) */
   uint8_t *alive;                 /* (B,) */
   const double *thr;              /* (B,) per-bond tensile threshold, or NULL */
   uint8_t *failure_mode;          /* (B,) 1 tensile, 2 shear, 3 both */
@@ -512,12 +515,15 @@ typedef struct tdm_d3_arrays {
 int tdm_d3_abi_sizes(size_t *params_bytes, size_t *arrays_bytes);
 /* Blocks (256 spheres each) for n spheres. */
 int tdm_d3_blocks(int n);
-size_t tdm_d3_workspace_bytes(int n, int ncells, int cell_cap, int kh);
+size_t tdm_d3_workspace_bytes(int n, int nbonds, int ncells, int cell_cap, int kh);
 /* Inverse reference covariance of every sphere, cleared cell tables,
  * friction springs and status; replaces IceDEM3D.__init__'s derived state
  * (dem3d.py:579-760). */
 int tdm_d3_setup(tdm_d3_arrays *a, const tdm_d3_params *p, void *stream);
 int tdm_d3_clear_status(tdm_d3_arrays *a, const tdm_d3_params *p, void *stream);
+/* Rebuilds the candidate lists in the next evaluation (after the caller
+ * moved spheres). */
+int tdm_d3_rebin(tdm_d3_arrays *a, const tdm_d3_params *p, void *stream);
 /* Drops every friction spring (a restart: the reference's state_dict does
  * not carry them, dem3d.py:1828-1852). */
 int tdm_d3_clear_history(tdm_d3_arrays *a, const tdm_d3_params *p, void *stream);
@@ -537,10 +543,10 @@ int tdm_d3_shear(tdm_d3_arrays *a, const tdm_d3_params *p, double *shear, void *
 int tdm_d3_graph_create(tdm_d3_arrays *a, const tdm_d3_params *p, int pairs, void *stream,
                         tdm_graph **out);
 /* As tdm_d3_step (nsteps even) with HIP events around each evaluation's
- * strain and step kernels; mean durations in microseconds.  Measurement
- * helper. */
+ * strain, bond and step kernels; mean durations in microseconds.
+ * Measurement helper. */
 int tdm_d3_step_timed(tdm_d3_arrays *a, const tdm_d3_params *p, int nsteps, void *stream,
-                      double *mean_step_us, double *mean_strain_us);
+                      double *mean_step_us, double *mean_strain_us, double *mean_bond_us);
 
 #ifdef __cplusplus
 }

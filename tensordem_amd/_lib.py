@@ -102,7 +102,7 @@ class D3Params(ctypes.Structure):
         "radius", "mass", "dt", "drag", "kb", "kc", "kt", "mu", "eps_t", "eps_s", "hz_pair",
         "hz_pair_u", "hz_rigid", "hz_kt", "zeta2", "gamma_mobile", "gamma_pinned", "cdamp",
         "tool_r", "platen_k", "platen_c", "top_z0", "bot_z0", "top_v", "bot_v", "lat_k", "lat_c",
-        "mg", "cell_w")] + [(name, ctypes.c_int32) for name in (
+        "mg", "cell_w", "skin")] + [(name, ctypes.c_int32) for name in (
         "hertz", "damp", "friction", "mindlin_kt", "top", "bot", "lateral", "gravity", "n",
         "nbonds", "ncells", "cell_cap", "kh", "nblocks")]
 
@@ -110,7 +110,7 @@ class D3Params(ctypes.Structure):
 class D3Arrays(ctypes.Structure):
     """Mirror of `tdm_d3_arrays`."""
     _fields_ = [(name, ctypes.c_void_p) for name in (
-        "x", "v", "x_alt", "v_alt", "x0", "fixed", "nbr", "bond_of_slot", "alive", "thr",
+        "x", "v", "x_alt", "v_alt", "x0", "fixed", "nbr", "bond_of_slot", "pairs", "alive", "thr",
         "failure_mode", "failure_time_s", "failure_extension_m", "failure_energy_J", "loads",
         "force", "pair_energy", "tool", "clock", "out", "workspace")] + [
         ("workspace_bytes", ctypes.c_size_t), ("evals", ctypes.c_int64),
@@ -118,7 +118,7 @@ class D3Arrays(ctypes.Structure):
 
 
 D3_REACTION, D3_TOP_REACTION, D3_BOTTOM_REACTION, D3_LATERAL = 0, 3, 4, 5
-D3_FRICTION_POWER, D3_CONTACTS, D3_BREAKS, D3_STATUS, D3_NOUT = 9, 10, 11, 12, 13
+D3_FRICTION_POWER, D3_CONTACTS, D3_BREAKS, D3_STATUS, D3_REBUILDS, D3_NOUT = 9, 10, 11, 12, 13, 14
 D3_UPDATE_FRACTURE, D3_UPDATE_HISTORY = 1, 2
 ST_HIST_OVERFLOW = 16
 
@@ -186,10 +186,11 @@ _SIGNATURES = {
     "tdm_d3_abi_sizes": ([ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)],
                          ctypes.c_int),
     "tdm_d3_blocks": ([ctypes.c_int], ctypes.c_int),
-    "tdm_d3_workspace_bytes": ([ctypes.c_int] * 4, ctypes.c_size_t),
+    "tdm_d3_workspace_bytes": ([ctypes.c_int] * 5, ctypes.c_size_t),
     "tdm_d3_setup": ([ctypes.c_void_p] * 3, ctypes.c_int),
     "tdm_d3_clear_status": ([ctypes.c_void_p] * 3, ctypes.c_int),
     "tdm_d3_clear_history": ([ctypes.c_void_p] * 3, ctypes.c_int),
+    "tdm_d3_rebin": ([ctypes.c_void_p] * 3, ctypes.c_int),
     "tdm_d3_step": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p],
                     ctypes.c_int),
     "tdm_d3_forces": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p],
@@ -198,9 +199,8 @@ _SIGNATURES = {
     "tdm_d3_shear": ([ctypes.c_void_p] * 4, ctypes.c_int),
     "tdm_d3_graph_create": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
                              ctypes.POINTER(ctypes.c_void_p)], ctypes.c_int),
-    "tdm_d3_step_timed": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
-                           ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)],
-                          ctypes.c_int),
+    "tdm_d3_step_timed": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+                          + [ctypes.POINTER(ctypes.c_double)] * 3, ctypes.c_int),
 }
 
 _lib = None

@@ -2,22 +2,24 @@
 // Euler step) of the reference's IceDEM3D (src/tensordem/dem3d.py:1300-1660)
 // on the device, FP64, one thread per sphere.
 //
-// Three launches per evaluation:
+// Four launches per evaluation:
 //
 //   d3_strain_kernel   objective shear strain of every sphere (the
 //                      least-squares deformation gradient over its bonds,
 //                      dem3d.py:875-901) and the sphere's entry in the hashed
 //                      cell table of this evaluation (cell width 2r, so every
 //                      overlapping pair lies in 27 adjacent cells);
-//   d3_step_kernel     per sphere: fracture of its bonds (both ends decide
-//                      from the same operands; the lower end writes the
-//                      failure record), bond forces, contacts of the 27 cells
-//                      sorted by partner, Cundall-Strack springs, tool,
-//                      platens, integration; clears the other cell table;
+//   d3_bond_kernel     per bond: extension, fracture commit and failure
+//                      record, the bond force on the bond's lower sphere;
+//   d3_step_kernel     per sphere: its bond forces gathered in order,
+//                      contacts of the 27 cells sorted by partner,
+//                      Cundall-Strack springs, tool, platens, integration;
+//                      clears the other cell table;
 //   d3_finish_kernel   sums the per-block partials in block order and
 //                      advances the clock and the lateral platens.
 //
-// No float atomics: a sphere's force is summed in one fixed order (drag,
+// No float atomics: each bond's force is computed once, like the oracle's,
+// and a sphere's force is summed in one fixed order (drag,
 // gravity, load, bonds to higher partners ascending, bonds to lower partners
 // ascending, contacts by ascending partner, tool, platens), the order of the
 // oracle, so FP64 runs compare bit for bit.  A pair's spring state is kept
@@ -29,31 +31,37 @@ namespace tdm {
 namespace {
 
 constexpr int kSlots = 26;   // lattice neighbours of a sphere
-constexpr int kCand = 24;    // contacts per sphere (LDS list)
+constexpr int kCand = 24;    // candidate partners per sphere (list capacity)
 constexpr int kNch = 12;     // partial channels
 
 enum { OP_STEP = 0, OP_FORCES = 1, OP_ENERGY = 2 };
 enum { DAMP_NONE = 0, DAMP_RESTITUTION = 1, DAMP_ABSOLUTE = 2 };
 enum {
   C_RX = 0, C_RY, C_RZ, C_TOP, C_BOT, C_LAT0, C_LAT1, C_LAT2, C_LAT3, C_FPOW, C_CONTACTS,
-  C_BREAKS
+  C_BREAKS  // written by the finish kernel from the bond kernel's count
 };
 
 struct D3Ws {
   double *inv;        // [n][9]
   uint8_t *degen;     // [n]
   double *shear;      // [n]
-  int *count;         // [2][ncells]
-  int *tab;           // [2][ncells][cap]
+  int *count;         // [2][ncells]: cleared by the evaluation before the one that fills it
+  int *tab;           // [ncells][cap], filled and read by rebuild evaluations only
   int *hq;            // [2][n][kh]
   double *hxi;        // [2][n][kh][3]
   int *hcnt;          // [2][n]
   double *partials;   // [nblocks][kNch]
-  int *status;        // [1]
+  int *status;        // [0] status bits, [1] bonds broken by this evaluation,
+                      // [2] rebuild the candidate lists in this evaluation,
+                      // [3] rebuild requested by this evaluation, [4] rebuilds
+  double *bf;         // [nbonds][3] bond force on the lower sphere
+  int *list;          // [n][kCand] partners within 2r + skin at the last build, ascending
+  int *lcnt;          // [n]
+  double *xb;         // [n][3] positions at the last build
 };
 
-__host__ __device__ inline void carve_d3(void *base, int n, int ncells, int cap, int kh,
-                                         int nblocks, D3Ws *w, size_t *total)
+__host__ __device__ inline void carve_d3(void *base, int n, int nbonds, int ncells, int cap,
+                                         int kh, int nblocks, D3Ws *w, size_t *total)
 {
   size_t off = 0;
   char *b = static_cast<char *>(base);
@@ -67,12 +75,16 @@ __host__ __device__ inline void carve_d3(void *base, int n, int ncells, int cap,
   w->degen = reinterpret_cast<uint8_t *>(take(N));
   w->shear = reinterpret_cast<double *>(take(sizeof(double) * N));
   w->count = reinterpret_cast<int *>(take(sizeof(int) * 2 * size_t(ncells)));
-  w->tab = reinterpret_cast<int *>(take(sizeof(int) * 2 * size_t(ncells) * size_t(cap)));
+  w->tab = reinterpret_cast<int *>(take(sizeof(int) * size_t(ncells) * size_t(cap)));
   w->hq = reinterpret_cast<int *>(take(sizeof(int) * 2 * N * size_t(kh)));
   w->hxi = reinterpret_cast<double *>(take(sizeof(double) * 6 * N * size_t(kh)));
   w->hcnt = reinterpret_cast<int *>(take(sizeof(int) * 2 * N));
   w->partials = reinterpret_cast<double *>(take(sizeof(double) * kNch * size_t(nblocks)));
   w->status = reinterpret_cast<int *>(take(sizeof(int) * 64));
+  w->bf = reinterpret_cast<double *>(take(sizeof(double) * 3 * size_t(nbonds > 0 ? nbonds : 1)));
+  w->list = reinterpret_cast<int *>(take(sizeof(int) * kCand * N));
+  w->lcnt = reinterpret_cast<int *>(take(sizeof(int) * N));
+  w->xb = reinterpret_cast<double *>(take(sizeof(double) * 3 * N));
   *total = off;
 }
 
@@ -160,8 +172,8 @@ __global__ __launch_bounds__(kThreads) void d3_strain_kernel(tdm_d3_arrays a, td
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= p.n) return;
   const double3 xi = ld3(x, i);
-  if (tab < 0) {
-    // strain only
+  if (tab < 0 || !w.status[2]) {
+    // strain only: no candidate search in this evaluation
   } else if (!isfinite(xi.x) || !isfinite(xi.y) || !isfinite(xi.z)) {
     atomicOr(w.status, TDM_ST_NONFINITE);
   } else {
@@ -169,32 +181,42 @@ __global__ __launch_bounds__(kThreads) void d3_strain_kernel(tdm_d3_arrays a, td
                             cell_coord(xi.z, p.cell_w), p.ncells - 1);
     const int slot = atomicAdd(w.count + size_t(tab) * p.ncells + h, 1);
     if (slot < p.cell_cap)
-      w.tab[(size_t(tab) * p.ncells + h) * p.cell_cap + slot] = i;
+      w.tab[size_t(h) * p.cell_cap + slot] = i;
     else
       atomicOr(w.status, TDM_ST_CELL_OVERFLOW);
   }
-  // objective shear strain (dem3d.py:875-901)
+  // objective shear strain (dem3d.py:875-901): sum q r^T over the bonds to
+  // higher partners, then lower partners, each in ascending partner order;
+  // absent slots (-1, after the present ones) load the sphere itself and add
+  // nothing, so every gather of a half is issued at once
   double cr[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   const double3 x0i = ld3(a.x0, i);
-  for (int s = 0; s < kSlots; ++s) {
-    const int q = a.nbr[size_t(i) * kSlots + s];
-    if (q < 0) {
-      if (s < 13) { s = 12; continue; }
-      break;
-    }
-    const double3 xq = ld3(x, q), x0q = ld3(a.x0, q);
-    double qq[3], rr[3];
-    if (s < 13) {
-      qq[0] = xq.x - xi.x; qq[1] = xq.y - xi.y; qq[2] = xq.z - xi.z;
-      rr[0] = x0q.x - x0i.x; rr[1] = x0q.y - x0i.y; rr[2] = x0q.z - x0i.z;
-    } else {
-      qq[0] = xi.x - xq.x; qq[1] = xi.y - xq.y; qq[2] = xi.z - xq.z;
-      rr[0] = x0i.x - x0q.x; rr[1] = x0i.y - x0q.y; rr[2] = x0i.z - x0q.z;
-    }
+  const int *row = a.nbr + size_t(i) * kSlots;
 #pragma unroll
-    for (int u = 0; u < 3; ++u)
+  for (int half = 0; half < 2; ++half) {
+    int qs[13];
 #pragma unroll
-      for (int v = 0; v < 3; ++v) cr[3 * u + v] += qq[u] * rr[v];
+    for (int k = 0; k < 13; ++k) qs[k] = row[half * 13 + k];
+#pragma unroll
+    for (int k = 0; k < 13; ++k) {
+      const bool present = qs[k] >= 0;
+      const int q = present ? qs[k] : i;
+      const double3 xq = ld3(x, q), x0q = ld3(a.x0, q);
+      double qq[3], rr[3];
+      if (half == 0) {
+        qq[0] = xq.x - xi.x; qq[1] = xq.y - xi.y; qq[2] = xq.z - xi.z;
+        rr[0] = x0q.x - x0i.x; rr[1] = x0q.y - x0i.y; rr[2] = x0q.z - x0i.z;
+      } else {
+        qq[0] = xi.x - xq.x; qq[1] = xi.y - xq.y; qq[2] = xi.z - xq.z;
+        rr[0] = x0i.x - x0q.x; rr[1] = x0i.y - x0q.y; rr[2] = x0i.z - x0q.z;
+      }
+      if (present) {
+#pragma unroll
+        for (int u = 0; u < 3; ++u)
+#pragma unroll
+          for (int v = 0; v < 3; ++v) cr[3 * u + v] += qq[u] * rr[v];
+      }
+    }
   }
   if (w.degen[i]) {
     w.shear[i] = 0.0;
@@ -227,6 +249,44 @@ __global__ __launch_bounds__(kThreads) void d3_strain_kernel(tdm_d3_arrays a, td
       s2 += d * d;
     }
   w.shear[i] = sqrt((2.0 / 3.0) * s2);
+}
+
+// ---------------------------------------------------------------- bonds
+// One thread per bond: extension, the fracture commit (dem3d.py:984-1009;
+// the bond's own record, one writer) and the bond force on its lower sphere
+// (dem3d.py:1327-1350), which both spheres then gather.
+__global__ __launch_bounds__(kThreads) void d3_bond_kernel(tdm_d3_arrays a, tdm_d3_params p, D3Ws w,
+                                                           const double *x, int commit)
+{
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= p.nbonds) return;
+  const int i = a.pairs[2 * b], j = a.pairs[2 * b + 1];
+  const double3 xi = ld3(x, i), xj = ld3(x, j), x0i = ld3(a.x0, i), x0j = ld3(a.x0, j);
+  const double3 d = make_double3(xj.x - xi.x, xj.y - xi.y, xj.z - xi.z);
+  const double3 d0 = make_double3(x0j.x - x0i.x, x0j.y - x0i.y, x0j.z - x0i.z);
+  const double dist = norm3(d), l0 = norm3(d0);
+  const double ext = dist - l0;
+  if (commit && a.alive[b]) {
+    const double thr = a.thr ? a.thr[b] : p.eps_t;
+    const bool tensile = ext / l0 > thr;
+    const bool shear = 0.5 * (w.shear[i] + w.shear[j]) > p.eps_s;
+    if (tensile || shear) {
+      a.alive[b] = 0;
+      a.failure_mode[b] = uint8_t(int(tensile) + 2 * int(shear));
+      a.failure_time_s[b] = a.clock[0];
+      a.failure_extension_m[b] = ext;
+      a.failure_energy_J[b] = 0.5 * p.kb * (ext * ext);
+      atomicAdd(w.status + 1, 1);
+    }
+  }
+  const double mag = p.kb * ext;
+  const double den = l0 > kEps64 ? l0 : kEps64;
+  const double nx = dist > kEps64 ? d.x / dist : d0.x / den;
+  const double ny = dist > kEps64 ? d.y / dist : d0.y / den;
+  const double nz = dist > kEps64 ? d.z / dist : d0.z / den;
+  w.bf[3 * size_t(b)] = mag * nx;
+  w.bf[3 * size_t(b) + 1] = mag * ny;
+  w.bf[3 * size_t(b) + 2] = mag * nz;
 }
 
 // ---------------------------------------------------------------- step
@@ -271,69 +331,47 @@ __global__ __launch_bounds__(kThreads) void d3_step_kernel(tdm_d3_arrays a, tdm_
       f[1] += a.loads[3 * i + 1];
       f[2] += a.loads[3 * i + 2];
     }
-    const double sh_i = w.shear[i];
     const double3 x0i = ld3(a.x0, i);
     const int *row = a.nbr + size_t(i) * kSlots;
     const int *brow = a.bond_of_slot + size_t(i) * kSlots;
+    // bond forces of the bond kernel: bonds to higher partners added, to
+    // lower partners subtracted, each in ascending partner order
     uint32_t alive_mask = 0;
-    // bonds (dem3d.py:1327-1350) with the fracture commit (dem3d.py:984-1009)
-    for (int sl = 0; sl < kSlots; ++sl) {
-      const int q = row[sl];
-      if (q < 0) {
-        if (sl < 13) { sl = 12; continue; }
-        break;
-      }
-      const bool up = sl < 13;
-      const int b = brow[sl];
-      const double3 xq = ld3(s.x, q), x0q = ld3(a.x0, q);
-      // bond vector from its lower sphere to its upper sphere
-      const double3 d = up ? make_double3(xq.x - xi.x, xq.y - xi.y, xq.z - xi.z)
-                           : make_double3(xi.x - xq.x, xi.y - xq.y, xi.z - xq.z);
-      const double3 d0 = up ? make_double3(x0q.x - x0i.x, x0q.y - x0i.y, x0q.z - x0i.z)
-                            : make_double3(x0i.x - x0q.x, x0i.y - x0q.y, x0i.z - x0q.z);
-      const double dist = norm3(d), l0 = norm3(d0);
-      const double ext = dist - l0;
-      bool alive = a.alive[b] != 0;
-      if (s.commit && alive) {
-        const double thr = a.thr ? a.thr[b] : p.eps_t;
-        const double sh_q = w.shear[q];
-        const bool tensile = ext / l0 > thr;
-        const bool shear = 0.5 * (up ? sh_i + sh_q : sh_q + sh_i) > p.eps_s;
-        if (tensile || shear) {
-          alive = false;
-          if (up) {
-            a.alive[b] = 0;
-            a.failure_mode[b] = uint8_t(int(tensile) + 2 * int(shear));
-            a.failure_time_s[b] = time;
-            a.failure_extension_m[b] = ext;
-            a.failure_energy_J[b] = 0.5 * p.kb * (ext * ext);
-            ch[C_BREAKS] += 1.0;
+#pragma unroll
+    for (int half = 0; half < 2; ++half) {
+      int bs[13];
+#pragma unroll
+      for (int k = 0; k < 13; ++k) bs[k] = brow[half * 13 + k];
+#pragma unroll
+      for (int k = 0; k < 13; ++k) {
+        const bool present = bs[k] >= 0;
+        const size_t b = present ? size_t(bs[k]) : 0;
+        const bool alive = present && a.alive[b] != 0;
+        const double bx = w.bf[3 * b], by = w.bf[3 * b + 1], bz = w.bf[3 * b + 2];
+        if (alive) {
+          alive_mask |= 1u << (half * 13 + k);
+          if (half == 0) {
+            f[0] += bx; f[1] += by; f[2] += bz;
+          } else {
+            f[0] -= bx; f[1] -= by; f[2] -= bz;
           }
         }
       }
-      if (!alive) continue;
-      alive_mask |= 1u << sl;
-      const double mag = p.kb * ext;
-      const double den = l0 > kEps64 ? l0 : kEps64;
-      const double nx = dist > kEps64 ? d.x / dist : d0.x / den;
-      const double ny = dist > kEps64 ? d.y / dist : d0.y / den;
-      const double nz = dist > kEps64 ? d.z / dist : d0.z / den;
-      const double bx = mag * nx, by = mag * ny, bz = mag * nz;
-      if (up) {
-        f[0] += bx; f[1] += by; f[2] += bz;
-      } else {
-        f[0] -= bx; f[1] -= by; f[2] -= bz;
-      }
     }
 
-    // contact candidates: 27 cells (distinct buckets), pairs closer than 2r
-    // without a live bond, kept sorted by partner
+    // candidate partners: every sphere within 2r + skin when the lists were
+    // last built, ascending.  A rebuild searches the 27 cells (of width
+    // 2r + skin) around the sphere; it is due when a sphere has moved 0.45
+    // skin since the last one, so every pair closer than 2r is listed.
+    const bool rebuild = w.status[2] != 0;
+    const double reach = two_r + p.skin;
+    int *lrow = w.list + size_t(i) * kCand;
     int nc = 0;
-    {
+    if (rebuild) {
       const long long cx = cell_coord(xi.x, p.cell_w), cy = cell_coord(xi.y, p.cell_w),
                       cz = cell_coord(xi.z, p.cell_w);
       const int *cnt = w.count + size_t(s.tab) * p.ncells;
-      const int *tb = w.tab + size_t(s.tab) * p.ncells * p.cell_cap;
+      const int *tb = w.tab;
       for (int c = 0; c < 27; ++c) {
         // two of the 27 cells may share a bucket: a sphere found twice is
         // dropped when it is inserted
@@ -344,15 +382,9 @@ __global__ __launch_bounds__(kThreads) void d3_step_kernel(tdm_d3_arrays a, tdm_
           const int q = tb[size_t(h) * p.cell_cap + e];
           if (q == i) continue;
           const double3 xq = ld3(s.x, q);
-          const double3 d = q > i ? make_double3(xq.x - xi.x, xq.y - xi.y, xq.z - xi.z)
-                                  : make_double3(xi.x - xq.x, xi.y - xq.y, xi.z - xq.z);
-          if (fabs(d.x) >= two_r || fabs(d.y) >= two_r || fabs(d.z) >= two_r) continue;
-          const double ov = two_r - norm3(d);
-          if (!(ov > 0.0)) continue;
-          bool bonded = false;
-          for (int sl = 0; sl < kSlots; ++sl)
-            if (row[sl] == q) bonded = (alive_mask >> sl) & 1u;
-          if (bonded) continue;
+          const double3 d = make_double3(xq.x - xi.x, xq.y - xi.y, xq.z - xi.z);
+          if (fabs(d.x) >= reach || fabs(d.y) >= reach || fabs(d.z) >= reach) continue;
+          if (!(norm3(d) < reach)) continue;
           bool seen = false;
           for (int e = 0; e < nc; ++e) seen |= cand[e][t] == q;
           if (seen) continue;
@@ -368,6 +400,11 @@ __global__ __launch_bounds__(kThreads) void d3_step_kernel(tdm_d3_arrays a, tdm_
           cand[pos][t] = q;
         }
       }
+      for (int e = 0; e < nc; ++e) lrow[e] = cand[e][t];
+      w.lcnt[i] = nc;
+      st3(w.xb, i, xi);
+    } else {
+      nc = w.lcnt[i];
     }
 
     const int kh = p.kh;
@@ -375,13 +412,20 @@ __global__ __launch_bounds__(kThreads) void d3_step_kernel(tdm_d3_arrays a, tdm_
     if (p.friction) hin_n = s.hcnt_in[i];
     double epair = 0.0;
     for (int c = 0; c < nc; ++c) {
-      const int q = cand[c][t];
+      const int q = lrow[c];
       const bool lower = i < q;
       const int ia = lower ? i : q, ja = lower ? q : i;
       const double3 xa = ld3(s.x, ia), xb = ld3(s.x, ja);
       const double3 cd = make_double3(xb.x - xa.x, xb.y - xa.y, xb.z - xa.z);
+      if (fabs(cd.x) >= two_r || fabs(cd.y) >= two_r || fabs(cd.z) >= two_r) continue;
       const double cdist = norm3(cd);
       const double ov = two_r - cdist;
+      if (!(ov > 0.0)) continue;
+      // pairs whose bond is alive are not contacts (dem3d.py:1374-1378)
+      bool bonded = false;
+      for (int sl = 0; sl < kSlots; ++sl)
+        if (row[sl] == q) bonded = (alive_mask >> sl) & 1u;
+      if (bonded) continue;
       if (s.op == OP_ENERGY) {
         if (lower) epair += p.hertz ? p.hz_pair_u * (ov * ov * sqrt(ov)) : (0.5 * p.kc) * (ov * ov);
         continue;
@@ -538,6 +582,10 @@ __global__ __launch_bounds__(kThreads) void d3_step_kernel(tdm_d3_arrays a, tdm_
           finite &= isfinite(xx) && isfinite(vv);
         }
         if (!finite) atomicOr(w.status, TDM_ST_NONFINITE);
+        const double3 x_b = ld3(w.xb, i);
+        const double dx = xn[0] - x_b.x, dy = xn[1] - x_b.y, dz = xn[2] - x_b.z;
+        const double lim = 0.45 * p.skin;
+        if (!(dx * dx + dy * dy + dz * dz <= lim * lim)) atomicOr(w.status + 3, 1);
         st3(s.x_out, i, make_double3(xn[0], xn[1], xn[2]));
         st3(s.v_out, i, make_double3(vnw[0], vnw[1], vnw[2]));
       }
@@ -563,7 +611,13 @@ __global__ __launch_bounds__(kThreads) void d3_finish_kernel(tdm_d3_arrays a, td
   block_sum<kNch>(ch, red);
   if (threadIdx.x != 0) return;
   for (int k = 0; k < kNch; ++k) a.out[k] = ch[k];
+  a.out[C_BREAKS] = double(w.status[1]);
+  w.status[1] = 0;
+  if (w.status[2]) w.status[4] += 1;
+  w.status[2] = w.status[3];
+  w.status[3] = 0;
   a.out[TDM_D3_STATUS] = double(*w.status);
+  a.out[TDM_D3_REBUILDS] = double(w.status[4]);
   if (op == OP_STEP) {
     double *c = a.clock;
     if (p.lateral)
@@ -581,7 +635,12 @@ __global__ void d3_copy_kernel(double *dst, const double *src, size_t n)
 
 __global__ void d3_status_kernel(const D3Ws w, double *out)
 {
-  if (threadIdx.x == 0 && blockIdx.x == 0) out[TDM_D3_STATUS] = double(*w.status);
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  if (w.status[2]) w.status[4] += 1;
+  w.status[2] = w.status[3];
+  w.status[3] = 0;
+  out[TDM_D3_STATUS] = double(*w.status);
+  out[TDM_D3_REBUILDS] = double(w.status[4]);
 }
 
 int check_d3(const tdm_d3_arrays *a, const tdm_d3_params *p, D3Ws *w)
@@ -591,19 +650,20 @@ int check_d3(const tdm_d3_arrays *a, const tdm_d3_params *p, D3Ws *w)
   if (p->ncells < 1 || (p->ncells & (p->ncells - 1))) return fail(TDM_EINVAL, "ncells must be a power of two");
   if (p->cell_cap < 1 || p->kh < 1) return fail(TDM_EINVAL, "bad cell or history capacity");
   if (p->nblocks != tdm_d3_blocks(p->n)) return fail(TDM_EINVAL, "nblocks does not match n");
-  if (!(p->cell_w > 0.0)) return fail(TDM_EINVAL, "cell width must be positive");
+  if (!(p->skin > 0.0) || !(p->cell_w >= 2.0 * p->radius + p->skin))
+    return fail(TDM_EINVAL, "cell width must cover 2 r + skin, skin positive");
   if (!a->x || !a->v || !a->x_alt || !a->v_alt || !a->x0 || !a->fixed || !a->nbr ||
       !a->bond_of_slot || !a->force || !a->clock || !a->out || !a->tool || !a->workspace)
     return fail(TDM_EINVAL, "missing array");
-  if (p->nbonds > 0 && (!a->alive || !a->failure_mode || !a->failure_time_s ||
+  if (p->nbonds > 0 && (!a->alive || !a->pairs || !a->failure_mode || !a->failure_time_s ||
                         !a->failure_extension_m || !a->failure_energy_J))
     return fail(TDM_EINVAL, "missing bond array");
   size_t need = 0;
-  carve_d3(nullptr, p->n, p->ncells, p->cell_cap, p->kh, p->nblocks, w, &need);
+  carve_d3(nullptr, p->n, p->nbonds, p->ncells, p->cell_cap, p->kh, p->nblocks, w, &need);
   if (a->workspace_bytes < need) return fail(TDM_ESPACE, "workspace too small");
   if (reinterpret_cast<uintptr_t>(a->workspace) & 255)
     return fail(TDM_EINVAL, "workspace must be 256-byte aligned");
-  carve_d3(a->workspace, p->n, p->ncells, p->cell_cap, p->kh, p->nblocks, w, &need);
+  carve_d3(a->workspace, p->n, p->nbonds, p->ncells, p->cell_cap, p->kh, p->nblocks, w, &need);
   return TDM_OK;
 }
 
@@ -632,6 +692,9 @@ int launch_eval(tdm_d3_arrays *a, const tdm_d3_params *p, const D3Ws &w, int op,
   s.commit = (flags & TDM_D3_UPDATE_FRACTURE) && op != OP_ENERGY;
   s.upd_hist = upd_hist;
   hipLaunchKernelGGL(d3_strain_kernel, dim3(nb), dim3(kThreads), 0, st, *a, *p, w, x, tab);
+  if (op != OP_ENERGY && p->nbonds > 0)
+    hipLaunchKernelGGL(d3_bond_kernel, dim3((p->nbonds + kThreads - 1) / kThreads), dim3(kThreads),
+                       0, st, *a, *p, w, x, s.commit);
   hipLaunchKernelGGL(d3_step_kernel, dim3(nb), dim3(kThreads), 0, st, *a, *p, w, s);
   if (op != OP_ENERGY)
     hipLaunchKernelGGL(d3_finish_kernel, dim3(1), dim3(kThreads), 0, st, *a, *p, w, op, nb);
@@ -680,12 +743,12 @@ int tdm_d3_abi_sizes(size_t *params_bytes, size_t *arrays_bytes)
 
 int tdm_d3_blocks(int n) { return n > 0 ? (n + kThreads - 1) / kThreads : 0; }
 
-size_t tdm_d3_workspace_bytes(int n, int ncells, int cell_cap, int kh)
+size_t tdm_d3_workspace_bytes(int n, int nbonds, int ncells, int cell_cap, int kh)
 {
-  if (n < 1 || ncells < 1 || cell_cap < 1 || kh < 1) return 0;
+  if (n < 1 || nbonds < 0 || ncells < 1 || cell_cap < 1 || kh < 1) return 0;
   D3Ws w;
   size_t total = 0;
-  carve_d3(nullptr, n, ncells, cell_cap, kh, tdm_d3_blocks(n), &w, &total);
+  carve_d3(nullptr, n, nbonds, ncells, cell_cap, kh, tdm_d3_blocks(n), &w, &total);
   return total;
 }
 
@@ -702,6 +765,7 @@ int tdm_d3_setup(tdm_d3_arrays *a, const tdm_d3_params *p, void *stream)
   hipLaunchKernelGGL(fill_kernel<int>, dim3(64), dim3(kThreads), 0, st, w.hcnt,
                      size_t(2) * p->n, 0);
   hipLaunchKernelGGL(fill_kernel<int>, dim3(1), dim3(64), 0, st, w.status, size_t(64), 0);
+  hipLaunchKernelGGL(fill_kernel<int>, dim3(1), dim3(64), 0, st, w.status + 2, size_t(1), 1);
   HIP_TRY(hipGetLastError());
   a->evals = 0;
   a->hist_flips = 0;
@@ -716,6 +780,17 @@ int tdm_d3_clear_status(tdm_d3_arrays *a, const tdm_d3_params *p, void *stream)
   if (rc) return rc;
   hipLaunchKernelGGL(fill_kernel<int>, dim3(1), dim3(64), 0, static_cast<hipStream_t>(stream),
                      w.status, size_t(64), 0);
+  HIP_TRY(hipGetLastError());
+  return TDM_OK;
+}
+
+int tdm_d3_rebin(tdm_d3_arrays *a, const tdm_d3_params *p, void *stream)
+{
+  D3Ws w;
+  int rc = check_d3(a, p, &w);
+  if (rc) return rc;
+  hipLaunchKernelGGL(fill_kernel<int>, dim3(1), dim3(64), 0, static_cast<hipStream_t>(stream),
+                     w.status + 2, size_t(1), 1);
   HIP_TRY(hipGetLastError());
   return TDM_OK;
 }
@@ -798,7 +873,7 @@ int tdm_d3_graph_create(tdm_d3_arrays *a, const tdm_d3_params *p, int pairs, voi
 }
 
 int tdm_d3_step_timed(tdm_d3_arrays *a, const tdm_d3_params *p, int nsteps, void *stream,
-                      double *mean_step_us, double *mean_strain_us)
+                      double *mean_step_us, double *mean_strain_us, double *mean_bond_us)
 {
   D3Ws w;
   int rc = check_d3(a, p, &w);
@@ -807,7 +882,7 @@ int tdm_d3_step_timed(tdm_d3_arrays *a, const tdm_d3_params *p, int nsteps, void
   hipStream_t st = static_cast<hipStream_t>(stream);
   hipEvent_t e[4];
   for (int k = 0; k < 4; ++k) HIP_TRY(hipEventCreate(&e[k]));
-  double tot_step = 0.0, tot_strain = 0.0;
+  double tot_step = 0.0, tot_strain = 0.0, tot_bond = 0.0;
   for (int k = 0; k < nsteps; ++k) {
     const bool even = (k & 1) == 0;
     const double *x = even ? a->x : a->x_alt, *v = even ? a->v : a->v_alt;
@@ -827,16 +902,22 @@ int tdm_d3_step_timed(tdm_d3_arrays *a, const tdm_d3_params *p, int nsteps, void
     HIP_TRY(hipEventRecord(e[0], st));
     hipLaunchKernelGGL(d3_strain_kernel, dim3(p->nblocks), dim3(kThreads), 0, st, *a, *p, w, x, tab);
     HIP_TRY(hipEventRecord(e[1], st));
+    if (p->nbonds > 0)
+      hipLaunchKernelGGL(d3_bond_kernel, dim3((p->nbonds + kThreads - 1) / kThreads),
+                         dim3(kThreads), 0, st, *a, *p, w, x, 1);
+    HIP_TRY(hipEventRecord(e[3], st));
     hipLaunchKernelGGL(d3_step_kernel, dim3(p->nblocks), dim3(kThreads), 0, st, *a, *p, w, s);
     HIP_TRY(hipEventRecord(e[2], st));
     hipLaunchKernelGGL(d3_finish_kernel, dim3(1), dim3(kThreads), 0, st, *a, *p, w, int(OP_STEP),
                        p->nblocks);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventSynchronize(e[2]));
-    float ms0 = 0.f, ms1 = 0.f;
+    float ms0 = 0.f, ms1 = 0.f, ms2 = 0.f;
     HIP_TRY(hipEventElapsedTime(&ms0, e[0], e[1]));
-    HIP_TRY(hipEventElapsedTime(&ms1, e[1], e[2]));
+    HIP_TRY(hipEventElapsedTime(&ms2, e[1], e[3]));
+    HIP_TRY(hipEventElapsedTime(&ms1, e[3], e[2]));
     tot_strain += ms0;
+    tot_bond += ms2;
     tot_step += ms1;
     a->evals += 1;
     if (upd) a->hist_flips += 1;
@@ -846,6 +927,7 @@ int tdm_d3_step_timed(tdm_d3_arrays *a, const tdm_d3_params *p, int nsteps, void
   for (int k = 0; k < 4; ++k) (void)hipEventDestroy(e[k]);
   if (mean_step_us) *mean_step_us = 1e3 * tot_step / nsteps;
   if (mean_strain_us) *mean_strain_us = 1e3 * tot_strain / nsteps;
+  if (mean_bond_us) *mean_bond_us = 1e3 * tot_bond / nsteps;
   return TDM_OK;
 }
 

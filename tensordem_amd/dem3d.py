@@ -38,8 +38,9 @@ from .dem3dconfig import (DEM3DConfig, build_lattice, hertz_effective_radius,
                           hertz_reduced_modulus, hertz_reduced_shear_modulus)
 from .strength import weibull_strengths
 
-CELL_CAP = 16
+CELL_CAP = 32
 HISTORY_SLOTS = 16
+SKIN_RADII = 0.2
 
 
 def _unsupported(cfg: DEM3DConfig) -> list[str]:
@@ -167,20 +168,24 @@ class IceDEM3D:
         # kernel buffers
         self._nbr = torch.tensor(lat.nbr, dtype=torch.int32, device=dev)
         self._bos = torch.tensor(lat.bond_of_slot, dtype=torch.int32, device=dev)
+        self._pairs32 = torch.tensor(lat.pairs, dtype=torch.int32, device=dev)
         self._force = torch.zeros((n, 3), dtype=f64, device=dev)
         self._pair_energy = torch.zeros(n, dtype=f64, device=dev)
         self._out = torch.zeros(_lib.D3_NOUT, dtype=f64, device=dev)
         self._loads_t: torch.Tensor | None = None
         self._graphs: dict = {}
-        ncells = 1 << max(10, (2 * n - 1).bit_length())
+        # about 8 buckets per sphere: a cell of width 2r + skin holds up to 8
+        # lattice spheres, so collisions must stay rare
+        ncells = 1 << max(10, (8 * n - 1).bit_length())
         kh = HISTORY_SLOTS if _friction_mu(config) > 0.0 else 1
         self._params = self._make_params(n, nb, ncells, kh)
-        ws = int(self.lib.tdm_d3_workspace_bytes(n, ncells, CELL_CAP, kh))
+        ws = int(self.lib.tdm_d3_workspace_bytes(n, nb, ncells, CELL_CAP, kh))
         self._ws = torch.empty(ws + 256, dtype=torch.uint8, device=dev)
         self._arrays = _lib.D3Arrays()
         self._bind_arrays()
         _lib.check(self.lib.tdm_d3_setup(ctypes.byref(self._arrays), ctypes.byref(self._params),
                                          self._stream_ptr))
+        self._seen_version = self._x._version
 
     # ------------------------------------------------------------------ set-up
     @property
@@ -219,7 +224,10 @@ class IceDEM3D:
         p.lat_c = (c.lateral_platen_damping if c.lateral_platen_damping is not None
                    else c.platen_damping)
         p.mg = c.mass * c.gravity
-        p.cell_w = 2.0 * c.radius
+        # candidate lists of partners within 2r + skin, rebuilt on the device
+        # when a sphere has moved 0.45 skin
+        p.skin = SKIN_RADII * c.radius
+        p.cell_w = 2.0 * c.radius + p.skin
         p.hertz = int(c.contact_model == "hertz")
         p.damp = 2 if c.contact_damping_model == "absolute" else (
             1 if c.restitution_coefficient < 1.0 else 0)
@@ -238,6 +246,7 @@ class IceDEM3D:
         a.x0 = self.initial_positions.data_ptr()
         a.fixed = self._fixed.data_ptr()
         a.nbr, a.bond_of_slot = self._nbr.data_ptr(), self._bos.data_ptr()
+        a.pairs = self._pairs32.data_ptr() if len(self.pairs) else None
         nb = len(self.pairs)
         a.alive = self._alive.data_ptr() if nb else None
         a.thr = self.bond_breaking_strain.data_ptr() if self.bond_breaking_strain is not None \
@@ -257,6 +266,17 @@ class IceDEM3D:
         # platen starts are plain attributes a caller may move between steps
         self._params.top_z0 = float(self.top_platen_start_z)
         self._params.bot_z0 = float(self.bottom_platen_start_z)
+        # positions written by the caller (assigned or edited in place, which
+        # torch counts in the tensor's version) invalidate the candidate lists
+        if self._x._version != self._seen_version:
+            _lib.check(self.lib.tdm_d3_rebin(ctypes.byref(self._arrays),
+                                             ctypes.byref(self._params), self._stream_ptr))
+            self._seen_version = self._x._version
+
+    @property
+    def rebuilds(self) -> int:
+        """Candidate-list rebuilds so far."""
+        return int(self._out[_lib.D3_REBUILDS].item())
 
     # ------------------------------------------------------------------ state
     def _assign(self, dst: torch.Tensor, value, name: str) -> None:
@@ -478,18 +498,18 @@ class IceDEM3D:
         if st & _lib.ST_NONFINITE:
             raise FloatingPointError("non-finite DEM3D state; reduce dt or contact stiffness")
 
-    def step_timed(self, steps: int) -> tuple[float, float]:
-        """`steps` (even) steps with HIP events on each step and strain kernel
-        launch; mean durations in microseconds (measurement helper)."""
+    def step_timed(self, steps: int) -> tuple[float, float, float]:
+        """`steps` (even) steps with HIP events on each step, strain and bond
+        kernel launch; mean durations in microseconds (measurement helper)."""
         self._sync_params()
-        step_us, strain_us = ctypes.c_double(), ctypes.c_double()
+        step_us, strain_us, bond_us = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
         _lib.check(self.lib.tdm_d3_step_timed(ctypes.byref(self._arrays),
                                               ctypes.byref(self._params), int(steps),
                                               self._stream_ptr, ctypes.byref(step_us),
-                                              ctypes.byref(strain_us)))
+                                              ctypes.byref(strain_us), ctypes.byref(bond_us)))
         self._advance_clock(int(steps))
         self._finish_steps()
-        return step_us.value, strain_us.value
+        return step_us.value, strain_us.value, bond_us.value
 
     def shear_strain(self) -> torch.Tensor:
         """Objective shear strain of every sphere at the current state."""
@@ -507,6 +527,7 @@ class IceDEM3D:
         return 0.5 * self.config.bond_stiffness * extension.square()
 
     def _pair_contact_energy(self) -> float:
+        self._sync_params()
         _lib.check(self.lib.tdm_d3_energy(ctypes.byref(self._arrays), ctypes.byref(self._params),
                                           self._stream_ptr))
         return float(self._pair_energy.sum())

@@ -102,6 +102,7 @@ def test_indentation_with_fracture_bit_exact():
             same_state(sim, ref, f"step {k}")
             assert np.allclose(np_(sim.last_reaction), ref.last_reaction, rtol=1e-12, atol=1e-15)
     assert sim.broken_bonds > 20 and sim.broken_bonds == ref.broken_bonds
+    assert sim.rebuilds > 2          # candidate lists rebuilt on the device as spheres moved
     assert set(np.unique(ref.failure_mode[ref.alive == 0])) >= {1, 2}
     assert ref.out[10] > 0                    # broken pairs in contact
     assert sim.step_count == 600 and sim.time == ref.time
@@ -239,6 +240,13 @@ def test_broken_bond_pair_becomes_a_contact():
     sim.alive[0] = False
     force, _ = sim.forces(update_fracture=False)
     assert float(sim._out[10]) == 1
+    # an in-place edit of positions (tests/test_dem3d.py:100-114) is seen too
+    sim.positions[j] = sim.positions[i] + torch.tensor([1.5 * sim.config.radius, 0.0, 0.0],
+                                                       dtype=torch.float64, device=pos.device)
+    sim.forces(update_fracture=False)
+    assert float(sim._out[10]) == 1
+    sim.positions[j] = sim.positions[i] + torch.tensor([sim.config.radius, 0.0, 0.0],
+                                                       dtype=torch.float64, device=pos.device)
     e = sim.energy()
     k = sim.config.contact_stiffness
     assert e["particle_contact_energy_J"] == pytest.approx(0.5 * k * sim.config.radius ** 2,
