@@ -43,6 +43,12 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "particle-steps/s FP32-force vs FP64 plain on 1 GPU; p50/p99 batched solve latency"
 HBM_PEAK_GBS = 8000.0
+_T0 = time.perf_counter()
+
+
+def progress(msg: str) -> None:
+    """One line on stderr as each phase of the run ends."""
+    print(f"bench.py [{time.perf_counter() - _T0:7.1f} s]: {msg}", file=sys.stderr, flush=True)
 # compulsory bytes per disk per step of plate_step_kernel: x, v read (32) and
 # written (32), link bits read+written (2), fixed flag (1), candidate count (1),
 # list-build offset xb (8)
@@ -553,15 +559,16 @@ def bench_bed(args):
     warm.close()
     del warm
 
-    def run(eng, timed):
+    def run(eng, timed, label):
         """Walk the windows: advance untimed to each start, then `timed(eng,
         k)`, which returns (result, steps it advanced)."""
         at, out = 0, []
-        for s0, k in wins:
+        for w, (s0, k) in enumerate(wins):
             _advance(eng, s0 - at)
             res, adv = timed(eng, k)
             out.append(res)
             at = s0 + adv
+            progress(f"bed {label}: window {w + 1}/{len(wins)} done")
         return out
 
     # a short window replays its graph once untimed first (the first replay
@@ -596,7 +603,7 @@ def bench_bed(args):
     out = {}
     for fp32 in (True, False):
         eng = engine(fp32)
-        secs = run(eng, graph_window)
+        secs = run(eng, graph_window, "FP32 graph" if fp32 else "FP64 graph")
         gpu_s = sum(secs)
         acc = eng.accounting[0].cpu().numpy()
         out["fp32" if fp32 else "fp64"] = dict(
@@ -637,7 +644,7 @@ def bench_bed(args):
                 half = (eng.x.cpu().numpy(), eng.v.cpu().numpy())
         return (us, byts), lk + kk
 
-    ev = run(eng, event_window)
+    ev = run(eng, event_window, "kernel events")
     eng.close()
     del eng
     us_all = [u for w_us, _ in ev for u in w_us]
@@ -1376,6 +1383,7 @@ def run_workloads(args):
     # headline: the 1M-disk column (the configuration the FP32-force /
     # FP64-state target is stated at); the other configurations ride along
     bed = bench_bed(args)
+    progress("bed column done")
     line = {"metric": METRIC, "value": bed.pop("value"), "unit": "particle-steps/s",
             "n_gpus": 1, "steps": bed["steps"], "warmup": bed.pop("warmup"),
             "ms_per_step": bed["ms_per_step"], "higher_is_better": True, "scaling": "weak",
@@ -1386,10 +1394,15 @@ def run_workloads(args):
     c2 = bench_plate(args)
     c2.pop("metric")
     line["c2_plate"] = c2
+    progress("1024x128 plate done")
     line["c4_ensemble"] = bench_ensemble(args)
+    progress("512-plate ensemble done")
     line["c5_serve"] = bench_serve(args)
+    progress("request loop done")
     line["disk_system"] = bench_disk(args)
+    progress("1M-disk box done")
     line["dem3d_scale"] = bench_dem3d(args)
+    progress("47^3 sphere lattice done")
     print(json.dumps(line))
     return 0
 
@@ -1439,6 +1452,7 @@ def bench_plate(args):
             rate=n * steps / gpu_s, kernel_s=k_s, status=int(st), broken=broken)
         eng.close()
         del eng
+        progress(f"plate {'FP32' if fp32 else 'FP64'} forces done")
     r32, r64 = results["fp32"], results["fp64"]
     for name, r in results.items():
         if r["status"]:
