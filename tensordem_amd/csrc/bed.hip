@@ -181,92 +181,120 @@ __device__ inline BedConst load_const(const tdm_bed_member *mp)
   return c;
 }
 
-// One pair p-q seen from p, lo -> hi oriented.  Returns false without
-// touching anything when the disks do not overlap (decided in FP64 for both
-// precisions).  On contact: F is the force on hi (lo receives -F), xi is
-// updated in place, and the owner's ledger channels are filled.
-template <typename R>
-__device__ inline bool hm_pair(const BedConst &m, double2 x_lo, double2 x_hi,
-                               double2 v_lo, double2 v_hi, double r_lo, double r_hi, double2 &xi,
-                               double &Fx, double &Fy, double &dash, double &fric)
+// The pair-law constants of a member, in scalar registers for the whole step
+// (the rest of the member record is read where it is used, so the step
+// kernel's scalar register budget holds without spills).
+struct BedLaw {
+  double e_star, g_star, damping, friction, rho_pi_t, dt;
+};
+
+__device__ inline BedLaw load_law(const tdm_bed_member *mp)
 {
-  const double dx = x_hi.x - x_lo.x, dy = x_hi.y - x_lo.y;
-  const double s2 = dx * dx + dy * dy;
-  const double rs = r_lo + r_hi;
-  // exact reject without the square root: above (rs^2)(1 + 1e-15) the
-  // correctly rounded sqrt(s2) is >= rs, so (rs - dist) > 0 is false
-  if (s2 > (rs * rs) * (1.0 + 1e-15)) return false;
-  const double dist = sqrt(s2);
+  BedLaw c;
+  c.e_star = uniform(mp->e_star);
+  c.g_star = uniform(mp->g_star);
+  c.damping = uniform(mp->damping);
+  c.friction = uniform(mp->friction);
+  c.rho_pi_t = uniform(mp->rho_pi_t);
+  c.dt = uniform(mp->dt);
+  return c;
+}
+
+// FP32 copies of the law constants, formed once per thread.
+struct BedLaw32 {
+  float two_e, eight_g, damping, friction, rho_pi_t, dt;
+};
+
+__device__ inline BedLaw32 law32(const BedLaw &m)
+{
+  return BedLaw32{2.0f * (float)m.e_star, 8.0f * (float)m.g_star, (float)m.damping,
+                  (float)m.friction, (float)m.rho_pi_t, (float)m.dt};
+}
+
+// Pair law of one overlapping pair, lo -> hi oriented: (dx, dy) = x_hi - x_lo
+// and s2 = dx^2 + dy^2 as the contact test formed them, rs = r_p + r_q,
+// rs2 = rs * rs, (vrx, vry) = v_lo - v_hi.  The radii enter only through
+// r_p r_q, r_p + r_q and the masses' product and sum, so either disk of a pair
+// may pass them in its own order and get the same bits.  On return F is the
+// force on hi (lo receives -F) and xi the pair's updated spring history.
+//
+// FP64: the reference's arithmetic (oracle/bed2d.c), bit for bit.  FP32: the
+// overlap d = rs - |dx| is formed as (rs^2 - s2) / (rs + |dx|), with the
+// numerator in FP64, so no FP64 square root is needed (relative error a few
+// FP32 ulps); whether the pair touches was decided in FP64 beforehand.
+__device__ inline void hm_pair64(const BedLaw &m, double dx, double dy, double dist, double rs,
+                                 double rp, double rq, double vrx, double vry, double2 &xi,
+                                 double &Fx, double &Fy, double &dash, double &fric)
+{
   const double d = rs - dist;
-  if (!(d > 0)) return false;
-  const double vrx = v_lo.x - v_hi.x, vry = v_lo.y - v_hi.y;
-  if constexpr (Prec<R>::fp64) {
-    const double dd = dist > 1e-300 ? dist : 1e-300;
-    const double nx = dx / dd, ny = dy / dd;
-    const double reff = (r_lo * r_hi) / (r_lo + r_hi);
-    const double mlo = m.rho_pi_t * (r_lo * r_lo), mhi = m.rho_pi_t * (r_hi * r_hi);
-    const double meff = (mlo * mhi) / (mlo + mhi);
-    const double sn = (2 * m.e_star) * sqrt(reff * d);
-    const double fel = ((2.0 / 3.0) * sn) * d;
-    const double vn = vrx * nx + vry * ny;
-    double fn = fel + (m.damping * sqrt(sn * meff)) * vn;
-    if (fn < 0) fn = 0;
-    const double st = (8 * m.g_star) * sqrt(reff * d);
-    const double vtx = vrx - vn * nx, vty = vry - vn * ny;
-    double ex = xi.x + vtx * m.dt, ey = xi.y + vty * m.dt;
-    const double mag = sqrt(ex * ex + ey * ey);
-    const double cap = m.friction * fn;
-    if (mag > 0) {
-      const double den = st * mag;
-      double sc = cap / (den > 1e-300 ? den : 1e-300);
-      if (sc > 1) sc = 1;
-      ex = ex * sc;
-      ey = ey * sc;
-    }
-    xi = make_double2(ex, ey);
-    Fx = fn * nx + st * ex;
-    Fy = fn * ny + st * ey;
-    dash = (fn - fel) * vn;
-    fric = st * (ex * vrx + ey * vry);
-  } else {
-    const float df = (float)d, distf = (float)dist;
-    const float inv = rcp32(fmaxf(distf, 1e-30f));
-    const float nx = (float)dx * inv, ny = (float)dy * inv;
-    const float rl = (float)r_lo, rh = (float)r_hi;
-    const float reff = (rl * rh) * rcp32(rl + rh);
-    const float k = (float)m.rho_pi_t;
-    const float mlo = k * (rl * rl), mhi = k * (rh * rh);
-    const float meff = (mlo * mhi) * rcp32(mlo + mhi);
-    const float sq = sqrt32(reff * df);
-    const float sn = (2.0f * (float)m.e_star) * sq;
-    const float fel = ((2.0f / 3.0f) * sn) * df;
-    const float vn = (float)vrx * nx + (float)vry * ny;
-    const float fn = fmaxf(fel + ((float)m.damping * sqrt32(sn * meff)) * vn, 0.0f);
-    const float st = (8.0f * (float)m.g_star) * sq;
-    const float vtx = (float)vrx - vn * nx, vty = (float)vry - vn * ny;
-    const float dtf = (float)m.dt;
-    float ex = (float)xi.x + vtx * dtf, ey = (float)xi.y + vty * dtf;
-    const float mag = sqrt32(ex * ex + ey * ey);
-    const float cap = (float)m.friction * fn;
-    if (mag > 0) {
-      const float sc = fminf(cap * rcp32(fmaxf(st * mag, 1e-30f)), 1.0f);
-      ex *= sc;
-      ey *= sc;
-    }
-    xi = make_double2((double)ex, (double)ey);
-    Fx = (double)(fn * nx + st * ex);
-    Fy = (double)(fn * ny + st * ey);
-    dash = (double)((fn - fel) * vn);
-    fric = (double)(st * (ex * (float)vrx + ey * (float)vry));
+  const double dd = dist > 1e-300 ? dist : 1e-300;
+  const double nx = dx / dd, ny = dy / dd;
+  const double reff = (rp * rq) / rs;
+  const double mlo = m.rho_pi_t * (rp * rp), mhi = m.rho_pi_t * (rq * rq);
+  const double meff = (mlo * mhi) / (mlo + mhi);
+  const double sn = (2 * m.e_star) * sqrt(reff * d);
+  const double fel = ((2.0 / 3.0) * sn) * d;
+  const double vn = vrx * nx + vry * ny;
+  double fn = fel + (m.damping * sqrt(sn * meff)) * vn;
+  if (fn < 0) fn = 0;
+  const double st = (8 * m.g_star) * sqrt(reff * d);
+  const double vtx = vrx - vn * nx, vty = vry - vn * ny;
+  double ex = xi.x + vtx * m.dt, ey = xi.y + vty * m.dt;
+  const double mag = sqrt(ex * ex + ey * ey);
+  const double cap = m.friction * fn;
+  if (mag > 0) {
+    const double den = st * mag;
+    double sc = cap / (den > 1e-300 ? den : 1e-300);
+    if (sc > 1) sc = 1;
+    ex = ex * sc;
+    ey = ey * sc;
   }
-  return true;
+  xi = make_double2(ex, ey);
+  Fx = fn * nx + st * ex;
+  Fy = fn * ny + st * ey;
+  dash = (fn - fel) * vn;
+  fric = st * (ex * vrx + ey * vry);
+}
+
+__device__ inline void hm_pair32(const BedLaw32 &m, double dx, double dy, double s2, double rs,
+                                 double rs2, float rpf, float rqf, double vrx, double vry,
+                                 float2 &xi, double &Fx, double &Fy, double &dash, double &fric)
+{
+  const float distf = sqrt32((float)s2);
+  const float df = fmaxf((float)(rs2 - s2) * rcp32((float)rs + distf), 0.0f);
+  const float inv = rcp32(fmaxf(distf, 1e-30f));
+  const float nx = (float)dx * inv, ny = (float)dy * inv;
+  const float reff = (rpf * rqf) * rcp32(rpf + rqf);
+  const float mlo = m.rho_pi_t * (rpf * rpf), mhi = m.rho_pi_t * (rqf * rqf);
+  const float meff = (mlo * mhi) * rcp32(mlo + mhi);
+  const float sq = sqrt32(reff * df);
+  const float sn = m.two_e * sq;
+  const float fel = ((2.0f / 3.0f) * sn) * df;
+  const float vxf = (float)vrx, vyf = (float)vry;
+  const float vn = vxf * nx + vyf * ny;
+  const float fn = fmaxf(fel + (m.damping * sqrt32(sn * meff)) * vn, 0.0f);
+  const float st = m.eight_g * sq;
+  const float vtx = vxf - vn * nx, vty = vyf - vn * ny;
+  float ex = xi.x + vtx * m.dt, ey = xi.y + vty * m.dt;
+  const float mag = sqrt32(ex * ex + ey * ey);
+  const float cap = m.friction * fn;
+  if (mag > 0) {
+    const float sc = fminf(cap * rcp32(fmaxf(st * mag, 1e-30f)), 1.0f);
+    ex *= sc;
+    ey *= sc;
+  }
+  xi = make_float2(ex, ey);
+  Fx = (double)(fn * nx + st * ex);
+  Fy = (double)(fn * ny + st * ey);
+  dash = (double)((fn - fel) * vn);
+  fric = (double)(st * (ex * vxf + ey * vyf));
 }
 
 // Wall contact along the inward unit normal (nx, ny) at distance `gap`.
 template <typename R>
-__device__ inline bool hm_wall(const BedConst &m, double r, double mass, double gap,
-                               double nx, double ny, double2 v, double2 &xi, double &Fx,
-                               double &Fy, double &dash, double &fric)
+__device__ inline bool hm_wall(const BedLaw &m, double wall_friction, double r, double mass,
+                               double gap, double nx, double ny, double2 v, double2 &xi,
+                               double &Fx, double &Fy, double &dash, double &fric)
 {
   const double d = r - gap;
   if (!(d > 0)) return false;
@@ -280,7 +308,7 @@ __device__ inline bool hm_wall(const BedConst &m, double r, double mass, double 
     const double vtx = v.x - vn * nx, vty = v.y - vn * ny;
     double ex = xi.x + vtx * m.dt, ey = xi.y + vty * m.dt;
     const double mag = sqrt(ex * ex + ey * ey);
-    const double cap = m.wall_friction * fn;
+    const double cap = wall_friction * fn;
     if (mag > 0) {
       const double den = st * mag;
       double sc = cap / (den > 1e-300 ? den : 1e-300);
@@ -305,7 +333,7 @@ __device__ inline bool hm_wall(const BedConst &m, double r, double mass, double 
     const float dtf = (float)m.dt;
     float ex = (float)xi.x + vtx * dtf, ey = (float)xi.y + vty * dtf;
     const float mag = sqrt32(ex * ex + ey * ey);
-    const float cap = (float)m.wall_friction * fn;
+    const float cap = (float)wall_friction * fn;
     if (mag > 0) {
       const float sc = fminf(cap * rcp32(fmaxf(st * mag, 1e-30f)), 1.0f);
       ex *= sc;
@@ -345,9 +373,27 @@ __device__ inline int cell_of(double ox, double oy, double h, int ncx, int ncy, 
 // length and mask; *status gets CAND_OVERFLOW if S is short.
 constexpr int kScan = 4;
 
+// The member's contact grid, read when a step bins or searches.
+struct BedGrid {
+  double ox, oy, h;
+  int ncx, ncy, first;
+};
+
+__device__ inline BedGrid load_grid(const tdm_bed_member *mp)
+{
+  BedGrid g;
+  g.ox = uniform(mp->grid_origin[0]);
+  g.oy = uniform(mp->grid_origin[1]);
+  g.h = uniform(mp->cell_size);
+  g.ncx = uniform(mp->ncx);
+  g.ncy = uniform(mp->ncy);
+  g.first = uniform(mp->cell_first);
+  return g;
+}
+
 template <typename R>
-__device__ inline int rebuild_list(const BKArgs &k, const BedConst &m, int p, double2 xp,
-                                   double rp, int lp, int nc, uint32_t mask,
+__device__ inline int rebuild_list(const BKArgs &k, const BedGrid &m, double skin, int tb,
+                                   int p, double2 xp, double rp, int lp, int nc, uint32_t mask,
                                    int (&cq)[kPre], uint32_t *nmask_out, int *status)
 {
   const tdm_bed_arrays &a = k.a;
@@ -358,9 +404,9 @@ __device__ inline int rebuild_list(const BKArgs &k, const BedConst &m, int p, do
   int *ncl = k.w.cand + (lp ^ 1) * sn + p;
   Hist<R> *nxi = reinterpret_cast<Hist<R> *>(k.w.xi) + (lp ^ 1) * sn + p;
   int cx, cy;
-  cell_of(m.ox, m.oy, m.cell_size, m.ncx, m.ncy, m.cell_first, xp, &cx, &cy);
-  const int *cntab = k.w.cell_count + (size_t)m.tab * a.ncells;
-  const int *tab = k.w.cell_tab + (size_t)m.tab * a.ncells * a.cell_cap;
+  cell_of(m.ox, m.oy, m.h, m.ncx, m.ncy, m.first, xp, &cx, &cy);
+  const int *cntab = k.w.cell_count + (size_t)tb * a.ncells;
+  const int *tab = k.w.cell_tab + (size_t)tb * a.ncells * a.cell_cap;
   // entries of the 3x3 block, flattened: cells o = 0..8 hold pre[o]..pre[o+1]-1
   int pre[10];
   pre[0] = 0;
@@ -368,7 +414,7 @@ __device__ inline int rebuild_list(const BKArgs &k, const BedConst &m, int p, do
   for (int o = 0; o < 9; ++o) {
     const int gx = cx + o % 3 - 1, gy = cy + o / 3 - 1;
     const bool in = gx >= 0 && gx < m.ncx && gy >= 0 && gy < m.ncy;
-    pre[o + 1] = in ? min(cntab[m.cell_first + gy * m.ncx + gx], a.cell_cap) : 0;
+    pre[o + 1] = in ? min(cntab[m.first + gy * m.ncx + gx], a.cell_cap) : 0;
   }
 #pragma unroll
   for (int o = 0; o < 9; ++o) pre[o + 1] += pre[o];
@@ -391,7 +437,7 @@ __device__ inline int rebuild_list(const BKArgs &k, const BedConst &m, int p, do
 #pragma unroll
         for (int c = 0; c < 9; ++c)
           if (c == o) { oo = c; pr = pre[c]; }
-        const int cell = m.cell_first + (cy + oo / 3 - 1) * m.ncx + (cx + oo % 3 - 1);
+        const int cell = m.first + (cy + oo / 3 - 1) * m.ncx + (cx + oo % 3 - 1);
         qs[u] = tab[(size_t)cell * a.cell_cap + (t - pr)];
       }
     }
@@ -410,7 +456,7 @@ __device__ inline int rebuild_list(const BKArgs &k, const BedConst &m, int p, do
       const double dx = xq[u].x - xp.x, dy = xq[u].y - xp.y;
       // candidates: within r_p + r_q + skin (the slightly inclusive squared
       // form only ever adds pairs, which the contact test then rejects)
-      const double cut = (rp + rq[u]) + m.skin;
+      const double cut = (rp + rq[u]) + skin;
       if (!(dx * dx + dy * dy < (cut * cut) * (1.0 + 1e-15))) continue;
       if (nn == S) { *status |= TDM_ST_CAND_OVERFLOW; continue; }
       // sorted insert: one compare-exchange pass over the register list
@@ -474,6 +520,7 @@ __global__ __launch_bounds__(kThreads) void bed_step_kernel(BKArgs k)
   const int p = p0 + (valid ? (int)threadIdx.x : 0);
   const int n = a.n, S = a.cand_slots;
   const size_t sn = (size_t)S * n;
+  const tdm_bed_member *mp = a.members + mi;
   // own state, list header and the first kPre candidate slots in one round
   const double2 xp = k.x_in[p], vp = k.v_in[p];
   const double rp = a.radius[p];
@@ -483,7 +530,10 @@ __global__ __launch_bounds__(kThreads) void bed_step_kernel(BKArgs k)
   int cq[kPre];  // the first list slots; later slots are read when reached
 #pragma unroll
   for (int e = 0; e < kPre; ++e) cq[e] = e < S ? k.w.cand[lp0 * sn + (size_t)e * n + p] : p;
-  BedConst m = load_const(a.members + mi);
+  const BedLaw law = load_law(mp);
+  const double gravity_y = uniform(mp->gravity_y), skin = uniform(mp->skin);
+  int mode = uniform(mp->mode), tab = uniform(mp->tab);
+  double time = uniform(mp->time);
   BedDyn pd, cur;
   int pflags = 0;
   if (kDeferred) {
@@ -493,89 +543,105 @@ __global__ __launch_bounds__(kThreads) void bed_step_kernel(BKArgs k)
     const int4 pf4 = reinterpret_cast<const int4 *>(k.w.mflags)[mi];
     const int r = (int)(pd.step % 3);
     pflags = r == 0 ? pf4.x : (r == 1 ? pf4.y : pf4.z);
-    cur = pd.fresh ? pd : bed_advance(pd, pflags, m.dt);
-    m.mode = cur.mode;
-    m.tab = cur.tab;
-    m.time = cur.time;
+    cur = pd.fresh ? pd : bed_advance(pd, pflags, law.dt);
+    mode = cur.mode;
+    tab = cur.tab;
+    time = cur.time;
   }
   if (threadIdx.x < 2) sflags[threadIdx.x] = 0;
-  double ch[kStepCh];
-#pragma unroll
-  for (int c = 0; c < kStepCh; ++c) ch[c] = 0.0;
+  // ledger channels, gathered into ch[] after the forces (the pair loop
+  // carries only its own two)
+  double c_dash = 0.0, c_fric = 0.0, c_wdash = 0.0, c_wfric = 0.0, c_wfx = 0.0, c_wfy = 0.0;
+  double c_counts = 0.0, c_wwork = 0.0;
   int status = 0, req = 0;
   int lp = lp0;
 
   if (valid) {
-    const double mass = m.rho_pi_t * (rp * rp);
-    if (m.mode == MODE_SEARCH) {
+    const double mass = law.rho_pi_t * (rp * rp);
+    if (mode == MODE_SEARCH) {
       uint32_t nmask;
-      nc = rebuild_list<R>(k, m, p, xp, rp, lp0, nc, mask, cq, &nmask, &status);
+      nc = rebuild_list<R>(k, load_grid(mp), skin, tab, p, xp, rp, lp0, nc, mask, cq, &nmask,
+                           &status);
       mask = nmask;
       lp = lp0 ^ 1;
       xb = xp;
       k.w.xb[p] = xp;
     } else {
       const double ddx = xp.x - xb.x, ddy = xp.y - xb.y;
-      const double lim = 0.5 * m.skin * 0.999;
+      const double lim = 0.5 * skin * 0.999;
       if (ddx * ddx + ddy * ddy >= lim * lim) status |= TDM_ST_STALE_LIST;
     }
     Hist<R> *xis = reinterpret_cast<Hist<R> *>(k.w.xi) + lp * sn + p;
+    const BedLaw32 law_f = law32(law);
+    const float rpf = (float)rp;
 
-    // ---- forces: gravity, pairs by ascending partner, floor, back wall
-    double fx = 0.0, fy = mass * m.gravity_y;
+    // ---- forces: gravity, pairs by ascending partner, floor, back wall.
+    // Slots past the list length read the disk itself (cache hits) so that
+    // every gather of a chunk is issued unconditionally.
+    double fx = 0.0, fy = mass * gravity_y;
     uint32_t nmask = 0;
     int npair = 0, nwall = 0;
 #pragma unroll
     for (int c0 = 0; c0 < kMaxSlots; c0 += kChunk) {
       if (!__any(c0 < nc)) break;
-      double2 xq[kChunk], vq[kChunk], hq[kChunk];
-      double rq[kChunk];
       int qq[kChunk];
-      uint32_t hi_bits = 0;  // bit j: partner j has the higher index (p is lo)
+      double2 xq[kChunk], vq[kChunk];
+      double rq[kChunk];
+      Hist<R> hq[kChunk];
 #pragma unroll
       for (int j = 0; j < kChunk; ++j) {
         const int e = c0 + j;
-        xq[j] = vq[j] = hq[j] = make_double2(0.0, 0.0);
-        rq[j] = 1.0;
-        qq[j] = p;
-        if (e < nc) {
-          const int q = e < kPre ? cq[e] : k.w.cand[lp * sn + (size_t)e * n + p];
-          qq[j] = q;
-          hi_bits |= (uint32_t)(q > p) << j;
-          xq[j] = k.x_in[q];
-          rq[j] = a.radius[q];
-          // a partner's velocity is needed only in contact: fetched with the
-          // round for the pairs in contact last step, on demand for new ones
-          if ((mask >> e) & 1u) {
-            vq[j] = k.v_in[q];
-            hq[j] = widen(xis[(size_t)e * n]);
-          }
+        if (e < kPre) qq[j] = e < nc ? cq[e] : p;
+        else qq[j] = e < nc ? k.w.cand[lp * sn + (size_t)e * n + p] : p;
+        xq[j] = k.x_in[qq[j]];
+        rq[j] = a.radius[qq[j]];
+        // a partner's velocity and the pair's history are needed only in
+        // contact: fetched with the round for the pairs in contact last
+        // step, on demand for new ones
+        if ((mask >> e) & 1u) {
+          vq[j] = k.v_in[qq[j]];
+          hq[j] = xis[(size_t)e * n];
         }
       }
 #pragma unroll
       for (int j = 0; j < kChunk; ++j) {
         const int e = c0 + j;
         if (e >= nc) continue;
-        const bool self_lo = (hi_bits >> j) & 1u;  // the pair law is evaluated lo -> hi
-        const double2 xl = self_lo ? xp : xq[j], xh = self_lo ? xq[j] : xp;
-        const double rl = self_lo ? rp : rq[j], rh = self_lo ? rq[j] : rp;
-        {
-          // hm_pair's overlap test, before the velocity is needed
-          const double dx = xh.x - xl.x, dy = xh.y - xl.y;
-          const double s2 = dx * dx + dy * dy;
-          const double rs = rl + rh;
-          if (s2 > (rs * rs) * (1.0 + 1e-15) || !(rs - sqrt(s2) > 0)) continue;
+        // partner minus self; the lo -> hi difference is this or its
+        // negation (exact), and s2 is the same either way
+        double dx = xq[j].x - xp.x, dy = xq[j].y - xp.y;
+        const double s2 = dx * dx + dy * dy;
+        const double rs = rp + rq[j];
+        const double rs2 = rs * rs;
+        // contact iff rs - sqrt(s2) > 0, decided in FP64 for both precisions:
+        // above rs2 (1 + 1e-15) the correctly rounded root is >= rs, below
+        // rs2 (1 - 1e-15) it is < rs; only the band between needs the root
+        if (s2 > rs2 * (1.0 + 1e-15)) continue;
+        double dist = 0.0;
+        if constexpr (Prec<R>::fp64) {
+          dist = sqrt(s2);
+          if (!(rs - dist > 0)) continue;
+        } else {
+          if (!(s2 < rs2 * (1.0 - 1e-15)) && !(rs - sqrt(s2) > 0)) continue;
         }
-        if (!((mask >> e) & 1u)) vq[j] = k.v_in[qq[j]];
-        const double2 vl = self_lo ? vp : vq[j], vh = self_lo ? vq[j] : vp;
-        double2 h = hq[j];
+        const bool self_lo = qq[j] > p;
+        if (!((mask >> e) & 1u)) {
+          vq[j] = k.v_in[qq[j]];
+          hq[j] = Hist<R>{0, 0};
+        }
+        double vrx = vp.x - vq[j].x, vry = vp.y - vq[j].y;  // self minus partner
+        if (!self_lo) { dx = -dx; dy = -dy; vrx = -vrx; vry = -vry; }
         double Fx, Fy, dash, fric;
-        if (!hm_pair<R>(m, xl, xh, vl, vh, rl, rh, h, Fx, Fy, dash, fric)) continue;
-        xis[(size_t)e * n] = narrow<Hist<R>>(h);
+        if constexpr (Prec<R>::fp64)
+          hm_pair64(law, dx, dy, dist, rs, rp, rq[j], vrx, vry, hq[j], Fx, Fy, dash, fric);
+        else
+          hm_pair32(law_f, dx, dy, s2, rs, rs2, rpf, (float)rq[j], vrx, vry, hq[j], Fx, Fy,
+                    dash, fric);
+        xis[(size_t)e * n] = hq[j];
         nmask |= 1u << e;
         if (self_lo) {
           fx += -Fx; fy += -Fy;
-          ch[C_DASH] += dash; ch[C_FRIC] += fric; ++npair;
+          c_dash += dash; c_fric += fric; ++npair;
         } else {
           fx += Fx; fy += Fy;
         }
@@ -583,75 +649,84 @@ __global__ __launch_bounds__(kThreads) void bed_step_kernel(BKArgs k)
     }
     // floor, back wall, right wall: one loop so the law is inlined once
     double2 *xw = reinterpret_cast<double2 *>(a.xi_wall) + 3 * p;
+    const int use_floor = uniform(mp->use_floor), use_back = uniform(mp->use_back_wall);
+    const int use_right = uniform(mp->use_right_wall);
+    const double floor_y = uniform(mp->floor_y), back_x = uniform(mp->back_wall_x);
+    const double right_x = uniform(mp->right_wall_x), right_v = uniform(mp->right_wall_v);
+    const double wall_mu = uniform(mp->wall_friction);
 #pragma unroll 1
     for (int w = 0; w < 3; ++w) {
-      if (!(w == 0 ? m.use_floor : (w == 1 ? m.use_back_wall : m.use_right))) continue;
+      if (!(w == 0 ? use_floor : (w == 1 ? use_back : use_right))) continue;
       // the law sees the disk's velocity relative to the wall (kernel.py:381-389);
       // the floor and the back wall stand still
-      const double wv = w == 2 ? m.right_v : 0.0;
-      const double gap = w == 0 ? xp.y - m.floor_y
-                                : (w == 1 ? xp.x - m.back_wall_x
-                                          : (m.right_x + m.right_v * m.time) - xp.x);
+      const double wv = w == 2 ? right_v : 0.0;
+      const double gap = w == 0 ? xp.y - floor_y
+                                : (w == 1 ? xp.x - back_x : (right_x + right_v * time) - xp.x);
       const double nx = w == 0 ? 0.0 : (w == 1 ? 1.0 : -1.0), ny = w == 0 ? 1.0 : 0.0;
       const uint32_t bit = kFloorBit << w;
       double2 h = (mask & bit) ? xw[w] : make_double2(0.0, 0.0);
       const double2 vr = make_double2(vp.x - wv, vp.y);
       double Fx, Fy, dash, fric;
-      if (hm_wall<R>(m, rp, mass, gap, nx, ny, vr, h, Fx, Fy, dash, fric)) {
+      if (hm_wall<R>(law, wall_mu, rp, mass, gap, nx, ny, vr, h, Fx, Fy, dash, fric)) {
         xw[w] = h;
         nmask |= bit;
         fx += Fx; fy += Fy;
-        ch[C_WDASH] += dash; ch[C_WFRIC] += fric; ch[C_WFX] += Fx; ch[C_WFY] += Fy;
-        ch[C_WWORK] += Fx * wv;
+        c_wdash += dash; c_wfric += fric; c_wfx += Fx; c_wfy += Fy;
+        c_wwork += Fx * wv;
         ++nwall;
       }
     }
     k.w.mask[(size_t)lp * n + p] = nmask;
-    ch[C_COUNTS] = (double)npair + kWallCount * (double)nwall;
+    c_counts = (double)npair + kWallCount * (double)nwall;
 
     // ---- semi-implicit Euler (dem.py:363-365) -------------------------
     double vx, vy;
     if constexpr (Prec<R>::fp64) {
-      vx = vp.x + (m.dt * fx) / mass;
-      vy = vp.y + (m.dt * fy) / mass;
+      vx = vp.x + (law.dt * fx) / mass;
+      vy = vp.y + (law.dt * fy) / mass;
     } else {
-      const double dt_m = m.dt / mass;
+      const double dt_m = law.dt / mass;
       vx = vp.x + fx * dt_m;
       vy = vp.y + fy * dt_m;
     }
-    const double xn = xp.x + m.dt * vx, yn = xp.y + m.dt * vy;
+    const double xn = xp.x + law.dt * vx, yn = xp.y + law.dt * vy;
     k.x_out[p] = make_double2(xn, yn);
     k.v_out[p] = make_double2(vx, vy);
     if (!isfinite(xn) || !isfinite(yn) || !isfinite(vx) || !isfinite(vy)) status |= TDM_ST_NONFINITE;
     const double ddx = xn - xb.x, ddy = yn - xb.y;
-    const double q4 = kRebuildFrac * m.skin;
+    const double q4 = kRebuildFrac * skin;
     if (ddx * ddx + ddy * ddy >= q4 * q4) req = 1;
-    if (m.mode == MODE_BIN) {
+    if (mode == MODE_BIN) {
+      const BedGrid g = load_grid(mp);
       int cx, cy;
-      const int cell = cell_of(m.ox, m.oy, m.cell_size, m.ncx, m.ncy, m.cell_first,
-                               make_double2(xn, yn), &cx, &cy);
-      const int t = m.tab ^ 1;
+      const int cell = cell_of(g.ox, g.oy, g.h, g.ncx, g.ncy, g.first, make_double2(xn, yn), &cx,
+                               &cy);
+      const int t = tab ^ 1;
       const int slot = atomicAdd(k.w.cell_count + (size_t)t * a.ncells + cell, 1);
       if (slot < a.cell_cap) k.w.cell_tab[((size_t)t * a.ncells + cell) * a.cell_cap + slot] = p;
       else status |= TDM_ST_CELL_OVERFLOW;
     }
   }
-  if (m.mode == MODE_SEARCH) {
+  if (mode == MODE_SEARCH) {
     // clear the other hash table for the next bin phase
-    const int ncell_m = m.ncx * m.ncy;
-    int *cntc = k.w.cell_count + (size_t)(m.tab ^ 1) * a.ncells + m.cell_first;
-    for (int c = blk.w * kThreads + threadIdx.x; c < ncell_m; c += m.nblocks * kThreads)
+    const int ncell_m = uniform(mp->ncx) * uniform(mp->ncy);
+    const int nblocks_m = uniform(mp->nblocks);
+    int *cntc = k.w.cell_count + (size_t)(tab ^ 1) * a.ncells + uniform(mp->cell_first);
+    for (int c = blk.w * kThreads + threadIdx.x; c < ncell_m; c += nblocks_m * kThreads)
       cntc[c] = 0;
   }
   __syncthreads();
   if (status) atomicOr(&sflags[0], status);
   if (req) atomicOr(&sflags[1], 1);
+  double ch[kStepCh];
+  ch[C_DASH] = c_dash; ch[C_FRIC] = c_fric; ch[C_WDASH] = c_wdash; ch[C_WFRIC] = c_wfric;
+  ch[C_WFX] = c_wfx; ch[C_WFY] = c_wfy; ch[C_COUNTS] = c_counts; ch[C_WWORK] = c_wwork;
   block_sum(ch, red);
   const int pslot = kDeferred ? (int)(cur.step & 1) : 0;
   if (threadIdx.x == 0) {
     // partials by the member's logical block index: the reduction order does
     // not depend on where the member sits in the launch
-    const size_t slot = (size_t)(m.block_first + blk.w);
+    const size_t slot = (size_t)(uniform(mp->block_first) + blk.w);
 #pragma unroll
     for (int c = 0; c < kStepCh; ++c)
       k.w.partials[((size_t)pslot * kNch + c) * a.nblocks + slot] = ch[c];
@@ -662,7 +737,7 @@ __global__ __launch_bounds__(kThreads) void bed_step_kernel(BKArgs k)
       if (f) atomicOr(k.w.mflags + 4 * mi + (int)(cur.step % 3), f);
       // this block alone reads its table entry: the next launch uses the
       // lists this search wrote
-      if (m.mode == MODE_SEARCH) k.w.blocks[blockIdx.x].z = blk.z ^ kLpBit;
+      if (mode == MODE_SEARCH) k.w.blocks[blockIdx.x].z = blk.z ^ kLpBit;
     }
   }
   if (kDeferred && blk.w == 0) {
