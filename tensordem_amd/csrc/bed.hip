@@ -202,12 +202,13 @@ __device__ inline BedLaw load_law(const tdm_bed_member *mp)
 
 // FP32 copies of the law constants, formed once per thread.
 struct BedLaw32 {
-  float two_e, eight_g, damping, friction, rho_pi_t, dt;
+  float two_e, eight_g, inv_eight_g, damping, friction, rho_pi_t, dt;
 };
 
 __device__ inline BedLaw32 law32(const BedLaw &m)
 {
-  return BedLaw32{2.0f * (float)m.e_star, 8.0f * (float)m.g_star, (float)m.damping,
+  const float eight_g = 8.0f * (float)m.g_star;
+  return BedLaw32{2.0f * (float)m.e_star, eight_g, 1.0f / eight_g, (float)m.damping,
                   (float)m.friction, (float)m.rho_pi_t, (float)m.dt};
 }
 
@@ -221,7 +222,8 @@ __device__ inline BedLaw32 law32(const BedLaw &m)
 // FP64: the reference's arithmetic (oracle/bed2d.c), bit for bit.  FP32: the
 // overlap d = rs - |dx| is formed as (rs^2 - s2) / (rs + |dx|), with the
 // numerator in FP64, so no FP64 square root is needed (relative error a few
-// FP32 ulps); whether the pair touches was decided in FP64 beforehand.
+// FP32 ulps), and roots and reciprocals share hardware rsq/rcp results;
+// whether the pair touches was decided in FP64 beforehand.
 __device__ inline void hm_pair64(const BedLaw &m, double dx, double dy, double dist, double rs,
                                  double rp, double rq, double vrx, double vry, double2 &xi,
                                  double &Fx, double &Fy, double &dash, double &fric)
@@ -260,14 +262,23 @@ __device__ inline void hm_pair32(const BedLaw32 &m, double dx, double dy, double
                                  double rs2, float rpf, float rqf, double vrx, double vry,
                                  float2 &xi, double &Fx, double &Fy, double &dash, double &fric)
 {
-  const float distf = sqrt32((float)s2);
+  // six transcendental operations per pair: 1/|dx| and |dx| from one rsq,
+  // R* and m* over one common reciprocal, sqrt(R* d) and its inverse from
+  // one rsq (the inverse feeds the Coulomb scale), the damping root and the
+  // spring's inverse length
+  const float s2f = (float)s2;
+  const float inv = rsq32(fmaxf(s2f, 1e-37f));
+  const float distf = s2f * inv;
   const float df = fmaxf((float)(rs2 - s2) * rcp32((float)rs + distf), 0.0f);
-  const float inv = rcp32(fmaxf(distf, 1e-30f));
   const float nx = (float)dx * inv, ny = (float)dy * inv;
-  const float reff = (rpf * rqf) * rcp32(rpf + rqf);
-  const float mlo = m.rho_pi_t * (rpf * rpf), mhi = m.rho_pi_t * (rqf * rqf);
-  const float meff = (mlo * mhi) * rcp32(mlo + mhi);
-  const float sq = sqrt32(reff * df);
+  const float rp2 = rpf * rpf, rq2 = rqf * rqf;
+  const float rsum = rpf + rqf, r2sum = rp2 + rq2;
+  const float rden = rcp32(rsum * r2sum);
+  const float reff = ((rpf * rqf) * r2sum) * rden;
+  const float meff = (m.rho_pi_t * (rp2 * rq2)) * (rsum * rden);
+  const float t = reff * df;
+  const float it = rsq32(fmaxf(t, 1e-37f));
+  const float sq = t * it;
   const float sn = m.two_e * sq;
   const float fel = ((2.0f / 3.0f) * sn) * df;
   const float vxf = (float)vrx, vyf = (float)vry;
@@ -276,10 +287,10 @@ __device__ inline void hm_pair32(const BedLaw32 &m, double dx, double dy, double
   const float st = m.eight_g * sq;
   const float vtx = vxf - vn * nx, vty = vyf - vn * ny;
   float ex = xi.x + vtx * m.dt, ey = xi.y + vty * m.dt;
-  const float mag = sqrt32(ex * ex + ey * ey);
-  const float cap = m.friction * fn;
-  if (mag > 0) {
-    const float sc = fminf(cap * rcp32(fmaxf(st * mag, 1e-30f)), 1.0f);
+  const float m2 = ex * ex + ey * ey;
+  if (m2 > 0) {
+    // |xi| s_t capped at mu f_n: scale = mu f_n / (s_t |xi|)
+    const float sc = fminf(((m.friction * fn) * (it * m.inv_eight_g)) * rsq32(m2), 1.0f);
     ex *= sc;
     ey *= sc;
   }

@@ -28,6 +28,7 @@ __host__ __device__ inline size_t align256(size_t v) { return (v + 255) & ~size_
 // FP32 margin are re-done in FP64, so these never decide a fracture.
 __device__ inline float rcp32(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ inline float sqrt32(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ inline float rsq32(float x) { return __builtin_amdgcn_rsqf(x); }
 
 // Deterministic wave-wide sum of a double: a fixed DPP shift/broadcast tree
 // (row_shr 1,2,4,8 inside each 16-lane row, then row_bcast 15 and 31), both

@@ -190,10 +190,12 @@ struct KArgs {
 // with the lo -> hi orientation so both ends compute identical bits.  Returns
 // false when the pair does not overlap; else (Fx, Fy) is the force on lo's
 // partner side convention of the reference (`pair_force`, applied +F to lo,
-// -F to hi) and `diss` the pair's dissipated power (dem.py:327-328).
-template <typename R, typename F>
+// -F to hi) and `diss` the pair's dissipated power (dem.py:327-328).  The
+// normal of coincident disks falls back to the pair's unit rest vector
+// (dem.py:241-242), which n0() forms only in that case.
+template <typename R, typename F, typename N0>
 __device__ inline bool contact_pair(double2 x_lo, double2 x_hi, double2 v_lo, double2 v_hi,
-                                    double2 n0, double radius, double kc, double cdamp,
+                                    N0 &&n0, double radius, double kc, double cdamp,
                                     F &Fx, F &Fy, double &diss)
 {
   const double two_r = 2 * radius;
@@ -203,7 +205,7 @@ __device__ inline bool contact_pair(double2 x_lo, double2 x_hi, double2 v_lo, do
     if (!(d < two_r)) return false;
     double nx, ny;
     if (d > 0) { double dd = d > kEps64 ? d : kEps64; nx = dx / dd; ny = dy / dd; }
-    else { nx = n0.x; ny = n0.y; }
+    else { const double2 u = n0(); nx = u.x; ny = u.y; }
     double ov = two_r - d;
     double vn = (v_hi.x - v_lo.x) * nx + (v_hi.y - v_lo.y) * ny;
     double rep = kc * ov - cdamp * vn;
@@ -218,7 +220,7 @@ __device__ inline bool contact_pair(double2 x_lo, double2 x_hi, double2 v_lo, do
     if (!(d < tr)) return false;
     float nx, ny;
     if (d > 0) { float r = rcp32(fmaxf(d, kEps32)); nx = dx * r; ny = dy * r; }
-    else { nx = (float)n0.x; ny = (float)n0.y; }
+    else { const double2 u = n0(); nx = (float)u.x; ny = (float)u.y; }
     float ov = tr - d;
     float vn = (float)(v_hi.x - v_lo.x) * nx + (float)(v_hi.y - v_lo.y) * ny;
     float kcf = (float)kc;
@@ -449,6 +451,16 @@ __device__ __forceinline__ void plate_step_body(const KArgs &k)
       if (ddx * ddx + ddy * ddy >= lim * lim) status |= TDM_ST_STALE_LIST;
     }
     int split = 0;  // list slots below split hold partners below p
+    // unit rest vector lo -> hi of p and a non-neighbour q (the normal of
+    // coincident disks): the two lattice sites' difference, dem.py:119-129
+    auto rest_dir = [&](int q, bool self_lo) {
+      const int ql = q - first, jx = ql % nx, jy = ql / nx;
+      const double2 sq = lattice_site(jx, jy, half_span, two_r);
+      const double2 s_lo = self_lo ? sp : sq, s_hi = self_lo ? sq : sp;
+      const double ix0 = s_hi.x - s_lo.x, iy0 = s_hi.y - s_lo.y;
+      const double id = sqrt(ix0 * ix0 + iy0 * iy0);
+      return make_double2(ix0 / id, iy0 / id);
+    };
 
     // unbonded candidate q, lo -> hi oriented; accumulates into f
     auto unbonded = [&](int q, double &fx, double &fy, bool energy) {
@@ -463,14 +475,9 @@ __device__ __forceinline__ void plate_step_body(const KArgs &k)
         return;
       }
       const double2 vq = k.v_in[q];
-      const int ql = q - first, jx = ql % nx, jy = ql / nx;
-      const double2 sq = lattice_site(jx, jy, half_span, two_r);
-      const double2 s_lo = self_lo ? sp : sq, s_hi = self_lo ? sq : sp;
-      double ix0 = s_hi.x - s_lo.x, iy0 = s_hi.y - s_lo.y;
-      double id = sqrt(ix0 * ix0 + iy0 * iy0);
       double Fx, Fy, diss;
       if (contact_pair<R>(self_lo ? xp : xq, self_lo ? xq : xp, self_lo ? vp : vq,
-                          self_lo ? vq : vp, make_double2(ix0 / id, iy0 / id), m.radius, kc,
+                          self_lo ? vq : vp, [&] { return rest_dir(q, self_lo); }, m.radius, kc,
                           cdamp, Fx, Fy, diss)) {
         if (self_lo) { fx += Fx; fy += Fy; dsc += diss; }
         else { fx += -Fx; fy += -Fy; }
@@ -547,15 +554,10 @@ __device__ __forceinline__ void plate_step_body(const KArgs &k)
             const int q = qs[u];
             cQ[e][threadIdx.x] = q;
             const bool self_lo = p < q;
-            const int ql = q - first, jx = ql % nx, jy = ql / nx;
-            const double2 sq = lattice_site(jx, jy, half_span, two_r);
-            const double2 s_lo = self_lo ? sp : sq, s_hi = self_lo ? sq : sp;
-            const double ix0 = s_hi.x - s_lo.x, iy0 = s_hi.y - s_lo.y;
-            const double id = sqrt(ix0 * ix0 + iy0 * iy0);
             double Fx, Fy, diss;
             if (contact_pair<R>(self_lo ? xp : xq[u], self_lo ? xq[u] : xp, self_lo ? vp : vq[u],
-                                self_lo ? vq[u] : vp, make_double2(ix0 / id, iy0 / id), m.radius,
-                                kc, cdamp, Fx, Fy, diss)) {
+                                self_lo ? vq[u] : vp, [&] { return rest_dir(q, self_lo); },
+                                m.radius, kc, cdamp, Fx, Fy, diss)) {
               // the signed contribution the sum adds (FP32 forces are float
               // values, so the narrowing store is exact)
               chas |= 1u << e;
@@ -665,8 +667,8 @@ __device__ __forceinline__ void plate_step_body(const KArgs &k)
             const double2 vq = k.v_in[q];
             double diss;
             if (contact_pair<double>(x_lo, x_hi, self_lo ? vp : vq, self_lo ? vq : vp,
-                                     make_double2(rx / rest, ry / rest), m.radius, kc, cdamp,
-                                     Fx[c], Fy[c], diss)) {
+                                     [&] { return make_double2(rx / rest, ry / rest); }, m.radius,
+                                     kc, cdamp, Fx[c], Fy[c], diss)) {
               has |= 1u << c;
               if (self_lo) ch[C_PAIR_DISS] += diss;
             }
@@ -718,11 +720,14 @@ __device__ __forceinline__ void plate_step_body(const KArgs &k)
             has |= 1u << c;
           } else {
             const double2 vq = k.v_in[q];
-            double diss, rx, ry, rest;
-            site_vec(rx, ry, rest);
-            if (contact_pair<float>(x_lo, x_hi, self_lo ? vp : vq, self_lo ? vq : vp,
-                                    make_double2(rx / rest, ry / rest), m.radius, kc, cdamp,
-                                    Fx[c], Fy[c], diss)) {
+            double diss;
+            auto n0 = [&] {
+              double rx, ry, rest;
+              site_vec(rx, ry, rest);
+              return make_double2(rx / rest, ry / rest);
+            };
+            if (contact_pair<float>(x_lo, x_hi, self_lo ? vp : vq, self_lo ? vq : vp, n0,
+                                    m.radius, kc, cdamp, Fx[c], Fy[c], diss)) {
               has |= 1u << c;
               if (self_lo) ch[C_PAIR_DISS] += diss;
             }

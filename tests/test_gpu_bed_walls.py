@@ -92,7 +92,10 @@ def test_moving_wall_graph_replays_and_restart(fp32):
 
 
 def test_settled_release_state():
-    state, info = settled_column(20000, R0, seed=1, max_steps=60000)
+    # settling ends once the kinetic energy is below 1e-6 of m g H (checked
+    # every 10 000 steps); the step count that takes depends on the FP32
+    # rounding of the forces, so the cap leaves room
+    state, info = settled_column(20000, R0, seed=1, max_steps=150000)
     print(info)
     assert info["kinetic_rel"] < 1e-6
     n = len(state["radii"])
