@@ -35,7 +35,7 @@ constexpr int kCandLds = 6;     // candidate slots whose pair forces are staged 
 constexpr int kStageMaxTile = 128;
 constexpr int kMaxLdsStage = (kStageMaxTile + 4) * (kThreads / kStageMaxTile + 4);
 constexpr int kMaxRingStage = (kStageMaxTile + 2) * (kThreads / kStageMaxTile + 2);
-constexpr int kCandBatch = 2;   // candidates gathered per round of the force pre-pass
+constexpr int kCandBatch = kCandLds;  // candidates gathered per round of the force pre-pass
 
 // ledger partial channels
 enum {
@@ -122,8 +122,8 @@ __device__ inline uint32_t present_links(int ix, int iy, int nx, int ny)
 // ---------------------------------------------------------------------------
 // Shear strain (dem.py:182-208).  Bonds are visited as index_add_ visits
 // them: partners above the disk by ascending index (codes 4..7), then
-// partners below (codes 0..3).  pinv of the 2x2 covariance is the closed-form
-// inverse.
+// partners below (codes 0..3).  pinv of the 2x2 covariance C = sum r r^T is
+// the closed-form inverse.
 // ---------------------------------------------------------------------------
 struct Tile {
   const double2 *sx;  // LDS positions, row stride lw, origin (tx0-2, ty0-2)
@@ -132,47 +132,76 @@ struct Tile {
   __device__ double2 x(int ix, int iy) const { return sx[(iy - ty0 + kHalo) * lw + (ix - tx0 + kHalo)]; }
 };
 
-__device__ double strain64(const Tile &t, int ix, int iy)
+// Rest vector of bond code c of disk (ix, iy), oriented as the strain sums
+// see it (from the disk to partners above, from partners below to the disk).
+__device__ inline void rest_vec(int ix, int iy, int c, double half_span, double two_r,
+                                double &rx, double &ry)
 {
-  double c00 = 0, c01 = 0, c10 = 0, c11 = 0, x00 = 0, x01 = 0, x10 = 0, x11 = 0;
-  const double2 xp = t.x(ix, iy);
-  const double2 sp = lattice_site(ix, iy, t.half_span, t.two_r);
+  const double2 sp = lattice_site(ix, iy, half_span, two_r);
+  const double2 sq = lattice_site(ix + dxc(c), iy + dyc(c), half_span, two_r);
+  if (c >= 4) { rx = sq.x - sp.x; ry = sq.y - sp.y; }
+  else        { rx = sp.x - sq.x; ry = sp.y - sq.y; }
+}
+
+// Strains of U disks of the tile ring, interleaved for instruction-level
+// parallelism when U = 2 (a thread with one disk passes it twice).  An
+// absent bond contributes (+0) x (+0) to each sum of C and X, which leaves
+// it unchanged (the sums start at +0 and never become -0), so no branch is
+// needed.
+template <int U>
+__device__ void strain64x(const Tile &t, const int (&hx)[U], const int (&hy)[U], double (&s)[U])
+{
+  double x00[U], x01[U], x10[U], x11[U], c00[U], c01[U], c10[U], c11[U];
+  double2 xp[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    x00[u] = x01[u] = x10[u] = x11[u] = 0.0;
+    c00[u] = c01[u] = c10[u] = c11[u] = 0.0;
+    xp[u] = t.x(hx[u], hy[u]);
+  }
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
     const int c = (k + 4) & 7;
-    const int jx = ix + dxc(c), jy = iy + dyc(c);
-    if (!on_lattice(jx, jy, t.nx, t.ny)) continue;
-    const double2 xq = t.x(jx, jy);
-    const double2 sq = lattice_site(jx, jy, t.half_span, t.two_r);
-    double rx, ry, qx, qy;
-    if (c >= 4) { rx = sq.x - sp.x; ry = sq.y - sp.y; qx = xq.x - xp.x; qy = xq.y - xp.y; }
-    else        { rx = sp.x - sq.x; ry = sp.y - sq.y; qx = xp.x - xq.x; qy = xp.y - xq.y; }
-    c00 += rx * rx; c01 += rx * ry; c10 += ry * rx; c11 += ry * ry;
-    x00 += qx * rx; x01 += qx * ry; x10 += qy * rx; x11 += qy * ry;
-  }
-  double det = c00 * c11 - c01 * c10;
-  double i00, i01, i10, i11;
-  if (det != 0.0) {
-    i00 = c11 / det; i01 = -c01 / det; i10 = -c10 / det; i11 = c00 / det;
-  } else {
-    double tr = c00 + c11;
-    if (tr > 0.0) {
-      double t2 = tr * tr;
-      i00 = c00 / t2; i01 = c01 / t2; i10 = c10 / t2; i11 = c11 / t2;
-    } else {
-      i00 = i01 = i10 = i11 = 0.0;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int jx = hx[u] + dxc(c), jy = hy[u] + dyc(c);
+      const bool pr = on_lattice(jx, jy, t.nx, t.ny);
+      const double2 xq = t.x(jx, jy);
+      double rx, ry;
+      rest_vec(hx[u], hy[u], c, t.half_span, t.two_r, rx, ry);
+      double qx = c >= 4 ? xq.x - xp[u].x : xp[u].x - xq.x;
+      double qy = c >= 4 ? xq.y - xp[u].y : xp[u].y - xq.y;
+      if (!pr) { rx = 0.0; ry = 0.0; qx = 0.0; qy = 0.0; }
+      c00[u] += rx * rx; c01[u] += rx * ry; c10[u] += ry * rx; c11[u] += ry * ry;
+      x00[u] += qx * rx; x01[u] += qx * ry; x10[u] += qy * rx; x11[u] += qy * ry;
     }
   }
-  double f00 = x00 * i00 + x01 * i10;
-  double f01 = x00 * i01 + x01 * i11;
-  double f10 = x10 * i00 + x11 * i10;
-  double f11 = x10 * i01 + x11 * i11;
-  double g00 = f00 * f00 + f10 * f10;
-  double g01 = f00 * f01 + f10 * f11;
-  double g11 = f01 * f01 + f11 * f11;
-  double exx = 0.5 * (g00 - 1.0), exy = 0.5 * (g01 - 0.0), eyy = 0.5 * (g11 - 1.0);
-  double dxy = exx - eyy;
-  return sqrt(dxy * dxy + 4.0 * (exy * exy));
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    double i00, i01, i10, i11;
+    const double det = c00[u] * c11[u] - c01[u] * c10[u];
+    if (det != 0.0) {
+      i00 = c11[u] / det; i01 = -c01[u] / det; i10 = -c10[u] / det; i11 = c00[u] / det;
+    } else {
+      const double tr = c00[u] + c11[u];
+      if (tr > 0.0) {
+        const double t2 = tr * tr;
+        i00 = c00[u] / t2; i01 = c01[u] / t2; i10 = c10[u] / t2; i11 = c11[u] / t2;
+      } else {
+        i00 = i01 = i10 = i11 = 0.0;
+      }
+    }
+    double f00 = x00[u] * i00 + x01[u] * i10;
+    double f01 = x00[u] * i01 + x01[u] * i11;
+    double f10 = x10[u] * i00 + x11[u] * i10;
+    double f11 = x10[u] * i01 + x11[u] * i11;
+    double g00 = f00 * f00 + f10 * f10;
+    double g01 = f00 * f01 + f10 * f11;
+    double g11 = f01 * f01 + f11 * f11;
+    double exx = 0.5 * (g00 - 1.0), exy = 0.5 * (g01 - 0.0), eyy = 0.5 * (g11 - 1.0);
+    double dxy = exx - eyy;
+    s[u] = sqrt(dxy * dxy + 4.0 * (exy * exy));
+  }
 }
 
 struct KArgs {
@@ -234,6 +263,23 @@ __device__ inline bool contact_pair(double2 x_lo, double2 x_hi, double2 v_lo, do
 __device__ void member_ledger(const KArgs &k, int mi, double time, int flags, int pslot,
                               double (*wsum)[kCh]);
 
+// A member's clock and search phase at the start of a step, from the state
+// at the start of the previous step and the flags that step raised (the
+// finalize and flush launches advance the member record the same way).
+__device__ inline MemberDyn advance_dyn(const MemberDyn &pd, int pf, double dt)
+{
+  MemberDyn d = pd;
+  d.fresh = 0;
+  d.pad = 0;
+  if (!pd.fresh) {
+    d.time = pd.time + dt;
+    d.step = pd.step + 1;
+    if (pd.mode == MODE_BIN) { d.tab = pd.tab ^ 1; d.mode = MODE_SEARCH; }
+    else d.mode = (pf >> 16) ? MODE_BIN : MODE_IDLE;
+  }
+  return d;
+}
+
 // kStage: the first kCandLds candidates of a disk are gathered ahead of the
 // bonds and their pair forces staged in LDS (18 KB FP32 / 30 KB FP64 more
 // per block, tiles at most kStageMaxTile wide).
@@ -252,6 +298,23 @@ __device__ __forceinline__ void plate_step_body(const KArgs &k)
   __shared__ R cFx[kCandLds][kThreads], cFy[kCandLds][kThreads];
 
   const tdm_plate_arrays &a = k.a;
+  if (kDeferred && (int)blockIdx.x >= a.nblocks) {
+    // one extra block per member carries the member forward (this step's
+    // clock and search phase for the next launch) and reduces the previous
+    // step's ledger, whose partials and flags are complete now; kept apart
+    // from the tiles so that no tile waits on the reduction
+    const int mi = (int)blockIdx.x - a.nblocks;
+    const MemberDyn pd = k.w.dyn[2 * mi + (k.parity ^ 1)];
+    const int4 pf4 = reinterpret_cast<const int4 *>(k.w.mflags)[mi];
+    const int pf = reinterpret_cast<const int *>(&pf4)[pd.step % 3];
+    const MemberDyn nd = advance_dyn(pd, pf, a.members[mi].dt);
+    if (threadIdx.x == 0) {
+      k.w.dyn[2 * mi + k.parity] = nd;
+      k.w.mflags[4 * mi + (int)((nd.step + 1) % 3)] = 0;
+    }
+    if (!pd.fresh) member_ledger(k, mi, pd.time, pf, (int)(pd.step & 1), red);
+    return;
+  }
   // block table entry: (member, tile x0, tile y0, rank), (nx, ny, first, tile_x)
   const int4 blk = k.w.blocks[2 * blockIdx.x];
   const int4 geo = k.w.blocks[2 * blockIdx.x + 1];
@@ -293,6 +356,8 @@ __device__ __forceinline__ void plate_step_body(const KArgs &k)
           reinterpret_cast<const uint64_t *>(k.w.dyn + 2 * mi + (k.parity ^ 1))[t];
     if (t == 0) sprevf = reinterpret_cast<const int4 *>(k.w.mflags)[mi];
   }
+  const int rw = TX + 2, nring = rw * (TY + 2);
+  auto ring_site = [&](int t, int &hx, int &hy) { hx = tx0 - 1 + t % rw; hy = ty0 - 1 + t / rw; };
   // stage positions of tile + 2-disk halo
   // (at most kMaxLds / kThreads + 1 = 6 rounds; loads of a round are issued together)
   for (int t0 = 0; t0 < lw * lh; t0 += 2 * kThreads) {
@@ -322,24 +387,44 @@ __device__ __forceinline__ void plate_step_body(const KArgs &k)
   long long step = m.step_count;
   if (kDeferred) {
     const MemberDyn pd = sdyn;
-    mode = pd.mode; tab = pd.tab; time = pd.time; step = pd.step;
-    if (!pd.fresh) {
-      const int pf = reinterpret_cast<const int *>(&sprevf)[pd.step % 3];
-      time = pd.time + m.dt;
-      step = pd.step + 1;
-      if (pd.mode == MODE_BIN) { tab = pd.tab ^ 1; mode = MODE_SEARCH; }
-      else mode = (pf >> 16) ? MODE_BIN : MODE_IDLE;
+    const MemberDyn cd =
+        advance_dyn(pd, reinterpret_cast<const int *>(&sprevf)[pd.step % 3], m.dt);
+    mode = cd.mode; tab = cd.tab; time = cd.time; step = cd.step;
+  }
+  // partner velocities of the bonds already broken (they act as contacts and
+  // need them): with kStage (two waves per SIMD, registers to spare) all
+  // eight are requested here, so their latency overlaps the strain stage;
+  // otherwise each broken bond loads its partner's when it is evaluated
+  uint32_t brk = 0;
+  double2 vbrk[8];
+  if (valid && k.op != OP_ENERGY) {
+    brk = present_links(ix, iy, nx, ny) & ~links;
+    if constexpr (kStage) {
+#pragma unroll
+      for (int c = 0; c < 8; ++c)
+        if ((brk >> c) & 1u) vbrk[c] = k.v_in[p + dyc(c) * nx + dxc(c)];
     }
   }
   Tile tile{sx, lw, tx0, ty0, nx, ny, half_span, two_r};
-  const int rw = TX + 2;
-  for (int t = threadIdx.x; t < rw * (TY + 2); t += kThreads) {
-    int hx = tx0 - 1 + t % rw, hy = ty0 - 1 + t / rw;
-    // FP64 in both precisions: the strain stage is a small part of the
-    // step, and exact strains make every shear-break decision the FP64 one
-    double s = 0;
-    if (on_lattice(hx, hy, nx, ny)) s = strain64(tile, hx, hy);
-    ss[t] = s;
+  // FP64 in both precisions: exact strains make every shear-break decision
+  // the FP64 one.  Ring disks t and t + 256 are computed together (a thread
+  // whose second index is past the ring computes its first twice).
+  // two ring disks per pass where registers allow (two waves per SIMD), else one
+  constexpr int kU = kStage ? 2 : 1;
+  for (int t0 = threadIdx.x; t0 < nring; t0 += kU * kThreads) {
+    int hx[kU], hy[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const int t = t0 + u * kThreads;
+      ring_site(t < nring ? t : t0, hx[u], hy[u]);
+    }
+    double sv[kU];
+    strain64x<kU>(tile, hx, hy, sv);
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const int t = t0 + u * kThreads;
+      if (t < nring) ss[t] = on_lattice(hx[u], hy[u], nx, ny) ? sv[u] : 0.0;
+    }
   }
   __syncthreads();
 
@@ -525,11 +610,11 @@ __device__ __forceinline__ void plate_step_body(const KArgs &k)
       ch[E_VSQ] += vp.x * vp.x + vp.y * vp.y;
     } else {
       // ---- unbonded candidates first: the first kCandLds list slots are
-      // gathered kCandBatch at a time (indices, then partner state) and their
-      // signed pair forces are left in this thread's LDS slots for the
-      // ordered sum below, so a rubble disk waits on a few gather rounds
-      // instead of two dependent loads per candidate; slots past kCandLds
-      // (rare) are evaluated inline during the sum
+      // gathered in one round (indices, then partner state) and their signed
+      // pair forces are left in this thread's LDS slots for the ordered sum
+      // below, so a rubble disk waits on one gather round instead of two
+      // dependent loads per candidate; slots past kCandLds (rare) are
+      // evaluated inline during the sum
       uint32_t chas = 0;
       const int ncl = min(nc, kCandLds);
       if constexpr (kStage) {
@@ -664,7 +749,8 @@ __device__ __forceinline__ void plate_step_body(const KArgs &k)
             Fy[c] = mag * nyv;
             has |= 1u << c;
           } else {
-            const double2 vq = k.v_in[q];
+            // broken before this step: prefetched (kStage); else loaded here
+            const double2 vq = (kStage && ((brk >> c) & 1u)) ? vbrk[c] : k.v_in[q];
             double diss;
             if (contact_pair<double>(x_lo, x_hi, self_lo ? vp : vq, self_lo ? vq : vp,
                                      [&] { return make_double2(rx / rest, ry / rest); }, m.radius,
@@ -719,7 +805,7 @@ __device__ __forceinline__ void plate_step_body(const KArgs &k)
             Fy[c] = mag * nyv;
             has |= 1u << c;
           } else {
-            const double2 vq = k.v_in[q];
+            const double2 vq = (kStage && ((brk >> c) & 1u)) ? vbrk[c] : k.v_in[q];
             double diss;
             auto n0 = [&] {
               double rx, ry, rest;
@@ -920,21 +1006,6 @@ __device__ __forceinline__ void plate_step_body(const KArgs &k)
     if (!kDeferred) k.w.bflags[slot] = f;
     else if (f) atomicOr(k.w.mflags + 4 * mi + (int)(step % 3), f);
   }
-  if (kDeferred && blk.w == 0) {
-    // the member's first tile also carries the member forward: this step's
-    // state for the next launch, and the ledger of the previous step (whose
-    // partials and flags are complete now)
-    if (threadIdx.x == 0) {
-      MemberDyn nd;
-      nd.time = time; nd.step = step; nd.mode = mode; nd.tab = tab; nd.fresh = 0; nd.pad = 0;
-      k.w.dyn[2 * mi + k.parity] = nd;
-      k.w.mflags[4 * mi + (int)((step + 1) % 3)] = 0;
-    }
-    const MemberDyn pd = sdyn;
-    if (!pd.fresh)
-      member_ledger(k, mi, pd.time, reinterpret_cast<const int *>(&sprevf)[pd.step % 3],
-                    (int)(pd.step & 1), red);
-  }
 }
 
 // 4 waves per SIMD: both precisions fit 128 VGPRs without spills (the
@@ -953,7 +1024,7 @@ void plate_step_kernel(KArgs k)
 // wave and the 4 wave totals in order, so the order depends only on the
 // member's block count (independent of the batch around it, and the same
 // whether the reduction runs in the finalize launch of a single step, in the
-// next step's first tile or in the flush at the end of a multi-step call).
+// next step's ledger block or in the flush at the end of a multi-step call).
 // Separate launches make every block's partials visible without device-scope
 // fences inside a step (such fences write back the L2 and cost more than a
 // launch).
@@ -1348,7 +1419,7 @@ int tdm_plate_setup(const tdm_plate_arrays *a, void *stream)
   // counters, requests, hash tables, candidate counts
   fill_kernel<int><<<64, 256, 0, s>>>(w.bflags, (size_t)a->nblocks, 0);
   fill_kernel<int><<<1024, 256, 0, s>>>(w.cell_count, 2 * (size_t)a->ncells, 0);
-  (void)hipMemsetAsync(w.ncand, 0, (size_t)a->n, s);
+  fill_kernel<uint8_t><<<1024, 256, 0, s>>>(w.ncand, (size_t)a->n, (uint8_t)0);
   HIP_TRY(hipGetLastError());
   rc = tdm_plate_sync_links(a, stream);
   if (rc) return rc;
@@ -1413,11 +1484,13 @@ static int launch_step(const tdm_plate_arrays *a, const Workspace &w, int parity
   const bool stage = a->max_tile_x > 0 && a->max_tile_x <= kStageMaxTile &&
                      (fp32 || a->nblocks <= 2 * cu_count());
   void (*kern)(KArgs) = fp32 ? pick_step<float>(deferred, stage) : pick_step<double>(deferred, stage);
+  // a multi-step launch carries one ledger block per member after the tiles
+  const int grid = a->nblocks + (deferred ? a->nmembers : 0);
   if (ev0) {
     // events carried by the dispatch itself: they time exactly this kernel
-    hipExtLaunchKernelGGL(kern, dim3(a->nblocks), dim3(kThreads), 0, s, ev0, ev1, 0, k);
+    hipExtLaunchKernelGGL(kern, dim3(grid), dim3(kThreads), 0, s, ev0, ev1, 0, k);
   } else {
-    kern<<<a->nblocks, kThreads, 0, s>>>(k);
+    kern<<<grid, kThreads, 0, s>>>(k);
   }
   HIP_TRY(hipGetLastError());
   if (!deferred) {
@@ -1428,14 +1501,12 @@ static int launch_step(const tdm_plate_arrays *a, const Workspace &w, int parity
 }
 
 // Batches of at most this many members carry the ledger from launch to
-// launch; larger batches keep one finalize launch per step (there, one
-// reduction block per member in its own launch is cheaper than a reduction
-// inside the first tile of every member).
+// launch; larger batches keep one finalize launch per step.
 constexpr int kDeferMaxMembers = 4;
 
 // nsteps steps from `parity`: one launch per step.  For two or more steps
 // of a small batch the per-step finalize launches are replaced by the
-// deferred ledger (each step's first tile reduces the previous step) between
+// deferred ledger (each step's ledger blocks reduce the previous step) between
 // an init and a flush launch.  ev (optional) holds 2 events per step around
 // each step launch.
 static int launch_steps(const tdm_plate_arrays *a, const Workspace &w, int nsteps, int parity,
