@@ -35,7 +35,7 @@ constexpr int kCandLds = 6;     // candidate slots whose pair forces are staged 
 constexpr int kStageMaxTile = 128;
 constexpr int kMaxLdsStage = (kStageMaxTile + 4) * (kThreads / kStageMaxTile + 4);
 constexpr int kMaxRingStage = (kStageMaxTile + 2) * (kThreads / kStageMaxTile + 2);
-constexpr int kCandBatch = kCandLds;  // candidates gathered per round of the force pre-pass
+constexpr int kCandBatch = 2;   // candidates gathered per round of the force pre-pass
 
 // ledger partial channels
 enum {
@@ -122,8 +122,8 @@ __device__ inline uint32_t present_links(int ix, int iy, int nx, int ny)
 // ---------------------------------------------------------------------------
 // Shear strain (dem.py:182-208).  Bonds are visited as index_add_ visits
 // them: partners above the disk by ascending index (codes 4..7), then
-// partners below (codes 0..3).  pinv of the 2x2 covariance C = sum r r^T is
-// the closed-form inverse.
+// partners below (codes 0..3).  pinv of the 2x2 covariance is the closed-form
+// inverse.
 // ---------------------------------------------------------------------------
 struct Tile {
   const double2 *sx;  // LDS positions, row stride lw, origin (tx0-2, ty0-2)
@@ -132,76 +132,47 @@ struct Tile {
   __device__ double2 x(int ix, int iy) const { return sx[(iy - ty0 + kHalo) * lw + (ix - tx0 + kHalo)]; }
 };
 
-// Rest vector of bond code c of disk (ix, iy), oriented as the strain sums
-// see it (from the disk to partners above, from partners below to the disk).
-__device__ inline void rest_vec(int ix, int iy, int c, double half_span, double two_r,
-                                double &rx, double &ry)
+__device__ double strain64(const Tile &t, int ix, int iy)
 {
-  const double2 sp = lattice_site(ix, iy, half_span, two_r);
-  const double2 sq = lattice_site(ix + dxc(c), iy + dyc(c), half_span, two_r);
-  if (c >= 4) { rx = sq.x - sp.x; ry = sq.y - sp.y; }
-  else        { rx = sp.x - sq.x; ry = sp.y - sq.y; }
-}
-
-// Strains of U disks of the tile ring, interleaved for instruction-level
-// parallelism when U = 2 (a thread with one disk passes it twice).  An
-// absent bond contributes (+0) x (+0) to each sum of C and X, which leaves
-// it unchanged (the sums start at +0 and never become -0), so no branch is
-// needed.
-template <int U>
-__device__ void strain64x(const Tile &t, const int (&hx)[U], const int (&hy)[U], double (&s)[U])
-{
-  double x00[U], x01[U], x10[U], x11[U], c00[U], c01[U], c10[U], c11[U];
-  double2 xp[U];
-#pragma unroll
-  for (int u = 0; u < U; ++u) {
-    x00[u] = x01[u] = x10[u] = x11[u] = 0.0;
-    c00[u] = c01[u] = c10[u] = c11[u] = 0.0;
-    xp[u] = t.x(hx[u], hy[u]);
-  }
+  double c00 = 0, c01 = 0, c10 = 0, c11 = 0, x00 = 0, x01 = 0, x10 = 0, x11 = 0;
+  const double2 xp = t.x(ix, iy);
+  const double2 sp = lattice_site(ix, iy, t.half_span, t.two_r);
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
     const int c = (k + 4) & 7;
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int jx = hx[u] + dxc(c), jy = hy[u] + dyc(c);
-      const bool pr = on_lattice(jx, jy, t.nx, t.ny);
-      const double2 xq = t.x(jx, jy);
-      double rx, ry;
-      rest_vec(hx[u], hy[u], c, t.half_span, t.two_r, rx, ry);
-      double qx = c >= 4 ? xq.x - xp[u].x : xp[u].x - xq.x;
-      double qy = c >= 4 ? xq.y - xp[u].y : xp[u].y - xq.y;
-      if (!pr) { rx = 0.0; ry = 0.0; qx = 0.0; qy = 0.0; }
-      c00[u] += rx * rx; c01[u] += rx * ry; c10[u] += ry * rx; c11[u] += ry * ry;
-      x00[u] += qx * rx; x01[u] += qx * ry; x10[u] += qy * rx; x11[u] += qy * ry;
-    }
+    const int jx = ix + dxc(c), jy = iy + dyc(c);
+    if (!on_lattice(jx, jy, t.nx, t.ny)) continue;
+    const double2 xq = t.x(jx, jy);
+    const double2 sq = lattice_site(jx, jy, t.half_span, t.two_r);
+    double rx, ry, qx, qy;
+    if (c >= 4) { rx = sq.x - sp.x; ry = sq.y - sp.y; qx = xq.x - xp.x; qy = xq.y - xp.y; }
+    else        { rx = sp.x - sq.x; ry = sp.y - sq.y; qx = xp.x - xq.x; qy = xp.y - xq.y; }
+    c00 += rx * rx; c01 += rx * ry; c10 += ry * rx; c11 += ry * ry;
+    x00 += qx * rx; x01 += qx * ry; x10 += qy * rx; x11 += qy * ry;
   }
-#pragma unroll
-  for (int u = 0; u < U; ++u) {
-    double i00, i01, i10, i11;
-    const double det = c00[u] * c11[u] - c01[u] * c10[u];
-    if (det != 0.0) {
-      i00 = c11[u] / det; i01 = -c01[u] / det; i10 = -c10[u] / det; i11 = c00[u] / det;
+  double det = c00 * c11 - c01 * c10;
+  double i00, i01, i10, i11;
+  if (det != 0.0) {
+    i00 = c11 / det; i01 = -c01 / det; i10 = -c10 / det; i11 = c00 / det;
+  } else {
+    double tr = c00 + c11;
+    if (tr > 0.0) {
+      double t2 = tr * tr;
+      i00 = c00 / t2; i01 = c01 / t2; i10 = c10 / t2; i11 = c11 / t2;
     } else {
-      const double tr = c00[u] + c11[u];
-      if (tr > 0.0) {
-        const double t2 = tr * tr;
-        i00 = c00[u] / t2; i01 = c01[u] / t2; i10 = c10[u] / t2; i11 = c11[u] / t2;
-      } else {
-        i00 = i01 = i10 = i11 = 0.0;
-      }
+      i00 = i01 = i10 = i11 = 0.0;
     }
-    double f00 = x00[u] * i00 + x01[u] * i10;
-    double f01 = x00[u] * i01 + x01[u] * i11;
-    double f10 = x10[u] * i00 + x11[u] * i10;
-    double f11 = x10[u] * i01 + x11[u] * i11;
-    double g00 = f00 * f00 + f10 * f10;
-    double g01 = f00 * f01 + f10 * f11;
-    double g11 = f01 * f01 + f11 * f11;
-    double exx = 0.5 * (g00 - 1.0), exy = 0.5 * (g01 - 0.0), eyy = 0.5 * (g11 - 1.0);
-    double dxy = exx - eyy;
-    s[u] = sqrt(dxy * dxy + 4.0 * (exy * exy));
   }
+  double f00 = x00 * i00 + x01 * i10;
+  double f01 = x00 * i01 + x01 * i11;
+  double f10 = x10 * i00 + x11 * i10;
+  double f11 = x10 * i01 + x11 * i11;
+  double g00 = f00 * f00 + f10 * f10;
+  double g01 = f00 * f01 + f10 * f11;
+  double g11 = f01 * f01 + f11 * f11;
+  double exx = 0.5 * (g00 - 1.0), exy = 0.5 * (g01 - 0.0), eyy = 0.5 * (g11 - 1.0);
+  double dxy = exx - eyy;
+  return sqrt(dxy * dxy + 4.0 * (exy * exy));
 }
 
 struct KArgs {
@@ -391,40 +362,15 @@ __device__ __forceinline__ void plate_step_body(const KArgs &k)
         advance_dyn(pd, reinterpret_cast<const int *>(&sprevf)[pd.step % 3], m.dt);
     mode = cd.mode; tab = cd.tab; time = cd.time; step = cd.step;
   }
-  // partner velocities of the bonds already broken (they act as contacts and
-  // need them): with kStage (two waves per SIMD, registers to spare) all
-  // eight are requested here, so their latency overlaps the strain stage;
-  // otherwise each broken bond loads its partner's when it is evaluated
-  uint32_t brk = 0;
-  double2 vbrk[8];
-  if (valid && k.op != OP_ENERGY) {
-    brk = present_links(ix, iy, nx, ny) & ~links;
-    if constexpr (kStage) {
-#pragma unroll
-      for (int c = 0; c < 8; ++c)
-        if ((brk >> c) & 1u) vbrk[c] = k.v_in[p + dyc(c) * nx + dxc(c)];
-    }
-  }
   Tile tile{sx, lw, tx0, ty0, nx, ny, half_span, two_r};
-  // FP64 in both precisions: exact strains make every shear-break decision
-  // the FP64 one.  Ring disks t and t + 256 are computed together (a thread
-  // whose second index is past the ring computes its first twice).
-  // two ring disks per pass where registers allow (two waves per SIMD), else one
-  constexpr int kU = kStage ? 2 : 1;
-  for (int t0 = threadIdx.x; t0 < nring; t0 += kU * kThreads) {
-    int hx[kU], hy[kU];
-#pragma unroll
-    for (int u = 0; u < kU; ++u) {
-      const int t = t0 + u * kThreads;
-      ring_site(t < nring ? t : t0, hx[u], hy[u]);
-    }
-    double sv[kU];
-    strain64x<kU>(tile, hx, hy, sv);
-#pragma unroll
-    for (int u = 0; u < kU; ++u) {
-      const int t = t0 + u * kThreads;
-      if (t < nring) ss[t] = on_lattice(hx[u], hy[u], nx, ny) ? sv[u] : 0.0;
-    }
+  for (int t = threadIdx.x; t < nring; t += kThreads) {
+    int hx, hy;
+    ring_site(t, hx, hy);
+    // FP64 in both precisions: the strain stage is a small part of the
+    // step, and exact strains make every shear-break decision the FP64 one
+    double s = 0;
+    if (on_lattice(hx, hy, nx, ny)) s = strain64(tile, hx, hy);
+    ss[t] = s;
   }
   __syncthreads();
 
@@ -610,11 +556,11 @@ __device__ __forceinline__ void plate_step_body(const KArgs &k)
       ch[E_VSQ] += vp.x * vp.x + vp.y * vp.y;
     } else {
       // ---- unbonded candidates first: the first kCandLds list slots are
-      // gathered in one round (indices, then partner state) and their signed
-      // pair forces are left in this thread's LDS slots for the ordered sum
-      // below, so a rubble disk waits on one gather round instead of two
-      // dependent loads per candidate; slots past kCandLds (rare) are
-      // evaluated inline during the sum
+      // gathered kCandBatch at a time (indices, then partner state) and their
+      // signed pair forces are left in this thread's LDS slots for the
+      // ordered sum below, so a rubble disk waits on a few gather rounds
+      // instead of two dependent loads per candidate; slots past kCandLds
+      // (rare) are evaluated inline during the sum
       uint32_t chas = 0;
       const int ncl = min(nc, kCandLds);
       if constexpr (kStage) {
@@ -749,8 +695,7 @@ __device__ __forceinline__ void plate_step_body(const KArgs &k)
             Fy[c] = mag * nyv;
             has |= 1u << c;
           } else {
-            // broken before this step: prefetched (kStage); else loaded here
-            const double2 vq = (kStage && ((brk >> c) & 1u)) ? vbrk[c] : k.v_in[q];
+            const double2 vq = k.v_in[q];
             double diss;
             if (contact_pair<double>(x_lo, x_hi, self_lo ? vp : vq, self_lo ? vq : vp,
                                      [&] { return make_double2(rx / rest, ry / rest); }, m.radius,
@@ -805,7 +750,7 @@ __device__ __forceinline__ void plate_step_body(const KArgs &k)
             Fy[c] = mag * nyv;
             has |= 1u << c;
           } else {
-            const double2 vq = (kStage && ((brk >> c) & 1u)) ? vbrk[c] : k.v_in[q];
+            const double2 vq = k.v_in[q];
             double diss;
             auto n0 = [&] {
               double rx, ry, rest;
@@ -1500,20 +1445,16 @@ static int launch_step(const tdm_plate_arrays *a, const Workspace &w, int parity
   return TDM_OK;
 }
 
-// Batches of at most this many members carry the ledger from launch to
-// launch; larger batches keep one finalize launch per step.
-constexpr int kDeferMaxMembers = 4;
 
 // nsteps steps from `parity`: one launch per step.  For two or more steps
-// of a small batch the per-step finalize launches are replaced by the
-// deferred ledger (each step's ledger blocks reduce the previous step) between
-// an init and a flush launch.  ev (optional) holds 2 events per step around
-// each step launch.
+// the per-step finalize launches are replaced by the deferred ledger (each
+// step's ledger blocks reduce the previous step) between an init and a flush
+// launch.  ev (optional) holds 2 events per step around each step launch.
 static int launch_steps(const tdm_plate_arrays *a, const Workspace &w, int nsteps, int parity,
                         int fp32, hipStream_t s, hipEvent_t *ev = nullptr)
 {
   if (nsteps <= 0) return TDM_OK;
-  if (nsteps == 1 || a->nmembers > kDeferMaxMembers) {
+  if (nsteps == 1) {
     for (int i = 0; i < nsteps; ++i) {
       int rc = launch_step(a, w, parity, fp32, OP_STEP, s, ev ? ev[2 * i] : nullptr,
                            ev ? ev[2 * i + 1] : nullptr);
