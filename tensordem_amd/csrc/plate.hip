@@ -678,50 +678,60 @@ __device__ __forceinline__ void plate_step_body(const KArgs &k)
           ch[C_ACQUIRED] += (0.5 * kc) * (ov * ov);
           ch[C_BREAKS] += 1.0;
         };
+        // a bond broken before this step is a contact only: its geometry and
+        // break test are skipped (rubble tiles run whole waves of those)
+        auto broken_contact = [&]() {
+          const double2 vq = k.v_in[q];
+          double diss;
+          auto n0 = [&] {
+            double rx, ry, rest;
+            site_vec(rx, ry, rest);
+            return make_double2(rx / rest, ry / rest);
+          };
+          if (contact_pair<R>(x_lo, x_hi, self_lo ? vp : vq, self_lo ? vq : vp, n0, m.radius, kc,
+                              cdamp, Fx[c], Fy[c], diss)) {
+            has |= 1u << c;
+            if (self_lo) ch[C_PAIR_DISS] += diss;
+          }
+        };
         if constexpr (Prec<R>::fp64) {
-          double rx, ry, rest, d, nxv, nyv;
-          site_vec(rx, ry, rest);
-          geom64(rx, ry, rest, d, nxv, nyv);
-          double ext = d - rest;
           if (alive) {
+            double rx, ry, rest, d, nxv, nyv;
+            site_vec(rx, ry, rest);
+            geom64(rx, ry, rest, d, nxv, nyv);
+            double ext = d - rest;
             double tstrain = ext / rest;
             double bshear = 0.5 * (sh_lo + sh_hi);
             int sf = bshear > eps_s, tf = tstrain > eps_t;
-            if (sf || tf) { alive = false; record(tf, sf, d, rest); }
-          }
-          if (alive) {
-            double mag = kb * ext;
-            Fx[c] = mag * nxv;
-            Fy[c] = mag * nyv;
-            has |= 1u << c;
-          } else {
-            const double2 vq = k.v_in[q];
-            double diss;
-            if (contact_pair<double>(x_lo, x_hi, self_lo ? vp : vq, self_lo ? vq : vp,
-                                     [&] { return make_double2(rx / rest, ry / rest); }, m.radius,
-                                     kc, cdamp, Fx[c], Fy[c], diss)) {
+            if (sf || tf) {
+              alive = false;
+              record(tf, sf, d, rest);
+            } else {
+              double mag = kb * ext;
+              Fx[c] = mag * nxv;
+              Fy[c] = mag * nyv;
               has |= 1u << c;
-              if (self_lo) ch[C_PAIR_DISS] += diss;
             }
           }
+          if (!alive) broken_contact();
         } else {
           // FP32: displacement relative to the nominal lattice vector keeps
           // the extension free of cancellation; exact FP64 site geometry is
           // only formed for break decisions inside the margin and records.
-          const double rnx = (self_lo ? dxc(c) : -dxc(c)) * two_r;
-          const double rny = (self_lo ? dyc(c) : -dyc(c)) * two_r;
-          const float rfx = (float)rnx, rfy = (float)rny;
-          const float restf = (dxc(c) != 0 && dyc(c) != 0) ? (float)(two_r * 1.4142135623730951)
-                                                             : (float)two_r;
-          float ux = (float)((x_hi.x - x_lo.x) - rnx), uy = (float)((x_hi.y - x_lo.y) - rny);
-          float dvx = rfx + ux, dvy = rfy + uy;
-          float d = sqrt32(dvx * dvx + dvy * dvy);
-          float ext = (2.0f * (rfx * ux + rfy * uy) + (ux * ux + uy * uy)) * rcp32(d + restf);
-          const float inv_rest = rcp32(restf);
-          float nxv, nyv;
-          if (d > 0) { float r = rcp32(fmaxf(d, kEps32)); nxv = dvx * r; nyv = dvy * r; }
-          else { nxv = rfx * inv_rest; nyv = rfy * inv_rest; }
           if (alive) {
+            const double rnx = (self_lo ? dxc(c) : -dxc(c)) * two_r;
+            const double rny = (self_lo ? dyc(c) : -dyc(c)) * two_r;
+            const float rfx = (float)rnx, rfy = (float)rny;
+            const float restf = (dxc(c) != 0 && dyc(c) != 0) ? (float)(two_r * 1.4142135623730951)
+                                                               : (float)two_r;
+            float ux = (float)((x_hi.x - x_lo.x) - rnx), uy = (float)((x_hi.y - x_lo.y) - rny);
+            float dvx = rfx + ux, dvy = rfy + uy;
+            float d = sqrt32(dvx * dvx + dvy * dvy);
+            float ext = (2.0f * (rfx * ux + rfy * uy) + (ux * ux + uy * uy)) * rcp32(d + restf);
+            const float inv_rest = rcp32(restf);
+            float nxv, nyv;
+            if (d > 0) { float r = rcp32(fmaxf(d, kEps32)); nxv = dvx * r; nyv = dvy * r; }
+            else { nxv = rfx * inv_rest; nyv = rfy * inv_rest; }
             // the shear test is the FP64 one (strains are FP64); the tensile
             // test is redone in FP64 inside the margin of its threshold
             const float tstrain = ext * inv_rest;
@@ -742,27 +752,14 @@ __device__ __forceinline__ void plate_step_body(const KArgs &k)
               site_vec(rx, ry, rest);
               geom64(rx, ry, rest, d64, n1, n2);
               record(tf, sf, d64, rest);
-            }
-          }
-          if (alive) {
-            float mag = (float)kb * ext;
-            Fx[c] = mag * nxv;
-            Fy[c] = mag * nyv;
-            has |= 1u << c;
-          } else {
-            const double2 vq = k.v_in[q];
-            double diss;
-            auto n0 = [&] {
-              double rx, ry, rest;
-              site_vec(rx, ry, rest);
-              return make_double2(rx / rest, ry / rest);
-            };
-            if (contact_pair<float>(x_lo, x_hi, self_lo ? vp : vq, self_lo ? vq : vp, n0,
-                                    m.radius, kc, cdamp, Fx[c], Fy[c], diss)) {
+            } else {
+              float mag = (float)kb * ext;
+              Fx[c] = mag * nxv;
+              Fy[c] = mag * nyv;
               has |= 1u << c;
-              if (self_lo) ch[C_PAIR_DISS] += diss;
             }
           }
+          if (!alive) broken_contact();
         }
       };
 
