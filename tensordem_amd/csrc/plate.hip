@@ -19,6 +19,8 @@
 //     a disk has moved a quarter of the Verlet skin.
 #include "common.h"
 
+#include <vector>
+
 namespace {
 
 using namespace tdm;
@@ -1324,10 +1326,13 @@ int tdm_plate_setup(const tdm_plate_arrays *a, void *stream)
     return fail(TDM_EINVAL, "max_tile_x is narrower than a member's tile");
   }
   // Logical tile order keeps a member's tiles contiguous (row-major tiles) so
-  // its ledger partials sum in a fixed order.  The physical launch order is
-  // permuted so that runs of consecutive tiles share one XCD (blocks are
-  // dealt round-robin over the 8 XCDs): launch slot b runs logical tile
-  // (b % 8) * ceil(nb/8) + b / 8 when that tile exists.
+  // its ledger partials sum in a fixed order; the kernels index partials and
+  // flags by logical tile, so the physical launch order is free.  Blocks are
+  // dealt round-robin over the 8 XCDs (launch slot s runs on XCD s % 8), and
+  // a tile's 2-disk halo is read from its neighbours' rows, so neighbouring
+  // tiles should share one XCD's L2: a member of at most kXcdWhole tiles runs
+  // entirely on one XCD (the one with the fewest tiles so far), a larger
+  // member is cut into 8 contiguous runs of tiles, one per XCD.
   int4 *hb = new int4[2 * total];
   int4 *logical = new int4[total];
   int li = 0;
@@ -1338,19 +1343,33 @@ int tdm_plate_setup(const tdm_plate_arrays *a, void *stream)
       for (int bx = 0; bx < ntx; ++bx)
         logical[li++] = make_int4(i, bx * tx, by * ty, by * ntx + bx);
   }
-  // Within a member, launch slots are grouped by the XCD they land on
-  // ((global slot) % 8); each XCD's slots take a contiguous run of logical
-  // tiles, so neighbouring tiles share that XCD's L2.  This is a bijection
-  // between the member's slots and its tiles.
+  constexpr int kXcd = 8, kXcdWhole = 32;
+  std::vector<int> queue[kXcd];
   for (int i = 0; i < nm; ++i) {
-    int b0 = hm[i].block_first, nb = hm[i].nblocks;
-    int next = 0;
-    for (int lane = 0; lane < 8; ++lane)
-      for (int b = 0; b < nb; ++b)
-        if ((b0 + b) % 8 == lane) {
-          hb[2 * (b0 + b)] = logical[b0 + next++];
-          hb[2 * (b0 + b) + 1] = make_int4(hm[i].nx, hm[i].ny, hm[i].first, hm[i].tile_x);
-        }
+    const int b0 = hm[i].block_first, nb = hm[i].nblocks;
+    if (nb <= kXcdWhole) {
+      int x = 0;
+      for (int c = 1; c < kXcd; ++c)
+        if (queue[c].size() < queue[x].size()) x = c;
+      for (int b = 0; b < nb; ++b) queue[x].push_back(b0 + b);
+    } else {
+      for (int r = 0; r < kXcd; ++r)
+        for (int b = r * nb / kXcd; b < (r + 1) * nb / kXcd; ++b) queue[r].push_back(b0 + b);
+    }
+  }
+  size_t head[kXcd] = {};
+  for (int slot = 0; slot < total; ++slot) {
+    int x = slot % kXcd;
+    if (head[x] == queue[x].size()) {
+      // this XCD's share is spent (shares differ by at most one member):
+      // take the next tile of the XCD with the most left
+      for (int c = 0; c < kXcd; ++c)
+        if (queue[c].size() - head[c] > queue[x].size() - head[x]) x = c;
+    }
+    const int item = queue[x][head[x]++];
+    const int i = logical[item].x;
+    hb[2 * slot] = logical[item];
+    hb[2 * slot + 1] = make_int4(hm[i].nx, hm[i].ny, hm[i].first, hm[i].tile_x);
   }
   delete[] logical;
   e = hipMemcpyAsync(w.blocks, hb, 2 * sizeof(int4) * total, hipMemcpyHostToDevice, s);
