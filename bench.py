@@ -1081,6 +1081,8 @@ def bench_ensemble(args):
         del eng
     eng = PlateEngine(tuned, fp32=True, **tune.engine_kwargs())
     kernel_s = eng.step_timed(steps) * 1e-6
+    # tiles at most 128 wide take the kernel that stages candidate forces in LDS
+    staged = 0 < eng.max_tile_x <= 128
     eng.close()
     del eng
     r32, r64 = out["fp32"], out["fp64"]
@@ -1104,7 +1106,8 @@ def bench_ensemble(args):
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic_from_profile("plate_step_c4_counters.md"),
-                     "kernel": "plate_step_kernel<float, false, false>", "kernel_us": kernel_s * 1e6,
+                     "kernel": f"plate_step_kernel<float, true, {str(staged).lower()}>",
+                     "kernel_us": kernel_s * 1e6,
                      "bytes_per_disk": bytes_per_disk, "bytes_per_launch": bytes_per_disk * n},
     }
     if not args.no_plain:
