@@ -26,6 +26,8 @@
 // than one per partner.
 #include "common.h"
 
+#include <vector>
+
 namespace {
 
 using namespace tdm;
@@ -742,14 +744,11 @@ __global__ __launch_bounds__(kThreads) void bed_step_kernel(BKArgs k)
     for (int c = 0; c < kStepCh; ++c)
       k.w.partials[((size_t)pslot * kNch + c) * a.nblocks + slot] = ch[c];
     const int f = sflags[0] | (sflags[1] << 16);
-    if (!kDeferred) {
-      k.w.bflags[slot] = f;
-    } else {
-      if (f) atomicOr(k.w.mflags + 4 * mi + (int)(cur.step % 3), f);
-      // this block alone reads its table entry: the next launch uses the
-      // lists this search wrote
-      if (mode == MODE_SEARCH) k.w.blocks[blockIdx.x].z = blk.z ^ kLpBit;
-    }
+    if (!kDeferred) k.w.bflags[slot] = f;
+    else if (f) atomicOr(k.w.mflags + 4 * mi + (int)(cur.step % 3), f);
+    // this block alone reads its table entry: the next launch uses the
+    // lists this search wrote
+    if (mode == MODE_SEARCH) k.w.blocks[blockIdx.x].z = blk.z ^ kLpBit;
   }
   if (kDeferred && blk.w == 0) {
     // the member's first block carries the member forward: this step's
@@ -982,14 +981,7 @@ __global__ __launch_bounds__(kThreads) void bed_finalize_kernel(BKArgs k)
     return;
   }
   const BedDyn d = bed_record_state(mp);
-  if (d.mode == MODE_SEARCH) {
-    // the step just rebuilt the lists into buffer lp^1: mirror the new
-    // parity into the member's block-table entries
-    for (int b = threadIdx.x; b < nb; b += kThreads) {
-      const int z = k.w.blocks[b0 + b].z;
-      k.w.blocks[b0 + b].z = (d.lp ^ 1) ? (z | kLpBit) : (z & ~kLpBit);
-    }
-  }
+  // (a search step's blocks flipped their own table entries' list parity)
   bed_member_ledger(k, mi, 0, flags, wsum);
   if (threadIdx.x == 0) bed_write_record(mp, bed_advance(d, flags, mp->dt));
 }
@@ -1213,22 +1205,46 @@ int tdm_bed_setup(const tdm_bed_arrays *a, void *stream)
     total += hm[i].nblocks;
   }
   if (total != a->nblocks) { delete[] hm; return fail(TDM_EINVAL, "bed nblocks mismatch"); }
-  int4 *hb = new int4[total];
-  int *order = new int[total];
+  // Block table: launch slot -> (member, first disk, count | list parity,
+  // block index within the member).  Partials are indexed by the member's
+  // own block index, so the launch order is free: blocks are dealt
+  // round-robin over the 8 XCDs (slot s runs on XCD s % 8), and a disk
+  // gathers its partners from the member's other blocks, so a member of at
+  // most kXcdWhole blocks runs entirely on one XCD (the one with the fewest
+  // blocks so far) and a larger one in 8 contiguous runs, one per XCD.
+  constexpr int kXcd = 8, kXcdWhole = 32;
+  std::vector<int2> queue[kXcd];  // (member, block index)
   for (int i = 0; i < nm; ++i) {
-    const int b0 = hm[i].block_first, nb = hm[i].nblocks;
-    xcd_order(b0, nb, order);
-    for (int b = 0; b < nb; ++b) {
-      const int item = order[b];
-      const int first = hm[i].first + item * kThreads;
-      const int count = min(kThreads, hm[i].first + hm[i].n - first);
-      hb[b0 + b] = make_int4(i, first, count | (hm[i].list_parity ? kLpBit : 0), item);
+    const int nb = hm[i].nblocks;
+    if (nb <= kXcdWhole) {
+      int x = 0;
+      for (int c = 1; c < kXcd; ++c)
+        if (queue[c].size() < queue[x].size()) x = c;
+      for (int b = 0; b < nb; ++b) queue[x].push_back(make_int2(i, b));
+    } else {
+      for (int r = 0; r < kXcd; ++r)
+        for (int b = r * nb / kXcd; b < (r + 1) * nb / kXcd; ++b)
+          queue[r].push_back(make_int2(i, b));
     }
+  }
+  int4 *hb = new int4[total];
+  size_t head[kXcd] = {};
+  for (int slot = 0; slot < total; ++slot) {
+    int x = slot % kXcd;
+    if (head[x] == queue[x].size()) {
+      // this XCD's share is spent: take the next block of the XCD with the most left
+      for (int c = 0; c < kXcd; ++c)
+        if (queue[c].size() - head[c] > queue[x].size() - head[x]) x = c;
+    }
+    const int2 it = queue[x][head[x]++];
+    const int i = it.x, item = it.y;
+    const int first = hm[i].first + item * kThreads;
+    const int count = min(kThreads, hm[i].first + hm[i].n - first);
+    hb[slot] = make_int4(i, first, count | (hm[i].list_parity ? kLpBit : 0), item);
   }
   e = hipMemcpyAsync(w.blocks, hb, sizeof(int4) * total, hipMemcpyHostToDevice, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   delete[] hb;
-  delete[] order;
   delete[] hm;
   if (e != hipSuccess) return fail(TDM_EHIP, hipGetErrorString(e));
   fill_kernel<int><<<64, 256, 0, s>>>(w.bflags, (size_t)a->nblocks, 0);

@@ -99,17 +99,6 @@ int fail(int code, const std::string &msg);
       return tdm::fail(TDM_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_));   \
   } while (0)
 
-// XCD-aware launch order: within a group of nb consecutive launch slots
-// starting at b0, slots landing on the same XCD ((global slot) % 8) take a
-// contiguous run of logical work items.  slot_to_item[b] = logical index.
-inline void xcd_order(int b0, int nb, int *slot_to_item)
-{
-  int next = 0;
-  for (int lane = 0; lane < 8; ++lane)
-    for (int b = 0; b < nb; ++b)
-      if ((b0 + b) % 8 == lane) slot_to_item[b] = next++;
-}
-
 }  // namespace tdm
 
 // captured step sequence (tdm_graph in the C ABI)
