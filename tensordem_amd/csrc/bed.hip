@@ -420,17 +420,23 @@ __device__ inline int rebuild_list(const BKArgs &k, const BedGrid &m, double ski
   cell_of(m.ox, m.oy, m.h, m.ncx, m.ncy, m.first, xp, &cx, &cy);
   const int *cntab = k.w.cell_count + (size_t)tb * a.ncells;
   const int *tab = k.w.cell_tab + (size_t)tb * a.ncells * a.cell_cap;
-  // entries of the 3x3 block, flattened: cells o = 0..8 hold pre[o]..pre[o+1]-1
-  int pre[10];
+  // entries of the 3x3 block, flattened: cells o = 0..8 hold pre[o]..pre[o+1]-1,
+  // and entry t of cell o sits at table offset base[o] + t
+  int pre[10], base[9];
   pre[0] = 0;
 #pragma unroll
   for (int o = 0; o < 9; ++o) {
     const int gx = cx + o % 3 - 1, gy = cy + o / 3 - 1;
     const bool in = gx >= 0 && gx < m.ncx && gy >= 0 && gy < m.ncy;
-    pre[o + 1] = in ? min(cntab[m.first + gy * m.ncx + gx], a.cell_cap) : 0;
+    const int cell = in ? m.first + gy * m.ncx + gx : 0;
+    pre[o + 1] = in ? min(cntab[cell], a.cell_cap) : 0;
+    base[o] = cell * a.cell_cap;
   }
 #pragma unroll
-  for (int o = 0; o < 9; ++o) pre[o + 1] += pre[o];
+  for (int o = 0; o < 9; ++o) {
+    base[o] -= pre[o];
+    pre[o + 1] += pre[o];
+  }
   const int total = pre[9];
   int nl[kMaxSlots];
 #pragma unroll
@@ -443,15 +449,12 @@ __device__ inline int rebuild_list(const BKArgs &k, const BedGrid &m, double ski
       const int t = t0 + u;
       qs[u] = p;
       if (t < total) {
-        int o = 0;
+        // the cell holding entry t is the last one starting at or before t
+        // (an empty cell starts where the next one does, which overrides it)
+        int off = base[0];
 #pragma unroll
-        for (int c = 1; c < 9; ++c) o += t >= pre[c];
-        int oo = 0, pr = 0;  // o and pre[o] without dynamic register indexing
-#pragma unroll
-        for (int c = 0; c < 9; ++c)
-          if (c == o) { oo = c; pr = pre[c]; }
-        const int cell = m.first + (cy + oo / 3 - 1) * m.ncx + (cx + oo % 3 - 1);
-        qs[u] = tab[(size_t)cell * a.cell_cap + (t - pr)];
+        for (int c = 1; c < 9; ++c) off = t >= pre[c] ? base[c] : off;
+        qs[u] = tab[(size_t)(off + t)];
       }
     }
     double2 xq[kScan];
@@ -1046,6 +1049,8 @@ int check_bed(const tdm_bed_arrays *a, BedWs *w)
     return fail(TDM_EINVAL, "empty bed batch");
   if (a->cand_slots <= 0 || a->cand_slots > kMaxSlots || a->cell_cap <= 0)
     return fail(TDM_EINVAL, "cand_slots must be in [1,16] and cell_cap > 0");
+  if ((long long)a->ncells * a->cell_cap >= (1LL << 31))
+    return fail(TDM_EINVAL, "bed cell table too large (ncells x cell_cap must stay below 2^31)");
   size_t need = 0;
   carve(a->workspace, a->n, a->nblocks, a->nmembers, a->ncells, a->cand_slots, a->cell_cap, w,
         &need);
