@@ -413,17 +413,24 @@ __device__ __forceinline__ void plate_step_body(const KArgs &k)
       const int *ctab = k.w.cell_tab + (size_t)tab * a.ncells * a.cell_cap;
       // the nine cell counts are fetched together; the 3x3 block's entries
       // are then visited kScan at a time (one table load and one position
-      // gather per batch) and inserted into a sorted list held in registers
-      int pre[10];
+      // gather per batch) and inserted into a sorted list held in registers;
+      // cells o = 0..8 hold entries pre[o]..pre[o+1]-1, entry t of cell o at
+      // table offset base[o] + t
+      int pre[10], base[9];
       pre[0] = 0;
 #pragma unroll
       for (int o = 0; o < 9; ++o) {
         const int gx = cx + o % 3 - 1, gy = cy + o / 3 - 1;
         const bool in = gx >= 0 && gx < m.ncx && gy >= 0 && gy < m.ncy;
-        pre[o + 1] = in ? min(cnt[m.cell_first + gy * m.ncx + gx], a.cell_cap) : 0;
+        const int cell = in ? m.cell_first + gy * m.ncx + gx : 0;
+        pre[o + 1] = in ? min(cnt[cell], a.cell_cap) : 0;
+        base[o] = cell * a.cell_cap;
       }
 #pragma unroll
-      for (int o = 0; o < 9; ++o) pre[o + 1] += pre[o];
+      for (int o = 0; o < 9; ++o) {
+        base[o] -= pre[o];
+        pre[o + 1] += pre[o];
+      }
       const int total = pre[9];
       int nl[kMaxCand];
 #pragma unroll
@@ -435,15 +442,12 @@ __device__ __forceinline__ void plate_step_body(const KArgs &k)
           const int t = t0 + u;
           qs[u] = p;
           if (t < total) {
-            int o = 0;
+            // the cell holding entry t is the last one starting at or before
+            // t (an empty cell starts where the next one does)
+            int off = base[0];
 #pragma unroll
-            for (int c = 1; c < 9; ++c) o += t >= pre[c];
-            int oo = 0, pr = 0;  // o and pre[o] without dynamic register indexing
-#pragma unroll
-            for (int c = 0; c < 9; ++c)
-              if (c == o) { oo = c; pr = pre[c]; }
-            const int cell = m.cell_first + (cy + oo / 3 - 1) * m.ncx + (cx + oo % 3 - 1);
-            qs[u] = ctab[(size_t)cell * a.cell_cap + (t - pr)];
+            for (int c = 1; c < 9; ++c) off = t >= pre[c] ? base[c] : off;
+            qs[u] = ctab[(size_t)(off + t)];
           }
         }
         double2 xq[kScan];
@@ -1222,6 +1226,8 @@ int check_arrays(const tdm_plate_arrays *a, Workspace *w)
     return fail(TDM_EINVAL, "empty batch");
   if (a->cand_slots < 0 || a->cand_slots > kMaxCand || a->cell_cap <= 0)
     return fail(TDM_EINVAL, "cand_slots must be in [0,16] and cell_cap > 0");
+  if ((long long)a->ncells * a->cell_cap >= (1LL << 31))
+    return fail(TDM_EINVAL, "cell table too large (ncells x cell_cap must stay below 2^31)");
   size_t need = 0;
   carve(a->workspace, a->n, a->nblocks, a->nmembers, a->ncells, a->cand_slots, a->cell_cap, w,
         &need);
