@@ -133,6 +133,7 @@ struct BKArgs {
   double2 *x_out, *v_out;
   int op;
   int parity;  // 0: x, v -> x_alt, v_alt; 1: the reverse
+  int first;   // first launch of a multi-step call: the member record holds the state
 };
 
 // The member constants one step reads, moved to scalar registers (every lane
@@ -149,6 +150,12 @@ __device__ inline double uniform(double v)
                           __builtin_amdgcn_readfirstlane(__double2loint(v)));
 }
 __device__ inline int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ inline long long uniform(long long v)
+{
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((int)(v >> 32));
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((int)v);
+  return (long long)(((uint64_t)hi << 32) | lo);
+}
 
 __device__ inline BedConst load_const(const tdm_bed_member *mp)
 {
@@ -514,16 +521,17 @@ __device__ inline int rebuild_list(const BKArgs &k, const BedGrid &m, double ski
   return nn;
 }
 
-template <int U>
+template <int U, int G>
 __device__ void bed_member_ledger(const BKArgs &k, int mi, long long step, int flags,
                                   double (*wsum)[kCh]);
 __device__ inline BedDyn bed_advance(const BedDyn &d, int flags, double dt);
 
-// kDeferred: a launch inside a multi-step call.  The member's clock and
-// search phase come from the previous launch's state and flags instead of
-// the member record, each block flips its own block-table list parity after
-// a rebuild, and the member's first block reduces the previous step's
-// ledger after its own work (one launch per step, no finalize launch).
+// kDeferred: a launch inside a multi-step call.  After the call's first
+// launch the member's clock and search phase come from the previous
+// launch's state and flags instead of the member record, each block flips
+// its own block-table list parity after a rebuild, and the member's first
+// block reduces the previous step's ledger after its own work (one launch
+// per step, no finalize launch).
 template <typename R, bool kDeferred>
 __global__ __launch_bounds__(kThreads) void bed_step_kernel(BKArgs k)
 {
@@ -552,7 +560,18 @@ __global__ __launch_bounds__(kThreads) void bed_step_kernel(BKArgs k)
   double time = uniform(mp->time);
   BedDyn pd, cur;
   int pflags = 0;
-  if (kDeferred) {
+  if (kDeferred && k.first) {
+    // the call's first step starts from the member record
+    pd.time = time;
+    pd.step = uniform((long long)mp->step_count);
+    pd.mode = mode;
+    pd.tab = tab;
+    pd.lp = uniform(mp->list_parity);
+    pd.rebuilds = uniform(mp->rebuilds);
+    pd.fresh = 1;
+    pd.pad = 0;
+    cur = pd;
+  } else if (kDeferred) {
     // the state at the start of the previous step and the flags that step
     // raised (written by the previous launch); uniform, scalar loads
     pd = k.w.dyn[2 * mi + (k.parity ^ 1)];
@@ -763,7 +782,7 @@ __global__ __launch_bounds__(kThreads) void bed_step_kernel(BKArgs k)
       k.w.dyn[2 * mi + k.parity] = nd;
       k.w.mflags[4 * mi + (int)((cur.step + 1) % 3)] = 0;
     }
-    if (!pd.fresh) bed_member_ledger<1>(k, mi, pd.step, pflags, red);
+    if (!pd.fresh) bed_member_ledger<1, 1>(k, mi, pd.step, pflags, red);
   }
 }
 
@@ -850,34 +869,42 @@ __global__ __launch_bounds__(kThreads) void bed_energy_kernel(BKArgs k)
 // of the batch around the member.
 // ---------------------------------------------------------------------------
 // U: blocks whose loads are issued together (the order of the sum does not
-// depend on it; the step kernel uses 1 to keep its register budget)
-template <int NC, int U = 4>
+// depend on it; the step kernel uses 1 to keep its register budget).
+// G: 256-thread groups of the block (blockDim = 256 G); group g takes the
+// channels [g NC/G, (g+1) NC/G) with the same thread-to-block assignment as a
+// single group, so the sums are the same for every G.
+template <int NC, int U = 4, int G = 1>
 __device__ void bed_reduce(const BKArgs &k, int b0, int nb, int pslot, int c_lo,
                            double (*wsum)[kCh], double *stot)
 {
-  double acc[NC];
+  static_assert(NC % G == 0, "channels split evenly over the groups");
+  constexpr int NG = NC / G;
+  const int g = G == 1 ? 0 : (int)threadIdx.x / kThreads;
+  const int t = G == 1 ? (int)threadIdx.x : (int)threadIdx.x % kThreads;
+  double acc[NG];
 #pragma unroll
-  for (int c = 0; c < NC; ++c) acc[c] = 0.0;
-  const double *part = k.w.partials + ((size_t)pslot * kNch + c_lo) * k.a.nblocks + b0;
+  for (int c = 0; c < NG; ++c) acc[c] = 0.0;
+  const double *part =
+      k.w.partials + ((size_t)pslot * kNch + c_lo + g * NG) * k.a.nblocks + b0;
   for (int base = 0; base < nb; base += U * kThreads) {
-    double v[U][NC];
+    double v[U][NG];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const int b = base + u * kThreads + (int)threadIdx.x;
+      const int b = base + u * kThreads + t;
 #pragma unroll
-      for (int c = 0; c < NC; ++c) v[u][c] = b < nb ? part[(size_t)c * k.a.nblocks + b] : 0.0;
+      for (int c = 0; c < NG; ++c) v[u][c] = b < nb ? part[(size_t)c * k.a.nblocks + b] : 0.0;
     }
 #pragma unroll
     for (int u = 0; u < U; ++u)
 #pragma unroll
-      for (int c = 0; c < NC; ++c) acc[c] += v[u][c];
+      for (int c = 0; c < NG; ++c) acc[c] += v[u][c];
   }
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = t & 63, wave = t >> 6;
 #pragma unroll
-  for (int c = 0; c < NC; ++c) {
-    double t = 0.0;
-    if (__ballot(acc[c] != 0.0)) t = wave_sum(acc[c]);
-    if (lane == 0) wsum[wave][c] = t;
+  for (int c = 0; c < NG; ++c) {
+    double tot = 0.0;
+    if (__ballot(acc[c] != 0.0)) tot = wave_sum(acc[c]);
+    if (lane == 0) wsum[wave][g * NG + c] = tot;
   }
   __syncthreads();
   if (threadIdx.x < NC)
@@ -906,8 +933,8 @@ __device__ inline BedDyn bed_advance(const BedDyn &d, int flags, double dt)
 // Step ledger of member mi: the partials of the step that started at clock
 // d.step (parity slot d.step & 1) are reduced and added to the accounting;
 // status bits are merged into the member record.  Called by every thread of
-// a 256-thread block; thread 0 writes.
-template <int U = 4>
+// a block of 256 G threads; thread 0 writes.
+template <int U = 4, int G = 1>
 __device__ void bed_member_ledger(const BKArgs &k, int mi, long long step, int flags,
                                   double (*wsum)[kCh])
 {
@@ -915,7 +942,7 @@ __device__ void bed_member_ledger(const BKArgs &k, int mi, long long step, int f
   tdm_bed_member *mp = k.a.members + mi;
   const int b0 = mp->block_first, nb = mp->nblocks;
   __syncthreads();  // wsum may be the caller's reduction scratch
-  bed_reduce<kStepCh, U>(k, b0, nb, (int)(step & 1), 0, wsum, stot);
+  bed_reduce<kStepCh, U, G>(k, b0, nb, (int)(step & 1), 0, wsum, stot);
   if (threadIdx.x != 0) return;
   double *acc = k.a.accounting + (size_t)mi * TDM_BED_NACC;
   const double dt = mp->dt;
@@ -989,30 +1016,24 @@ __global__ __launch_bounds__(kThreads) void bed_finalize_kernel(BKArgs k)
   if (threadIdx.x == 0) bed_write_record(mp, bed_advance(d, flags, mp->dt));
 }
 
-// Start of a multi-step call: the member records seed the launch-to-launch
-// state (slot read by the first launch) and the flag words are cleared.
-__global__ void bed_init_kernel(const tdm_bed_member *members, int nm, BedDyn *dyn, int *mflags,
-                                int slot)
-{
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nm) return;
-  dyn[2 * t + slot] = bed_record_state(members + t);
-#pragma unroll
-  for (int c = 0; c < 4; ++c) mflags[4 * t + c] = 0;
-}
-
-// End of a multi-step call: ledger of the last step and the member record
-// brought to the state after it.  k.parity is the last launch's parity.
-__global__ __launch_bounds__(kThreads) void bed_flush_kernel(BKArgs k)
+// End of a multi-step call: ledger of the last step, the member record
+// brought to the state after it, and the member's flag words cleared for the
+// next call (its first launch raises flags into a word it does not clear).
+// k.parity is the last launch's parity.  Four 256-thread groups split the
+// channels, so the partials of up to 16 x 256 blocks are loaded in one round.
+constexpr int kFlushGroups = 4;
+constexpr int kFlushU = 16;
+__global__ __launch_bounds__(kThreads * kFlushGroups) void bed_flush_kernel(BKArgs k)
 {
   __shared__ double wsum[4][kCh];
   const int mi = blockIdx.x;
   const BedDyn st = k.w.dyn[2 * mi + k.parity];
   const int f = k.w.mflags[4 * mi + (int)(st.step % 3)];
-  bed_member_ledger(k, mi, st.step, f, wsum);
+  bed_member_ledger<kFlushU, kFlushGroups>(k, mi, st.step, f, wsum);
   if (threadIdx.x == 0) {
     tdm_bed_member *mp = k.a.members + mi;
     bed_write_record(mp, bed_advance(st, f, mp->dt));
+    reinterpret_cast<int4 *>(k.w.mflags)[mi] = make_int4(0, 0, 0, 0);
   }
 }
 
@@ -1060,13 +1081,14 @@ int check_bed(const tdm_bed_arrays *a, BedWs *w)
   return TDM_OK;
 }
 
-BKArgs bed_args(const tdm_bed_arrays *a, const BedWs &w, int parity, int op)
+BKArgs bed_args(const tdm_bed_arrays *a, const BedWs &w, int parity, int op, int first = 0)
 {
   BKArgs k;
   k.a = *a;
   k.w = w;
   k.op = op;
   k.parity = parity;
+  k.first = first;
   const double2 *x = reinterpret_cast<const double2 *>(a->x);
   const double2 *v = reinterpret_cast<const double2 *>(a->v);
   const double2 *xa = reinterpret_cast<const double2 *>(a->x_alt);
@@ -1079,12 +1101,12 @@ BKArgs bed_args(const tdm_bed_arrays *a, const BedWs &w, int parity, int op)
 }
 
 // One launch of the step (or energy) kernel; a single launch (deferred = 0)
-// is followed by its finalize.
+// is followed by its finalize.  first: the first launch of a multi-step call.
 int launch_bed(const tdm_bed_arrays *a, const BedWs &w, int parity, int fp32, int op,
                hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr,
-               int deferred = 0)
+               int deferred = 0, int first = 0)
 {
-  const BKArgs k = bed_args(a, w, parity, op);
+  const BKArgs k = bed_args(a, w, parity, op, first);
   if (op == OP_ENERGY) {
     bed_energy_kernel<<<a->nblocks, kThreads, 0, s>>>(k);
   } else {
@@ -1114,8 +1136,8 @@ constexpr int kDeferMaxMembers = 4;
 
 // nsteps steps from parity 0: one launch per step.  For two or more steps of
 // a small batch the per-step finalize launches are replaced by the deferred
-// ledger between an init and a flush launch.  ev (optional) holds 2 events
-// per step around each step launch.
+// ledger, closed by a flush launch.  ev (optional) holds 2 events per step
+// around each step launch.
 int launch_bed_steps(const tdm_bed_arrays *a, const BedWs &w, int nsteps, int fp32,
                      hipStream_t s, hipEvent_t *ev = nullptr)
 {
@@ -1130,17 +1152,14 @@ int launch_bed_steps(const tdm_bed_arrays *a, const BedWs &w, int nsteps, int fp
     }
     return TDM_OK;
   }
-  bed_init_kernel<<<(a->nmembers + 255) / 256, 256, 0, s>>>(a->members, a->nmembers, w.dyn,
-                                                            w.mflags, parity ^ 1);
-  HIP_TRY(hipGetLastError());
   for (int i = 0; i < nsteps; ++i) {
     int rc = launch_bed(a, w, parity, fp32, OP_STEP, s, ev ? ev[2 * i] : nullptr,
-                        ev ? ev[2 * i + 1] : nullptr, 1);
+                        ev ? ev[2 * i + 1] : nullptr, 1, i == 0);
     if (rc) return rc;
     parity ^= 1;
   }
   const BKArgs k = bed_args(a, w, parity ^ 1, OP_STEP);
-  bed_flush_kernel<<<a->nmembers, kThreads, 0, s>>>(k);
+  bed_flush_kernel<<<a->nmembers, kThreads * kFlushGroups, 0, s>>>(k);
   HIP_TRY(hipGetLastError());
   return TDM_OK;
 }
@@ -1253,6 +1272,7 @@ int tdm_bed_setup(const tdm_bed_arrays *a, void *stream)
   delete[] hm;
   if (e != hipSuccess) return fail(TDM_EHIP, hipGetErrorString(e));
   fill_kernel<int><<<64, 256, 0, s>>>(w.bflags, (size_t)a->nblocks, 0);
+  fill_kernel<int><<<64, 256, 0, s>>>(w.mflags, 4 * (size_t)a->nmembers, 0);
   fill_kernel<uint32_t><<<1024, 256, 0, s>>>(w.mask, 2 * (size_t)a->n, 0u);
   fill_kernel<double><<<1024, 256, 0, s>>>(a->xi_wall, 6 * (size_t)a->n, 0.0);
   HIP_TRY(hipGetLastError());
