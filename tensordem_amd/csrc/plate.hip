@@ -185,6 +185,7 @@ struct KArgs {
   int op;        // OP_FORCES, OP_STEP or OP_ENERGY
   int parity;    // 0: x, v -> x_alt, v_alt; 1: the reverse
   int deferred;  // OP_STEP inside a multi-step call: clock from `dyn`, ledger one launch later
+  int first;     // the first launch of a multi-step call: the member records hold the clock
 };
 
 
@@ -277,9 +278,21 @@ __device__ __forceinline__ void plate_step_body(const KArgs &k)
     // step's ledger, whose partials and flags are complete now; kept apart
     // from the tiles so that no tile waits on the reduction
     const int mi = (int)blockIdx.x - a.nblocks;
-    const MemberDyn pd = k.w.dyn[2 * mi + (k.parity ^ 1)];
-    const int4 pf4 = reinterpret_cast<const int4 *>(k.w.mflags)[mi];
-    const int pf = reinterpret_cast<const int *>(&pf4)[pd.step % 3];
+    MemberDyn pd;
+    int pf = 0;
+    if (k.first) {
+      const tdm_plate_member *mp = a.members + mi;
+      pd.time = mp->time;
+      pd.step = mp->step_count;
+      pd.mode = mp->mode;
+      pd.tab = mp->tab;
+      pd.fresh = 1;
+      pd.pad = 0;
+    } else {
+      pd = k.w.dyn[2 * mi + (k.parity ^ 1)];
+      const int4 pf4 = reinterpret_cast<const int4 *>(k.w.mflags)[mi];
+      pf = reinterpret_cast<const int *>(&pf4)[pd.step % 3];
+    }
     const MemberDyn nd = advance_dyn(pd, pf, a.members[mi].dt);
     if (threadIdx.x == 0) {
       k.w.dyn[2 * mi + k.parity] = nd;
@@ -320,7 +333,7 @@ __device__ __forceinline__ void plate_step_body(const KArgs &k)
     reinterpret_cast<uint64_t *>(&sm)[threadIdx.x] =
         reinterpret_cast<const uint64_t *>(a.members + mi)[threadIdx.x];
   if (threadIdx.x < 2) sflags[threadIdx.x] = 0;
-  if (kDeferred && threadIdx.x >= 64 && threadIdx.x < 68) {
+  if (kDeferred && !k.first && threadIdx.x >= 64 && threadIdx.x < 68) {
     // the member's state at the start of the previous step and the flags
     // that step raised (written by the previous launch)
     const int t = threadIdx.x - 64;
@@ -358,7 +371,7 @@ __device__ __forceinline__ void plate_step_body(const KArgs &k)
   int mode = m.mode, tab = m.tab;
   double time = m.time;
   long long step = m.step_count;
-  if (kDeferred) {
+  if (kDeferred && !k.first) {
     const MemberDyn pd = sdyn;
     const MemberDyn cd =
         advance_dyn(pd, reinterpret_cast<const int *>(&sprevf)[pd.step % 3], m.dt);
@@ -1126,27 +1139,10 @@ __global__ __launch_bounds__(kFinThreads) void plate_finalize_kernel(KArgs k)
   if (flags & 0xFFFF) mp->status = m.status | (flags & 0xFFFF);
 }
 
-// Start of a multi-step call: the member records seed the launch-to-launch
-// state (slot read by the first launch) and the flag words are cleared.
-__global__ void plate_init_kernel(const tdm_plate_member *members, int nm, MemberDyn *dyn,
-                                  int *mflags, int slot)
-{
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nm) return;
-  MemberDyn d;
-  d.time = members[t].time;
-  d.step = members[t].step_count;
-  d.mode = members[t].mode;
-  d.tab = members[t].tab;
-  d.fresh = 1;
-  d.pad = 0;
-  dyn[2 * t + slot] = d;
-#pragma unroll
-  for (int c = 0; c < 4; ++c) mflags[4 * t + c] = 0;
-}
-
-// End of a multi-step call: ledger of the last step and the member record
-// brought to the state after it.  k.parity is the last launch's parity.
+// End of a multi-step call: ledger of the last step, the member record
+// brought to the state after it, and the member's flag words cleared for the
+// next call (its first launch raises flags into a word it does not clear).
+// k.parity is the last launch's parity.
 __global__ __launch_bounds__(kFinThreads) void plate_flush_kernel(KArgs k)
 {
   __shared__ double wsum[kFinThreads / 64][kCh];
@@ -1154,7 +1150,10 @@ __global__ __launch_bounds__(kFinThreads) void plate_flush_kernel(KArgs k)
   const MemberDyn st = k.w.dyn[2 * mi + k.parity];
   const int f = k.w.mflags[4 * mi + (int)(st.step % 3)];
   member_ledger(k, mi, st.time, f, (int)(st.step & 1), wsum);
-  if (threadIdx.x == 0) advance_record(k.a.members + mi, st.time, st.step, st.mode, st.tab, f);
+  if (threadIdx.x == 0) {
+    advance_record(k.a.members + mi, st.time, st.step, st.mode, st.tab, f);
+    reinterpret_cast<int4 *>(k.w.mflags)[mi] = make_int4(0, 0, 0, 0);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1385,6 +1384,7 @@ int tdm_plate_setup(const tdm_plate_arrays *a, void *stream)
   if (e != hipSuccess) return fail(TDM_EHIP, hipGetErrorString(e));
   // counters, requests, hash tables, candidate counts
   fill_kernel<int><<<64, 256, 0, s>>>(w.bflags, (size_t)a->nblocks, 0);
+  fill_kernel<int><<<64, 256, 0, s>>>(w.mflags, 4 * (size_t)a->nmembers, 0);
   fill_kernel<int><<<1024, 256, 0, s>>>(w.cell_count, 2 * (size_t)a->ncells, 0);
   fill_kernel<uint8_t><<<1024, 256, 0, s>>>(w.ncand, (size_t)a->n, (uint8_t)0);
   HIP_TRY(hipGetLastError());
@@ -1419,7 +1419,7 @@ int tdm_plate_rebin(const tdm_plate_arrays *a, void *stream)
 }
 
 static KArgs make_args(const tdm_plate_arrays *a, const Workspace &w, int parity, int op,
-                       int deferred)
+                       int deferred, int first = 0)
 {
   KArgs k;
   k.a = *a;
@@ -1427,6 +1427,7 @@ static KArgs make_args(const tdm_plate_arrays *a, const Workspace &w, int parity
   k.op = op;
   k.parity = parity;
   k.deferred = deferred;
+  k.first = first;
   const double2 *x = reinterpret_cast<const double2 *>(a->x);
   const double2 *v = reinterpret_cast<const double2 *>(a->v);
   const double2 *xa = reinterpret_cast<const double2 *>(a->x_alt);
@@ -1445,9 +1446,9 @@ static KArgs make_args(const tdm_plate_arrays *a, const Workspace &w, int parity
 // whole grid is resident at two blocks per unit.
 static int launch_step(const tdm_plate_arrays *a, const Workspace &w, int parity, int fp32,
                        int op, hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr,
-                       int deferred = 0)
+                       int deferred = 0, int first = 0)
 {
-  const KArgs k = make_args(a, w, parity, op, deferred);
+  const KArgs k = make_args(a, w, parity, op, deferred, first);
   const bool stage = a->max_tile_x > 0 && a->max_tile_x <= kStageMaxTile &&
                      (fp32 || a->nblocks <= 2 * cu_count());
   void (*kern)(KArgs) = fp32 ? pick_step<float>(deferred, stage) : pick_step<double>(deferred, stage);
@@ -1470,8 +1471,9 @@ static int launch_step(const tdm_plate_arrays *a, const Workspace &w, int parity
 
 // nsteps steps from `parity`: one launch per step.  For two or more steps
 // the per-step finalize launches are replaced by the deferred ledger (each
-// step's ledger blocks reduce the previous step) between an init and a flush
-// launch.  ev (optional) holds 2 events per step around each step launch.
+// step's ledger blocks reduce the previous step), closed by a flush launch;
+// the first launch reads the member records.  ev (optional) holds 2 events
+// per step around each step launch.
 static int launch_steps(const tdm_plate_arrays *a, const Workspace &w, int nsteps, int parity,
                         int fp32, hipStream_t s, hipEvent_t *ev = nullptr)
 {
@@ -1485,12 +1487,9 @@ static int launch_steps(const tdm_plate_arrays *a, const Workspace &w, int nstep
     }
     return TDM_OK;
   }
-  plate_init_kernel<<<(a->nmembers + 255) / 256, 256, 0, s>>>(a->members, a->nmembers, w.dyn,
-                                                              w.mflags, parity ^ 1);
-  HIP_TRY(hipGetLastError());
   for (int i = 0; i < nsteps; ++i) {
     int rc = launch_step(a, w, parity, fp32, OP_STEP, s, ev ? ev[2 * i] : nullptr,
-                         ev ? ev[2 * i + 1] : nullptr, 1);
+                         ev ? ev[2 * i + 1] : nullptr, 1, i == 0);
     if (rc) return rc;
     parity ^= 1;
   }
