@@ -39,6 +39,8 @@ extern "C" {
 #define TDM_ST_CAND_OVERFLOW 4  /* more unbonded contact candidates than slots */
 #define TDM_ST_CELL_OVERFLOW 8  /* more disks in one hash cell than its capacity */
 #define TDM_ST_HIST_OVERFLOW 16 /* more friction springs on one sphere than history slots */
+#define TDM_ST_INTEGRATOR 32    /* a velocity-Verlet bed member stepped by a batch whose
+                                   tdm_bed_arrays.verlet is 0 (it was stepped by Euler) */
 
 /* ledger layout (accounting, per member, float64) -- the dict of dem.py:139-144 */
 #define TDM_ACC_EXT_WORK 0
@@ -285,7 +287,18 @@ typedef struct tdm_bed_member {
   int32_t block_first, nblocks;
   int32_t mode, tab, list_parity, status;
   int32_t rebuilds;             /* candidate-list rebuilds so far */
-  int32_t use_right_wall, pad_;
+  int32_t use_right_wall;
+  /* integrator: 0 semi-implicit Euler (v and x are the state after the last
+   * step); 1 the floe bed's kick-drift-kick velocity-Verlet
+   * (scale_ceiling/kernel.py:410-421), one force evaluation per step: a step
+   * evaluates the force at the current x with the current v, kicks v by
+   * dt/2 F/m twice (the second half of the previous step, then the first
+   * half of this one) and drifts x by dt v.  v is then the half-step velocity
+   * the law sees; tdm_bed_velocity gives the full-step one.  verlet_start = 1
+   * marks v as a full-step velocity (a new or edited state: the first step
+   * skips the first kick, as the reference's first step evaluates the force
+   * of the state it is given); the steps clear it. */
+  int32_t integrator, verlet_start, pad_;
 } tdm_bed_member;
 
 typedef struct tdm_bed_arrays {
@@ -301,7 +314,10 @@ typedef struct tdm_bed_arrays {
   int32_t fp32_histories;        /* 1: the spring histories in the workspace are float2 (a bed
                                     stepped with FP32 forces), 0: double2; every step call
                                     must pass the matching fp32 flag */
-  int32_t pad_;
+  int32_t verlet;                /* nonzero iff some member has integrator 1: such batches
+                                    run kernels that carry the velocity-Verlet update (the
+                                    others do not pay for it); setup rejects a Verlet member
+                                    in a batch without the flag */
 } tdm_bed_arrays;
 
 /* sizeof(tdm_bed_member) and sizeof(tdm_bed_arrays) as compiled. */
@@ -340,6 +356,12 @@ int tdm_bed_step_timed(const tdm_bed_arrays *a, int nsteps, int fp32, void *stre
  * energy of the current state into `energy`; replaces FloeBed.kinetic_energy /
  * gravitational_energy / contact_energy (scale_ceiling/kernel.py:455-482). */
 int tdm_bed_energy(const tdm_bed_arrays *a, void *stream);
+/* Full-step velocity of the current state into v_alt (N,2): for
+ * velocity-Verlet members v + (dt/2 F) / m with F the force of the current
+ * x and v (the floe bed's `vel` after a step, scale_ceiling/kernel.py:
+ * 417-418), springs and ledger untouched; for the others, and for a state
+ * marked verlet_start, v itself.  The next step overwrites v_alt. */
+int tdm_bed_velocity(const tdm_bed_arrays *a, int fp32, void *stream);
 /* 2*pairs bed steps captured as a graph (state ends in x/v). */
 int tdm_bed_graph_create(const tdm_bed_arrays *a, int pairs, int fp32, void *stream,
                          tdm_graph **out);

@@ -34,8 +34,19 @@ def _bind():
 
 
 class OracleBed:
+    """config.integrator "euler": semi-implicit Euler.  "verlet": the floe bed's
+    kick-drift-kick velocity-Verlet (scale_ceiling/kernel.py:410-421) in the
+    device's arrangement, one force evaluation per step: a step evaluates
+    the force of the current (x, v) and commits its springs, kicks v by
+    (dt/2 F)/m twice -- the second half kick of the previous step, skipped
+    while `verlet_start` marks v as a full-step velocity, then the first half
+    kick of this step -- and drifts x by dt v.  `velocities` is then the
+    half-step velocity; `full_velocity()` gives the reference's `vel`."""
+
     def __init__(self, config, x, r, dt=None):
         self.config = config
+        self.integrator = getattr(config, "integrator", "euler")
+        self.verlet_start = True
         self.lib = _bind()
         self.positions = np.ascontiguousarray(x, dtype=np.float64).copy()
         self.radii = np.ascontiguousarray(r, dtype=np.float64).copy()
@@ -87,26 +98,62 @@ class OracleBed:
     def right_wall_position(self) -> float:
         return self.right_x + self.right_v * self.time
 
-    def step(self, integrate: bool = True) -> None:
+    def _evaluate(self, integrate: bool, walls=None):
+        """orb_step on the current state: the force into self.force, the new
+        pair springs and the step scalars returned; wall springs updated in
+        the arrays of `walls` (default: the oracle's own)."""
         n = len(self.radii)
         cap = max(16, 4 * n)
         ok = np.zeros(cap, dtype=np.int64)
         ox = np.zeros((cap, 2))
         scal = np.zeros(BS_NSCAL)
+        xf, xl, xr = walls if walls is not None else (self.xi_floor, self.xi_left, self.xi_right)
         d, i64 = ctypes.c_double, ctypes.c_int64
         cnt = self.lib.orb_step(_p(self._params(), d), n, _p(self.positions, d),
                                 _p(self.velocities, d), _p(self.radii, d), len(self.hist_keys),
                                 _p(self.hist_keys, i64), _p(self.hist_xi, d), cap, _p(ok, i64),
-                                _p(ox, d), _p(self.xi_floor, d), _p(self.xi_left, d),
-                                _p(self.xi_right, d), _p(self.force, d), _p(scal, d),
-                                int(integrate))
+                                _p(ox, d), _p(xf, d), _p(xl, d), _p(xr, d), _p(self.force, d),
+                                _p(scal, d), int(integrate))
         if cnt == -1:
             raise RuntimeError("oracle contact capacity exceeded")
         if cnt == -2:
             raise FloatingPointError("Non-finite simulation state; reduce dt")
-        self.hist_keys = ok[:cnt].copy()
-        self.hist_xi = ox[:cnt].copy()
+        return ok[:cnt].copy(), ox[:cnt].copy(), scal
+
+    def _verlet_kicks(self):
+        """(dt/2 F)/m per disk, in the reference's order of operations."""
+        inv_m = 1.0 / self.masses
+        return ((0.5 * self.dt) * self.force) * inv_m[:, None]
+
+    def full_velocity(self) -> np.ndarray:
+        """The full-step velocity of the current state: for a velocity-Verlet
+        state v + (dt/2 F)/m with the force of the current (x, v) (springs
+        not committed), else v."""
+        if self.integrator != "verlet" or self.verlet_start:
+            return self.velocities.copy()
+        keep = self.force.copy()
+        walls = (self.xi_floor.copy(), self.xi_left.copy(), self.xi_right.copy())
+        self._evaluate(False, walls)
+        out = self.velocities + self._verlet_kicks()
+        self.force[:] = keep
+        return out
+
+    def step(self, integrate: bool = True) -> None:
+        verlet = self.integrator == "verlet" and integrate
+        ok, ox, scal = self._evaluate(integrate and not verlet)
+        self.hist_keys = ok
+        self.hist_xi = ox
         self.last = scal.copy()
+        if verlet:
+            kick = self._verlet_kicks()
+            u = self.velocities if self.verlet_start else self.velocities + kick
+            v = u + kick
+            x = self.positions + self.dt * v
+            if not (np.isfinite(v).all() and np.isfinite(x).all()):
+                raise FloatingPointError("Non-finite simulation state; reduce dt")
+            self.velocities[:] = v
+            self.positions[:] = x
+            self.verlet_start = False
         if integrate:
             dt = self.dt
             self.acc[0] += dt * scal[BS_DASH]

@@ -79,7 +79,8 @@ class BedMember(ctypes.Structure):
         ("cell_first", ctypes.c_int32), ("block_first", ctypes.c_int32),
         ("nblocks", ctypes.c_int32), ("mode", ctypes.c_int32), ("tab", ctypes.c_int32),
         ("list_parity", ctypes.c_int32), ("status", ctypes.c_int32), ("rebuilds", ctypes.c_int32),
-        ("use_right_wall", ctypes.c_int32), ("pad_", ctypes.c_int32),
+        ("use_right_wall", ctypes.c_int32), ("integrator", ctypes.c_int32),
+        ("verlet_start", ctypes.c_int32), ("pad_", ctypes.c_int32),
     ]
 
 
@@ -92,7 +93,7 @@ class BedArrays(ctypes.Structure):
         ("workspace", ctypes.c_void_p), ("workspace_bytes", ctypes.c_size_t),
         ("n", ctypes.c_int32), ("nmembers", ctypes.c_int32), ("nblocks", ctypes.c_int32),
         ("ncells", ctypes.c_int32), ("cand_slots", ctypes.c_int32), ("cell_cap", ctypes.c_int32),
-        ("fp32_histories", ctypes.c_int32), ("pad_", ctypes.c_int32),
+        ("fp32_histories", ctypes.c_int32), ("verlet", ctypes.c_int32),
     ]
 
 
@@ -121,6 +122,7 @@ D3_REACTION, D3_TOP_REACTION, D3_BOTTOM_REACTION, D3_LATERAL = 0, 3, 4, 5
 D3_FRICTION_POWER, D3_CONTACTS, D3_BREAKS, D3_STATUS, D3_REBUILDS, D3_NOUT = 9, 10, 11, 12, 13, 14
 D3_UPDATE_FRACTURE, D3_UPDATE_HISTORY = 1, 2
 ST_HIST_OVERFLOW = 16
+ST_INTEGRATOR = 32
 
 
 BED_DASHPOT, BED_FRICTION, BED_WALL_DASHPOT, BED_WALL_FRICTION, BED_WALL_IMPULSE = 0, 1, 2, 3, 4
@@ -165,6 +167,7 @@ _SIGNATURES = {
     "tdm_bed_step_timed": ([ctypes.POINTER(BedArrays), ctypes.c_int, ctypes.c_int,
                             ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
     "tdm_bed_energy": ([ctypes.POINTER(BedArrays), ctypes.c_void_p], ctypes.c_int),
+    "tdm_bed_velocity": ([ctypes.POINTER(BedArrays), ctypes.c_int, ctypes.c_void_p], ctypes.c_int),
     "tdm_bed_graph_create": ([ctypes.POINTER(BedArrays), ctypes.c_int, ctypes.c_int,
                               ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)], ctypes.c_int),
     "tdm_graph_destroy": ([ctypes.c_void_p], ctypes.c_int),

@@ -120,6 +120,8 @@ def bed_parts(s: BedSpec, *, device="cuda", skin_factor: float = 0.4,
     m.floor_y, m.back_wall_x = c.floor_y, c.back_wall_x
     m.right_wall_x, m.right_wall_v = c.right_wall_x, c.right_wall_speed
     m.use_right_wall = int(c.right_wall)
+    m.integrator = int(c.integrator == "verlet")
+    m.verlet_start = 1          # the given velocities are full-step ones
     m.time, m.step_count = 0.0, 0
     r0 = float(r.mean())
     m.skin = skin_factor * r0
@@ -185,6 +187,7 @@ class BedEngine:
         self.accounting = torch.zeros((nm, _lib.BED_NACC), **f64)
         self.energy = torch.zeros((nm, 4), **f64)
         self._members_host = members
+        self._verlet = any(members[k].integrator for k in range(nm))
         self.members = torch.frombuffer(bytearray(bytes(members)), dtype=torch.uint8).to(dev)
         args = (n, blocks, nm, cells, cand_slots, cell_cap)
         wsb = self.lib.tdm_bed_workspace_bytes(*args)
@@ -216,6 +219,7 @@ class BedEngine:
         a.n, a.nmembers, a.nblocks, a.ncells = self.n, self.nmembers, self.nblocks, self.ncells
         a.cand_slots, a.cell_cap = self.cand_slots, self.cell_cap
         a.fp32_histories = int(self.fp32)
+        a.verlet = int(self._verlet)
 
     def _swap(self) -> None:
         self.x, self.x_alt = self.x_alt, self.x
@@ -237,6 +241,16 @@ class BedEngine:
 
     def rebin(self) -> None:
         _lib.check(self.lib.tdm_bed_rebin(ctypes.byref(self._arrays), self.stream))
+
+    def full_velocities(self) -> torch.Tensor:
+        """(n,2) full-step velocities of the current state: `v` for Euler
+        members; for velocity-Verlet members v + (dt/2 F)/m with the force of
+        the current state (the floe bed's `vel` after a step,
+        scale_ceiling/kernel.py:417-418), springs and ledger untouched."""
+        self._refresh()
+        _lib.check(self.lib.tdm_bed_velocity(ctypes.byref(self._arrays), int(self.fp32),
+                                             self.stream))
+        return self.v_alt.clone()
 
     def energies(self) -> torch.Tensor:
         _lib.check(self.lib.tdm_bed_energy(ctypes.byref(self._arrays), self.stream))
@@ -403,6 +417,8 @@ class BedEngine:
             "pair_xi": hx[order].copy(),
             "wall_xi": self.xi_wall[sl].cpu().numpy().copy(),
             "wall_active": ((mask >> 16) & 7).astype(np.uint8),
+            # velocity-Verlet: 0 while `velocities` holds the half-step velocity
+            "verlet_start": int(m.verlet_start),
         }
 
     @classmethod
@@ -450,6 +466,7 @@ class BedEngine:
         eng.accounting[0].copy_(torch.from_numpy(acc).to(dev))
         ms = eng.read_members()
         ms[0].time, ms[0].step_count = float(state["time"]), int(state["step_count"])
+        ms[0].verlet_start = int(state.get("verlet_start", 1))
         eng.write_members(ms)
         return eng
 
@@ -502,8 +519,10 @@ def validate_bed_restart(state: dict) -> None:
         raise ValueError("not a tensordem_amd bed restart state")
     need = {"schema", "config", "dt", "time", "step_count", "positions", "velocities", "radii",
             "accounting", "pairs", "pair_xi", "wall_xi", "wall_active"}
-    if set(state) != need:
+    if not need <= set(state) <= need | {"verlet_start"}:
         raise ValueError("bed restart fields missing or unknown")
+    if "verlet_start" in state and state["verlet_start"] not in (0, 1):
+        raise ValueError("invalid bed restart verlet_start")
     x = np.asarray(state["positions"], np.float64)
     n = len(x)
     shapes = {"positions": [(n, 2)], "velocities": [(n, 2)], "radii": [(n,)],
@@ -545,7 +564,8 @@ def save_bed_restart(state: dict, path) -> None:
     and the law as JSON inside it; read back by `load_bed_restart` without
     unpickling anything."""
     import json
-    meta = {k: state[k] for k in ("schema", "config", "dt", "time", "step_count")}
+    meta = {k: state[k] for k in ("schema", "config", "dt", "time", "step_count", "verlet_start")
+            if k in state}
     arrays = {k: np.asarray(v) for k, v in state.items() if k not in meta}
     np.savez(path, meta=np.frombuffer(json.dumps(meta).encode(), dtype=np.uint8), **arrays)
 

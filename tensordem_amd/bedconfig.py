@@ -38,6 +38,10 @@ class BedConfig:
     right_wall: bool = False
     right_wall_x: float = 0.0       # position at t = 0 [m]
     right_wall_speed: float = 0.0   # velocity along +x [m/s]
+    # "euler": semi-implicit Euler (the plate's integrator); "verlet": the
+    # floe bed's kick-drift-kick velocity-Verlet (scale_ceiling/kernel.py:
+    # 410-421), whose state between steps holds the half-step velocity
+    integrator: str = "euler"
 
     def __post_init__(self) -> None:
         for name in ("young", "density", "thickness"):
@@ -57,6 +61,8 @@ class BedConfig:
         for name in ("right_wall_x", "right_wall_speed"):
             if not math.isfinite(getattr(self, name)):
                 raise ValueError(f"{name} must be finite")
+        if self.integrator not in ("euler", "verlet"):
+            raise ValueError("integrator must be 'euler' or 'verlet'")
 
     # -- contact constants (scale_ceiling/kernel.py:119-129, 216-227) --------
     @property

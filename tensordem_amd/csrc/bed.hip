@@ -532,8 +532,12 @@ __device__ inline BedDyn bed_advance(const BedDyn &d, int flags, double dt);
 // its own block-table list parity after a rebuild, and the member's first
 // block reduces the previous step's ledger after its own work (one launch
 // per step, no finalize launch).
-template <typename R, bool kDeferred>
-__global__ __launch_bounds__(kThreads) void bed_step_kernel(BKArgs k)
+// kReport: the velocity readout: the forces of the current state on the
+// current lists, and only the full-step velocity stored (into v_out).
+// kVerlet: the batch has velocity-Verlet members (the update is compiled in
+// only then: it costs the Euler-only kernels 1 % in registers).
+template <typename R, bool kDeferred, bool kReport, bool kVerlet>
+__device__ __forceinline__ void bed_step_body(const BKArgs &k)
 {
   __shared__ double red[4][kCh];
   __shared__ int sflags[2];
@@ -583,6 +587,7 @@ __global__ __launch_bounds__(kThreads) void bed_step_kernel(BKArgs k)
     tab = cur.tab;
     time = cur.time;
   }
+  if (kReport) mode = MODE_IDLE;  // the current lists hold every contact
   if (threadIdx.x < 2) sflags[threadIdx.x] = 0;
   // ledger channels, gathered into ch[] after the forces (the pair loop
   // carries only its own two)
@@ -672,7 +677,7 @@ __global__ __launch_bounds__(kThreads) void bed_step_kernel(BKArgs k)
         else
           hm_pair32(law_f, dx, dy, s2, rs, rs2, rpf, (float)rq[j], vrx, vry, hq[j], Fx, Fy,
                     dash, fric);
-        xis[(size_t)e * n] = hq[j];
+        if constexpr (!kReport) xis[(size_t)e * n] = hq[j];
         nmask |= 1u << e;
         if (self_lo) {
           fx += -Fx; fy += -Fy;
@@ -703,7 +708,7 @@ __global__ __launch_bounds__(kThreads) void bed_step_kernel(BKArgs k)
       const double2 vr = make_double2(vp.x - wv, vp.y);
       double Fx, Fy, dash, fric;
       if (hm_wall<R>(law, wall_mu, rp, mass, gap, nx, ny, vr, h, Fx, Fy, dash, fric)) {
-        xw[w] = h;
+        if constexpr (!kReport) xw[w] = h;
         nmask |= bit;
         fx += Fx; fy += Fy;
         c_wdash += dash; c_wfric += fric; c_wfx += Fx; c_wfy += Fy;
@@ -711,37 +716,66 @@ __global__ __launch_bounds__(kThreads) void bed_step_kernel(BKArgs k)
         ++nwall;
       }
     }
-    k.w.mask[(size_t)lp * n + p] = nmask;
-    c_counts = (double)npair + kWallCount * (double)nwall;
-
-    // ---- semi-implicit Euler (dem.py:363-365) -------------------------
-    double vx, vy;
-    if constexpr (Prec<R>::fp64) {
-      vx = vp.x + (law.dt * fx) / mass;
-      vy = vp.y + (law.dt * fy) / mass;
+    if constexpr (kReport) {
+      // the full-step velocity: v itself, or v + (dt/2 F) / m for a
+      // velocity-Verlet state (the reference's second half kick)
+      double2 vf = vp;
+      if (uniform(mp->integrator) && !uniform(mp->verlet_start)) {
+        const double hk = 0.5 * law.dt, inv_m = 1.0 / mass;
+        vf = make_double2(vp.x + (hk * fx) * inv_m, vp.y + (hk * fy) * inv_m);
+      }
+      k.v_out[p] = vf;
     } else {
-      const double dt_m = law.dt / mass;
-      vx = vp.x + fx * dt_m;
-      vy = vp.y + fy * dt_m;
-    }
-    const double xn = xp.x + law.dt * vx, yn = xp.y + law.dt * vy;
-    k.x_out[p] = make_double2(xn, yn);
-    k.v_out[p] = make_double2(vx, vy);
-    if (!isfinite(xn) || !isfinite(yn) || !isfinite(vx) || !isfinite(vy)) status |= TDM_ST_NONFINITE;
-    const double ddx = xn - xb.x, ddy = yn - xb.y;
-    const double q4 = kRebuildFrac * skin;
-    if (ddx * ddx + ddy * ddy >= q4 * q4) req = 1;
-    if (mode == MODE_BIN) {
-      const BedGrid g = load_grid(mp);
-      int cx, cy;
-      const int cell = cell_of(g.ox, g.oy, g.h, g.ncx, g.ncy, g.first, make_double2(xn, yn), &cx,
-                               &cy);
-      const int t = tab ^ 1;
-      const int slot = atomicAdd(k.w.cell_count + (size_t)t * a.ncells + cell, 1);
-      if (slot < a.cell_cap) k.w.cell_tab[((size_t)t * a.ncells + cell) * a.cell_cap + slot] = p;
-      else status |= TDM_ST_CELL_OVERFLOW;
+      k.w.mask[(size_t)lp * n + p] = nmask;
+      c_counts = (double)npair + kWallCount * (double)nwall;
+
+      // velocity-Verlet members: v is a full-step velocity only on a new or
+      // edited state, which only a call's first step sees (read here, where
+      // the scalar registers are free)
+      double vx, vy;
+      if (kVerlet && uniform(mp->integrator)) {
+        const bool vfull = (!kDeferred || k.first) && uniform(mp->verlet_start) != 0;
+        // ---- velocity-Verlet (scale_ceiling/kernel.py:410-421): the second
+        // half kick of the previous step (none from a full-step velocity), the
+        // first half kick of this one, each (dt/2 F) / m in the reference's
+        // order, then the drift below
+        const double hk = 0.5 * law.dt, inv_m = 1.0 / mass;
+        double ux = vp.x, uy = vp.y;
+        if (!vfull) {
+          ux = vp.x + (hk * fx) * inv_m;
+          uy = vp.y + (hk * fy) * inv_m;
+        }
+        vx = ux + (hk * fx) * inv_m;
+        vy = uy + (hk * fy) * inv_m;
+      } else if constexpr (Prec<R>::fp64) {
+        // ---- semi-implicit Euler (dem.py:363-365) -------------------------
+        vx = vp.x + (law.dt * fx) / mass;
+        vy = vp.y + (law.dt * fy) / mass;
+      } else {
+        const double dt_m = law.dt / mass;
+        vx = vp.x + fx * dt_m;
+        vy = vp.y + fy * dt_m;
+      }
+      const double xn = xp.x + law.dt * vx, yn = xp.y + law.dt * vy;
+      k.x_out[p] = make_double2(xn, yn);
+      k.v_out[p] = make_double2(vx, vy);
+      if (!isfinite(xn) || !isfinite(yn) || !isfinite(vx) || !isfinite(vy)) status |= TDM_ST_NONFINITE;
+      const double ddx = xn - xb.x, ddy = yn - xb.y;
+      const double q4 = kRebuildFrac * skin;
+      if (ddx * ddx + ddy * ddy >= q4 * q4) req = 1;
+      if (mode == MODE_BIN) {
+        const BedGrid g = load_grid(mp);
+        int cx, cy;
+        const int cell = cell_of(g.ox, g.oy, g.h, g.ncx, g.ncy, g.first, make_double2(xn, yn), &cx,
+                                 &cy);
+        const int t = tab ^ 1;
+        const int slot = atomicAdd(k.w.cell_count + (size_t)t * a.ncells + cell, 1);
+        if (slot < a.cell_cap) k.w.cell_tab[((size_t)t * a.ncells + cell) * a.cell_cap + slot] = p;
+        else status |= TDM_ST_CELL_OVERFLOW;
+      }
     }
   }
+  if constexpr (kReport) return;
   if (mode == MODE_SEARCH) {
     // clear the other hash table for the next bin phase
     const int ncell_m = uniform(mp->ncx) * uniform(mp->ncy);
@@ -784,6 +818,24 @@ __global__ __launch_bounds__(kThreads) void bed_step_kernel(BKArgs k)
     }
     if (!pd.fresh) bed_member_ledger<1, 1>(k, mi, pd.step, pflags, red);
   }
+}
+
+template <typename R, bool kDeferred>
+__global__ __launch_bounds__(kThreads) void bed_step_kernel(BKArgs k)
+{
+  bed_step_body<R, kDeferred, false, false>(k);
+}
+
+template <typename R, bool kDeferred>
+__global__ __launch_bounds__(kThreads) void bed_verlet_step_kernel(BKArgs k)
+{
+  bed_step_body<R, kDeferred, false, true>(k);
+}
+
+template <typename R>
+__global__ __launch_bounds__(kThreads) void bed_velocity_kernel(BKArgs k)
+{
+  bed_step_body<R, false, true, true>(k);
 }
 
 // Energies of the current state on the current lists (no rebuild, no
@@ -967,6 +1019,7 @@ __device__ inline void bed_write_record(tdm_bed_member *mp, const BedDyn &d)
   mp->tab = d.tab;
   mp->list_parity = d.lp;
   mp->rebuilds = d.rebuilds;
+  mp->verlet_start = 0;  // a step has been taken from the state
 }
 
 __device__ inline BedDyn bed_record_state(const tdm_bed_member *mp)
@@ -1013,7 +1066,10 @@ __global__ __launch_bounds__(kThreads) void bed_finalize_kernel(BKArgs k)
   const BedDyn d = bed_record_state(mp);
   // (a search step's blocks flipped their own table entries' list parity)
   bed_member_ledger(k, mi, 0, flags, wsum);
-  if (threadIdx.x == 0) bed_write_record(mp, bed_advance(d, flags, mp->dt));
+  if (threadIdx.x == 0) {
+    bed_write_record(mp, bed_advance(d, flags, mp->dt));
+    if (mp->integrator && !k.a.verlet) mp->status |= TDM_ST_INTEGRATOR;
+  }
 }
 
 // End of a multi-step call: ledger of the last step, the member record
@@ -1033,6 +1089,7 @@ __global__ __launch_bounds__(kThreads * kFlushGroups) void bed_flush_kernel(BKAr
   if (threadIdx.x == 0) {
     tdm_bed_member *mp = k.a.members + mi;
     bed_write_record(mp, bed_advance(st, f, mp->dt));
+    if (mp->integrator && !k.a.verlet) mp->status |= TDM_ST_INTEGRATOR;
     reinterpret_cast<int4 *>(k.w.mflags)[mi] = make_int4(0, 0, 0, 0);
   }
 }
@@ -1111,8 +1168,13 @@ int launch_bed(const tdm_bed_arrays *a, const BedWs &w, int parity, int fp32, in
     bed_energy_kernel<<<a->nblocks, kThreads, 0, s>>>(k);
   } else {
     void (*kern)(BKArgs) =
-        deferred ? (fp32 ? bed_step_kernel<float, true> : bed_step_kernel<double, true>)
-                 : (fp32 ? bed_step_kernel<float, false> : bed_step_kernel<double, false>);
+        a->verlet
+            ? (deferred ? (fp32 ? bed_verlet_step_kernel<float, true>
+                                : bed_verlet_step_kernel<double, true>)
+                        : (fp32 ? bed_verlet_step_kernel<float, false>
+                                : bed_verlet_step_kernel<double, false>))
+            : (deferred ? (fp32 ? bed_step_kernel<float, true> : bed_step_kernel<double, true>)
+                        : (fp32 ? bed_step_kernel<float, false> : bed_step_kernel<double, false>));
     if (ev0) {
       // events carried by the dispatch itself: they time exactly this kernel
       hipExtLaunchKernelGGL(kern, dim3(a->nblocks), dim3(kThreads), 0, s, ev0, ev1, 0, k);
@@ -1225,6 +1287,11 @@ int tdm_bed_setup(const tdm_bed_arrays *a, void *stream)
     if (hm[i].block_first != total || hm[i].nblocks != tdm_bed_blocks(hm[i].n)) {
       delete[] hm;
       return fail(TDM_EINVAL, "bed member block_first/nblocks do not match tdm_bed_blocks");
+    }
+    if ((hm[i].integrator != 0 && hm[i].integrator != 1) || (hm[i].integrator && !a->verlet)) {
+      delete[] hm;
+      return fail(TDM_EINVAL, "bed member integrator must be 0 or 1, and a batch with "
+                              "velocity-Verlet members needs tdm_bed_arrays.verlet");
     }
     total += hm[i].nblocks;
   }
@@ -1346,6 +1413,23 @@ int tdm_bed_energy(const tdm_bed_arrays *a, void *stream)
   if (rc) return rc;
   if (!a->energy) return fail(TDM_EINVAL, "energy output required");
   return launch_bed(a, w, 0, 0, OP_ENERGY, static_cast<hipStream_t>(stream));
+}
+
+int tdm_bed_velocity(const tdm_bed_arrays *a, int fp32, void *stream)
+{
+  BedWs w;
+  int rc = check_bed(a, &w);
+  if (rc) return rc;
+  if ((fp32 != 0) != (a->fp32_histories != 0))
+    return fail(TDM_EINVAL, "the bed's spring histories are stored in the other precision "
+                            "(fp32_histories must match the fp32 flag)");
+  // parity 0: reads x, v, writes v_alt
+  const BKArgs k = bed_args(a, w, 0, OP_STEP);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (fp32) bed_velocity_kernel<float><<<a->nblocks, kThreads, 0, s>>>(k);
+  else bed_velocity_kernel<double><<<a->nblocks, kThreads, 0, s>>>(k);
+  HIP_TRY(hipGetLastError());
+  return TDM_OK;
 }
 
 int tdm_bed_graph_create(const tdm_bed_arrays *a, int pairs, int fp32, void *stream,

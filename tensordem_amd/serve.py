@@ -215,7 +215,7 @@ def _carry_bed(old: BedEngine, old_members, eng: BedEngine, members, keep) -> No
     ti, tj = torch.tensor(si, device=dev), torch.tensor(di, device=dev)
     eng.accounting[tj] = old.accounting[ti]
     _copy_fields(eng.members, old.members, di, si, _lib.BedMember,
-                 ("time", "step_count", "status", "rebuilds"))
+                 ("time", "step_count", "status", "rebuilds", "verlet_start"))
     # the carried lists are current: no rebuild is pending for these members
     _zero_fields(eng.members, di, _lib.BedMember, ("mode", "tab", "list_parity"))
 

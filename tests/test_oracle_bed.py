@@ -373,3 +373,98 @@ def test_deposited_column_is_loose_overlap_free_and_supported():
     phi = math.pi * (r * r).sum() / (width * h)
     assert 0.73 < phi < 0.78
     assert h / width == pytest.approx(2.0, rel=0.05)
+
+
+# ---- velocity-Verlet (the floe bed's integrator, scale_ceiling/kernel.py:410-421)
+
+def test_verlet_free_flight_under_gravity_is_exact():
+    """Constant acceleration: kick-drift-kick reproduces x0 + v0 t + g t^2 / 2
+    and v0 + g t (to rounding); semi-implicit Euler is off by g dt t / 2."""
+    g, dt, steps = 9.81, 1e-3, 400
+    x0, v0 = np.array([[0.3, 2.0]]), np.array([[0.7, 1.5]])
+    t = steps * dt
+    want_x = x0 + v0 * t + np.array([[0.0, -0.5 * g * t * t]])
+    want_v = v0 + np.array([[0.0, -g * t]])
+    out = {}
+    for integ in ("verlet", "euler"):
+        b = OracleBed(free(gravity=g, integrator=integ), x0, np.array([R]), dt=dt)
+        b.velocities[:] = v0
+        for _ in range(steps):
+            b.step()
+        out[integ] = (b.positions.copy(), b.full_velocity())
+    xv, vv = out["verlet"]
+    assert np.allclose(xv, want_x, rtol=0, atol=1e-13)
+    assert np.allclose(vv, want_v, rtol=0, atol=1e-12)
+    xe, _ = out["euler"]
+    assert abs(xe[0, 1] - want_x[0, 1]) == pytest.approx(0.5 * g * dt * t, rel=1e-6)
+
+
+def _reference_floe_steps(cfg, x, r, v, dt, steps):
+    """kernel.py:410-421 literally: a force evaluation (springs committed)
+    when no force is cached, v += 0.5 dt F / m, x += dt v, a force
+    evaluation at the new positions, v += 0.5 dt F / m.  The force engine is
+    the oracle's own evaluation (its Euler instance, never integrated)."""
+    f = OracleBed(BedConfig(**{**cfg.__dict__, "integrator": "euler"}), x, r, dt=dt)
+    f.velocities[:] = v
+    inv_m = 1.0 / f.masses
+    force = None
+    for _ in range(steps):
+        if force is None:
+            f.step(integrate=False)
+            force = f.force.copy()
+        f.velocities += ((0.5 * dt) * force) * inv_m[:, None]
+        f.positions += dt * f.velocities
+        f.time += dt
+        f.step(integrate=False)
+        force = f.force.copy()
+        f.velocities += ((0.5 * dt) * force) * inv_m[:, None]
+    return f
+
+
+def test_verlet_launch_order_equals_the_reference_step_sequence():
+    """The device arrangement (one force evaluation per step, the previous
+    step's second half kick folded in) is the reference's step sequence:
+    after N steps the positions and the full-step velocities are identical,
+    and after one more device step the springs are too (the reference's
+    N-th step ends with the force evaluation the device does next)."""
+    x, r = column_packing(120, R, seed=4)
+    x = x * 0.82                         # overlapping at the start: springs from step 1
+    x[:, 1] += r.max() - x[:, 1].min()   # resting on the floor
+    cfg = BedConfig(integrator="verlet", right_wall=True,
+                    right_wall_x=float((x[:, 0] + r).max()) - 2e-4, right_wall_speed=-0.05,
+                    friction=0.4, wall_friction=0.3)
+    v = np.random.default_rng(4).normal(scale=0.2, size=x.shape)
+    dt = cfg.stable_dt(r.min())
+    steps = 300
+    ref = _reference_floe_steps(cfg, x, r, v, dt, steps)
+    dev = OracleBed(cfg, x, r, dt=dt)
+    dev.velocities[:] = v
+    for _ in range(steps):
+        dev.step()
+    assert len(ref.hist_keys) > 25
+    assert (ref.xi_right != 0).any() and (ref.xi_floor != 0).any()
+    assert np.array_equal(dev.positions, ref.positions)
+    assert np.array_equal(dev.full_velocity(), ref.velocities)
+    dev.step()
+    assert np.array_equal(dev.hist_keys, ref.hist_keys)
+    assert np.array_equal(dev.hist_xi, ref.hist_xi)
+    for a, b in ((dev.xi_floor, ref.xi_floor), (dev.xi_left, ref.xi_left),
+                 (dev.xi_right, ref.xi_right)):
+        assert np.array_equal(a, b)
+
+
+def test_verlet_full_velocity_commits_nothing():
+    cfg = BedConfig(integrator="verlet")
+    x, r = column_packing(60, R, seed=2)
+    x = x * 0.82
+    b = OracleBed(cfg, x, r)
+    for _ in range(50):
+        b.step()
+    keys, xi, xf = b.hist_keys.copy(), b.hist_xi.copy(), b.xi_floor.copy()
+    v1 = b.full_velocity()
+    v2 = b.full_velocity()
+    assert np.array_equal(v1, v2)
+    assert np.array_equal(keys, b.hist_keys) and np.array_equal(xi, b.hist_xi)
+    assert np.array_equal(xf, b.xi_floor)
+    assert len(keys) > 20
+    assert not np.array_equal(v1, b.velocities)   # the half kick is there
