@@ -453,10 +453,12 @@ BED_R0 = 0.01
 BED_BYTES_FIXED = 97
 
 
-def bed_bytes_per_disk(cand_mean, contacts_per_disk, wall_per_disk):
-    """+ 4 B per candidate index, + 32 B per pair-history copy (read and
-    written), + 32 B per wall history (read and written)."""
-    return BED_BYTES_FIXED + 4.0 * cand_mean + 32.0 * contacts_per_disk + 32.0 * wall_per_disk
+def bed_bytes_per_disk(cand_mean, contacts_per_disk, wall_per_disk, hist_bytes):
+    """+ 4 B per candidate index, + 2 hist_bytes per pair-history copy in
+    contact (read and written: float2, 8 B, with FP32 forces; double2, 16 B,
+    with FP64 forces), + 32 B per wall history (double2, read and written)."""
+    return (BED_BYTES_FIXED + 4.0 * cand_mean + 2.0 * hist_bytes * contacts_per_disk
+            + 32.0 * wall_per_disk)
 
 
 def graph_steps(k, cap):
@@ -639,7 +641,7 @@ def bench_bed(args):
             st = eng.list_stats()
             acc = eng.accounting[0].cpu().numpy()
             byts.append(bed_bytes_per_disk(st["cand_mean"], st["contacts_per_disk"],
-                                           acc[_lib.BED_WALL_CONTACTS] / n))
+                                           acc[_lib.BED_WALL_CONTACTS] / n, hist_bytes=8))
             if half is None and eng.read_members()[0].step_count - start >= half_at:
                 half = (eng.x.cpu().numpy(), eng.v.cpu().numpy())
         return (us, byts), lk + kk
