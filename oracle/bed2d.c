@@ -22,7 +22,9 @@
  *     starts at time t, inward normal -x, the law evaluated on the velocity
  *     relative to the wall; it doubles as the column's gate (v_right = 0);
  *   - semi-implicit Euler v += dt F / m, x += dt v as IceDEM.step
- *     (dem.py:363-365).
+ *     (dem.py:363-365); the floe bed's velocity-Verlet
+ *     (scale_ceiling/kernel.py:410-421) is driven from oracle/bed.py on this
+ *     force evaluation (integrate = 0: springs committed, no update).
  * Per disk the force is gravity, then pair forces by ascending partner
  * index, then floor, then back wall.  (The reference's scatter order,
  * torch index_add_, is not specified; this order is the one the GPU kernel
