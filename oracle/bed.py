@@ -180,11 +180,11 @@ class OracleBed:
         return self.config.rho_pi_t * (self.radii * self.radii)
 
     def energies(self) -> dict:
-        """Kinetic, gravitational (relative to y = 0) and Hertz normal potential
-        (2/5) f_el d of the current contacts."""
+        """Kinetic (of the full-step velocity), gravitational (relative to
+        y = 0) and Hertz normal potential (2/5) f_el d of the current contacts."""
         c = self.config
         m = self.masses
-        kin = 0.5 * float((m[:, None] * self.velocities ** 2).sum())
+        kin = 0.5 * float((m[:, None] * self.full_velocity() ** 2).sum())
         pot = float((m * c.gravity * self.positions[:, 1]).sum())
         el = 0.0
         for i, j in self.contacts():

@@ -253,7 +253,18 @@ class BedEngine:
         return self.v_alt.clone()
 
     def energies(self) -> torch.Tensor:
-        _lib.check(self.lib.tdm_bed_energy(ctypes.byref(self._arrays), self.stream))
+        """(nmembers, 4): kinetic, gravitational, Hertz normal and tangential
+        spring energy of the current state; the kinetic term of a
+        velocity-Verlet member is that of its full-step velocity (the floe
+        bed's `kinetic_energy`, scale_ceiling/kernel.py:455-457)."""
+        arr = self._arrays
+        if self._verlet:
+            self._refresh()
+            _lib.check(self.lib.tdm_bed_velocity(ctypes.byref(self._arrays), int(self.fp32),
+                                                 self.stream))
+            arr = _lib.BedArrays.from_buffer_copy(self._arrays)
+            arr.v = self.v_alt.data_ptr()
+        _lib.check(self.lib.tdm_bed_energy(ctypes.byref(arr), self.stream))
         return self.energy.clone()
 
     def graph(self, pairs: int):

@@ -75,6 +75,17 @@ def test_verlet_fp64_bit_exact_over_mixed_calls():
     for i in range(6):
         assert acc[i] == pytest.approx(ref.acc[i], rel=1e-11, abs=1e-15), i
     assert acc[_lib.BED_WALL_WORK] == pytest.approx(ref.acc[6], rel=1e-11, abs=1e-15)
+    # energies: the kinetic term of the full-step velocity, as the reference's
+    e = eng.energies()[0].cpu().numpy()
+    oe = ref.energies()
+    assert e[0] == pytest.approx(oe["kinetic_J"], rel=1e-12)
+    assert e[1] == pytest.approx(oe["potential_J"], rel=1e-12)
+    assert e[2] == pytest.approx(oe["elastic_normal_J"], rel=1e-10)
+    # reading energies or velocities changes nothing: one more step still matches
+    eng.step(1)
+    ref.step()
+    assert np.array_equal(eng.x.cpu().numpy(), ref.positions)
+    assert np.array_equal(eng.v.cpu().numpy(), ref.velocities)
 
 
 def test_verlet_restart_and_graph_replays_continue_bit_for_bit():
