@@ -146,10 +146,24 @@ def bed_parts(s: BedSpec, *, device="cuda", skin_factor: float = 0.4,
                     radius=torch.tensor(r, **f64))
 
 
+def cell_capacity(cell_size: float, r_min: float, floor: int = 8) -> int:
+    """Entries per hash cell that no packing can overflow: disks of radius
+    >= r_min centred in a cell of side h lie inside the square of side
+    h + 2 r_min, at a packing density of at most pi / sqrt(12), so at most
+    (h + 2 r_min)^2 pi/sqrt(12) / (pi r_min^2) of them fit; never below
+    `floor`."""
+    span = cell_size + 2.0 * r_min
+    bound = int(math.floor(span * span / (math.sqrt(12.0) * r_min * r_min) + 1e-9))
+    return max(int(floor), bound)
+
+
 class BedEngine:
     def __init__(self, specs: list[BedSpec], *, device="cuda", fp32: bool = False,
-                 skin_factor: float = 0.4, cand_slots: int = 16, cell_cap: int = 8,
+                 skin_factor: float = 0.4, cand_slots: int = 16, cell_cap: int | None = None,
                  runout: float = 1.0, parts: list[BedParts] | None = None):
+        """cell_cap None: the largest `cell_capacity` of the members (8 for
+        radii in [0.8, 1.2] r0; 22 for the floe bed's widest spread, radii in
+        [0.5, 1.5] r0)."""
         self.lib = _lib.load()
         self.device = torch.device(device)
         if self.device.type != "cuda" or not torch.cuda.is_available():
@@ -173,7 +187,10 @@ class BedEngine:
             blocks += pt.nblocks
             cells += pt.ncells
         self.n, self.nmembers, self.nblocks, self.ncells = n, nm, blocks, cells
-        self.cand_slots, self.cell_cap = cand_slots, cell_cap
+        if cell_cap is None:
+            cell_cap = max(cell_capacity(pt.member.cell_size, float(pt.radius.min()))
+                           for pt in parts)
+        self.cand_slots, self.cell_cap = cand_slots, int(cell_cap)
         dev = self.device
         f64 = dict(dtype=torch.float64, device=dev)
         self.x = torch.cat([pt.x0 for pt in parts])

@@ -203,3 +203,28 @@ def test_bed_restart_continues_bit_for_bit(tmp_path):
         assert np.array_equal(batch.accounting[1].cpu().numpy(), back.accounting[0].cpu().numpy())
     assert np.array_equal(batch.snapshot(1)["pair_xi"], back.snapshot()["pair_xi"])
     assert back.read_members()[0].time == batch.read_members()[1].time
+
+
+def test_polydisperse_bed_sizes_its_hash_and_matches_oracle():
+    """The floe bed's widest radius spread (polydispersity 1: radii in
+    [0.5, 1.5] r0, a 3:1 ratio): the engine sizes its hash cells by the
+    packing bound and stays bit-identical to the oracle, with no overflow."""
+    x, r = column_packing(800, R0, polydispersity=0.5, seed=21)
+    cfg = BedConfig()
+    pre = BedEngine([BedSpec(cfg, x, r)])
+    pre.step(40000)                        # fallen and collapsing: dense contacts
+    assert pre.status()[0] == 0 and pre.cell_cap == 22
+    x, v = pre.x.cpu().numpy(), pre.v.cpu().numpy()
+    ref = OracleBed(cfg, x, r)
+    ref.velocities[:] = v
+    eng = BedEngine([BedSpec(cfg, x, r, dt=ref.dt, velocities=v)])
+    for k in range(1, 17):
+        eng.step(250)
+        for _ in range(250):
+            ref.step()
+        assert eng.status()[0] == 0, f"step {250 * k}"
+        assert np.array_equal(eng.x.cpu().numpy(), ref.positions), f"step {250 * k}"
+        assert np.array_equal(eng.v.cpu().numpy(), ref.velocities), f"step {250 * k}"
+    got = eng.contact_pairs() - eng.read_members()[0].first
+    assert np.array_equal(got, oracle_pairs(ref))
+    assert len(ref.hist_keys) > 300
