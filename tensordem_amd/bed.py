@@ -146,14 +146,16 @@ def bed_parts(s: BedSpec, *, device="cuda", skin_factor: float = 0.4,
                     radius=torch.tensor(r, **f64))
 
 
-def cell_capacity(cell_size: float, r_min: float, floor: int = 8) -> int:
+def cell_capacity(cell_size: float, r_min: float, floor: int = 8,
+                  cell_size_y: float | None = None) -> int:
     """Entries per hash cell that no packing can overflow: disks of radius
-    >= r_min centred in a cell of side h lie inside the square of side
-    h + 2 r_min, at a packing density of at most pi / sqrt(12), so at most
-    (h + 2 r_min)^2 pi/sqrt(12) / (pi r_min^2) of them fit; never below
-    `floor`."""
-    span = cell_size + 2.0 * r_min
-    bound = int(math.floor(span * span / (math.sqrt(12.0) * r_min * r_min) + 1e-9))
+    >= r_min centred in an h x h_y cell lie inside the rectangle
+    (h + 2 r_min) x (h_y + 2 r_min), at a packing density of at most
+    pi / sqrt(12), so at most that area x pi/sqrt(12) / (pi r_min^2) of them
+    fit; never below `floor`."""
+    hy = cell_size if cell_size_y is None else cell_size_y
+    area = (cell_size + 2.0 * r_min) * (hy + 2.0 * r_min)
+    bound = int(math.floor(area / (math.sqrt(12.0) * r_min * r_min) + 1e-9))
     return max(int(floor), bound)
 
 

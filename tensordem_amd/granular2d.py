@@ -20,7 +20,8 @@ Differences a caller can see:
   first read of `contacts`); before that, the first read evaluates the
   current state.
 * A disk has at most `cand_slots` (24) candidate partners within
-  2*max(r) + skin and a grid cell holds at most `cell_cap` disks; beyond
+  2*max(r) + skin and a grid cell holds at most `cell_cap` disks (by default
+  the packing bound of the smallest disks in one cell, at least 16); beyond
   that `step` raises RuntimeError instead of computing a wrong answer.
 """
 from __future__ import annotations
@@ -230,7 +231,7 @@ class DiskSystem:
     def __init__(self, positions, radii, config: DiskConfig | None = None, *,
                  walls=None, hull=None, periodic_x: float | None = None,
                  periodic_y: float | None = None, wall_velocity=None, device="cuda",
-                 cand_slots: int = 24, cell_cap: int = 16) -> None:
+                 cand_slots: int = 24, cell_cap: int | None = None) -> None:
         self.config = cfg = config or DiskConfig()
         pos = np.array(positions, dtype=np.float64)
         rad = np.array(radii, dtype=np.float64).reshape(-1)
@@ -318,6 +319,11 @@ class DiskSystem:
         p.reach = 2.0 * float(rad.max()) + cfg.skin
         self._grid(pos)
         p.n, p.n_walls, p.n_hull = n, len(self.walls), len(self.hull)
+        if cell_cap is None:
+            # the packing bound of the smallest disks in one cell (at least 16)
+            from .bed import cell_capacity
+            cell_cap = cell_capacity(p.cell_h[0], float(rad.min()), floor=16,
+                                     cell_size_y=p.cell_h[1])
         p.cand_slots, p.cell_cap = int(cand_slots), int(cell_cap)
         p.nblocks = self.lib.tdm_disk_blocks(n)
         wsb = self.lib.tdm_disk_workspace_bytes(n, p.ncell[0] * p.ncell[1], p.cell_cap,

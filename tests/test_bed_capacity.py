@@ -35,3 +35,11 @@ def test_column_keeps_eight_and_wide_spreads_grow():
     # the floe bed's widest spread (polydispersity 1: radii in [0.5, 1.5] r0)
     assert cell_capacity(2 * 0.015 + 0.004, 0.005) == 22
     assert cell_capacity(0.01, 0.01) == 8          # never below the floor
+
+
+def test_rectangular_cells():
+    # (h + 2r)(hy + 2r) / (sqrt(12) r^2) for an elongated cell
+    h, hy, r = 0.03, 0.06, 0.005
+    want = math.floor((h + 2 * r) * (hy + 2 * r) / (math.sqrt(12.0) * r * r))
+    assert cell_capacity(h, r, floor=0, cell_size_y=hy) == want
+    assert cell_capacity(h, r, floor=0, cell_size_y=h) == cell_capacity(h, r, floor=0)
