@@ -726,7 +726,9 @@ __device__ __forceinline__ void bed_step_body(const BKArgs &k)
       }
       k.v_out[p] = vf;
     } else {
-      k.w.mask[(size_t)lp * n + p] = nmask;
+      // the mask changes only when a contact forms or breaks (a search step
+      // writes the new list's buffer): most lanes skip the store
+      if (mode == MODE_SEARCH || nmask != mask) k.w.mask[(size_t)lp * n + p] = nmask;
       c_counts = (double)npair + kWallCount * (double)nwall;
 
       // velocity-Verlet members: v is a full-step velocity only on a new or
