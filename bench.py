@@ -449,8 +449,9 @@ BED_N = 1 << 20
 BED_R0 = 0.01
 # compulsory bytes per disk per step of bed_step_kernel, before the list- and
 # contact-dependent terms: x, v read (32) and written (32), radius (8), list
-# origin xb (16), contact mask read + written (8), candidate count (1)
-BED_BYTES_FIXED = 97
+# origin xb (16), contact mask read (4; written only when a contact forms or
+# breaks), candidate count (1)
+BED_BYTES_FIXED = 93
 
 
 def bed_bytes_per_disk(cand_mean, contacts_per_disk, wall_per_disk, hist_bytes):
