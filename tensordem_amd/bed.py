@@ -149,13 +149,17 @@ def bed_parts(s: BedSpec, *, device="cuda", skin_factor: float = 0.4,
 def cell_capacity(cell_size: float, r_min: float, floor: int = 8,
                   cell_size_y: float | None = None) -> int:
     """Entries per hash cell that no packing can overflow: disks of radius
-    >= r_min centred in an h x h_y cell lie inside the rectangle
-    (h + 2 r_min) x (h_y + 2 r_min), at a packing density of at most
-    pi / sqrt(12), so at most that area x pi/sqrt(12) / (pi r_min^2) of them
-    fit; never below `floor`."""
+    >= r_min whose pairwise overlaps stay below 2 % of r_min hold disjoint
+    concentric disks of radius r = 0.99 r_min; those centred in an h x h_y
+    cell lie inside the rectangle (h + 2 r) x (h_y + 2 r), at a packing
+    density of at most pi / sqrt(12), so at most that area x pi/sqrt(12) /
+    (pi r^2) of them fit; never below `floor`.  (Contact overlaps of a
+    settled bed are of order 1e-4 of a radius; a cell that overflows anyway
+    is flagged, never dropped silently.)"""
     hy = cell_size if cell_size_y is None else cell_size_y
-    area = (cell_size + 2.0 * r_min) * (hy + 2.0 * r_min)
-    bound = int(math.floor(area / (math.sqrt(12.0) * r_min * r_min) + 1e-9))
+    r = 0.99 * r_min
+    area = (cell_size + 2.0 * r) * (hy + 2.0 * r)
+    bound = int(math.floor(area / (math.sqrt(12.0) * r * r) + 1e-9))
     return max(int(floor), bound)
 
 
