@@ -13,6 +13,10 @@ merge), which runs a short horizon on the host.
   step's contact-pair set equals the oracle's (decided on the same state),
   and after K steps positions stay within 1e-9 m (1e-7 of the disk radius)
   of the oracle's.
+* The floe bed's velocity-Verlet (scale_ceiling/kernel.py:410-421) from the
+  same state, FP64 forces: contact pairs, springs, positions, half-step
+  velocities and the full-step readout bit-identical to the oracle's, over
+  single steps and a multi-step call.
 """
 import numpy as np
 import pytest
@@ -95,4 +99,33 @@ def test_c3_fp32_contact_set_and_positions(state):
     print(f"fp32 after {K} steps: max |dx| = {err:.3e} m, max |dv| = {verr:.3e} m/s, "
           f"contacts {len(snap['pairs'])} vs {len(ref.hist_keys)}", flush=True)
     assert err <= 1e-9
+    eng.close()
+
+
+def test_c3_verlet_fp64_bit_identical_to_oracle(state):
+    vstate = dict(state, config=dict(state["config"], integrator="verlet"), verlet_start=1)
+    ref = _oracle(vstate)
+    assert ref.integrator == "verlet" and ref.verlet_start
+    eng = BedEngine.from_snapshot(vstate, fp32=False)
+
+    def same(tag):
+        snap = eng.snapshot()
+        assert np.array_equal(_keys(snap), ref.hist_keys), f"contact-pair set differs {tag}"
+        assert np.array_equal(snap["pair_xi"], ref.hist_xi), f"spring histories differ {tag}"
+        assert np.array_equal(snap["positions"], ref.positions), f"positions differ {tag}"
+        assert np.array_equal(snap["velocities"], ref.velocities), f"velocities differ {tag}"
+        return snap
+
+    for k in range(1, 6):
+        eng.step(1)
+        ref.step()
+        snap = same(f"at step {k}")
+        print(f"verlet step {k}: {len(snap['pairs'])} contacts, identical", flush=True)
+    eng.step(6)
+    for _ in range(6):
+        ref.step()
+    same("after a 6-step call")
+    assert np.array_equal(eng.full_velocities().cpu().numpy(), ref.full_velocity()), \
+        "full-step velocities differ"
+    assert int(eng.status()[0]) == 0
     eng.close()
